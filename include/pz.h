@@ -1,0 +1,164 @@
+/*
+ * pz.h -- C ABI of the MI355X (gfx950) bootstrapping engine for
+ * phantom-zone's boolean FHE.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns
+ * an int status (PZ_OK = 0, negative on error; pz_last_error() gives the
+ * message) and never aborts or throws across the boundary.  Buffers named
+ * d_* are device pointers (HBM) and all such calls are ordered on `stream`
+ * (a hipStream_t, NULL = the context's stream).  Buffers named h_* are host
+ * pointers.  A context is single-stream and not thread-safe, like the
+ * reference's thread-local evaluator; a key handle is immutable and may be
+ * shared by contexts on the same device, like the reference's OnceLock key.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (gausslabs/phantom-zone, Rust).  Layouts are the reference's: an RLWE
+ * ciphertext is [a | b] (2 x N u64), an LWE ciphertext is [b, a_1..a_n],
+ * matrices are row-major rows of N words.
+ */
+#ifndef PZ_H
+#define PZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PZ_OK 0
+#define PZ_ERR_ARG (-1)
+#define PZ_ERR_HIP (-2)
+#define PZ_ERR_UNSUPPORTED (-3)
+#define PZ_ERR_STATE (-4)
+
+/* BoolParameters (src/bool/parameters.rs:96-167). */
+typedef struct pz_params {
+    uint64_t rlwe_q;      /* Q, prime                                 */
+    uint32_t rlwe_n;      /* N = 512 .. 4096                          */
+    uint32_t lwe_n;       /* n                                        */
+    uint64_t lwe_q;       /* q_ks, power of two <= 2^17              */
+    uint32_t br_q;        /* blind-rotation modulus                   */
+    uint32_t lwe_logb, lwe_d;
+    uint32_t rlrg_logb, rlrg_da, rlrg_db;
+    uint32_t auto_logb, auto_d;
+    uint32_t g, w;
+    uint32_t lwe_secret_gaussian;
+} pz_params;
+
+typedef struct pz_ctx pz_ctx;
+typedef struct pz_key pz_key;
+
+/* Gate ids of the BooleanGates seam (src/bool/mod.rs:22-71). */
+enum { PZ_NAND = 0, PZ_AND = 1, PZ_OR = 2, PZ_NOR = 3, PZ_XOR = 4, PZ_XNOR = 5 };
+
+/* Named constant parameter sets, src/bool/parameters.rs:526-726:
+ * "NI_2P", "NI_4P_HB_FR", "NI_8P", "SP_TEST", "I_2P_LB_SR", "I_4P", "I_8P",
+ * "NI_4P_LB_SR". */
+int pz_params_named(const char *name, pz_params *out);
+
+const char *pz_last_error(void);
+
+/* ---- context: BoolEvaluator::new (src/bool/evaluator.rs:651-784) with
+ * NttInit::new (src/ntt.rs:388-394), ModInit::new
+ * (src/backend/modulus_u64.rs:18-43) and the decomposers
+ * (src/decomposer.rs:158-178). */
+int pz_ctx_create(const pz_params *p, int device, pz_ctx **out);
+int pz_ctx_destroy(pz_ctx *ctx);
+/* psi (the 2N-th root NttBackendU64 derives from ChaCha8 seed 0,
+ * src/ntt.rs:323-328), and the number of compute units / resident PBS work
+ * groups the context uses. */
+int pz_ctx_info(pz_ctx *ctx, uint64_t *psi, uint32_t *num_cu, uint32_t *pbs_groups_per_cu);
+/* Host copy of the NTT tables: psi_bo, psi_bo_shoup, psi_inv_bo,
+ * psi_inv_bo_shoup (N each), n_inv, n_inv_shoup (src/ntt.rs:310-386). */
+int pz_ctx_ntt_tables(pz_ctx *ctx, uint64_t *h_out);
+int pz_ctx_stream(pz_ctx *ctx, void **stream);
+
+/* ---- device memory helpers ---- */
+int pz_malloc(pz_ctx *ctx, size_t bytes, void **d_ptr);
+int pz_free(pz_ctx *ctx, void *d_ptr);
+int pz_memcpy_htod(pz_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int pz_memcpy_dtoh(pz_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int pz_sync(pz_ctx *ctx);
+
+/* ---- NTT: Ntt::{forward_lazy, forward, backward_lazy, backward}
+ * (src/ntt.rs:16-22, impl :396-442), in place on nrows rows of N words.
+ * Output ranges as the reference: forward [0,q), forward_lazy [0,2q),
+ * backward [0,q), backward_lazy [0,2q). */
+int pz_ntt_forward(pz_ctx *ctx, uint64_t *d_rows, size_t nrows, void *stream);
+int pz_ntt_forward_lazy(pz_ctx *ctx, uint64_t *d_rows, size_t nrows, void *stream);
+int pz_ntt_backward(pz_ctx *ctx, uint64_t *d_rows, size_t nrows, void *stream);
+int pz_ntt_backward_lazy(pz_ctx *ctx, uint64_t *d_rows, size_t nrows, void *stream);
+
+/* ---- ShoupMatrixFMA::shoup_matrix_fma (src/backend/mod.rs:137-141, impl
+ * src/backend/modulus_u64.rs:231-264): d_out[j] += sum_r a[r][j]*b[r][j],
+ * a/a_shoup/b are d rows of n words; d_out stays in [0,2q). */
+int pz_shoup_matrix_fma(pz_ctx *ctx, uint64_t *d_out, const uint64_t *d_a,
+                        const uint64_t *d_a_shoup, const uint64_t *d_b, uint32_t d, size_t n,
+                        void *stream);
+
+/* ---- Decomposer::decompose_iter over `count` values, digits scattered as
+ * decompose_r does (src/rgsw/runtime.rs:382-399): d_digits[r*count + i] is
+ * digit r of value i.  which: 0 RLWE x RGSW part a, 1 part b, 2 auto,
+ * 3 LWE key switch (src/bool/parameters.rs decomposer getters). */
+int pz_decompose(pz_ctx *ctx, int which, const uint64_t *d_in, uint64_t *d_digits,
+                 size_t count, void *stream);
+
+/* ---- server key: NonInteractiveServerKeyEvaluationDomain +
+ * ShoupNonInteractiveServerKeyEvaluationDomain (src/bool/keys.rs:851-1005,
+ * :1068-1173) implementing PbsKey (src/pbs.rs:16-27).  Host inputs in the
+ * reference layout: h_rgsw[n][2da+2db][N] and h_auto[w+1][2*d_auto][N] in the
+ * evaluation domain, h_lwe_ksk[N*d_ks][n+1].  Shoup twins are computed on the
+ * device. */
+int pz_key_upload(pz_ctx *ctx, const uint64_t *h_rgsw, const uint64_t *h_auto,
+                  const uint64_t *h_lwe_ksk, pz_key **out);
+int pz_key_free(pz_key *key);
+/* Device bytes held by the key (serving admission by HBM footprint). */
+int pz_key_bytes(pz_key *key, size_t *bytes);
+
+/* ---- rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) on B RLWE
+ * ciphertexts d_rlwe[B][2][N]; RLWE b is multiplied by RGSW(X^{z_i}) with
+ * i = d_rgsw_idx[b]; d_trivial[b] != 0 skips part a (trivial RLWE). */
+int pz_rlwe_by_rgsw(pz_ctx *ctx, pz_key *key, uint64_t *d_rlwe, const uint32_t *d_rgsw_idx,
+                    const uint8_t *d_trivial, size_t B, void *stream);
+
+/* ---- rlwe_auto_shoup (src/rgsw/runtime.rs:530-643): automorphism with key
+ * index d_auto_idx[b] in [0, w] (0 = -g, k = g^k; src/bool/evaluator.rs:
+ * 730-743). */
+int pz_rlwe_auto(pz_ctx *ctx, pz_key *key, uint64_t *d_rlwe, const uint32_t *d_auto_idx,
+                 const uint8_t *d_trivial, size_t B, void *stream);
+
+/* ---- lwe_key_switch (src/lwe.rs:14-44): d_in[B][N+1] (mod q_ks) ->
+ * d_out[B][n+1] (mod q_ks). */
+int pz_lwe_key_switch(pz_ctx *ctx, pz_key *key, const uint64_t *d_in, uint64_t *d_out,
+                      size_t B, void *stream);
+
+/* ---- pbs (src/pbs.rs:104-233): B independent bootstraps of
+ * d_lwe[B][N+1] (mod Q, overwritten), test vector d_gate[b] (gate id).
+ * Runs moddown + key switch + persistent blind rotation + sample extract. */
+int pz_pbs(pz_ctx *ctx, pz_key *key, const uint32_t *d_gate, uint64_t *d_lwe, size_t B,
+           void *stream);
+
+/* ---- BooleanGates batch (src/bool/evaluator.rs:2136-2250): d_out[b] =
+ * gate d_ops[b] of (d_c0[b], d_c1[b]).  The first call for a buffer set and
+ * batch size captures the three launches in a hipGraph; later calls with the
+ * same buffers replay it. */
+int pz_gates(pz_ctx *ctx, pz_key *key, const uint32_t *d_ops, const uint64_t *d_c0,
+             const uint64_t *d_c1, uint64_t *d_out, size_t B, void *stream);
+/* not_inplace (src/bool/evaluator.rs:2236-2239) on B ciphertexts. */
+int pz_not(pz_ctx *ctx, uint64_t *d_c, size_t B, void *stream);
+
+/* Automorphism count of each bootstrap of the last pz_pbs / pz_gates batch
+ * (the blind rotation always runs n RLWE x RGSW products; the number of
+ * automorphisms depends on the input).  Used for the algorithmic work count. */
+int pz_last_auto_counts(pz_ctx *ctx, uint32_t *h_counts, size_t B);
+
+/* Per-kernel timing of the last pz_gates / pz_pbs call made with timing
+ * enabled (milliseconds from HIP events on the launch stream). */
+int pz_set_timing(pz_ctx *ctx, int enabled);
+int pz_last_timing(pz_ctx *ctx, float *ms_prep, float *ms_ks, float *ms_blind_rotate);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,183 @@
+/*
+ * pz_oracle.h -- CPU restatement of phantom-zone's bootstrapping (PBS) path.
+ *
+ * ORACLE / TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or
+ * calls this code; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg do, and only as the checker (or as the timed CPU baseline).
+ *
+ * Every function follows one reference routine loop for loop (same Shoup /
+ * Barrett formulas, same lazy ranges, same stage order, same bucket order,
+ * same f64 moddown); the citation is on each definition in pz_oracle.c.
+ *
+ * Parity status per function (how each is pinned) is listed in the header
+ * comment of pz_oracle.c.
+ */
+#ifndef PZ_ORACLE_H
+#define PZ_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Parameter set (src/bool/parameters.rs:96-167). */
+typedef struct pzo_params {
+    uint64_t rlwe_q;        /* Q (prime, non-native)                          */
+    uint32_t rlwe_n;        /* N (ring size)                                  */
+    uint32_t lwe_n;         /* n (LWE dimension)                              */
+    uint64_t lwe_q;         /* q_ks (power of two)                            */
+    uint32_t br_q;          /* blind-rotation modulus                         */
+    uint32_t lwe_logb, lwe_d;               /* LWE key switch decomposer      */
+    uint32_t rlrg_logb, rlrg_da, rlrg_db;   /* RLWE x RGSW decomposer (a, b)  */
+    uint32_t auto_logb, auto_d;             /* automorphism decomposer        */
+    uint32_t g, w;                          /* generator, window              */
+    uint32_t lwe_secret_gaussian;           /* 1: error dist, 0: ternary      */
+} pzo_params;
+
+/* Named parameter sets: 0 NI_2P, 1 NI_4P_HB_FR, 2 NI_8P, 3 SP_TEST_BOOL_PARAMS,
+ * 4 I_2P_LB_SR, 5 I_4P, 6 I_8P, 7 NI_4P_LB_SR. Returns 0 on success. */
+int pzo_params_named(int which, pzo_params *out);
+
+/* ---------------- ChaCha (rand_chacha 0.3.1 state layout) ---------------- */
+void pzo_chacha_block(const uint8_t key[32], uint64_t counter, uint64_t stream,
+                      int rounds, uint32_t out[16]);
+typedef struct pzo_rng pzo_rng;
+pzo_rng *pzo_rng_new(const uint8_t seed[32]);
+void pzo_rng_free(pzo_rng *r);
+uint32_t pzo_rng_next_u32(pzo_rng *r);
+uint64_t pzo_rng_next_u64(pzo_rng *r);
+void pzo_rng_fill_bytes(pzo_rng *r, uint8_t *out, size_t n);
+uint64_t pzo_rng_uniform(pzo_rng *r, uint64_t range);   /* [0, range) */
+int64_t pzo_rng_gauss_round(pzo_rng *r);
+int64_t pzo_rng_gauss_trunc(pzo_rng *r);
+void pzo_ternary_hw(pzo_rng *r, int32_t *out, uint32_t n, uint32_t hw);
+
+/* ---------------- scalar arithmetic ---------------- */
+uint64_t pzo_shoup_repr(uint64_t v, uint64_t q);
+uint64_t pzo_shoup_mul(uint64_t a, uint64_t b, uint64_t b_shoup, uint64_t q);
+uint64_t pzo_mul_mod_fast(uint64_t q, uint64_t a, uint64_t b);
+uint64_t pzo_mul_mod_fast_lazy(uint64_t q, uint64_t a, uint64_t b);
+uint64_t pzo_mod_exponent(uint64_t a, uint64_t b, uint64_t q);
+void pzo_shoup_matrix_fma(uint64_t q, uint64_t *out, const uint64_t *a,
+                          const uint64_t *a_shoup, const uint64_t *b,
+                          uint32_t d, uint32_t n);
+
+/* ---------------- NTT (src/ntt.rs) ---------------- */
+typedef struct pzo_ntt pzo_ntt;
+pzo_ntt *pzo_ntt_new(uint64_t q, uint32_t n);
+void pzo_ntt_free(pzo_ntt *t);
+uint64_t pzo_ntt_psi(const pzo_ntt *t);
+/* tables: [psi_bo, psi_bo_shoup, psi_inv_bo, psi_inv_bo_shoup] each n, then
+ * n_inv, n_inv_shoup  (4n + 2 words) */
+void pzo_ntt_tables(const pzo_ntt *t, uint64_t *out);
+void pzo_ntt_forward_lazy(const pzo_ntt *t, uint64_t *v);
+void pzo_ntt_forward(const pzo_ntt *t, uint64_t *v);
+void pzo_ntt_backward_lazy(const pzo_ntt *t, uint64_t *v);
+void pzo_ntt_backward(const pzo_ntt *t, uint64_t *v);
+void pzo_ntt_rows(const pzo_ntt *t, int which, uint64_t *v, uint32_t rows);
+void pzo_negacyclic_mul_schoolbook(uint64_t q, const uint64_t *a,
+                                   const uint64_t *b, uint64_t *out, uint32_t n);
+
+/* ---------------- decomposer (src/decomposer.rs) ---------------- */
+/* digits out: count x d, row-major per value */
+void pzo_decompose_iter(uint64_t q, uint32_t logb, uint32_t d,
+                        const uint64_t *v, uint64_t *digits, size_t count);
+void pzo_decompose_to_vec(uint64_t q, uint32_t logb, uint32_t d,
+                          const uint64_t *v, uint64_t *digits, size_t count);
+uint64_t pzo_round_value(uint64_t v, uint32_t ignore_bits);
+uint64_t pzo_recompose(uint64_t q, uint32_t logb, uint32_t d, const uint64_t *limbs);
+uint32_t pzo_decomposer_ignore_bits(uint64_t q, uint32_t logb, uint32_t d);
+
+/* ---------------- evaluator (BoolEvaluator + BoolPbsInfo) ---------------- */
+typedef struct pzo_eval pzo_eval;
+pzo_eval *pzo_eval_new(const pzo_params *p);
+void pzo_eval_free(pzo_eval *e);
+const pzo_ntt *pzo_eval_ntt(const pzo_eval *e);
+void pzo_eval_dlog_map(const pzo_eval *e, uint32_t *out);            /* br_q */
+int64_t pzo_eval_auto_element(const pzo_eval *e, uint32_t k);
+void pzo_eval_auto_map(const pzo_eval *e, uint32_t k, uint32_t *idx, uint8_t *sign);
+/* gate ids: 0 NAND, 1 AND, 2 OR, 3 NOR, 4 XOR, 5 XNOR */
+void pzo_eval_test_vec(const pzo_eval *e, uint32_t gate, uint64_t *out);  /* br_q/2 */
+void pzo_generate_auto_map(uint32_t ring_size, int64_t k, uint32_t *idx, uint8_t *sign);
+
+/* key sizes in u64 words */
+typedef struct pzo_key_sizes {
+    size_t rgsw_rows;       /* 2*da + 2*db */
+    size_t rgsw_words;      /* lwe_n * rgsw_rows * N */
+    size_t auto_rows;       /* 2*d_auto */
+    size_t auto_words;      /* (w+1) * auto_rows * N */
+    size_t ksk_rows;        /* N * lwe_d */
+    size_t ksk_words;       /* ksk_rows * (lwe_n + 1) */
+} pzo_key_sizes;
+void pzo_key_sizes_get(const pzo_eval *e, pzo_key_sizes *out);
+
+/* In-memory server key, reference layout: rgsw[i][row][N] and auto[k][row][N]
+ * in evaluation domain with Shoup twins; lwe_ksk[row][n+1]. Caller owns. */
+typedef struct pzo_key {
+    uint64_t *rgsw, *rgsw_shoup;
+    uint64_t *auto_key, *auto_shoup;
+    uint64_t *lwe_ksk;
+} pzo_key;
+
+/* seeded key generation (this build's own procedure, see pz_oracle.c) */
+void pzo_keygen_secrets(const pzo_eval *e, const uint8_t seed[32], uint32_t parties,
+                        int32_t *s_rlwe, int32_t *z_lwe);
+void pzo_keygen_server(const pzo_eval *e, const uint8_t seed[32],
+                       const int32_t *s_rlwe, const int32_t *z_lwe, pzo_key *key);
+void pzo_key_shoup(const pzo_eval *e, pzo_key *key);
+
+/* runtime ops (src/rgsw/runtime.rs, src/lwe.rs, src/pbs.rs) */
+void pzo_rlwe_by_rgsw_shoup(const pzo_eval *e, uint64_t *rlwe, const uint64_t *rgsw,
+                            const uint64_t *rgsw_shoup, int is_trivial);
+void pzo_rlwe_by_rgsw(const pzo_eval *e, uint64_t *rlwe, const uint64_t *rgsw,
+                      int is_trivial);
+void pzo_rlwe_auto_shoup(const pzo_eval *e, uint64_t *rlwe, const uint64_t *ksk,
+                         const uint64_t *ksk_shoup, uint32_t k, int is_trivial);
+void pzo_rlwe_auto(const pzo_eval *e, uint64_t *rlwe, const uint64_t *ksk,
+                   uint32_t k, int is_trivial);
+void pzo_lwe_key_switch(const pzo_eval *e, uint64_t *lwe_out, const uint64_t *lwe_in,
+                        const uint64_t *ksk);
+uint64_t pzo_moddown(const pzo_eval *e, uint64_t v);
+uint32_t pzo_mod_switch_odd(double v, double from_q, double to_q);
+void pzo_sample_extract(const pzo_eval *e, uint64_t *lwe_out, const uint64_t *rlwe,
+                        uint32_t index);
+void pzo_monomial_mul(uint64_t q, const uint64_t *p_in, uint64_t *p_out,
+                      uint32_t mon_exp, int mon_sign, uint32_t ring_size);
+
+/* PBS statistics: counts of each operation in the blind rotation */
+typedef struct pzo_pbs_stats {
+    uint32_t ext_products;
+    uint32_t autos;
+    uint32_t trivial_autos;
+} pzo_pbs_stats;
+void pzo_pbs(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint64_t *lwe,
+             pzo_pbs_stats *stats);
+/* Blind-rotation schedule of one PBS input (after key switch): a list of
+ * ops, each (kind, arg, trivial) packed as kind | arg<<8 | trivial<<31:
+ * kind 0 = RLWE x RGSW with LWE index arg, kind 1 = auto with key index arg.
+ * Returns the number of ops written (cap = max ops). */
+uint32_t pzo_br_schedule(const pzo_eval *e, const uint64_t *lwe_ks, uint32_t *ops,
+                         uint32_t cap);
+/* The gate pre-processing + PBS (src/bool/evaluator.rs:2118-2250). */
+void pzo_gate(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint64_t *c0,
+              const uint64_t *c1);
+void pzo_not(const pzo_eval *e, uint64_t *c0);
+/* batched PBS timing helper for the CPU baseline: returns seconds */
+double pzo_pbs_batch_threads(const pzo_eval *e, const pzo_key *key, uint32_t gate,
+                             uint64_t *lwes, uint32_t count, uint32_t threads);
+
+/* encryption helpers (tests) */
+void pzo_encrypt_lwe(const pzo_eval *e, const uint8_t seed[32], const int32_t *s,
+                     uint32_t dim, uint64_t m, uint64_t *out);
+uint64_t pzo_decrypt_lwe(const pzo_eval *e, const uint64_t *lwe, const int32_t *s,
+                         uint32_t dim);
+int pzo_decode_bool(const pzo_eval *e, uint64_t m);
+uint64_t pzo_true_el(const pzo_eval *e);
+uint64_t pzo_false_el(const pzo_eval *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

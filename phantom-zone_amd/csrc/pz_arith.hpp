@@ -1,0 +1,157 @@
+// pz_arith.hpp -- device integer arithmetic for the PBS path on gfx950.
+//
+// Every routine is the exact integer function the reference computes (same
+// Shoup quotient, same lazy ranges, same decomposition carries), written for
+// 64-wide wavefronts: 64-bit words live in VGPR pairs and the 64x64->128
+// high product is emulated from 32-bit v_mad_u64_u32 / v_mul_hi_u32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+#define PZ_DEV __device__ __forceinline__
+
+// hi 64 bits of a*b (the Shoup quotient). __umul64hi lowers to three
+// v_mad_u64_u32 plus one v_mul_hi_u32 on CDNA.
+PZ_DEV u64 mulhi64(u64 a, u64 b) { return __umul64hi(a, b); }
+
+// ShoupMul::mul, src/utils.rs:43-46: b*a - q*floor(b_shoup*a/2^64) (wrapping),
+// result in [0, 2q) for any a < 2^64.
+PZ_DEV u64 shoup_mul(u64 a, u64 w, u64 ws, u64 q)
+{
+    return w * a - q * mulhi64(ws, a);
+}
+
+// forward_butterly_0_to_4q, src/ntt.rs:31-49
+PZ_DEV void fb_4q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)
+{
+    u64 xx = x >= q2 ? x - q2 : x;
+    u64 t = shoup_mul(y, w, ws, q);
+    x = xx + t;
+    y = xx + q2 - t;
+}
+
+PZ_DEV u64 umin(u64 a, u64 b) { return a < b ? a : b; }
+
+// forward_butterly_0_to_2q, src/ntt.rs:51-75
+PZ_DEV void fb_2q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)
+{
+    u64 xx = x >= q2 ? x - q2 : x;
+    u64 t = shoup_mul(y, w, ws, q);
+    u64 ox = xx + t, oy = xx - t;
+    x = umin(ox, ox - q2);
+    y = umin(oy, oy + q2);
+}
+
+// inverse_butterfly_0_to_2q, src/ntt.rs:84-104
+PZ_DEV void ib_2q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)
+{
+    u64 xd = x + y;
+    xd = xd >= q2 ? xd - q2 : xd;
+    u64 t = x + q2 - y;
+    x = xd;
+    y = shoup_mul(t, w, ws, q);
+}
+
+// ModularOpsU64 add_mod_fast / sub_mod_fast / neg (src/backend/modulus_u64.rs:46-77,
+// :147-149)
+PZ_DEV u64 add_mod(u64 a, u64 b, u64 q)
+{
+    u64 o = a + b;
+    return o >= q ? o - q : o;
+}
+PZ_DEV u64 sub_mod(u64 a, u64 b, u64 q) { return a >= b ? a - b : (q + a) - b; }
+PZ_DEV u64 neg_mod(u64 a, u64 q) { return q - a; }
+
+// One Shoup multiply-accumulate step of shoup_matrix_fma
+// (src/backend/modulus_u64.rs:252-260): out in [0,2q) -> [0,2q).
+PZ_DEV u64 shoup_fma(u64 out, u64 a, u64 as, u64 b, u64 q, u64 q2)
+{
+    u64 quot = mulhi64(as, b);
+    u64 v = a * b + out;
+    v = v - q * quot;
+    return v >= q2 ? v - q2 : v;
+}
+
+// Barrett mul_mod_fast (src/backend/modulus_u64.rs:106-130) for keygen-side
+// products. Constants: logq = 64 - clz(q+1), mu = 2^(2logq+3)/q, alpha = logq+3.
+struct Barrett {
+    u64 q;
+    u32 logq, alpha;
+    u64 mu_lo, mu_hi;
+};
+
+PZ_DEV void mul128(u64 a, u64 b, u64 &lo, u64 &hi)
+{
+    lo = a * b;
+    hi = __umul64hi(a, b);
+}
+
+PZ_DEV u64 barrett_mul(const Barrett &br, u64 a, u64 b)
+{
+    u64 ab_lo, ab_hi;
+    mul128(a, b, ab_lo, ab_hi);
+    // tmp = ab >> (logq-2)
+    u32 s = br.logq - 2;
+    u64 t_lo = (ab_lo >> s) | (ab_hi << (64 - s));
+    u64 t_hi = ab_hi >> s;
+    // k = (tmp * mu) >> (alpha + 2); tmp*mu < 2^128 by the reference's bounds
+    u64 p0_lo, p0_hi, p1_lo, p1_hi, p2_lo, p2_hi;
+    mul128(t_lo, br.mu_lo, p0_lo, p0_hi);
+    mul128(t_lo, br.mu_hi, p1_lo, p1_hi);
+    mul128(t_hi, br.mu_lo, p2_lo, p2_hi);
+    u64 mid = p0_hi + p1_lo;
+    u64 c1 = mid < p0_hi;
+    u64 mid2 = mid + p2_lo;
+    u64 c2 = mid2 < mid;
+    u64 top = p1_hi + p2_hi + t_hi * br.mu_hi + c1 + c2;
+    // (p0_lo, mid2, top) is the product's low 192 bits; take >> (alpha+2)
+    u32 sh = br.alpha + 2;  // < 64 for logq <= 60
+    u64 k_lo = (mid2 >> sh) | (top << (64 - sh));
+    (void)p0_lo;
+    // out = (ab - k*q) mod 2^64
+    u64 out = ab_lo - k_lo * br.q;
+    return out >= br.q ? out - br.q : out;
+}
+
+// Gadget decomposer (src/decomposer.rs:158-178, :217-234, :264-309).
+struct Decomp {
+    u64 q;
+    u32 logb, d, ignore_bits;
+};
+
+// Digit `r` (0-based) of the signed decomposition of v. Release-build
+// semantics: for v == q the !(q - v) + 1 wraps to 0.
+PZ_DEV u64 decomp_prepare(const Decomp &dc, u64 value)
+{
+    if (value >= (dc.q >> 1)) value = ~(dc.q - value) + 1;
+    u32 ib = dc.ignore_bits;
+    if (ib != 0) {
+        u64 msb = (value & ((1ull << ib) - 1)) >> (ib - 1);
+        value = (value >> ib) + msb;
+    }
+    return value;
+}
+
+// One DecomposerIter::next step; returns the digit and advances `value`.
+PZ_DEV u64 decomp_next(const Decomp &dc, u64 &value)
+{
+    u64 bmask = (1ull << dc.logb) - 1;
+    u64 bby2 = 1ull << (dc.logb - 1);
+    u64 k = value & bmask;
+    value = (value - k) >> dc.logb;
+    u64 carry = (k > bby2) || (k == bby2 && (value & 1) == 1);
+    u64 neg_carry = 0 - carry;
+    value = value + carry;
+    return (neg_carry & dc.q) + k - (carry << dc.logb);
+}
+
+PZ_DEV u64 decomp_digit(const Decomp &dc, u64 v, u32 r)
+{
+    u64 value = decomp_prepare(dc, v);
+    u64 dg = 0;
+    for (u32 i = 0; i <= r; i++) dg = decomp_next(dc, value);
+    return dg;
+}

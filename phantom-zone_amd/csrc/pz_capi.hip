@@ -1,0 +1,617 @@
+// pz_capi.hip -- the extern "C" boundary (include/pz.h): contexts, resident
+// server keys, and the launch sequences of every operator.
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/pz.h"
+#include "pz_host.hpp"
+#include "pz_internal.hpp"
+
+using namespace pz;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                     \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            g_err = std::string(#x) + ": " + hipGetErrorString(e_);                   \
+            return PZ_ERR_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+struct GraphKey {
+    const void *ops, *c0, *c1, *out;
+    size_t B;
+    bool operator<(const GraphKey &o) const
+    {
+        return std::tie(ops, c0, c1, out, B) < std::tie(o.ops, o.c0, o.c1, o.out, o.B);
+    }
+};
+
+struct pz_ctx {
+    pz_params p;
+    int device = 0;
+    int logn = 0;
+    hipStream_t stream = nullptr;
+    DevCtx dc;
+    host::HostTables ht;
+    // device allocations of the tables
+    u64 *d_tables = nullptr;   // 4N
+    u32 *d_dlog = nullptr;
+    u64 *d_tv = nullptr;
+    // scratch of the PBS pipeline
+    int8_t *d_digits = nullptr;
+    u32 *d_ksb = nullptr, *d_ks = nullptr, *d_stats = nullptr;
+    size_t cap = 0;
+    int num_cu = 0, br_per_cu = 0;
+    // captured gate batches
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    const pz_key *graph_key = nullptr;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float t_prep = 0, t_ks = 0, t_br = 0;
+};
+
+struct pz_key {
+    pz_ctx *owner = nullptr;
+    DevKey dk;
+    u64 *rgsw = nullptr, *rgsw_s = nullptr, *auto_k = nullptr, *auto_s = nullptr;
+    void *ksk = nullptr;
+    size_t bytes = 0;
+};
+
+extern "C" const char *pz_last_error(void) { return g_err.c_str(); }
+
+extern "C" int pz_params_named(const char *name, pz_params *out)
+{
+    if (!name || !out) return fail(PZ_ERR_ARG, "null argument");
+    if (host::params_named(name, out) != 0) return fail(PZ_ERR_ARG, "unknown parameter set");
+    return PZ_OK;
+}
+
+static int check_params(const pz_params *p, int *logn)
+{
+    uint32_t N = p->rlwe_n;
+    int l = 0;
+    while ((1u << l) < N) l++;
+    if ((1u << l) != N || (l != 9 && l != 11))
+        return fail(PZ_ERR_UNSUPPORTED, "rlwe_n must be 512 or 2048");
+    if (p->rlwe_q >= (1ull << 61) || p->rlwe_q % (2ull * N) != 1)
+        return fail(PZ_ERR_ARG, "rlwe_q must be < 2^61 and 1 mod 2N");
+    if (p->lwe_q & (p->lwe_q - 1) || p->lwe_q > (1ull << 17) || p->lwe_q < 2)
+        return fail(PZ_ERR_UNSUPPORTED, "lwe_q must be a power of two <= 2^17");
+    if (p->br_q & (p->br_q - 1) || p->br_q > 2 * N || p->br_q < 16)
+        return fail(PZ_ERR_ARG, "br_q must be a power of two <= 2N");
+    if (p->w + 1 > (uint32_t)MAX_AUTO) return fail(PZ_ERR_UNSUPPORTED, "window too large");
+    if (p->lwe_n == 0 || p->lwe_n > 4096) return fail(PZ_ERR_UNSUPPORTED, "lwe_n out of range");
+    if (p->lwe_logb > 7 || p->lwe_logb == 0) return fail(PZ_ERR_UNSUPPORTED, "lwe_logb must be 1..7");
+    if (p->rlrg_da == 0 || p->rlrg_db == 0 || p->auto_d == 0 || p->lwe_d == 0)
+        return fail(PZ_ERR_ARG, "decomposition counts must be > 0");
+    uint32_t lq = host::ref_log2(p->rlwe_q);
+    if (p->rlrg_logb * p->rlrg_da > lq || p->rlrg_logb * p->rlrg_db > lq ||
+        p->auto_logb * p->auto_d > lq || p->lwe_logb * p->lwe_d > host::ref_log2(p->lwe_q))
+        return fail(PZ_ERR_ARG, "decomposer wants logq >= logb*d");
+    *logn = l;
+    return PZ_OK;
+}
+
+static Decomp make_decomp(uint64_t q, uint32_t logb, uint32_t d)
+{
+    Decomp dc;
+    dc.q = q;
+    dc.logb = logb;
+    dc.d = d;
+    dc.ignore_bits = host::ref_log2(q) - logb * d;
+    return dc;
+}
+
+extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
+{
+    if (!p || !out) return fail(PZ_ERR_ARG, "null argument");
+    int logn = 0;
+    int rc = check_params(p, &logn);
+    if (rc) return rc;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(PZ_ERR_ARG, "no such device");
+    HIPCHK(hipSetDevice(device));
+    pz_ctx *c = new pz_ctx();
+    c->p = *p;
+    c->device = device;
+    c->logn = logn;
+    uint32_t N = p->rlwe_n;
+    c->ht = host::make_tables(p->rlwe_q, N);
+    if (c->ht.psi == 0) {
+        delete c;
+        return fail(PZ_ERR_ARG, "no 2N-th root of unity found");
+    }
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PZ_ERR_HIP, "stream create failed");
+    }
+    std::vector<u64> tabs(4 * (size_t)N);
+    memcpy(tabs.data(), c->ht.psi_bo.data(), 8 * N);
+    memcpy(tabs.data() + N, c->ht.psi_bo_s.data(), 8 * N);
+    memcpy(tabs.data() + 2 * N, c->ht.ipsi_bo.data(), 8 * N);
+    memcpy(tabs.data() + 3 * N, c->ht.ipsi_bo_s.data(), 8 * N);
+    std::vector<uint32_t> dlog = host::dlog_map(*p);
+    std::vector<uint64_t> tv = host::test_vectors(*p);
+    if (hipMalloc(&c->d_tables, 8 * tabs.size()) != hipSuccess ||
+        hipMalloc(&c->d_dlog, 4 * dlog.size()) != hipSuccess ||
+        hipMalloc(&c->d_tv, 8 * tv.size()) != hipSuccess) {
+        pz_ctx_destroy(c);
+        return fail(PZ_ERR_HIP, "table allocation failed");
+    }
+    hipMemcpy(c->d_tables, tabs.data(), 8 * tabs.size(), hipMemcpyHostToDevice);
+    hipMemcpy(c->d_dlog, dlog.data(), 4 * dlog.size(), hipMemcpyHostToDevice);
+    hipMemcpy(c->d_tv, tv.data(), 8 * tv.size(), hipMemcpyHostToDevice);
+
+    DevCtx &d = c->dc;
+    memset(&d, 0, sizeof(d));
+    d.tb.psi = c->d_tables;
+    d.tb.psis = c->d_tables + N;
+    d.tb.ipsi = c->d_tables + 2 * N;
+    d.tb.ipsis = c->d_tables + 3 * N;
+    d.tb.n_inv = c->ht.n_inv;
+    d.tb.n_inv_s = c->ht.n_inv_s;
+    d.tb.q = p->rlwe_q;
+    d.tb.q2 = 2 * p->rlwe_q;
+    d.dec_a = make_decomp(p->rlwe_q, p->rlrg_logb, p->rlrg_da);
+    d.dec_b = make_decomp(p->rlwe_q, p->rlrg_logb, p->rlrg_db);
+    d.dec_auto = make_decomp(p->rlwe_q, p->auto_logb, p->auto_d);
+    d.dec_lwe = make_decomp(p->lwe_q, p->lwe_logb, p->lwe_d);
+    d.da = p->rlrg_da;
+    d.db = p->rlrg_db;
+    d.dauto = p->auto_d;
+    d.lwe_d = p->lwe_d;
+    d.n_lwe = p->lwe_n;
+    d.br_q = p->br_q;
+    d.w = p->w;
+    d.g = p->g;
+    d.embedding = (2 * N) / p->br_q;
+    d.lwe_q_log = host::ref_log2(p->lwe_q);
+    d.lwe_mask = p->lwe_q - 1;
+    d.lwe_qf = (double)p->lwe_q;
+    d.rlwe_qf = (double)p->rlwe_q;
+    d.qby4 = p->rlwe_q >> 2;
+    d.br.q = p->rlwe_q;
+    d.br.logq = 64 - (uint32_t)__builtin_clzll(p->rlwe_q + 1);
+    d.br.alpha = d.br.logq + 3;
+    host::u128 mu = ((host::u128)1 << (d.br.logq * 2 + 3)) / p->rlwe_q;
+    d.br.mu_lo = (u64)mu;
+    d.br.mu_hi = (u64)(mu >> 64);
+    d.dlog = c->d_dlog;
+    d.test_vec = c->d_tv;
+    for (uint32_t k = 0; k <= p->w; k++) {
+        int64_t el = k == 0 ? -(int64_t)p->g : 0;
+        if (k > 0) {
+            uint64_t gp = 1;
+            for (uint32_t i = 0; i < k; i++) gp *= p->g;
+            el = (int64_t)(gp % p->br_q);
+        }
+        d.auto_el[k] = host::auto_exp(el, N);
+    }
+
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    int per = 0;
+    size_t lds = br_lds_bytes(logn, p->br_q, p->lwe_n);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, blind_rotate_kernel(logn),
+                                                     1 << (logn - 3), lds) != hipSuccess ||
+        per < 1)
+        per = 1;
+    c->br_per_cu = per;
+    for (int i = 0; i < 4; i++) hipEventCreate(&c->ev[i]);
+    *out = c;
+    return PZ_OK;
+}
+
+static void drop_graphs(pz_ctx *c)
+{
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+}
+
+extern "C" int pz_ctx_destroy(pz_ctx *c)
+{
+    if (!c) return PZ_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    hipFree(c->d_tables);
+    hipFree(c->d_dlog);
+    hipFree(c->d_tv);
+    hipFree(c->d_digits);
+    hipFree(c->d_ksb);
+    hipFree(c->d_ks);
+    hipFree(c->d_stats);
+    for (int i = 0; i < 4; i++)
+        if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return PZ_OK;
+}
+
+extern "C" int pz_ctx_info(pz_ctx *c, uint64_t *psi, uint32_t *num_cu, uint32_t *per_cu)
+{
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    if (psi) *psi = c->ht.psi;
+    if (num_cu) *num_cu = (uint32_t)c->num_cu;
+    if (per_cu) *per_cu = (uint32_t)c->br_per_cu;
+    return PZ_OK;
+}
+
+extern "C" int pz_ctx_ntt_tables(pz_ctx *c, uint64_t *h)
+{
+    if (!c || !h) return fail(PZ_ERR_ARG, "null argument");
+    uint32_t N = c->p.rlwe_n;
+    memcpy(h, c->ht.psi_bo.data(), 8 * N);
+    memcpy(h + N, c->ht.psi_bo_s.data(), 8 * N);
+    memcpy(h + 2 * N, c->ht.ipsi_bo.data(), 8 * N);
+    memcpy(h + 3 * N, c->ht.ipsi_bo_s.data(), 8 * N);
+    h[4 * N] = c->ht.n_inv;
+    h[4 * N + 1] = c->ht.n_inv_s;
+    return PZ_OK;
+}
+
+extern "C" int pz_ctx_stream(pz_ctx *c, void **s)
+{
+    if (!c || !s) return fail(PZ_ERR_ARG, "null argument");
+    *s = (void *)c->stream;
+    return PZ_OK;
+}
+
+static hipStream_t S(pz_ctx *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+
+extern "C" int pz_malloc(pz_ctx *c, size_t bytes, void **p)
+{
+    if (!c || !p) return fail(PZ_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMalloc(p, bytes ? bytes : 8));
+    return PZ_OK;
+}
+
+extern "C" int pz_free(pz_ctx *c, void *p)
+{
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    HIPCHK(hipFree(p));
+    return PZ_OK;
+}
+
+extern "C" int pz_memcpy_htod(pz_ctx *c, void *d, const void *h, size_t bytes)
+{
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PZ_OK;
+}
+
+extern "C" int pz_memcpy_dtoh(pz_ctx *c, void *h, const void *d, size_t bytes)
+{
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    HIPCHK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PZ_OK;
+}
+
+extern "C" int pz_sync(pz_ctx *c)
+{
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipDeviceSynchronize());
+    return PZ_OK;
+}
+
+static int ntt_common(pz_ctx *c, int mode, uint64_t *d, size_t nrows, void *s)
+{
+    if (!c || (!d && nrows)) return fail(PZ_ERR_ARG, "null argument");
+    if (nrows == 0) return PZ_OK;
+    launch_ntt(c->logn, mode, d, nrows, c->dc.tb, S(c, s));
+    HIPCHK(hipGetLastError());
+    return PZ_OK;
+}
+
+extern "C" int pz_ntt_forward_lazy(pz_ctx *c, uint64_t *d, size_t n, void *s) { return ntt_common(c, 0, d, n, s); }
+extern "C" int pz_ntt_forward(pz_ctx *c, uint64_t *d, size_t n, void *s) { return ntt_common(c, 1, d, n, s); }
+extern "C" int pz_ntt_backward_lazy(pz_ctx *c, uint64_t *d, size_t n, void *s) { return ntt_common(c, 2, d, n, s); }
+extern "C" int pz_ntt_backward(pz_ctx *c, uint64_t *d, size_t n, void *s) { return ntt_common(c, 3, d, n, s); }
+
+extern "C" int pz_shoup_matrix_fma(pz_ctx *c, uint64_t *out, const uint64_t *a,
+                                   const uint64_t *as, const uint64_t *b, uint32_t d, size_t n,
+                                   void *s)
+{
+    if (!c || !out || !a || !as || !b) return fail(PZ_ERR_ARG, "null argument");
+    if (n == 0) return PZ_OK;
+    launch_fma(out, a, as, b, d, n, c->p.rlwe_q, S(c, s));
+    HIPCHK(hipGetLastError());
+    return PZ_OK;
+}
+
+extern "C" int pz_decompose(pz_ctx *c, int which, const uint64_t *in, uint64_t *digits,
+                            size_t count, void *s)
+{
+    if (!c || !in || !digits) return fail(PZ_ERR_ARG, "null argument");
+    const Decomp *dc = nullptr;
+    switch (which) {
+    case 0: dc = &c->dc.dec_a; break;
+    case 1: dc = &c->dc.dec_b; break;
+    case 2: dc = &c->dc.dec_auto; break;
+    case 3: dc = &c->dc.dec_lwe; break;
+    default: return fail(PZ_ERR_ARG, "unknown decomposer");
+    }
+    if (count == 0) return PZ_OK;
+    launch_decompose(in, digits, count, *dc, S(c, s));
+    HIPCHK(hipGetLastError());
+    return PZ_OK;
+}
+
+extern "C" int pz_key_upload(pz_ctx *c, const uint64_t *h_rgsw, const uint64_t *h_auto,
+                             const uint64_t *h_ksk, pz_key **out)
+{
+    if (!c || !h_rgsw || !h_auto || !h_ksk || !out) return fail(PZ_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const pz_params &p = c->p;
+    uint32_t N = p.rlwe_n;
+    size_t rgsw_rows = 2 * (size_t)p.rlrg_da + 2 * (size_t)p.rlrg_db;
+    size_t auto_rows = 2 * (size_t)p.auto_d;
+    size_t rgsw_words = (size_t)p.lwe_n * rgsw_rows * N;
+    size_t auto_words = (size_t)(p.w + 1) * auto_rows * N;
+    size_t ksk_words = (size_t)N * p.lwe_d * (p.lwe_n + 1);
+    u32 kb = p.lwe_q <= 65536 ? 2 : 4;
+    pz_key *k = new pz_key();
+    k->owner = c;
+    u64 *tmp = nullptr;
+    bool ok = hipMalloc(&k->rgsw, 8 * rgsw_words) == hipSuccess &&
+              hipMalloc(&k->rgsw_s, 8 * rgsw_words) == hipSuccess &&
+              hipMalloc(&k->auto_k, 8 * auto_words) == hipSuccess &&
+              hipMalloc(&k->auto_s, 8 * auto_words) == hipSuccess &&
+              hipMalloc(&k->ksk, kb * ksk_words) == hipSuccess &&
+              hipMalloc(&tmp, 8 * ksk_words) == hipSuccess;
+    if (!ok) {
+        hipFree(tmp);
+        pz_key_free(k);
+        return fail(PZ_ERR_HIP, "key allocation failed");
+    }
+    hipStream_t s = c->stream;
+    hipMemcpyAsync(k->rgsw, h_rgsw, 8 * rgsw_words, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(k->auto_k, h_auto, 8 * auto_words, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(tmp, h_ksk, 8 * ksk_words, hipMemcpyHostToDevice, s);
+    launch_shoup(k->rgsw, k->rgsw_s, rgsw_words, p.rlwe_q, s);
+    launch_shoup(k->auto_k, k->auto_s, auto_words, p.rlwe_q, s);
+    launch_pack_ksk(tmp, k->ksk, ksk_words, kb, s);
+    hipError_t e = hipStreamSynchronize(s);
+    hipFree(tmp);
+    if (e != hipSuccess) {
+        pz_key_free(k);
+        return fail(PZ_ERR_HIP, hipGetErrorString(e));
+    }
+    k->dk.rgsw = k->rgsw;
+    k->dk.rgsw_s = k->rgsw_s;
+    k->dk.auto_k = k->auto_k;
+    k->dk.auto_s = k->auto_s;
+    k->dk.ksk = k->ksk;
+    k->dk.rgsw_rows = (u32)rgsw_rows;
+    k->dk.auto_rows = (u32)auto_rows;
+    k->dk.ksk_bytes = kb;
+    k->bytes = 16 * rgsw_words + 16 * auto_words + kb * ksk_words;
+    *out = k;
+    return PZ_OK;
+}
+
+extern "C" int pz_key_free(pz_key *k)
+{
+    if (!k) return PZ_OK;
+    if (k->owner) {
+        hipStreamSynchronize(k->owner->stream);
+        if (k->owner->graph_key == k) {
+            drop_graphs(k->owner);
+            k->owner->graph_key = nullptr;
+        }
+    }
+    hipFree(k->rgsw);
+    hipFree(k->rgsw_s);
+    hipFree(k->auto_k);
+    hipFree(k->auto_s);
+    hipFree(k->ksk);
+    delete k;
+    return PZ_OK;
+}
+
+extern "C" int pz_key_bytes(pz_key *k, size_t *bytes)
+{
+    if (!k || !bytes) return fail(PZ_ERR_ARG, "null argument");
+    *bytes = k->bytes;
+    return PZ_OK;
+}
+
+extern "C" int pz_rlwe_by_rgsw(pz_ctx *c, pz_key *k, uint64_t *rlwe, const uint32_t *idx,
+                               const uint8_t *triv, size_t B, void *s)
+{
+    if (!c || !k || !rlwe || !idx || !triv) return fail(PZ_ERR_ARG, "null argument");
+    if (B == 0) return PZ_OK;
+    launch_ext(c->logn, false, rlwe, idx, triv, B, c->dc, k->dk, S(c, s));
+    HIPCHK(hipGetLastError());
+    return PZ_OK;
+}
+
+extern "C" int pz_rlwe_auto(pz_ctx *c, pz_key *k, uint64_t *rlwe, const uint32_t *idx,
+                            const uint8_t *triv, size_t B, void *s)
+{
+    if (!c || !k || !rlwe || !idx || !triv) return fail(PZ_ERR_ARG, "null argument");
+    if (B == 0) return PZ_OK;
+    launch_ext(c->logn, true, rlwe, idx, triv, B, c->dc, k->dk, S(c, s));
+    HIPCHK(hipGetLastError());
+    return PZ_OK;
+}
+
+// grow the PBS pipeline scratch to hold B ciphertexts
+static int ensure_scratch(pz_ctx *c, size_t B)
+{
+    if (B <= c->cap) return PZ_OK;
+    drop_graphs(c);  // captured graphs reference the old buffers
+    hipStreamSynchronize(c->stream);
+    hipFree(c->d_digits);
+    hipFree(c->d_ksb);
+    hipFree(c->d_ks);
+    hipFree(c->d_stats);
+    c->d_stats = nullptr;
+    c->d_digits = nullptr;
+    c->d_ksb = nullptr;
+    c->d_ks = nullptr;
+    size_t rows = (size_t)c->p.rlwe_n * c->p.lwe_d;
+    size_t cap = B < 64 ? 64 : B;
+    HIPCHK(hipMalloc(&c->d_digits, rows * cap));
+    HIPCHK(hipMalloc(&c->d_ksb, 4 * cap));
+    HIPCHK(hipMalloc(&c->d_ks, 4 * cap * (c->p.lwe_n + 1)));
+    HIPCHK(hipMalloc(&c->d_stats, 4 * cap));
+    c->cap = cap;
+    return PZ_OK;
+}
+
+extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint64_t *out,
+                                 size_t B, void *s)
+{
+    if (!c || !k || !in || !out) return fail(PZ_ERR_ARG, "null argument");
+    if (B == 0) return PZ_OK;
+    int rc = ensure_scratch(c, B);
+    if (rc) return rc;
+    hipStream_t st = S(c, s);
+    const pz_params &p = c->p;
+    u32 n1 = p.lwe_n + 1;
+    launch_prep(PREP_KSONLY, in, nullptr, nullptr, p.rlwe_n + 1, B, c->d_digits, c->d_ksb, c->dc,
+                st);
+    launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1, B,
+              c->d_ks, (u32)(p.lwe_q - 1), st);
+    launch_widen(c->d_ks, out, B * n1, st);
+    HIPCHK(hipGetLastError());
+    return PZ_OK;
+}
+
+// moddown/KS/blind-rotate sequence on stream st
+static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, const uint64_t *c1,
+                         const uint32_t *ops, uint64_t *out, size_t B, hipStream_t st)
+{
+    const pz_params &p = c->p;
+    u32 n1 = p.lwe_n + 1;
+    if (c->timing) hipEventRecord(c->ev[0], st);
+    launch_prep(mode, c0, c1, ops, p.rlwe_n + 1, B, c->d_digits, c->d_ksb, c->dc, st);
+    if (c->timing) hipEventRecord(c->ev[1], st);
+    launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1, B,
+              c->d_ks, (u32)(p.lwe_q - 1), st);
+    if (c->timing) hipEventRecord(c->ev[2], st);
+    size_t resident = (size_t)c->num_cu * c->br_per_cu;
+    if (resident < 1) resident = 1;
+    int grid = (int)(B < resident ? B : resident);
+    launch_blind_rotate(c->logn, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk, st);
+    if (c->timing) hipEventRecord(c->ev[3], st);
+}
+
+static int collect_timing(pz_ctx *c, hipStream_t st)
+{
+    if (!c->timing) return PZ_OK;
+    HIPCHK(hipEventSynchronize(c->ev[3]));
+    hipEventElapsedTime(&c->t_prep, c->ev[0], c->ev[1]);
+    hipEventElapsedTime(&c->t_ks, c->ev[1], c->ev[2]);
+    hipEventElapsedTime(&c->t_br, c->ev[2], c->ev[3]);
+    (void)st;
+    return PZ_OK;
+}
+
+extern "C" int pz_pbs(pz_ctx *c, pz_key *k, const uint32_t *gate, uint64_t *lwe, size_t B,
+                      void *s)
+{
+    if (!c || !k || !gate || !lwe) return fail(PZ_ERR_ARG, "null argument");
+    if (B == 0) return PZ_OK;
+    int rc = ensure_scratch(c, B);
+    if (rc) return rc;
+    hipStream_t st = S(c, s);
+    pbs_sequence(c, k, PREP_PBS, lwe, nullptr, gate, lwe, B, st);
+    HIPCHK(hipGetLastError());
+    return collect_timing(c, st);
+}
+
+extern "C" int pz_gates(pz_ctx *c, pz_key *k, const uint32_t *ops, const uint64_t *c0,
+                        const uint64_t *c1, uint64_t *out, size_t B, void *s)
+{
+    if (!c || !k || !ops || !c0 || !c1 || !out) return fail(PZ_ERR_ARG, "null argument");
+    if (B == 0) return PZ_OK;
+    int rc = ensure_scratch(c, B);
+    if (rc) return rc;
+    hipStream_t st = S(c, s);
+    if (c->timing) {
+        // eager launches so the per-kernel events bracket each kernel
+        pbs_sequence(c, k, PREP_GATE, c0, c1, ops, out, B, st);
+        HIPCHK(hipGetLastError());
+        return collect_timing(c, st);
+    }
+    if (c->graph_key != k) {
+        drop_graphs(c);
+        c->graph_key = k;
+    }
+    GraphKey gk{ops, c0, c1, out, B};
+    auto it = c->graphs.find(gk);
+    if (it == c->graphs.end()) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        pbs_sequence(c, k, PREP_GATE, c0, c1, ops, out, B, c->stream);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess) return fail(PZ_ERR_HIP, hipGetErrorString(e));
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(PZ_ERR_HIP, hipGetErrorString(e));
+        if (c->graphs.size() > 32) drop_graphs(c);
+        it = c->graphs.emplace(gk, ge).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, st));
+    return PZ_OK;
+}
+
+extern "C" int pz_not(pz_ctx *c, uint64_t *d, size_t B, void *s)
+{
+    if (!c || !d) return fail(PZ_ERR_ARG, "null argument");
+    if (B == 0) return PZ_OK;
+    launch_not(d, B * (c->p.rlwe_n + 1), c->p.rlwe_q, S(c, s));
+    HIPCHK(hipGetLastError());
+    return PZ_OK;
+}
+
+extern "C" int pz_last_auto_counts(pz_ctx *c, uint32_t *h_counts, size_t B)
+{
+    if (!c || !h_counts) return fail(PZ_ERR_ARG, "null argument");
+    if (B > c->cap) return fail(PZ_ERR_ARG, "more counts than the last batch holds");
+    HIPCHK(hipMemcpyAsync(h_counts, c->d_stats, 4 * B, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PZ_OK;
+}
+
+extern "C" int pz_set_timing(pz_ctx *c, int enabled)
+{
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    c->timing = enabled != 0;
+    return PZ_OK;
+}
+
+extern "C" int pz_last_timing(pz_ctx *c, float *a, float *b, float *d)
+{
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    if (a) *a = c->t_prep;
+    if (b) *b = c->t_ks;
+    if (d) *d = c->t_br;
+    return PZ_OK;
+}

@@ -1,0 +1,180 @@
+// pz_device.hpp -- device-side context and the two RLWE operators of the
+// blind rotation as work-group-level routines (one RLWE per work group of
+// N/8 threads, coefficients resident in VGPRs).
+#pragma once
+#include "pz_ntt.hpp"
+
+namespace pz {
+
+constexpr int MAX_AUTO = 32;
+
+// Everything a kernel needs about the parameter set (passed by value).
+struct DevCtx {
+    Tables tb;
+    Decomp dec_a, dec_b, dec_auto, dec_lwe;
+    u32 da, db, dauto, lwe_d;
+    u32 n_lwe, br_q, w, g, embedding;
+    u32 lwe_q_log;
+    u64 lwe_mask;
+    double lwe_qf, rlwe_qf;
+    u64 qby4;
+    Barrett br;
+    const u32 *dlog;        // [br_q]
+    const u64 *test_vec;    // [6][br_q/2]
+    u64 auto_el[MAX_AUTO];  // automorphism exponents, reduced into [0, 2N)
+};
+
+// Device key (reference row layout, evaluation domain, Shoup twins apart).
+struct DevKey {
+    const u64 *rgsw, *rgsw_s;  // [n][rgsw_rows][N]
+    const u64 *auto_k, *auto_s;  // [w+1][auto_rows][N]
+    const void *ksk;           // [N*d][n+1], u16 or u32
+    u32 rgsw_rows, auto_rows, ksk_bytes;
+};
+
+// 8 consecutive words (evaluation-domain slice of one thread)
+PZ_DEV void load8(const u64 *__restrict__ p, u64 (&v)[8])
+{
+    const ulonglong2 *p2 = reinterpret_cast<const ulonglong2 *>(p);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        ulonglong2 t = p2[i];
+        v[2 * i] = t.x;
+        v[2 * i + 1] = t.y;
+    }
+}
+
+// out[r] = shoup_fma(out[r], K[8j+r], Ks[8j+r], D[r]) for the thread's slice
+PZ_DEV void fma_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+                    const u64 (&D)[8], u32 j, u64 q, u64 q2)
+{
+    u64 kv[8], ks[8];
+    load8(K + 8 * j, kv);
+    load8(Ks + 8 * j, ks);
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[r] = shoup_fma(acc[r], kv[r], ks[r], D[r], q, q2);
+}
+
+// rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) on the RLWE held in
+// registers (stride layout). K/Ks point at the RGSW's first row; rows are
+// [A(-sm) x da, B(-sm) x da, A(m) x db, B(m) x db], N words each.
+template <int LOGN>
+PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
+                            const u64 *__restrict__ Ks, bool trivial, const DevCtx &c,
+                            u32 j, u64 *lds)
+{
+    constexpr int N = 1 << LOGN;
+    const u64 q = c.tb.q, q2 = c.tb.q2;
+    u64 acc[2][8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[0][r] = acc[1][r] = 0;
+    const u32 da = c.da, db = c.db;
+    if (da == 1 && db == 1 && !trivial) {
+        u64 D[2][8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            D[0][r] = decomp_digit(c.dec_a, a[r], 0);
+            D[1][r] = decomp_digit(c.dec_b, b[r], 0);
+        }
+        ntt_fwd<LOGN, 2, false>(D, j, lds, c.tb);
+        fma_row(acc[0], K + 0 * N, Ks + 0 * N, D[0], j, q, q2);
+        fma_row(acc[0], K + 2 * N, Ks + 2 * N, D[1], j, q, q2);
+        fma_row(acc[1], K + 1 * N, Ks + 1 * N, D[0], j, q, q2);
+        fma_row(acc[1], K + 3 * N, Ks + 3 * N, D[1], j, q, q2);
+    } else {
+        if (!trivial) {
+            for (u32 rr = 0; rr < da; rr++) {
+                u64 D[1][8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_a, a[r], rr);
+                ntt_fwd<LOGN, 1, false>(D, j, lds, c.tb);
+                fma_row(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
+                fma_row(acc[1], K + (size_t)(da + rr) * N, Ks + (size_t)(da + rr) * N, D[0], j,
+                        q, q2);
+            }
+        }
+        for (u32 rr = 0; rr < db; rr++) {
+            u64 D[1][8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_b, b[r], rr);
+            ntt_fwd<LOGN, 1, false>(D, j, lds, c.tb);
+            size_t o0 = (size_t)(2 * da + rr) * N, o1 = (size_t)(2 * da + db + rr) * N;
+            fma_row(acc[0], K + o0, Ks + o0, D[0], j, q, q2);
+            fma_row(acc[1], K + o1, Ks + o1, D[0], j, q, q2);
+        }
+    }
+    ntt_inv<LOGN, 2, true>(acc, j, lds, c.tb);
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        a[r] = acc[0][r];
+        b[r] = acc[1][r];
+    }
+}
+
+// rlwe_auto_shoup (src/rgsw/runtime.rs:530-643) with the automorphism map of
+// generate_auto_map (src/rgsw/keygen.rs:16-41) computed on the fly:
+// to = i*k mod 2N, negated when it wraps past N.
+template <int LOGN>
+PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
+                     const u64 *__restrict__ Ks, u64 kexp, bool trivial, const DevCtx &c,
+                     u32 j, u64 *lds)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int W = Geom<LOGN>::LDS_WORDS;
+    const u64 q = c.tb.q, q2 = c.tb.q2;
+    const u64 SIGN = 1ull << 63;
+    if (trivial) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            u32 x = coef_x<LOGN>(j, r);
+            u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
+            bool pos = to < (u32)N;
+            to = pos ? to : to - N;
+            lds[lds_ix(to)] = pos ? b[r] : neg_mod(b[r], q);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; r++) b[r] = lds[lds_ix(coef_x<LOGN>(j, r))];
+        __syncthreads();
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        u32 x = coef_x<LOGN>(j, r);
+        u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
+        bool pos = to < (u32)N;
+        to = pos ? to : to - N;
+        lds[lds_ix(to)] = pos ? a[r] : neg_mod(a[r], q);
+        lds[W + lds_ix(to)] = pos ? b[r] : (b[r] | SIGN);
+    }
+    __syncthreads();
+    u64 bp[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        a[r] = lds[lds_ix(coef_x<LOGN>(j, r))];
+        bp[r] = lds[W + lds_ix(coef_x<LOGN>(j, r))];
+    }
+    __syncthreads();
+    u64 acc[2][8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[0][r] = acc[1][r] = 0;
+    const u32 d = c.dauto;
+    for (u32 rr = 0; rr < d; rr++) {
+        u64 D[1][8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_auto, a[r], rr);
+        ntt_fwd<LOGN, 1, false>(D, j, lds, c.tb);
+        fma_row(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
+        fma_row(acc[1], K + (size_t)(d + rr) * N, Ks + (size_t)(d + rr) * N, D[0], j, q, q2);
+    }
+    ntt_inv<LOGN, 2, true>(acc, j, lds, c.tb);
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        u64 v = bp[r] & ~SIGN;
+        bool neg = (bp[r] & SIGN) != 0;
+        a[r] = acc[0][r];
+        b[r] = neg ? sub_mod(acc[1][r], v, q) : add_mod(acc[1][r], v, q);
+    }
+}
+
+}  // namespace pz

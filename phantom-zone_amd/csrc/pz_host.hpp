@@ -1,0 +1,247 @@
+// pz_host.hpp -- host-side set-up of an evaluator: parameter sets, the
+// ChaCha8 generator that fixes the NTT root, NTT tables, Barrett constants,
+// dlog map, automorphism exponents and the gate test vectors.
+//
+// These mirror BoolEvaluator::new (src/bool/evaluator.rs:651-784) and
+// NttBackendU64::_new (src/ntt.rs:323-386); they run once per context.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pz.h"
+
+namespace pz {
+namespace host {
+
+typedef unsigned __int128 u128;
+
+// Named parameter sets, src/bool/parameters.rs:526-726.
+inline int params_named(const char *name, pz_params *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->rlwe_q = 18014398509404161ULL;
+    o->rlwe_n = 2048;
+    o->rlrg_logb = 17; o->rlrg_da = 1; o->rlrg_db = 1;
+    o->auto_logb = 24; o->auto_d = 1;
+    o->lwe_logb = 1;
+    o->g = 5; o->w = 10;
+    std::string s(name);
+    if (s == "NI_2P") {
+        o->lwe_q = 1ULL << 16; o->br_q = 1u << 12; o->lwe_n = 520; o->lwe_d = 13;
+        o->lwe_secret_gaussian = 1;
+    } else if (s == "NI_4P_HB_FR") {
+        o->lwe_q = 1ULL << 16; o->br_q = 1u << 11; o->lwe_n = 620; o->lwe_d = 13;
+    } else if (s == "NI_8P") {
+        o->lwe_q = 1ULL << 17; o->br_q = 1u << 12; o->lwe_n = 660; o->lwe_d = 14;
+    } else if (s == "SP_TEST") {
+        o->rlwe_q = 268369921ULL; o->rlwe_n = 512; o->lwe_q = 1ULL << 16;
+        o->br_q = 1u << 9; o->lwe_n = 100; o->lwe_logb = 4; o->lwe_d = 4;
+        o->rlrg_logb = 7; o->rlrg_da = 4; o->rlrg_db = 4;
+        o->auto_logb = 7; o->auto_d = 4; o->w = 5; o->lwe_secret_gaussian = 1;
+    } else if (s == "I_2P_LB_SR") {
+        o->lwe_q = 1ULL << 15; o->br_q = 1u << 11; o->lwe_n = 580; o->lwe_d = 12;
+    } else if (s == "I_4P") {
+        o->lwe_q = 1ULL << 16; o->br_q = 1u << 11; o->lwe_n = 620; o->lwe_d = 13;
+    } else if (s == "I_8P") {
+        o->lwe_q = 1ULL << 17; o->br_q = 1u << 12; o->lwe_n = 660; o->lwe_d = 14;
+    } else if (s == "NI_4P_LB_SR") {
+        o->lwe_q = 1ULL << 16; o->br_q = 1u << 12; o->lwe_n = 620; o->lwe_d = 13;
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+// ChaCha with rand_chacha's state layout (64-bit counter in words 12-13,
+// 64-bit stream in 14-15); 8 rounds = ChaCha8Rng.  Words are consumed in
+// order; next_u64 = lo | hi << 32 (rand_core BlockRng).
+struct ChaCha8 {
+    uint32_t key[8];
+    uint64_t counter = 0;
+    uint32_t buf[16];
+    int idx = 16;
+
+    explicit ChaCha8(const uint8_t seed[32])
+    {
+        for (int i = 0; i < 8; i++)
+            key[i] = (uint32_t)seed[4 * i] | ((uint32_t)seed[4 * i + 1] << 8) |
+                     ((uint32_t)seed[4 * i + 2] << 16) | ((uint32_t)seed[4 * i + 3] << 24);
+    }
+    static uint32_t rotl(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
+    static void qr(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d)
+    {
+        a += b; d ^= a; d = rotl(d, 16);
+        c += d; b ^= c; b = rotl(b, 12);
+        a += b; d ^= a; d = rotl(d, 8);
+        c += d; b ^= c; b = rotl(b, 7);
+    }
+    void refill()
+    {
+        uint32_t s[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+        for (int i = 0; i < 8; i++) s[4 + i] = key[i];
+        s[12] = (uint32_t)counter; s[13] = (uint32_t)(counter >> 32);
+        s[14] = 0; s[15] = 0;
+        uint32_t x[16];
+        memcpy(x, s, sizeof(s));
+        for (int r = 0; r < 8; r += 2) {
+            qr(x[0], x[4], x[8], x[12]); qr(x[1], x[5], x[9], x[13]);
+            qr(x[2], x[6], x[10], x[14]); qr(x[3], x[7], x[11], x[15]);
+            qr(x[0], x[5], x[10], x[15]); qr(x[1], x[6], x[11], x[12]);
+            qr(x[2], x[7], x[8], x[13]); qr(x[3], x[4], x[9], x[14]);
+        }
+        for (int i = 0; i < 16; i++) buf[i] = x[i] + s[i];
+        counter++;
+        idx = 0;
+    }
+    uint32_t next_u32()
+    {
+        if (idx >= 16) refill();
+        return buf[idx++];
+    }
+    uint64_t next_u64()
+    {
+        uint64_t lo = next_u32();
+        uint64_t hi = next_u32();
+        return lo | (hi << 32);
+    }
+    void fill_bytes(uint8_t *out, size_t n)
+    {
+        size_t i = 0;
+        while (i < n) {
+            uint32_t w = next_u32();
+            for (int b = 0; b < 4 && i < n; b++, i++) out[i] = (uint8_t)(w >> (8 * b));
+        }
+    }
+    // UniformInt<u64> sampling (widening multiply + rejection zone)
+    uint64_t uniform(uint64_t range)
+    {
+        uint64_t reject = (UINT64_MAX - range + 1) % range;
+        uint64_t zone = UINT64_MAX - reject;
+        for (;;) {
+            uint64_t v = next_u64();
+            u128 m = (u128)v * range;
+            if ((uint64_t)m <= zone) return (uint64_t)(m >> 64);
+        }
+    }
+};
+
+inline uint64_t mod_exp(uint64_t a, uint64_t b, uint64_t q)
+{
+    uint64_t acc = a, out = 1;
+    while (b) {
+        if (b & 1) out = (uint64_t)(((u128)acc * out) % q);
+        acc = (uint64_t)(((u128)acc * acc) % q);
+        b >>= 1;
+    }
+    return out;
+}
+
+inline uint64_t shoup_repr(uint64_t v, uint64_t q) { return (uint64_t)(((u128)v << 64) / q); }
+
+// find_primitive_root (src/ntt.rs:283-307) with ChaCha8Rng::from_seed([0;32])
+inline uint64_t find_psi(uint64_t q, uint64_t two_n)
+{
+    uint8_t seed[32] = {0};
+    ChaCha8 r(seed);
+    uint64_t t = (q - 1) / two_n;
+    for (int it = 0; it < 100; it++) {
+        uint64_t omega = r.next_u64() % q;
+        omega = mod_exp(omega, t, q);
+        if (mod_exp(omega, two_n >> 1, q) == 1) continue;
+        return omega;
+    }
+    return 0;
+}
+
+struct HostTables {
+    uint64_t psi = 0;
+    std::vector<uint64_t> psi_bo, psi_bo_s, ipsi_bo, ipsi_bo_s;
+    uint64_t n_inv = 0, n_inv_s = 0;
+};
+
+// NttBackendU64::_new (src/ntt.rs:323-386).  Powers are accumulated with
+// exact modular products (the reference's Barrett mul is exact).
+inline HostTables make_tables(uint64_t q, uint32_t n)
+{
+    HostTables t;
+    t.psi = find_psi(q, 2ull * n);
+    uint64_t psi_inv = mod_exp(t.psi, q - 2, q);
+    uint32_t logn = 0;
+    while ((1u << logn) < n) logn++;
+    t.psi_bo.resize(n); t.ipsi_bo.resize(n); t.psi_bo_s.resize(n); t.ipsi_bo_s.resize(n);
+    uint64_t rp = 1, rpi = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t bo = 0;
+        for (uint32_t b = 0; b < logn; b++) bo |= ((i >> b) & 1u) << (logn - 1 - b);
+        t.psi_bo[bo] = rp;
+        t.ipsi_bo[bo] = rpi;
+        rp = (uint64_t)(((u128)rp * t.psi) % q);
+        rpi = (uint64_t)(((u128)rpi * psi_inv) % q);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        t.psi_bo_s[i] = shoup_repr(t.psi_bo[i], q);
+        t.ipsi_bo_s[i] = shoup_repr(t.ipsi_bo[i], q);
+    }
+    t.n_inv = mod_exp(n, q - 2, q);
+    t.n_inv_s = shoup_repr(t.n_inv, q);
+    return t;
+}
+
+// utils::log2 (src/utils.rs:230-237)
+inline uint32_t ref_log2(uint64_t v)
+{
+    if ((v & (v - 1)) == 0) return 64 - (uint32_t)__builtin_clzll(v) - 1;
+    return 64 - (uint32_t)__builtin_clzll(v);
+}
+
+// generate_auto_map element k reduced into [0, 2*ring) (src/rgsw/keygen.rs:16-24)
+inline uint64_t auto_exp(int64_t k, uint64_t ring)
+{
+    uint64_t two_n = 2 * ring;
+    return k < 0 ? two_n - ((uint64_t)(-k) % two_n) : (uint64_t)k % two_n;
+}
+
+// Test vectors of BoolEvaluator::new (src/bool/evaluator.rs:680-726), each
+// mapped X -> X^{-g} over br_q/2 coefficients. Order: nand, and, or, nor,
+// xor, xnor.
+inline std::vector<uint64_t> test_vectors(const pz_params &p)
+{
+    uint64_t q = p.rlwe_q;
+    uint32_t qby2 = p.br_q >> 1, qby8 = p.br_q >> 3;
+    uint64_t tru = q >> 3, fal = (q - 1) - tru + 1;
+    struct TV { uint32_t part; uint64_t before, after; } tv[6] = {
+        {3 * qby8, tru, fal}, {3 * qby8, fal, tru}, {qby8, fal, tru},
+        {qby8, tru, fal},     {qby8, fal, tru},     {qby8, tru, fal},
+    };
+    std::vector<uint64_t> out(6 * (size_t)qby2);
+    uint64_t k = auto_exp(-(int64_t)p.g, qby2);
+    for (int g = 0; g < 6; g++) {
+        for (uint32_t i = 0; i < qby2; i++) {
+            uint64_t v = i < tv[g].part ? tv[g].before : tv[g].after;
+            uint64_t to = ((uint64_t)i * k) % (2ull * qby2);
+            bool pos = true;
+            if (to >= qby2) { to -= qby2; pos = false; }
+            out[(size_t)g * qby2 + to] = pos ? v : q - v;
+        }
+    }
+    return out;
+}
+
+// dlog map (src/bool/evaluator.rs:664-673)
+inline std::vector<uint32_t> dlog_map(const pz_params &p)
+{
+    uint32_t q = p.br_q;
+    std::vector<uint32_t> d(q, 0);
+    for (uint32_t i = 0; i < q / 4; i++) {
+        uint32_t v = (uint32_t)mod_exp(p.g, i, q);
+        d[v] = i;
+        d[q - v] = i + q / 4;
+    }
+    return d;
+}
+
+}  // namespace host
+}  // namespace pz

@@ -1,0 +1,196 @@
+// pz_ntt.hpp -- block-level negacyclic NTT over a 54-bit prime for gfx950.
+//
+// One ring element of N = 2^LOGN u64 coefficients is held by N/8 threads,
+// 8 coefficients per thread in VGPR pairs.  The 11 (for N=2048) radix-2
+// stages of the reference are grouped into radix-8 phases of three stages
+// that run entirely in registers; between phases the polynomial is
+// exchanged through LDS (padded one word in nine to spread banks).
+//
+// Forward (Cooley-Tukey, src/ntt.rs:110-180) takes coefficients in the
+// "stride" layout (thread j holds x = j + (N/8)*r) and leaves the evaluation
+// vector in the "contiguous" layout (thread j holds x = 8j + r).  The inverse
+// (Gentleman-Sande, src/ntt.rs:187-278) runs the phases backwards.  Each
+// element sees exactly the reference's butterfly sequence and twiddles, so
+// every lazy intermediate value is identical to the reference.
+#pragma once
+#include "pz_arith.hpp"
+
+namespace pz {
+
+__host__ __device__ constexpr int ilog_max1(int v) { return v > 0 ? v : 1; }
+
+// LDS index of coefficient x (one pad word every 8)
+PZ_DEV u32 lds_ix(u32 x) { return x + (x >> 3); }
+
+template <int LOGN>
+struct Geom {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int TPB = N / 8;
+    static constexpr int LDS_WORDS = N + N / 8;  // per polynomial
+};
+
+// Stride of the thread's 8 coefficients in the phase starting at stage S0.
+template <int LOGN, int S0>
+struct Phase {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int ST = ilog_max1(N >> (S0 + 3));
+    static constexpr int BS = 8 * ST;
+    static constexpr int NS = (LOGN - S0) < 3 ? (LOGN - S0) : 3;
+    PZ_DEV static u32 base(u32 j) { return (j / ST) * BS + (j % ST); }
+    PZ_DEV static u32 x(u32 j, int r) { return base(j) + ST * r; }
+};
+
+struct Tables {
+    const u64 *__restrict__ psi, *__restrict__ psis;    // forward, bit-reversed
+    const u64 *__restrict__ ipsi, *__restrict__ ipsis;  // inverse, bit-reversed
+    u64 n_inv, n_inv_s;
+    u64 q, q2;
+};
+
+// One forward stage S inside a phase of stride ST. KIND: 0 = 0_to_4q
+// butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q).
+template <int LOGN, int S, int ST, int NP, int KIND>
+PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int T = N >> (S + 1);
+    constexpr int LD = T / ST;
+    constexpr int M = 1 << S;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        if (r & LD) continue;
+        const u32 x = base + ST * r;
+        const u32 i = x / (2 * T);
+        const u64 w = tb.psi[M + i], ws = tb.psis[M + i];
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            if (KIND == 0) {
+                fb_4q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
+            } else {
+                fb_2q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
+                if (KIND == 2) {
+                    v[p][r] = umin(v[p][r], v[p][r] - tb.q);
+                    v[p][r + LD] = umin(v[p][r + LD], v[p][r + LD] - tb.q);
+                }
+            }
+        }
+    }
+}
+
+// One inverse stage S (m = 2^S, t = N >> (S+1)); S == 0 is the last stage,
+// which also scales by n^-1 (and reduces to [0,q) when STRICT).
+template <int LOGN, int S, int ST, int NP, bool STRICT>
+PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int T = N >> (S + 1);
+    constexpr int LD = T / ST;
+    constexpr int M = 1 << S;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        if (r & LD) continue;
+        const u32 x = base + ST * r;
+        const u32 i = x / (2 * T);
+        const u64 w = tb.ipsi[M + i], ws = tb.ipsis[M + i];
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            ib_2q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
+            if (S == 0) {
+                u64 ox = shoup_mul(v[p][r], tb.n_inv, tb.n_inv_s, tb.q);
+                u64 oy = shoup_mul(v[p][r + LD], tb.n_inv, tb.n_inv_s, tb.q);
+                if (STRICT) {
+                    ox = umin(ox, ox - tb.q);
+                    oy = umin(oy, oy - tb.q);
+                }
+                v[p][r] = ox;
+                v[p][r + LD] = oy;
+            }
+        }
+    }
+}
+
+// Forward phase starting at stage S0 (up to three stages).
+template <int LOGN, int S0, int NP, bool STRICT>
+PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
+{
+    using P = Phase<LOGN, S0>;
+    const u32 base = P::base(j);
+    constexpr int LAST = LOGN - 1;
+    constexpr int K_LAST = STRICT ? 2 : 1;
+    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : 0>(v, base, tb);
+    if constexpr (P::NS > 1)
+        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : 0>(v, base, tb);
+    if constexpr (P::NS > 2)
+        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : 0>(v, base, tb);
+}
+
+// Inverse phase covering stages S0+NS-1 .. S0 (descending).
+template <int LOGN, int S0, int NP, bool STRICT>
+PZ_DEV void inv_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
+{
+    using P = Phase<LOGN, S0>;
+    const u32 base = P::base(j);
+    if constexpr (P::NS > 2) inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT>(v, base, tb);
+    if constexpr (P::NS > 1) inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT>(v, base, tb);
+    inv_stage<LOGN, S0, P::ST, NP, STRICT>(v, base, tb);
+}
+
+// Store the thread's coefficients (layout of phase SA) into LDS, barrier,
+// load them back in the layout of phase SB, barrier.
+template <int LOGN, int SA, int SB, int NP>
+PZ_DEV void exchange(u64 (&v)[NP][8], u32 j, u64 *lds)
+{
+    constexpr int W = Geom<LOGN>::LDS_WORDS;
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int r = 0; r < 8; r++) lds[p * W + lds_ix(Phase<LOGN, SA>::x(j, r))] = v[p][r];
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[p][r] = lds[p * W + lds_ix(Phase<LOGN, SB>::x(j, r))];
+    __syncthreads();
+}
+
+// Forward NTT of NP polynomials: stride layout in, contiguous layout out.
+// STRICT = Ntt::forward (output in [0,q)), else forward_lazy ([0,2q)).
+template <int LOGN, int NP, bool STRICT>
+PZ_DEV void ntt_fwd(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
+{
+    static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
+    fwd_phase<LOGN, 0, NP, STRICT>(v, j, tb);
+    exchange<LOGN, 0, 3, NP>(v, j, lds);
+    fwd_phase<LOGN, 3, NP, STRICT>(v, j, tb);
+    exchange<LOGN, 3, 6, NP>(v, j, lds);
+    fwd_phase<LOGN, 6, NP, STRICT>(v, j, tb);
+    if constexpr (LOGN > 9) {
+        exchange<LOGN, 6, 9, NP>(v, j, lds);
+        fwd_phase<LOGN, 9, NP, STRICT>(v, j, tb);
+    }
+}
+
+// Inverse NTT of NP polynomials: contiguous layout in, stride layout out.
+// STRICT = Ntt::backward ([0,q)), else backward_lazy ([0,2q)).
+template <int LOGN, int NP, bool STRICT>
+PZ_DEV void ntt_inv(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
+{
+    if constexpr (LOGN > 9) {
+        inv_phase<LOGN, 9, NP, STRICT>(v, j, tb);
+        exchange<LOGN, 9, 6, NP>(v, j, lds);
+    }
+    inv_phase<LOGN, 6, NP, STRICT>(v, j, tb);
+    exchange<LOGN, 6, 3, NP>(v, j, lds);
+    inv_phase<LOGN, 3, NP, STRICT>(v, j, tb);
+    exchange<LOGN, 3, 0, NP>(v, j, lds);
+    inv_phase<LOGN, 0, NP, STRICT>(v, j, tb);
+}
+
+// Coefficient index of register r in the stride (coefficient-domain) layout
+// and in the contiguous (evaluation-domain) layout.
+template <int LOGN>
+PZ_DEV u32 coef_x(u32 j, int r) { return j + (u32)Geom<LOGN>::TPB * r; }
+template <int LOGN>
+PZ_DEV u32 eval_x(u32 j, int r) { return 8 * j + r; }
+
+}  // namespace pz

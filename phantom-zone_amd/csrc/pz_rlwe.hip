@@ -1,0 +1,267 @@
+// pz_rlwe.hip -- RLWE operators and the persistent blind rotation (gfx950).
+//
+//   k_ext             batched rlwe_by_rgsw_shoup
+//   k_auto            batched rlwe_auto_shoup
+//   k_blind_rotate    persistent: dlog buckets, test-vector rotation, LMKCY+
+//                     blind rotation, sample extract (one PBS per work group
+//                     at a time, RLWE resident in VGPRs)
+#include "pz_internal.hpp"
+
+namespace pz {
+
+// ---------------------------------------------------------------------------
+// Batched RLWE x RGSW / automorphism (one RLWE per work group).
+// rlwe: [B][2][N]; idx[b] selects the RGSW (LWE index) or auto key.
+// ---------------------------------------------------------------------------
+template <int LOGN>
+__global__ __launch_bounds__(1 << (LOGN - 3)) void k_ext(u64 *rlwe, const u32 *idx,
+                                                         const u8 *trivial, size_t B, DevCtx c,
+                                                         DevKey key)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ u64 lds[];
+    const u32 j = threadIdx.x;
+    for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
+        u64 *p = rlwe + bb * 2 * N;
+        u64 a[8], b[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            a[r] = p[coef_x<LOGN>(j, r)];
+            b[r] = p[N + coef_x<LOGN>(j, r)];
+        }
+        size_t off = (size_t)idx[bb] * key.rgsw_rows * N;
+        ext_product_blk<LOGN>(a, b, key.rgsw + off, key.rgsw_s + off, trivial[bb] != 0, c, j, lds);
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            p[coef_x<LOGN>(j, r)] = a[r];
+            p[N + coef_x<LOGN>(j, r)] = b[r];
+        }
+    }
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *kidx,
+                                                          const u8 *trivial, size_t B, DevCtx c,
+                                                          DevKey key)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ u64 lds[];
+    const u32 j = threadIdx.x;
+    for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
+        u64 *p = rlwe + bb * 2 * N;
+        u64 a[8], b[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            a[r] = p[coef_x<LOGN>(j, r)];
+            b[r] = p[N + coef_x<LOGN>(j, r)];
+        }
+        u32 k = kidx[bb];
+        size_t off = (size_t)k * key.auto_rows * N;
+        auto_blk<LOGN>(a, b, key.auto_k + off, key.auto_s + off, c.auto_el[k], trivial[bb] != 0,
+                       c, j, lds);
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            p[coef_x<LOGN>(j, r)] = a[r];
+            p[N + coef_x<LOGN>(j, r)] = b[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_blind_rotate: persistent PBS tail (src/pbs.rs:146-232).
+//   ks:   [B][n+1] key-switched LWE (mod q_ks)
+//   gate: [B] test-vector index
+//   out:  [B][N+1] output LWE (mod Q)
+// LDS: 2 polynomials (exchange / permutation) + bucket starts + index list.
+// ---------------------------------------------------------------------------
+template <int LOGN>
+__global__ __launch_bounds__(1 << (LOGN - 3)) void k_blind_rotate(const u32 *ks, const u32 *gate,
+                                                                  size_t B, u64 *out, u32 *stats,
+                                                                  DevCtx c, DevKey key)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int TPB = N / 8;
+    constexpr int W = Geom<LOGN>::LDS_WORDS;
+    extern __shared__ u64 lds[];
+    const u32 nb = c.br_q >> 1;
+    const u32 n = c.n_lwe;
+    u16 *bstart = reinterpret_cast<u16 *>(lds + 2 * W);  // [nb + 1]
+    u16 *bidx = bstart + nb + 1;                          // [n]
+    u32 *cnt = reinterpret_cast<u32 *>(lds);               // scratch in poly area
+    u16 *kbin = reinterpret_cast<u16 *>(cnt + nb + 1);     // scratch in poly area
+    const u32 j = threadIdx.x;
+    const u64 q = c.tb.q;
+
+    for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
+        const u32 *in = ks + bb * (n + 1);
+        // (3) odd moddown to br_q and dlog buckets (stable counting sort)
+        for (u32 i = j; i <= nb; i += TPB) cnt[i] = 0;
+        __syncthreads();
+        for (u32 i = j; i < n; i += TPB) {
+            u32 v = in[1 + i];
+            u32 odd = ((u32)(((u64)v * c.br_q) >> c.lwe_q_log)) | 1u;
+            u32 k = c.dlog[odd];
+            kbin[i] = (u16)k;
+            atomicAdd(&cnt[k + 1], 1u);
+        }
+        __syncthreads();
+        if (j == 0) {
+            u32 s = 0;
+            for (u32 i = 0; i <= nb; i++) {
+                s += cnt[i];
+                cnt[i] = s;
+                bstart[i] = (u16)s;
+            }
+            for (u32 i = 0; i < n; i++) bidx[cnt[kbin[i]]++] = (u16)i;
+        }
+        __syncthreads();
+
+        // (4) test vector times X^{g*b} (monomial_mul, src/pbs.rs:163-211)
+        const u32 oddb = ((u32)(((u64)in[0] * c.br_q) >> c.lwe_q_log)) | 1u;
+        u32 exp = (u32)(((u64)c.g * oddb) % c.br_q);
+        bool msign = true;
+        if (exp > nb) {
+            exp -= nb;
+            msign = false;
+        }
+        // gate ids outside 0..5 are invalid input; clamp so memory stays in bounds
+        const u32 gid = gate[bb] < 6 ? gate[bb] : 0;
+        const u64 *tv = c.test_vec + (size_t)gid * nb;
+        u64 a[8], b[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            u32 x = coef_x<LOGN>(j, r);
+            a[r] = 0;
+            u64 val = 0;
+            if (x % c.embedding == 0) {
+                u32 y = x / c.embedding;
+                u32 src;
+                bool s;
+                if (y >= exp) {
+                    src = y - exp;
+                    s = msign;
+                } else {
+                    src = y + nb - exp;
+                    s = !msign;
+                }
+                val = s ? tv[src] : neg_mod(tv[src], q);
+            }
+            b[r] = val;
+        }
+
+        // (5) blind rotation, src/pbs.rs:240-414
+        const size_t rs = (size_t)key.rgsw_rows * N, as = (size_t)key.auto_rows * N;
+        bool trivial = true;
+        u32 n_auto = 0;
+        const u32 qb4 = c.br_q >> 2;
+        for (int half = 0; half < 2; half++) {
+            const u32 off = half == 0 ? qb4 : 0;  // -(g^k) buckets first
+            u32 v = 0;
+            for (u32 i = qb4 - 1; i >= 1; i--) {
+                for (u32 t = bstart[off + i]; t < bstart[off + i + 1]; t++) {
+                    size_t si = bidx[t];
+                    ext_product_blk<LOGN>(a, b, key.rgsw + si * rs, key.rgsw_s + si * rs, trivial,
+                                          c, j, lds);
+                    trivial = false;
+                }
+                v += 1;
+                if (bstart[off + i] != bstart[off + i - 1] || v == c.w || i == 1) {
+                    auto_blk<LOGN>(a, b, key.auto_k + v * as, key.auto_s + v * as, c.auto_el[v],
+                                   trivial, c, j, lds);
+                    n_auto++;
+                    v = 0;
+                }
+            }
+            // -(g^0) then the -g automorphism / +(g^0) (no automorphism)
+            for (u32 t = bstart[off]; t < bstart[off + 1]; t++) {
+                size_t si = bidx[t];
+                ext_product_blk<LOGN>(a, b, key.rgsw + si * rs, key.rgsw_s + si * rs, trivial, c,
+                                      j, lds);
+                trivial = false;
+            }
+            if (half == 0) {
+                auto_blk<LOGN>(a, b, key.auto_k, key.auto_s, c.auto_el[0], trivial, c, j, lds);
+                n_auto++;
+            }
+        }
+
+        // (6) sample_extract at index 0 (src/pbs.rs:423-449)
+        if (stats && j == 0) stats[bb] = n_auto;
+        u64 *o = out + bb * (size_t)(N + 1);
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            u32 x = coef_x<LOGN>(j, r);
+            if (x == 0) {
+                o[1] = a[r];
+                o[0] = b[r];
+            } else {
+                o[1 + N - x] = neg_mod(a[r], q);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+template <int LOGN>
+static void launch_ext_t(bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv, size_t B,
+                         const DevCtx &c, const DevKey &k, hipStream_t s)
+{
+    constexpr int TPB = 1 << (LOGN - 3);
+    size_t lds = 2 * 8 * (size_t)Geom<LOGN>::LDS_WORDS;
+    int grid = B < 65535 ? (int)B : 65535;
+    if (grid < 1) return;
+    if (is_auto)
+        hipLaunchKernelGGL((k_auto<LOGN>), grid, TPB, lds, s, rlwe, idx, triv, B, c, k);
+    else
+        hipLaunchKernelGGL((k_ext<LOGN>), grid, TPB, lds, s, rlwe, idx, triv, B, c, k);
+}
+
+void launch_ext(int logn, bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv, size_t B,
+                const DevCtx &c, const DevKey &k, hipStream_t s)
+{
+    switch (logn) {
+    case 9: launch_ext_t<9>(is_auto, rlwe, idx, triv, B, c, k, s); break;
+    case 11: launch_ext_t<11>(is_auto, rlwe, idx, triv, B, c, k, s); break;
+    }
+}
+
+size_t br_lds_bytes(int logn, u32 br_q, u32 n_lwe)
+{
+    size_t words = (size_t)(1 << logn) + (size_t)(1 << logn) / 8;
+    size_t poly = 2 * 8 * words;
+    size_t scratch_need = 4 * (size_t)(br_q / 2 + 1) + 2 * (size_t)n_lwe;
+    if (scratch_need > poly) poly = scratch_need;
+    return poly + 2 * (size_t)(br_q / 2 + 1) + 2 * (size_t)n_lwe + 16;
+}
+
+template <int LOGN>
+static const void *br_kernel_ptr()
+{
+    return reinterpret_cast<const void *>(&k_blind_rotate<LOGN>);
+}
+
+const void *blind_rotate_kernel(int logn)
+{
+    switch (logn) {
+    case 9: return br_kernel_ptr<9>();
+    case 11: return br_kernel_ptr<11>();
+    }
+    return nullptr;
+}
+
+void launch_blind_rotate(int logn, int grid, const u32 *ks, const u32 *gate, size_t B, u64 *out,
+                         u32 *stats, const DevCtx &c, const DevKey &k, hipStream_t s)
+{
+    if (B == 0) return;
+    size_t lds = br_lds_bytes(logn, c.br_q, c.n_lwe);
+    int tpb = 1 << (logn - 3);
+    switch (logn) {
+    case 9: hipLaunchKernelGGL(k_blind_rotate<9>, grid, tpb, lds, s, ks, gate, B, out, stats, c, k); break;
+    case 11: hipLaunchKernelGGL(k_blind_rotate<11>, grid, tpb, lds, s, ks, gate, B, out, stats, c, k); break;
+    }
+}
+
+}  // namespace pz

@@ -1,0 +1,260 @@
+"""ctypes binding of libpz.so (the C ABI declared in include/pz.h).
+
+This is the Python face of the MI355X bootstrapping engine: device buffers,
+contexts, resident server keys and the batched operators.  It never falls back
+to the CPU: if libpz.so is missing or no GPU is visible, loading fails loudly.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpz.so")
+
+GATES = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5}
+
+
+class PzError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("rlwe_q", C.c_uint64), ("rlwe_n", C.c_uint32), ("lwe_n", C.c_uint32),
+        ("lwe_q", C.c_uint64), ("br_q", C.c_uint32),
+        ("lwe_logb", C.c_uint32), ("lwe_d", C.c_uint32),
+        ("rlrg_logb", C.c_uint32), ("rlrg_da", C.c_uint32), ("rlrg_db", C.c_uint32),
+        ("auto_logb", C.c_uint32), ("auto_d", C.c_uint32),
+        ("g", C.c_uint32), ("w", C.c_uint32), ("lwe_secret_gaussian", C.c_uint32),
+    ]
+
+
+# Every symbol declared in include/pz.h, with its ctypes signature.
+_vp = C.c_void_p
+SIGNATURES = {
+    "pz_params_named": (C.c_int, [C.c_char_p, C.POINTER(Params)]),
+    "pz_last_error": (C.c_char_p, []),
+    "pz_ctx_create": (C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(_vp)]),
+    "pz_ctx_destroy": (C.c_int, [_vp]),
+    "pz_ctx_info": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                              C.POINTER(C.c_uint32)]),
+    "pz_ctx_ntt_tables": (C.c_int, [_vp, _vp]),
+    "pz_ctx_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "pz_malloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "pz_free": (C.c_int, [_vp, _vp]),
+    "pz_memcpy_htod": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "pz_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "pz_sync": (C.c_int, [_vp]),
+    "pz_ntt_forward": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "pz_ntt_forward_lazy": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "pz_ntt_backward": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "pz_ntt_backward_lazy": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "pz_shoup_matrix_fma": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_size_t, _vp]),
+    "pz_decompose": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_size_t, _vp]),
+    "pz_key_upload": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "pz_key_free": (C.c_int, [_vp]),
+    "pz_key_bytes": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
+    "pz_rlwe_by_rgsw": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_rlwe_auto": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_lwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_pbs": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_gates": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_not": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "pz_last_auto_counts": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "pz_set_timing": (C.c_int, [_vp, C.c_int]),
+    "pz_last_timing": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+def build(jobs=3):
+    """Compile libpz.so with hipcc for gfx950 (phantom-zone_amd/Makefile)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-j%d" % jobs], check=True)
+
+
+def load():
+    """Load libpz.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PzError("libpz.so not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise PzError("pz error %d: %s" % (rc, load().pz_last_error().decode()))
+
+
+def params_named(name):
+    p = Params()
+    _check(load().pz_params_named(name.encode(), C.byref(p)))
+    return p
+
+
+class DeviceArray:
+    """A device allocation with numpy-shaped host transfers."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (shape,)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        ptr = C.c_void_p()
+        _check(ctx.L.pz_malloc(ctx.h, max(self.nbytes, 8), C.byref(ptr)))
+        self.ptr = ptr.value
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        _check(self.ctx.L.pz_memcpy_htod(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self.ctx.L.pz_memcpy_dtoh(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None:
+            self.ctx.L.pz_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Key:
+    """A server key resident in HBM (PbsKey)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    def nbytes(self):
+        n = C.c_size_t()
+        _check(self.ctx.L.pz_key_bytes(self.h, C.byref(n)))
+        return n.value
+
+    def free(self):
+        if self.h:
+            self.ctx.L.pz_key_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One evaluator on one device (BoolEvaluator over the GPU engine)."""
+
+    def __init__(self, params="NI_2P", device=0):
+        self.L = load()
+        self.p = params_named(params) if isinstance(params, str) else params
+        h = C.c_void_p()
+        _check(self.L.pz_ctx_create(C.byref(self.p), device, C.byref(h)))
+        self.h = h.value
+        self.N = self.p.rlwe_n
+        self.n = self.p.lwe_n
+        self.Q = self.p.rlwe_q
+
+    def close(self):
+        if self.h:
+            self.L.pz_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        psi, cu, per = C.c_uint64(), C.c_uint32(), C.c_uint32()
+        _check(self.L.pz_ctx_info(self.h, C.byref(psi), C.byref(cu), C.byref(per)))
+        return {"psi": psi.value, "num_cu": cu.value, "pbs_groups_per_cu": per.value}
+
+    def ntt_tables(self):
+        out = np.zeros(4 * self.N + 2, dtype=np.uint64)
+        _check(self.L.pz_ctx_ntt_tables(self.h, out.ctypes.data))
+        return out
+
+    def array(self, shape, dtype=np.uint64):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, host, dtype=None):
+        a = np.ascontiguousarray(host, dtype=dtype or host.dtype)
+        return DeviceArray(self, a.shape, a.dtype).upload(a)
+
+    def sync(self):
+        _check(self.L.pz_sync(self.h))
+
+    # ---- operators ----
+    def ntt(self, which, d_rows, nrows):
+        f = {"forward": self.L.pz_ntt_forward, "forward_lazy": self.L.pz_ntt_forward_lazy,
+             "backward": self.L.pz_ntt_backward,
+             "backward_lazy": self.L.pz_ntt_backward_lazy}[which]
+        _check(f(self.h, d_rows.ptr, nrows, None))
+
+    def shoup_matrix_fma(self, d_out, d_a, d_as, d_b, d, n):
+        _check(self.L.pz_shoup_matrix_fma(self.h, d_out.ptr, d_a.ptr, d_as.ptr, d_b.ptr, d, n,
+                                          None))
+
+    def decompose(self, which, d_in, d_digits, count):
+        _check(self.L.pz_decompose(self.h, which, d_in.ptr, d_digits.ptr, count, None))
+
+    def key_upload(self, rgsw, auto, lwe_ksk):
+        rgsw = np.ascontiguousarray(rgsw, dtype=np.uint64)
+        auto = np.ascontiguousarray(auto, dtype=np.uint64)
+        ksk = np.ascontiguousarray(lwe_ksk, dtype=np.uint64)
+        h = C.c_void_p()
+        _check(self.L.pz_key_upload(self.h, rgsw.ctypes.data, auto.ctypes.data,
+                                    ksk.ctypes.data, C.byref(h)))
+        return Key(self, h.value)
+
+    def rlwe_by_rgsw(self, key, d_rlwe, d_idx, d_triv, B):
+        _check(self.L.pz_rlwe_by_rgsw(self.h, key.h, d_rlwe.ptr, d_idx.ptr, d_triv.ptr, B, None))
+
+    def rlwe_auto(self, key, d_rlwe, d_idx, d_triv, B):
+        _check(self.L.pz_rlwe_auto(self.h, key.h, d_rlwe.ptr, d_idx.ptr, d_triv.ptr, B, None))
+
+    def lwe_key_switch(self, key, d_in, d_out, B):
+        _check(self.L.pz_lwe_key_switch(self.h, key.h, d_in.ptr, d_out.ptr, B, None))
+
+    def pbs(self, key, d_gate, d_lwe, B):
+        _check(self.L.pz_pbs(self.h, key.h, d_gate.ptr, d_lwe.ptr, B, None))
+
+    def gates(self, key, d_ops, d_c0, d_c1, d_out, B):
+        _check(self.L.pz_gates(self.h, key.h, d_ops.ptr, d_c0.ptr, d_c1.ptr, d_out.ptr, B, None))
+
+    def not_(self, d_c, B):
+        _check(self.L.pz_not(self.h, d_c.ptr, B, None))
+
+    def last_auto_counts(self, B):
+        out = np.zeros(B, dtype=np.uint32)
+        _check(self.L.pz_last_auto_counts(self.h, out.ctypes.data, B))
+        return out
+
+    def set_timing(self, on):
+        _check(self.L.pz_set_timing(self.h, 1 if on else 0))
+
+    def last_timing(self):
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        _check(self.L.pz_last_timing(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"prep_ms": a.value, "ks_ms": b.value, "blind_rotate_ms": c.value}
