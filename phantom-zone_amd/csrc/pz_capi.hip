@@ -144,11 +144,13 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
         delete c;
         return fail(PZ_ERR_HIP, "stream create failed");
     }
-    std::vector<u64> tabs(4 * (size_t)N);
-    memcpy(tabs.data(), c->ht.psi_bo.data(), 8 * N);
-    memcpy(tabs.data() + N, c->ht.psi_bo_s.data(), 8 * N);
-    memcpy(tabs.data() + 2 * N, c->ht.ipsi_bo.data(), 8 * N);
-    memcpy(tabs.data() + 3 * N, c->ht.ipsi_bo_s.data(), 8 * N);
+    std::vector<u64> tabs(4 * (size_t)N);  // [fw pairs (2N) | iw pairs (2N)]
+    for (uint32_t i = 0; i < N; i++) {
+        tabs[2 * i] = c->ht.psi_bo[i];
+        tabs[2 * i + 1] = c->ht.psi_bo_s[i];
+        tabs[2 * N + 2 * i] = c->ht.ipsi_bo[i];
+        tabs[2 * N + 2 * i + 1] = c->ht.ipsi_bo_s[i];
+    }
     std::vector<uint32_t> dlog = host::dlog_map(*p);
     std::vector<uint64_t> tv = host::test_vectors(*p);
     if (hipMalloc(&c->d_tables, 8 * tabs.size()) != hipSuccess ||
@@ -163,10 +165,8 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
 
     DevCtx &d = c->dc;
     memset(&d, 0, sizeof(d));
-    d.tb.psi = c->d_tables;
-    d.tb.psis = c->d_tables + N;
-    d.tb.ipsi = c->d_tables + 2 * N;
-    d.tb.ipsis = c->d_tables + 3 * N;
+    d.tb.fw = reinterpret_cast<const u64x2 *>(c->d_tables);
+    d.tb.iw = reinterpret_cast<const u64x2 *>(c->d_tables + 2 * N);
     d.tb.n_inv = c->ht.n_inv;
     d.tb.n_inv_s = c->ht.n_inv_s;
     d.tb.q = p->rlwe_q;

@@ -8,6 +8,17 @@ namespace pz {
 
 constexpr int MAX_AUTO = 32;
 
+// Forward NTT mode of the fused RLWE operators: FWD_FAST (unreduced) or
+// FWD_LAZY (the reference's forward_lazy values).  Both give identical
+// ciphertexts; only the evaluation-domain scratch values differ.
+#ifndef PZ_FWD_MODE
+#define PZ_FWD_MODE FWD_FAST
+#endif
+
+#ifndef PZ_MAC_FENCE
+#define PZ_MAC_FENCE 0
+#endif
+
 // Everything a kernel needs about the parameter set (passed by value).
 struct DevCtx {
     Tables tb;
@@ -48,6 +59,11 @@ PZ_DEV void load8(const u64 *__restrict__ p, u64 (&v)[8])
 PZ_DEV void fma_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
                     const u64 (&D)[8], u32 j, u64 q, u64 q2)
 {
+#if PZ_MAC_FENCE
+    // keep this row's key loads from being hoisted above the previous row /
+    // the NTT: hoisted, all rows' loads are live at once (~128 VGPRs)
+    __builtin_amdgcn_sched_barrier(0);
+#endif
     u64 kv[8], ks[8];
     load8(K + 8 * j, kv);
     load8(Ks + 8 * j, ks);
@@ -76,7 +92,7 @@ PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
             D[0][r] = decomp_digit(c.dec_a, a[r], 0);
             D[1][r] = decomp_digit(c.dec_b, b[r], 0);
         }
-        ntt_fwd<LOGN, 2, false>(D, j, lds, c.tb);
+        ntt_fwd<LOGN, 2, PZ_FWD_MODE>(D, j, lds, c.tb);
         fma_row(acc[0], K + 0 * N, Ks + 0 * N, D[0], j, q, q2);
         fma_row(acc[0], K + 2 * N, Ks + 2 * N, D[1], j, q, q2);
         fma_row(acc[1], K + 1 * N, Ks + 1 * N, D[0], j, q, q2);
@@ -87,7 +103,7 @@ PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
                 u64 D[1][8];
 #pragma unroll
                 for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_a, a[r], rr);
-                ntt_fwd<LOGN, 1, false>(D, j, lds, c.tb);
+                ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, lds, c.tb);
                 fma_row(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
                 fma_row(acc[1], K + (size_t)(da + rr) * N, Ks + (size_t)(da + rr) * N, D[0], j,
                         q, q2);
@@ -97,7 +113,7 @@ PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
             u64 D[1][8];
 #pragma unroll
             for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_b, b[r], rr);
-            ntt_fwd<LOGN, 1, false>(D, j, lds, c.tb);
+            ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, lds, c.tb);
             size_t o0 = (size_t)(2 * da + rr) * N, o1 = (size_t)(2 * da + db + rr) * N;
             fma_row(acc[0], K + o0, Ks + o0, D[0], j, q, q2);
             fma_row(acc[1], K + o1, Ks + o1, D[0], j, q, q2);
@@ -163,7 +179,7 @@ PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
         u64 D[1][8];
 #pragma unroll
         for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_auto, a[r], rr);
-        ntt_fwd<LOGN, 1, false>(D, j, lds, c.tb);
+        ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, lds, c.tb);
         fma_row(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
         fma_row(acc[1], K + (size_t)(d + rr) * N, Ks + (size_t)(d + rr) * N, D[0], j, q, q2);
     }

@@ -27,8 +27,8 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_ntt_rows(u64 *rows, size_t 
         if (MODE <= 1) {
 #pragma unroll
             for (int r = 0; r < 8; r++) v[0][r] = p[coef_x<LOGN>(j, r)];
-            if (MODE == 0) ntt_fwd<LOGN, 1, false>(v, j, lds, tb);
-            else ntt_fwd<LOGN, 1, true>(v, j, lds, tb);
+            if (MODE == 0) ntt_fwd<LOGN, 1, FWD_LAZY>(v, j, lds, tb);
+            else ntt_fwd<LOGN, 1, FWD_STRICT>(v, j, lds, tb);
 #pragma unroll
             for (int r = 0; r < 8; r++) p[eval_x<LOGN>(j, r)] = v[0][r];
         } else {

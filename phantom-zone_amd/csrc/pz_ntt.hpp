@@ -40,15 +40,35 @@ struct Phase {
     PZ_DEV static u32 x(u32 j, int r) { return base(j) + ST * r; }
 };
 
+// Load through an address_space(1) pointer: table pointers that went through
+// an asm barrier are generic, and generic (flat) loads also count against the
+// LDS counter, serialising every twiddle load with the LDS exchanges.
+typedef u64 u64x2 __attribute__((ext_vector_type(2)));
+
+PZ_DEV u64x2 gld2(const u64x2 *p)
+{
+    return *(const __attribute__((address_space(1))) u64x2 *)(p);
+}
+
+// NTT tables in device memory: (w, w_shoup) pairs interleaved so one 16-byte
+// load fetches both, powers in the reference's bit-reversed order.
 struct Tables {
-    const u64 *__restrict__ psi, *__restrict__ psis;    // forward, bit-reversed
-    const u64 *__restrict__ ipsi, *__restrict__ ipsis;  // inverse, bit-reversed
+    const u64x2 *__restrict__ fw;  // forward  (psi^bo, shoup)
+    const u64x2 *__restrict__ iw;  // inverse  (psi^-bo, shoup)
     u64 n_inv, n_inv_s;
     u64 q, q2;
 };
 
+// Forward modes: the reference's forward_lazy ([0,2q)) and forward ([0,q)),
+// and FWD_FAST for the fused blind rotation: no conditional subtractions, so
+// stage k leaves values below (2k+1)q (< 23q < 2^59 for N = 2048).  FAST is
+// only used where a Shoup product consumes the result (Shoup accepts any
+// 64-bit input), so every ciphertext stays identical to the reference.
+enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2 };
+
 // One forward stage S inside a phase of stride ST. KIND: 0 = 0_to_4q
-// butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q).
+// butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q),
+// 3 = unreduced (FWD_FAST).
 template <int LOGN, int S, int ST, int NP, int KIND>
 PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 {
@@ -61,11 +81,17 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
         if (r & LD) continue;
         const u32 x = base + ST * r;
         const u32 i = x / (2 * T);
-        const u64 w = tb.psi[M + i], ws = tb.psis[M + i];
+        const u64x2 tw = gld2(tb.fw + M + i);
+        const u64 w = tw.x, ws = tw.y;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
             if (KIND == 0) {
                 fb_4q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
+            } else if (KIND == 3) {
+                const u64 x = v[p][r];
+                const u64 t = shoup_mul(v[p][r + LD], w, ws, tb.q);
+                v[p][r] = x + t;
+                v[p][r + LD] = x + tb.q2 - t;
             } else {
                 fb_2q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
                 if (KIND == 2) {
@@ -91,7 +117,8 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
         if (r & LD) continue;
         const u32 x = base + ST * r;
         const u32 i = x / (2 * T);
-        const u64 w = tb.ipsi[M + i], ws = tb.ipsis[M + i];
+        const u64x2 tw = gld2(tb.iw + M + i);
+        const u64 w = tw.x, ws = tw.y;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
             ib_2q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
@@ -110,18 +137,19 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 }
 
 // Forward phase starting at stage S0 (up to three stages).
-template <int LOGN, int S0, int NP, bool STRICT>
+template <int LOGN, int S0, int NP, int MODE>
 PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
 {
     using P = Phase<LOGN, S0>;
     const u32 base = P::base(j);
     constexpr int LAST = LOGN - 1;
-    constexpr int K_LAST = STRICT ? 2 : 1;
-    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : 0>(v, base, tb);
+    constexpr int K_LAST = MODE == FWD_STRICT ? 2 : (MODE == FWD_LAZY ? 1 : 3);
+    constexpr int K_MID = MODE == FWD_FAST ? 3 : 0;
+    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID>(v, base, tb);
     if constexpr (P::NS > 1)
-        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : 0>(v, base, tb);
+        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID>(v, base, tb);
     if constexpr (P::NS > 2)
-        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : 0>(v, base, tb);
+        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID>(v, base, tb);
 }
 
 // Inverse phase covering stages S0+NS-1 .. S0 (descending).
@@ -153,20 +181,38 @@ PZ_DEV void exchange(u64 (&v)[NP][8], u32 j, u64 *lds)
     __syncthreads();
 }
 
+// Twiddle-table pointers passed through an empty asm before each phase.  The
+// compiler then cannot hoist the (loop-invariant) twiddle loads out of the
+// blind-rotation loop: hoisted, they pin ~160 VGPRs per thread and cap the
+// kernel at one wave per SIMD.
+#ifndef PZ_TWIDDLE_RELOAD
+#define PZ_TWIDDLE_RELOAD 1
+#endif
+PZ_DEV Tables fresh(const Tables &tb)
+{
+    Tables t = tb;
+#if PZ_TWIDDLE_RELOAD
+    asm volatile("" : "+s"(t.fw), "+s"(t.iw));
+#endif
+    return t;
+}
+
 // Forward NTT of NP polynomials: stride layout in, contiguous layout out.
-// STRICT = Ntt::forward (output in [0,q)), else forward_lazy ([0,2q)).
-template <int LOGN, int NP, bool STRICT>
+// MODE FWD_STRICT = Ntt::forward (output in [0,q)), FWD_LAZY = forward_lazy
+// ([0,2q)), FWD_FAST = unreduced (see above).
+template <int LOGN, int NP, int MODE>
 PZ_DEV void ntt_fwd(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
-    fwd_phase<LOGN, 0, NP, STRICT>(v, j, tb);
+    static_assert(MODE != FWD_FAST || LOGN <= 11, "unreduced values must stay below 2^64");
+    fwd_phase<LOGN, 0, NP, MODE>(v, j, fresh(tb));
     exchange<LOGN, 0, 3, NP>(v, j, lds);
-    fwd_phase<LOGN, 3, NP, STRICT>(v, j, tb);
+    fwd_phase<LOGN, 3, NP, MODE>(v, j, fresh(tb));
     exchange<LOGN, 3, 6, NP>(v, j, lds);
-    fwd_phase<LOGN, 6, NP, STRICT>(v, j, tb);
+    fwd_phase<LOGN, 6, NP, MODE>(v, j, fresh(tb));
     if constexpr (LOGN > 9) {
         exchange<LOGN, 6, 9, NP>(v, j, lds);
-        fwd_phase<LOGN, 9, NP, STRICT>(v, j, tb);
+        fwd_phase<LOGN, 9, NP, MODE>(v, j, fresh(tb));
     }
 }
 
@@ -176,14 +222,14 @@ template <int LOGN, int NP, bool STRICT>
 PZ_DEV void ntt_inv(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
 {
     if constexpr (LOGN > 9) {
-        inv_phase<LOGN, 9, NP, STRICT>(v, j, tb);
+        inv_phase<LOGN, 9, NP, STRICT>(v, j, fresh(tb));
         exchange<LOGN, 9, 6, NP>(v, j, lds);
     }
-    inv_phase<LOGN, 6, NP, STRICT>(v, j, tb);
+    inv_phase<LOGN, 6, NP, STRICT>(v, j, fresh(tb));
     exchange<LOGN, 6, 3, NP>(v, j, lds);
-    inv_phase<LOGN, 3, NP, STRICT>(v, j, tb);
+    inv_phase<LOGN, 3, NP, STRICT>(v, j, fresh(tb));
     exchange<LOGN, 3, 0, NP>(v, j, lds);
-    inv_phase<LOGN, 0, NP, STRICT>(v, j, tb);
+    inv_phase<LOGN, 0, NP, STRICT>(v, j, fresh(tb));
 }
 
 // Coefficient index of register r in the stride (coefficient-domain) layout

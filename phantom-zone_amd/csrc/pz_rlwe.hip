@@ -7,6 +7,11 @@
 //                     at a time, RLWE resident in VGPRs)
 #include "pz_internal.hpp"
 
+#ifndef PZ_BR_WAVES
+#define PZ_BR_WAVES 2
+#endif
+#define PZ_BR_ATTR __attribute__((amdgpu_waves_per_eu(PZ_BR_WAVES, PZ_BR_WAVES)))
+
 namespace pz {
 
 // ---------------------------------------------------------------------------
@@ -14,7 +19,7 @@ namespace pz {
 // rlwe: [B][2][N]; idx[b] selects the RGSW (LWE index) or auto key.
 // ---------------------------------------------------------------------------
 template <int LOGN>
-__global__ __launch_bounds__(1 << (LOGN - 3)) void k_ext(u64 *rlwe, const u32 *idx,
+__global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_ext(u64 *rlwe, const u32 *idx,
                                                          const u8 *trivial, size_t B, DevCtx c,
                                                          DevKey key)
 {
@@ -75,7 +80,7 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *
 // LDS: 2 polynomials (exchange / permutation) + bucket starts + index list.
 // ---------------------------------------------------------------------------
 template <int LOGN>
-__global__ __launch_bounds__(1 << (LOGN - 3)) void k_blind_rotate(const u32 *ks, const u32 *gate,
+__global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(const u32 *ks, const u32 *gate,
                                                                   size_t B, u64 *out, u32 *stats,
                                                                   DevCtx c, DevKey key)
 {

@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpz.so")
+LIB_PATH = os.environ.get("PZ_LIBPZ") or os.path.join(HERE, "libpz.so")
 
 GATES = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5}
 
