@@ -373,6 +373,8 @@ extern "C" int pz_key_upload(pz_ctx *c, const uint64_t *h_rgsw, const uint64_t *
     size_t auto_words = (size_t)(p.w + 1) * auto_rows * N;
     size_t ksk_words = (size_t)N * p.lwe_d * (p.lwe_n + 1);
     u32 kb = p.lwe_q <= 65536 ? 2 : 4;
+    u32 n1 = p.lwe_n + 1, ldk = (n1 + 63) / 64 * 64;
+    size_t ksk_dev = (size_t)N * p.lwe_d * ldk;
     pz_key *k = new pz_key();
     k->owner = c;
     u64 *tmp = nullptr;
@@ -380,7 +382,7 @@ extern "C" int pz_key_upload(pz_ctx *c, const uint64_t *h_rgsw, const uint64_t *
               hipMalloc(&k->rgsw_s, 8 * rgsw_words) == hipSuccess &&
               hipMalloc(&k->auto_k, 8 * auto_words) == hipSuccess &&
               hipMalloc(&k->auto_s, 8 * auto_words) == hipSuccess &&
-              hipMalloc(&k->ksk, kb * ksk_words) == hipSuccess &&
+              hipMalloc(&k->ksk, kb * ksk_dev) == hipSuccess &&
               hipMalloc(&tmp, 8 * ksk_words) == hipSuccess;
     if (!ok) {
         hipFree(tmp);
@@ -393,7 +395,7 @@ extern "C" int pz_key_upload(pz_ctx *c, const uint64_t *h_rgsw, const uint64_t *
     hipMemcpyAsync(tmp, h_ksk, 8 * ksk_words, hipMemcpyHostToDevice, s);
     launch_shoup(k->rgsw, k->rgsw_s, rgsw_words, p.rlwe_q, s);
     launch_shoup(k->auto_k, k->auto_s, auto_words, p.rlwe_q, s);
-    launch_pack_ksk(tmp, k->ksk, ksk_words, kb, s);
+    launch_pack_ksk(tmp, k->ksk, (size_t)N * p.lwe_d, n1, ldk, kb, s);
     hipError_t e = hipStreamSynchronize(s);
     hipFree(tmp);
     if (e != hipSuccess) {
@@ -408,7 +410,8 @@ extern "C" int pz_key_upload(pz_ctx *c, const uint64_t *h_rgsw, const uint64_t *
     k->dk.rgsw_rows = (u32)rgsw_rows;
     k->dk.auto_rows = (u32)auto_rows;
     k->dk.ksk_bytes = kb;
-    k->bytes = 16 * rgsw_words + 16 * auto_words + kb * ksk_words;
+    k->dk.ldk = ldk;
+    k->bytes = 16 * rgsw_words + 16 * auto_words + kb * ksk_dev;
     *out = k;
     return PZ_OK;
 }
@@ -475,6 +478,7 @@ static int ensure_scratch(pz_ctx *c, size_t B)
     c->d_ks = nullptr;
     size_t rows = (size_t)c->p.rlwe_n * c->p.lwe_d;
     size_t cap = B < 64 ? 64 : B;
+    cap = (cap + KS_CT - 1) / KS_CT * KS_CT;  // the key switch reads whole ciphertext tiles
     HIPCHK(hipMalloc(&c->d_digits, rows * cap));
     HIPCHK(hipMalloc(&c->d_ksb, 4 * cap));
     HIPCHK(hipMalloc(&c->d_ks, 4 * cap * (c->p.lwe_n + 1)));
@@ -493,11 +497,11 @@ extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint6
     hipStream_t st = S(c, s);
     const pz_params &p = c->p;
     u32 n1 = p.lwe_n + 1;
-    launch_prep(PREP_KSONLY, in, nullptr, nullptr, p.rlwe_n + 1, B, c->d_digits, c->d_ksb, c->dc,
-                st);
-    launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1, B,
-              c->d_ks, (u32)(p.lwe_q - 1), st);
-    launch_widen(c->d_ks, out, B * n1, st);
+    launch_prep(PREP_KSONLY, in, nullptr, nullptr, p.rlwe_n + 1, B, c->d_digits, c->d_ksb,
+                c->d_ks, c->dc, st);
+    launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
+              k->dk.ldk, B, c->d_ks, st);
+    launch_widen(c->d_ks, out, B * n1, (u32)(p.lwe_q - 1), st);
     HIPCHK(hipGetLastError());
     return PZ_OK;
 }
@@ -509,10 +513,10 @@ static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, con
     const pz_params &p = c->p;
     u32 n1 = p.lwe_n + 1;
     if (c->timing) hipEventRecord(c->ev[0], st);
-    launch_prep(mode, c0, c1, ops, p.rlwe_n + 1, B, c->d_digits, c->d_ksb, c->dc, st);
+    launch_prep(mode, c0, c1, ops, p.rlwe_n + 1, B, c->d_digits, c->d_ksb, c->d_ks, c->dc, st);
     if (c->timing) hipEventRecord(c->ev[1], st);
-    launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1, B,
-              c->d_ks, (u32)(p.lwe_q - 1), st);
+    launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
+              k->dk.ldk, B, c->d_ks, st);
     if (c->timing) hipEventRecord(c->ev[2], st);
     size_t resident = (size_t)c->num_cu * c->br_per_cu;
     if (resident < 1) resident = 1;

@@ -39,8 +39,8 @@ struct DevCtx {
 struct DevKey {
     const u64 *rgsw, *rgsw_s;  // [n][rgsw_rows][N]
     const u64 *auto_k, *auto_s;  // [w+1][auto_rows][N]
-    const void *ksk;           // [N*d][n+1], u16 or u32
-    u32 rgsw_rows, auto_rows, ksk_bytes;
+    const void *ksk;           // [N*d][ldk], u16 or u32, columns n+1..ldk-1 zero
+    u32 rgsw_rows, auto_rows, ksk_bytes, ldk;
 };
 
 // 8 consecutive words (evaluation-domain slice of one thread)

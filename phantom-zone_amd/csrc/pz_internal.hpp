@@ -14,9 +14,9 @@ enum { PREP_GATE = 0, PREP_PBS = 1, PREP_KSONLY = 2 };
 
 // LWE key-switch tiling: threads per group (one output column each),
 // ciphertexts per group, KSK rows staged per LDS round.
-constexpr int KS_TPB = 256;
-constexpr int KS_CT = 16;
-constexpr int KS_ROWS = 256;
+constexpr int KS_CT = 32;     // ciphertexts per work group
+constexpr int KS_ROWS = 256;  // KSK rows per LDS round
+constexpr int KS_SPLIT = 2;   // row splits (summed with atomics)
 
 void launch_ntt(int logn, int mode, u64 *rows, size_t nrows, const Tables &tb, hipStream_t s);
 void launch_fma(u64 *out, const u64 *a, const u64 *as, const u64 *b, u32 d, size_t n, u64 q,
@@ -26,16 +26,17 @@ void launch_decompose(const u64 *in, u64 *digits, size_t count, const Decomp &dc
 void launch_ext(int logn, bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv, size_t B,
                 const DevCtx &c, const DevKey &k, hipStream_t s);
 void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *ops, u32 N1, size_t B,
-                 int8_t *digits, u32 *ks_b, const DevCtx &c, hipStream_t s);
+                 int8_t *digits, u32 *ks_b, u32 *ks_out, const DevCtx &c, hipStream_t s);
 void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_bytes, u32 rows,
-               u32 n1, size_t B, u32 *out, u32 mask, hipStream_t s);
+               u32 n1, u32 ldk, size_t B, u32 *out, hipStream_t s);
 size_t br_lds_bytes(int logn, u32 br_q, u32 n_lwe);
 const void *blind_rotate_kernel(int logn);
 void launch_blind_rotate(int logn, int grid, const u32 *ks, const u32 *gate, size_t B, u64 *out,
                          u32 *stats, const DevCtx &c, const DevKey &k, hipStream_t s);
 void launch_shoup(const u64 *in, u64 *out, size_t n, u64 q, hipStream_t s);
 void launch_not(u64 *c0, size_t n, u64 q, hipStream_t s);
-void launch_widen(const u32 *in, u64 *out, size_t n, hipStream_t s);
-void launch_pack_ksk(const u64 *in, void *out, size_t n, u32 bytes, hipStream_t s);
+void launch_widen(const u32 *in, u64 *out, size_t n, u32 mask, hipStream_t s);
+void launch_pack_ksk(const u64 *in, void *out, size_t rows, u32 n1, u32 ldk, u32 bytes,
+                     hipStream_t s);
 
 }  // namespace pz

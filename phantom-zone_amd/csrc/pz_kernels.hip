@@ -77,11 +77,12 @@ __global__ void k_decompose(const u64 *in, u64 *digits, size_t count, Decomp dc)
 // mode: PREP_GATE (c0, c1, ops), PREP_PBS (in), PREP_KSONLY (in is mod q_ks).
 // ---------------------------------------------------------------------------
 __global__ void k_prep(int mode, const u64 *c0, const u64 *c1, const u32 *ops, u32 N1,
-                       size_t B, int8_t *digits, u32 *ks_b, DevCtx c)
+                       size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out, DevCtx c)
 {
     const size_t bb = blockIdx.x;
     const u32 col = blockIdx.y * blockDim.x + threadIdx.x;
     if (bb >= B || col >= N1) return;
+    if (col < c.n_lwe + 1) ks_out[bb * (c.n_lwe + 1) + col] = 0;  // k_ks accumulates
     const u64 q = c.tb.q;
     u64 v = c0[bb * N1 + col];
     if (mode == PREP_GATE) {
@@ -104,60 +105,127 @@ __global__ void k_prep(int mode, const u64 *c0, const u64 *c1, const u32 *ops, u
         ks_b[bb] = (u32)md;
         return;
     }
+    // digits layout [B/KS_CT][rows][KS_CT]: the key switch stages a row of
+    // KS_CT ciphertexts with contiguous loads
     const u32 d = c.lwe_d;
     const u64 lq = c.lwe_mask + 1;
-    int8_t *out = digits + bb * (size_t)(N1 - 1) * d + (size_t)(col - 1) * d;
+    const size_t rows = (size_t)(N1 - 1) * d;
+    int8_t *out = digits + (bb / KS_CT) * rows * KS_CT + (size_t)(col - 1) * d * KS_CT +
+                  (bb % KS_CT);
     u64 value = decomp_prepare(c.dec_lwe, md);
     for (u32 r = 0; r < d; r++) {
         u64 dg = decomp_next(c.dec_lwe, value);
         long long sv = dg >= (lq >> 1) ? (long long)dg - (long long)lq : (long long)dg;
-        out[r] = (int8_t)sv;
+        out[(size_t)r * KS_CT] = (int8_t)sv;
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_ks: out[b][col] = (sum_rows digit[b][row] * ksk[row][col] + (col==0)*b_in) mod q_ks
-// grid: (ceil((n+1)/KS_TPB), ceil(B/KS_CT)); block KS_TPB; each thread one
-// output column for KS_CT ciphertexts; digits staged through LDS.
+// k_ks: LWE key switch as a digits x KSK product, mod q_ks (src/lwe.rs:14-44).
+//   out[b][col] += sum_{rows in this split} digit[b][row] * ksk[row][col]
+// grid (ceil(B/KS_CT), KS_SPLIT); each thread owns one output unit for KS_CT
+// ciphertexts: for q_ks <= 2^16 a unit is two adjacent columns held as 16-bit
+// lanes of one register (v_pk_mad_u16: both lanes wrap mod 2^16, which is
+// the reference's mask arithmetic); otherwise one column in 32 bits.  Digits
+// are staged through LDS, pre-broadcast to both lanes.  The row splits meet
+// in u32 atomics; readers mask the sums with q_ks - 1 (q_ks divides 2^32).
+// out must be zero on entry (k_prep clears it) and gets b_in at column 0.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(KS_TPB) void k_ks(const int8_t *digits, const u32 *ks_b,
-                                               const T *ksk, u32 rows, u32 n1, size_t B,
-                                               u32 *out, u32 mask)
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+template <bool PACK16>
+__global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks_b,
+                                             const void *ksk, u32 rows, u32 n1, u32 ldk,
+                                             size_t B, u32 *out)
 {
-    __shared__ int8_t dig[KS_ROWS][KS_CT];
-    const u32 col = blockIdx.x * KS_TPB + threadIdx.x;
-    const size_t b0 = (size_t)blockIdx.y * KS_CT;
+    __shared__ __attribute__((aligned(16))) u32 dig[KS_ROWS][KS_CT];
+    const u32 unit = threadIdx.x;
+    const u32 units = PACK16 ? (n1 + 1) / 2 : n1;
+    const size_t b0 = (size_t)blockIdx.x * KS_CT;
+    const int8_t *tile = digits + (size_t)blockIdx.x * rows * KS_CT;
+    const u32 per = (rows + gridDim.y - 1) / gridDim.y;
+    const u32 rbeg = blockIdx.y * per;
+    const u32 rend = rbeg + per < rows ? rbeg + per : rows;
+    const uint32_t *k32 = reinterpret_cast<const uint32_t *>(ksk);
+    const u32 ld32 = PACK16 ? ldk / 2 : ldk;
     u32 acc[KS_CT];
 #pragma unroll
     for (int t = 0; t < KS_CT; t++) acc[t] = 0;
-    for (u32 r0 = 0; r0 < rows; r0 += KS_ROWS) {
+    for (u32 r0 = rbeg; r0 < rend; r0 += KS_ROWS) {
+        const u32 rmax = (rend - r0) < (u32)KS_ROWS ? (rend - r0) : (u32)KS_ROWS;
         __syncthreads();
-        for (u32 e = threadIdx.x; e < KS_ROWS * KS_CT; e += KS_TPB) {
-            u32 rr = e % KS_ROWS, t = e / KS_ROWS;
-            size_t bb = b0 + t;
-            int8_t dv = 0;
-            if (bb < B && r0 + rr < rows) dv = digits[bb * rows + r0 + rr];
-            dig[rr][t] = dv;
+        // stage: 4 digits (one u32 of int8) per step, broadcast to 16-bit lanes
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(tile + (size_t)r0 * KS_CT);
+        for (u32 e = threadIdx.x; e < rmax * KS_CT / 4; e += blockDim.x) {
+            u32 w4 = src[e];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int dv = (int)(int8_t)(w4 >> (8 * k));
+                (&dig[0][0])[4 * e + k] = PACK16 ? ((u32)(uint16_t)(int16_t)dv) * 0x10001u : (u32)dv;
+            }
         }
         __syncthreads();
-        if (col < n1) {
-            const u32 rmax = (rows - r0) < KS_ROWS ? (rows - r0) : KS_ROWS;
-            for (u32 rr = 0; rr < rmax; rr++) {
-                int kv = (int)ksk[(size_t)(r0 + rr) * n1 + col];
+        if (unit < units) {
+            const uint32_t *kp = k32 + (size_t)r0 * ld32 + unit;
+            u32 rr = 0;
+            for (; rr + 4 <= rmax; rr += 4) {
+                u32 kv[4];
 #pragma unroll
-                for (int t = 0; t < KS_CT; t++) acc[t] += (u32)((int)dig[rr][t] * kv);
+                for (int u = 0; u < 4; u++) kv[u] = kp[(size_t)(rr + u) * ld32];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const uint4 *dr = reinterpret_cast<const uint4 *>(&dig[rr + u][0]);
+#pragma unroll
+                    for (int t4 = 0; t4 < KS_CT / 4; t4++) {
+                        uint4 dd = dr[t4];
+                        u32 dv[4] = {dd.x, dd.y, dd.z, dd.w};
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            int t = 4 * t4 + k;
+                            if (PACK16) {
+                                u16x2 a = __builtin_bit_cast(u16x2, acc[t]);
+                                u16x2 dk = __builtin_bit_cast(u16x2, dv[k]);
+                                u16x2 kk = __builtin_bit_cast(u16x2, kv[u]);
+                                acc[t] = __builtin_bit_cast(u32, (u16x2)(dk * kk + a));
+                            } else {
+                                acc[t] += (u32)((int)dv[k] * (int)kv[u]);
+                            }
+                        }
+                    }
+                }
+            }
+            for (; rr < rmax; rr++) {
+                u32 kv = kp[(size_t)rr * ld32];
+#pragma unroll
+                for (int t = 0; t < KS_CT; t++) {
+                    if (PACK16) {
+                        u16x2 a = __builtin_bit_cast(u16x2, acc[t]);
+                        u16x2 dk = __builtin_bit_cast(u16x2, dig[rr][t]);
+                        u16x2 kk = __builtin_bit_cast(u16x2, kv);
+                        acc[t] = __builtin_bit_cast(u32, (u16x2)(dk * kk + a));
+                    } else {
+                        acc[t] += (u32)((int)dig[rr][t] * (int)kv);
+                    }
+                }
             }
         }
     }
-    if (col < n1) {
+    if (unit >= units) return;
 #pragma unroll
-        for (int t = 0; t < KS_CT; t++) {
-            size_t bb = b0 + t;
-            if (bb < B) {
-                u32 v = acc[t] + (col == 0 ? ks_b[bb] : 0u);
-                out[bb * n1 + col] = v & mask;
-            }
+    for (int t = 0; t < KS_CT; t++) {
+        size_t bb = b0 + t;
+        if (bb >= B) continue;
+        u32 *o = out + bb * n1;
+        if (PACK16) {
+            u32 c0 = 2 * unit;
+            u32 lo = acc[t] & 0xffffu, hi = acc[t] >> 16;
+            if (c0 == 0 && blockIdx.y == 0) lo += ks_b[bb];
+            atomicAdd(o + c0, lo);
+            if (c0 + 1 < n1) atomicAdd(o + c0 + 1, hi);
+        } else {
+            u32 v = acc[t];
+            if (unit == 0 && blockIdx.y == 0) v += ks_b[bb];
+            atomicAdd(o + unit, v);
         }
     }
 }
@@ -191,20 +259,25 @@ __global__ void k_not(u64 *c0, size_t n, u64 q)
 }
 
 // u32 LWE words -> u64
-__global__ void k_widen(const u32 *in, u64 *out, size_t n)
+__global__ void k_widen(const u32 *in, u64 *out, size_t n, u32 mask)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
-        out[i] = in[i];
+        out[i] = in[i] & mask;
 }
 
-// KSK rows (u64, reference layout) -> packed device words
+// KSK rows (u64, reference layout [rows][n1]) -> device words [rows][ldk]
+// (ldk >= n1, padding zero)
 template <typename T>
-__global__ void k_pack_ksk(const u64 *in, T *out, size_t n)
+__global__ void k_pack_ksk(const u64 *in, T *out, size_t rows, u32 n1, u32 ldk)
 {
+    size_t n = rows * ldk;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        out[i] = (T)in[i];
+         i += (size_t)gridDim.x * blockDim.x) {
+        size_t r = i / ldk;
+        u32 col = (u32)(i % ldk);
+        out[i] = col < n1 ? (T)in[r * n1 + col] : (T)0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -253,24 +326,26 @@ void launch_decompose(const u64 *in, u64 *digits, size_t count, const Decomp &dc
 }
 
 void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *ops, u32 N1, size_t B,
-                 int8_t *digits, u32 *ks_b, const DevCtx &c, hipStream_t s)
+                 int8_t *digits, u32 *ks_b, u32 *ks_out, const DevCtx &c, hipStream_t s)
 {
     if (B == 0) return;
     dim3 grid((unsigned)B, (N1 + 255) / 256);
-    hipLaunchKernelGGL(k_prep, grid, 256, 0, s, mode, c0, c1, ops, N1, B, digits, ks_b, c);
+    hipLaunchKernelGGL(k_prep, grid, 256, 0, s, mode, c0, c1, ops, N1, B, digits, ks_b, ks_out,
+                       c);
 }
 
 void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_bytes, u32 rows,
-               u32 n1, size_t B, u32 *out, u32 mask, hipStream_t s)
+               u32 n1, u32 ldk, size_t B, u32 *out, hipStream_t s)
 {
     if (B == 0) return;
-    dim3 grid((n1 + KS_TPB - 1) / KS_TPB, (unsigned)((B + KS_CT - 1) / KS_CT));
-    if (ksk_bytes == 2)
-        hipLaunchKernelGGL(k_ks<uint16_t>, grid, KS_TPB, 0, s, digits, ks_b,
-                           (const uint16_t *)ksk, rows, n1, B, out, mask);
+    const bool pack = ksk_bytes == 2;
+    u32 units = pack ? (n1 + 1) / 2 : n1;
+    u32 tpb = (units + 63) / 64 * 64;
+    dim3 grid((unsigned)((B + KS_CT - 1) / KS_CT), KS_SPLIT);
+    if (pack)
+        hipLaunchKernelGGL(k_ks<true>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk, B, out);
     else
-        hipLaunchKernelGGL(k_ks<uint32_t>, grid, KS_TPB, 0, s, digits, ks_b,
-                           (const uint32_t *)ksk, rows, n1, B, out, mask);
+        hipLaunchKernelGGL(k_ks<false>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk, B, out);
 }
 
 void launch_shoup(const u64 *in, u64 *out, size_t n, u64 q, hipStream_t s)
@@ -283,19 +358,21 @@ void launch_not(u64 *c0, size_t n, u64 q, hipStream_t s)
     hipLaunchKernelGGL(k_not, grid_for(n, 256), 256, 0, s, c0, n, q);
 }
 
-void launch_widen(const u32 *in, u64 *out, size_t n, hipStream_t s)
+void launch_widen(const u32 *in, u64 *out, size_t n, u32 mask, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_widen, grid_for(n, 256), 256, 0, s, in, out, n);
+    hipLaunchKernelGGL(k_widen, grid_for(n, 256), 256, 0, s, in, out, n, mask);
 }
 
-void launch_pack_ksk(const u64 *in, void *out, size_t n, u32 bytes, hipStream_t s)
+void launch_pack_ksk(const u64 *in, void *out, size_t rows, u32 n1, u32 ldk, u32 bytes,
+                     hipStream_t s)
 {
+    size_t n = rows * ldk;
     if (bytes == 2)
         hipLaunchKernelGGL(k_pack_ksk<uint16_t>, grid_for(n, 256), 256, 0, s, in,
-                           (uint16_t *)out, n);
+                           (uint16_t *)out, rows, n1, ldk);
     else
         hipLaunchKernelGGL(k_pack_ksk<uint32_t>, grid_for(n, 256), 256, 0, s, in,
-                           (uint32_t *)out, n);
+                           (uint32_t *)out, rows, n1, ldk);
 }
 
 }  // namespace pz

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
         for (u32 i = j; i <= nb; i += TPB) cnt[i] = 0;
         __syncthreads();
         for (u32 i = j; i < n; i += TPB) {
-            u32 v = in[1 + i];
+            u32 v = in[1 + i] & (u32)c.lwe_mask;
             u32 odd = ((u32)(((u64)v * c.br_q) >> c.lwe_q_log)) | 1u;
             u32 k = c.dlog[odd];
             kbin[i] = (u16)k;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
         __syncthreads();
 
         // (4) test vector times X^{g*b} (monomial_mul, src/pbs.rs:163-211)
-        const u32 oddb = ((u32)(((u64)in[0] * c.br_q) >> c.lwe_q_log)) | 1u;
+        const u32 oddb = ((u32)(((u64)(in[0] & (u32)c.lwe_mask) * c.br_q) >> c.lwe_q_log)) | 1u;
         u32 exp = (u32)(((u64)c.g * oddb) % c.br_q);
         bool msign = true;
         if (exp > nb) {
