@@ -1,0 +1,101 @@
+"""Pins of the oracle's PBS and gates (src/pbs.rs, src/bool/evaluator.rs).
+
+* Gate truth tables after decryption for every input pair of all six gates
+  (NAND/AND/OR/NOR/XOR/XNOR) and NOT, at NI_2P and SP_TEST.
+* The dlog map: (+-)g^dlog[a] == a (mod br_q) for every odd a
+  (src/bool/evaluator.rs:664-673).
+* The blind-rotation schedule: every LWE index gets exactly one RLWE x RGSW
+  product, the automorphism count matches the PBS statistics, and an
+  automorphism never follows another without a window reason
+  (src/pbs.rs:277-413).
+* Test vectors: entries are +-Q/8 and the NAND vector has the partition of
+  src/bool/evaluator.rs:690-726 after the X -> X^{-g} map.
+"""
+import numpy as np
+import pytest
+
+from conftest import keys_for, rng
+
+PLAIN = {"nand": lambda a, b: 1 - (a & b), "and": lambda a, b: a & b,
+         "or": lambda a, b: a | b, "nor": lambda a, b: 1 - (a | b),
+         "xor": lambda a, b: a ^ b, "xnor": lambda a, b: 1 - (a ^ b)}
+
+
+@pytest.mark.parametrize("name", ["SP_TEST", "NI_2P"])
+def test_gate_truth_tables(name):
+    o, s, z, key = keys_for(name)
+    seed = 1000
+    for g, f in PLAIN.items():
+        for a in (0, 1):
+            for b in (0, 1):
+                c0 = o.encrypt_bool(s, a, seed)
+                c1 = o.encrypt_bool(s, b, seed + 1)
+                seed += 2
+                out = o.gate(key, g, c0, c1)
+                assert o.decrypt_bool(s, out) == f(a, b), (g, a, b)
+    for a in (0, 1):
+        c = o.encrypt_bool(s, a, seed)
+        assert o.decrypt_bool(s, o.not_(c)) == 1 - a
+
+
+@pytest.mark.parametrize("name", ["NI_2P", "NI_8P", "NI_4P_HB_FR", "SP_TEST"])
+def test_dlog_map(name):
+    from conftest import oracle_for
+    o = oracle_for(name)
+    brq, g = o.p.br_q, o.p.g
+    dl = o.dlog_map()
+    for a in range(1, brq, 2):
+        k = int(dl[a])
+        if k < brq // 4:
+            assert pow(g, k, brq) == a
+        else:
+            assert (brq - pow(g, k - brq // 4, brq)) % brq == a
+
+
+def test_schedule_and_stats():
+    o, s, z, key = keys_for("NI_2P")
+    r = rng(70)
+    lq = o.p.lwe_q
+    for trial in range(20):
+        ks = r.integers(0, lq, size=o.n + 1, dtype=np.uint64)
+        ops = o.br_schedule(ks)
+        kinds = ops & 0xFF
+        args = (ops >> 8) & 0x7FFFFF
+        ext = sorted(int(a) for k, a in zip(kinds, args) if k == 0)
+        assert ext == list(range(o.n))
+        autos = int(np.count_nonzero(kinds == 1))
+        assert 400 < autos < 560
+        assert all(0 <= int(a) <= o.p.w for k, a in zip(kinds, args) if k == 1)
+        # the first op is never a non-trivial automorphism without a product
+        first_ext = int(np.argmax(kinds == 0))
+        assert all((int(op) >> 31) == 1 for op in ops[:first_ext])
+
+
+def test_pbs_stats_match_schedule():
+    o, s, z, key = keys_for("SP_TEST")
+    r = rng(71)
+    for _ in range(4):
+        x = r.integers(0, o.Q, size=o.N + 1, dtype=np.uint64)
+        out, st = o.pbs(key, "nand", x)
+        assert st.ext_products == o.n
+        assert st.autos >= 1
+
+
+@pytest.mark.parametrize("name", ["NI_2P", "SP_TEST"])
+def test_test_vectors(name):
+    from conftest import oracle_for
+    o = oracle_for(name)
+    q = o.Q
+    t, f = q >> 3, q - (q >> 3)
+    for g in ("nand", "and", "or", "nor", "xor", "xnor"):
+        tv = o.test_vec(g)
+        assert set(int(v) for v in tv) <= {t, f, q - t, q - f}
+    # NAND: before the X -> X^{-g} map, entries < 3*br_q/8 are true
+    qby2 = o.p.br_q // 2
+    tv = o.test_vec("nand")
+    kk = (2 * qby2 - o.p.g) % (2 * qby2)
+    for i in range(0, qby2, 5):
+        to = (i * kk) % (2 * qby2)
+        val = t if i < 3 * (o.p.br_q // 8) else f
+        got = int(tv[to]) if to < qby2 else (q - int(tv[to - qby2])) % q
+        assert got == val
