@@ -116,6 +116,32 @@ int pz_key_free(pz_key *key);
 /* Device bytes held by the key (serving admission by HBM footprint). */
 int pz_key_bytes(pz_key *key, size_t *bytes);
 
+/* Copy a key back to the host in the pz_key_upload layout (h_lwe_ksk as u64
+ * rows [N*d_ks][n+1]).  Any pointer may be NULL to skip that part. */
+int pz_key_download(pz_key *key, uint64_t *h_rgsw, uint64_t *h_auto, uint64_t *h_lwe_ksk);
+
+/* ---- seeded client-side key generation (this build's own seeded
+ * procedure; structure of ClientKey::sk_rlwe / sk_lwe, src/bool/keys.rs:
+ * 75-154, secret_key_encrypt_rgsw, seeded_auto_key_gen, seeded_lwe_ksk_keygen,
+ * src/rgsw/keygen.rs:68-430, src/lwe.rs:46-101).  The ideal secret is the sum
+ * of `parties` parties' secrets: h_s[N] (RLWE, ternary per party) and h_z[n]
+ * (LWE).  pz_keygen_server builds the evaluation-domain server key on the
+ * device under that ideal secret. */
+int pz_keygen_secrets(pz_ctx *ctx, const uint8_t *seed32, uint32_t parties, int32_t *h_s,
+                      int32_t *h_z);
+int pz_keygen_server(pz_ctx *ctx, const uint8_t *seed32, const int32_t *h_s, const int32_t *h_z,
+                     pz_key **out);
+
+/* ---- client-side boolean LWE encryption / decryption under h_s (dim
+ * coefficients): encrypt_lwe / decrypt_lwe (src/lwe.rs:104-162) with the
+ * bool encoding +-Q/8 and decode (src/bool/evaluator.rs:209-258).
+ * Ciphertext i uses the seed (seed32 + i) as a 256-bit little-endian integer.
+ * pz_decrypt_bools returns PZ_ERR_STATE if any value decodes outside {0,1}. */
+int pz_encrypt_bools(pz_ctx *ctx, const int32_t *h_s, uint32_t dim, const uint8_t *seed32,
+                     const uint8_t *h_bits, size_t count, uint64_t *h_out);
+int pz_decrypt_bools(pz_ctx *ctx, const int32_t *h_s, uint32_t dim, const uint64_t *h_cts,
+                     size_t count, uint8_t *h_bits);
+
 /* ---- rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) on B RLWE
  * ciphertexts d_rlwe[B][2][N]; RLWE b is multiplied by RGSW(X^{z_i}) with
  * i = d_rgsw_idx[b]; d_trivial[b] != 0 skips part a (trivial RLWE). */

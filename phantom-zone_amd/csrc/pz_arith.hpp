@@ -107,10 +107,11 @@ PZ_DEV u64 barrett_mul(const Barrett &br, u64 a, u64 b)
     u64 mid2 = mid + p2_lo;
     u64 c2 = mid2 < mid;
     u64 top = p1_hi + p2_hi + t_hi * br.mu_hi + c1 + c2;
-    // (p0_lo, mid2, top) is the product's low 192 bits; take >> (alpha+2)
-    u32 sh = br.alpha + 2;  // < 64 for logq <= 60
-    u64 k_lo = (mid2 >> sh) | (top << (64 - sh));
-    (void)p0_lo;
+    // (p0_lo, mid2, top) is the product's low 192 bits; k = product >> (alpha+2)
+    // (only its low 64 bits matter: the result is taken mod 2^64)
+    u32 sh = br.alpha + 2;  // in (0, 64) for 2 < logq <= 60
+    u64 k_lo = (p0_lo >> sh) | (mid2 << (64 - sh));
+    (void)top;
     // out = (ab - k*q) mod 2^64
     u64 out = ab_lo - k_lo * br.q;
     return out >= br.q ? out - br.q : out;

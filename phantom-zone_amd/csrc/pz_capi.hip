@@ -360,48 +360,52 @@ extern "C" int pz_decompose(pz_ctx *c, int which, const uint64_t *in, uint64_t *
     return PZ_OK;
 }
 
-extern "C" int pz_key_upload(pz_ctx *c, const uint64_t *h_rgsw, const uint64_t *h_auto,
-                             const uint64_t *h_ksk, pz_key **out)
+// Internal view of a context for the keygen translation unit.
+int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStream_t *s)
 {
-    if (!c || !h_rgsw || !h_auto || !h_ksk || !out) return fail(PZ_ERR_ARG, "null argument");
-    HIPCHK(hipSetDevice(c->device));
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(PZ_ERR_HIP, "set device failed");
+    *p = c->p;
+    *dc = c->dc;
+    *logn = c->logn;
+    *s = c->stream;
+    return PZ_OK;
+}
+
+// Build a key from device rows already in the reference layout (evaluation
+// domain RGSW / auto rows, u64 KSK rows [N*d][n+1]): computes the Shoup twins
+// and the packed KSK.  Takes ownership of rgsw and auto_k on success.
+int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows, pz_key **out)
+{
     const pz_params &p = c->p;
     uint32_t N = p.rlwe_n;
     size_t rgsw_rows = 2 * (size_t)p.rlrg_da + 2 * (size_t)p.rlrg_db;
     size_t auto_rows = 2 * (size_t)p.auto_d;
     size_t rgsw_words = (size_t)p.lwe_n * rgsw_rows * N;
     size_t auto_words = (size_t)(p.w + 1) * auto_rows * N;
-    size_t ksk_words = (size_t)N * p.lwe_d * (p.lwe_n + 1);
     u32 kb = p.lwe_q <= 65536 ? 2 : 4;
     u32 n1 = p.lwe_n + 1, ldk = (n1 + 63) / 64 * 64;
     size_t ksk_dev = (size_t)N * p.lwe_d * ldk;
     pz_key *k = new pz_key();
     k->owner = c;
-    u64 *tmp = nullptr;
-    bool ok = hipMalloc(&k->rgsw, 8 * rgsw_words) == hipSuccess &&
-              hipMalloc(&k->rgsw_s, 8 * rgsw_words) == hipSuccess &&
-              hipMalloc(&k->auto_k, 8 * auto_words) == hipSuccess &&
+    bool ok = hipMalloc(&k->rgsw_s, 8 * rgsw_words) == hipSuccess &&
               hipMalloc(&k->auto_s, 8 * auto_words) == hipSuccess &&
-              hipMalloc(&k->ksk, kb * ksk_dev) == hipSuccess &&
-              hipMalloc(&tmp, 8 * ksk_words) == hipSuccess;
+              hipMalloc(&k->ksk, kb * ksk_dev) == hipSuccess;
     if (!ok) {
-        hipFree(tmp);
         pz_key_free(k);
         return fail(PZ_ERR_HIP, "key allocation failed");
     }
     hipStream_t s = c->stream;
-    hipMemcpyAsync(k->rgsw, h_rgsw, 8 * rgsw_words, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(k->auto_k, h_auto, 8 * auto_words, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(tmp, h_ksk, 8 * ksk_words, hipMemcpyHostToDevice, s);
-    launch_shoup(k->rgsw, k->rgsw_s, rgsw_words, p.rlwe_q, s);
-    launch_shoup(k->auto_k, k->auto_s, auto_words, p.rlwe_q, s);
-    launch_pack_ksk(tmp, k->ksk, (size_t)N * p.lwe_d, n1, ldk, kb, s);
+    launch_shoup(rgsw, k->rgsw_s, rgsw_words, p.rlwe_q, s);
+    launch_shoup(auto_k, k->auto_s, auto_words, p.rlwe_q, s);
+    launch_pack_ksk(ksk_rows, k->ksk, (size_t)N * p.lwe_d, n1, ldk, kb, s);
     hipError_t e = hipStreamSynchronize(s);
-    hipFree(tmp);
     if (e != hipSuccess) {
         pz_key_free(k);
         return fail(PZ_ERR_HIP, hipGetErrorString(e));
     }
+    k->rgsw = rgsw;
+    k->auto_k = auto_k;
     k->dk.rgsw = k->rgsw;
     k->dk.rgsw_s = k->rgsw_s;
     k->dk.auto_k = k->auto_k;
@@ -414,6 +418,37 @@ extern "C" int pz_key_upload(pz_ctx *c, const uint64_t *h_rgsw, const uint64_t *
     k->bytes = 16 * rgsw_words + 16 * auto_words + kb * ksk_dev;
     *out = k;
     return PZ_OK;
+}
+
+extern "C" int pz_key_upload(pz_ctx *c, const uint64_t *h_rgsw, const uint64_t *h_auto,
+                             const uint64_t *h_ksk, pz_key **out)
+{
+    if (!c || !h_rgsw || !h_auto || !h_ksk || !out) return fail(PZ_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const pz_params &p = c->p;
+    uint32_t N = p.rlwe_n;
+    size_t rgsw_words = (size_t)p.lwe_n * (2 * (size_t)p.rlrg_da + 2 * (size_t)p.rlrg_db) * N;
+    size_t auto_words = (size_t)(p.w + 1) * 2 * (size_t)p.auto_d * N;
+    size_t ksk_words = (size_t)N * p.lwe_d * (p.lwe_n + 1);
+    u64 *rgsw = nullptr, *auto_k = nullptr, *tmp = nullptr;
+    bool ok = hipMalloc(&rgsw, 8 * rgsw_words) == hipSuccess &&
+              hipMalloc(&auto_k, 8 * auto_words) == hipSuccess &&
+              hipMalloc(&tmp, 8 * ksk_words) == hipSuccess;
+    if (ok) {
+        hipStream_t s = c->stream;
+        ok = hipMemcpyAsync(rgsw, h_rgsw, 8 * rgsw_words, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(auto_k, h_auto, 8 * auto_words, hipMemcpyHostToDevice, s) ==
+                 hipSuccess &&
+             hipMemcpyAsync(tmp, h_ksk, 8 * ksk_words, hipMemcpyHostToDevice, s) == hipSuccess;
+    }
+    int rc = ok ? pz_internal_key_adopt(c, rgsw, auto_k, tmp, out)
+                : fail(PZ_ERR_HIP, "key upload failed");
+    hipFree(tmp);
+    if (rc != PZ_OK) {
+        hipFree(rgsw);
+        hipFree(auto_k);
+    }
+    return rc;
 }
 
 extern "C" int pz_key_free(pz_key *k)
@@ -432,6 +467,33 @@ extern "C" int pz_key_free(pz_key *k)
     hipFree(k->auto_s);
     hipFree(k->ksk);
     delete k;
+    return PZ_OK;
+}
+
+extern "C" int pz_key_download(pz_key *k, uint64_t *h_rgsw, uint64_t *h_auto, uint64_t *h_ksk)
+{
+    if (!k || !k->owner) return fail(PZ_ERR_ARG, "null key");
+    pz_ctx *c = k->owner;
+    const pz_params &p = c->p;
+    uint32_t N = p.rlwe_n;
+    size_t rgsw_words = (size_t)p.lwe_n * k->dk.rgsw_rows * N;
+    size_t auto_words = (size_t)(p.w + 1) * k->dk.auto_rows * N;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h_rgsw) HIPCHK(hipMemcpy(h_rgsw, k->rgsw, 8 * rgsw_words, hipMemcpyDeviceToHost));
+    if (h_auto) HIPCHK(hipMemcpy(h_auto, k->auto_k, 8 * auto_words, hipMemcpyDeviceToHost));
+    if (h_ksk) {
+        size_t rows = (size_t)N * p.lwe_d, n1 = p.lwe_n + 1, ld = k->dk.ldk;
+        std::vector<uint8_t> packed(rows * ld * k->dk.ksk_bytes);
+        HIPCHK(hipMemcpy(packed.data(), k->ksk, packed.size(), hipMemcpyDeviceToHost));
+        for (size_t r = 0; r < rows; r++)
+            for (size_t col = 0; col < n1; col++) {
+                size_t i = r * ld + col;
+                h_ksk[r * n1 + col] = k->dk.ksk_bytes == 2
+                                          ? reinterpret_cast<const uint16_t *>(packed.data())[i]
+                                          : reinterpret_cast<const uint32_t *>(packed.data())[i];
+            }
+    }
     return PZ_OK;
 }
 

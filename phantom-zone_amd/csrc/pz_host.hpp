@@ -245,3 +245,117 @@ inline std::vector<uint32_t> dlog_map(const pz_params &p)
 
 }  // namespace host
 }  // namespace pz
+
+namespace pz {
+namespace host {
+
+// Noise tables: T[k] = floor(2^64 * P(|X| <= k)), X ~ N(0, 3.19^2), |X|
+// rounded half away from zero (ROUND, the error distribution of
+// src/random.rs:112-128 after map_element_from_f64) or truncated toward zero
+// (TRUNC, the Gaussian LWE secret of src/random.rs:145-160).
+static const uint64_t PZ_CDT_ROUND[42] = {
+    0x1fe283c57e62b000ULL, 0x5c9f0a1d68211000ULL, 0x9118b59b30188000ULL,
+    0xba394ad0e9bb6000ULL, 0xd776ad2ff2044000ULL, 0xea5237ec9550e000ULL,
+    0xf55a79de6976e000ULL, 0xfb3549a53711a000ULL, 0xfe06d0641332f000ULL,
+    0xff41e43843be2000ULL, 0xffbeb2df59999000ULL, 0xffeb8b424b33c000ULL,
+    0xfffa291785952000ULL, 0xfffe7b5bd6539000ULL, 0xffffa4085a2f0000ULL,
+    0xffffec331b5fb000ULL, 0xfffffc1f7ad22000ULL, 0xffffff4f4fc0a000ULL,
+    0xffffffe369496000ULL, 0xfffffffbcbf48000ULL, 0xffffffff7045c000ULL,
+    0xffffffffee900000ULL, 0xfffffffffe142000ULL, 0xffffffffffcec000ULL,
+    0xfffffffffffb8000ULL, 0xffffffffffffa000ULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+};
+static const uint64_t PZ_CDT_TRUNC[42] = {
+    0x3eff57680c66a400ULL, 0x7824f735e7db0000ULL, 0xa72b8006276f2000ULL,
+    0xca45da4d592f4000ULL, 0xe20ade4707b23000ULL, 0xf0a4971d1c5a8000ULL,
+    0xf8c73f4b85aae000ULL, 0xfce3eaa49015f000ULL, 0xfec68fa732ff1000ULL,
+    0xff8f4ba19fa56000ULL, 0xffdb06ac14e44000ULL, 0xfff4f143d774e000ULL,
+    0xfffcfcc974786000ULL, 0xffff40b3b3cca000ULL, 0xffffd4d199668000ULL,
+    0xfffff72233732000ULL, 0xfffffe5835846000ULL, 0xffffffb8129ae000ULL,
+    0xfffffff4e7dfa000ULL, 0xfffffffe71fbb000ULL, 0xffffffffcd54e000ULL,
+    0xfffffffffa245000ULL, 0xffffffffff628000ULL, 0xfffffffffff10000ULL,
+    0xfffffffffffeb000ULL, 0xffffffffffffe000ULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+    0xffffffffffffffffULL, 0xffffffffffffffffULL, 0xffffffffffffffffULL,
+};
+
+// One noise draw: u64 against the table, then a sign bit for k > 0.
+inline int64_t cdt_draw(ChaCha8 &r, const uint64_t *t)
+{
+    uint64_t u = r.next_u64();
+    int64_t k = 41;
+    for (int i = 0; i < 42; i++) {
+        if (u < t[i]) {
+            k = i;
+            break;
+        }
+    }
+    if (k > 0 && (r.next_u32() & 1u)) k = -k;
+    return k;
+}
+inline int64_t gauss_round(ChaCha8 &r) { return cdt_draw(r, PZ_CDT_ROUND); }
+inline int64_t gauss_trunc(ChaCha8 &r) { return cdt_draw(r, PZ_CDT_TRUNC); }
+
+// fill_random_ternary_secret_with_hamming_weight (src/utils.rs:75-110); the
+// sign bytes are Uniform<u8> draws (one word each).
+inline void ternary_hw(ChaCha8 &r, int32_t *out, uint32_t n, uint32_t hw)
+{
+    std::vector<uint8_t> bytes((hw + 7) / 8 + 1);
+    for (uint32_t i = 0; i < (hw + 7) / 8; i++) bytes[i] = (uint8_t)r.next_u32();
+    std::vector<uint32_t> idx(n);
+    for (uint32_t i = 0; i < n; i++) {
+        idx[i] = i;
+        out[i] = 0;
+    }
+    uint32_t len = n, bit = 0, byte = 0;
+    for (uint32_t i = 0; i < hw; i++) {
+        uint32_t s = (uint32_t)r.uniform(len);
+        out[idx[s]] = ((bytes[byte] >> bit) & 1u) ? 1 : -1;
+        idx[s] = idx[len - 1];
+        len--;
+        if (bit == 7) {
+            bit = 0;
+            byte++;
+        } else {
+            bit++;
+        }
+    }
+}
+
+inline uint64_t map_i64(uint64_t q, int64_t v) { return v < 0 ? q - (uint64_t)(-v) : (uint64_t)v; }
+
+// Secrets of `parties` parties summed into the ideal secret (ClientKey::
+// sk_rlwe / sk_lwe, src/bool/keys.rs:75-154; one seed per party drawn from
+// the master stream).
+inline void keygen_secrets(const pz_params &p, const uint8_t seed[32], uint32_t parties,
+                           int32_t *s_out, int32_t *z_out)
+{
+    uint32_t N = p.rlwe_n, n = p.lwe_n;
+    ChaCha8 master(seed);
+    std::vector<int32_t> sp(N), zp(n);
+    for (uint32_t i = 0; i < N; i++) s_out[i] = 0;
+    for (uint32_t i = 0; i < n; i++) z_out[i] = 0;
+    for (uint32_t q = 0; q < parties; q++) {
+        uint8_t ps[32];
+        master.fill_bytes(ps, 32);
+        ChaCha8 r(ps);
+        ternary_hw(r, sp.data(), N, N >> 1);
+        if (p.lwe_secret_gaussian) {
+            for (uint32_t i = 0; i < n; i++) zp[i] = (int32_t)gauss_trunc(r);
+        } else {
+            ternary_hw(r, zp.data(), n, n >> 1);
+        }
+        for (uint32_t i = 0; i < N; i++) s_out[i] += sp[i];
+        for (uint32_t i = 0; i < n; i++) z_out[i] += zp[i];
+    }
+}
+
+}  // namespace host
+}  // namespace pz

@@ -1,0 +1,358 @@
+// pz_keygen.hip -- seeded client-side key generation and LWE encryption.
+//
+// Secrets and all random polynomials are drawn on the host from ChaCha8
+// streams (the stream order is this build's own, fixed below); the products
+// a*s, the message terms and the transforms to the evaluation domain run on
+// the device with the same exact arithmetic as the reference's keygen:
+//   RGSW(X^{emb*z_i})   secret_key_encrypt_rgsw (src/rgsw/keygen.rs:68-160),
+//                       both RLWE'(m) halves explicit, then forward NTT of
+//                       every row (src/bool/keys.rs:881-885);
+//   auto keys           seeded_auto_key_gen / seeded_rlwe_ksk_gen
+//                       (src/rgsw/keygen.rs:306-430) for [-g, g, .., g^w];
+//   LWE KSK             seeded_lwe_ksk_keygen (src/lwe.rs:46-101).
+// Stream order (seed_key): for each RGSW i, for each of its d_a + d_b row
+// pairs: N uniform a, N noise e; then the auto keys likewise; then per KSK
+// row: n uniform a (mod q_ks), one noise e.
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/pz.h"
+#include "pz_host.hpp"
+#include "pz_internal.hpp"
+
+namespace pz {
+
+// One RLWE row pair of key material.
+struct RowPair {
+    const u64 *a;    // uniform a (coefficient domain)
+    const u64 *e;    // noise, already mod q
+    const u64 *as;   // a * s
+    const u64 *msg;  // full message polynomial (auto keys) or null
+    u64 *dst_a, *dst_b;
+    u64 beta;
+    u32 mono_pos;    // monomial message position (RGSW rows)
+    u64 mono_val;    // 1 or q - 1; 0 = no monomial
+    u32 msg_in_a;    // 1: message added to the a row (RLWE'(-sm)), 0: to b
+};
+
+__global__ void k_barrett_rows(u64 *rows, const u64 *s_eval, size_t nrows, u32 N, Barrett br)
+{
+    size_t total = nrows * N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x)
+        rows[i] = barrett_mul(br, rows[i], s_eval[i % N]);
+}
+
+// b = (e + a*s) + beta*msg_b ; a' = a + beta*msg_a  (add_mod_fast /
+// mul_mod_fast, src/backend/modulus_u64.rs:46-55, :106-130)
+__global__ void k_kg_combine(const RowPair *pairs, size_t npairs, u32 N, Barrett br)
+{
+    const u64 q = br.q;
+    size_t total = npairs * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const RowPair &p = pairs[t / N];
+        u32 i = (u32)(t % N);
+        u64 m = 0;
+        if (p.msg) m = p.msg[i];
+        else if (p.mono_val && i == p.mono_pos) m = p.mono_val;
+        u64 bm = barrett_mul(br, m, p.beta);
+        u64 a = p.a[i];
+        u64 b = add_mod(p.e[i], p.as[i], q);
+        if (p.msg_in_a) a = add_mod(a, bm, q);
+        else b = add_mod(b, bm, q);
+        p.dst_a[i] = a;
+        p.dst_b[i] = b;
+    }
+}
+
+// KSK row r = i*d + j: b = <a, z> + beta_j * (-s_i) + e (mod q_ks)
+__global__ void k_ksk_rows(u64 *ksk, const int32_t *s, const int32_t *z, const int64_t *err,
+                           u32 N, u32 d, u32 n, u32 logb, u32 ib, u64 mask)
+{
+    size_t rows = (size_t)N * d;
+    size_t n1 = (size_t)n + 1;
+    for (size_t r = blockIdx.x * (size_t)blockDim.x + threadIdx.x; r < rows;
+         r += (size_t)gridDim.x * blockDim.x) {
+        u64 *row = ksk + r * n1;
+        u64 az = 0;
+        for (u32 k = 0; k < n; k++) az += row[1 + k] * (u64)(int64_t)z[k];
+        u32 i = (u32)(r / d), j = (u32)(r % d);
+        u64 beta = 1ull << (logb * j + ib);
+        u64 neg_si = 0 - (u64)(int64_t)s[i];
+        u64 b = (az + beta * neg_si) & mask;
+        b = (b + (u64)err[r]) & mask;
+        row[0] = b;
+    }
+}
+
+static int grid_of(size_t n) { return (int)((n + 255) / 256 < 65535 * 4 ? (n + 255) / 256 : 65535 * 4); }
+
+}  // namespace pz
+
+using namespace pz;
+
+// defined in pz_capi.hip
+extern int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStream_t *s);
+extern int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows,
+                                 pz_key **out);
+
+static thread_local std::string kg_err;
+
+extern "C" int pz_keygen_secrets(pz_ctx *c, const uint8_t *seed, uint32_t parties,
+                                 int32_t *h_s, int32_t *h_z)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!seed || !h_s || !h_z || parties == 0) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    host::keygen_secrets(p, seed, parties, h_s, h_z);
+    return PZ_OK;
+}
+
+#define KG_CHK(x)                                 \
+    do {                                          \
+        if ((x) != hipSuccess) {                  \
+            ok = false;                           \
+            goto done;                            \
+        }                                         \
+    } while (0)
+
+extern "C" int pz_keygen_server(pz_ctx *c, const uint8_t *seed, const int32_t *h_s,
+                                const int32_t *h_z, pz_key **out)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!seed || !h_s || !h_z || !out) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    const u64 q = p.rlwe_q;
+    const u32 N = p.rlwe_n, n = p.lwe_n, da = p.rlrg_da, db = p.rlrg_db, dau = p.auto_d;
+    const u32 emb = (2 * N) / p.br_q;
+    const size_t rgsw_rows = 2 * (size_t)da + 2 * db, auto_rows = 2 * (size_t)dau;
+    const size_t n_rg_pairs = (size_t)n * (da + db), n_au_pairs = (size_t)(p.w + 1) * dau;
+    const size_t npairs = n_rg_pairs + n_au_pairs;
+    const size_t ksk_rows = (size_t)N * p.lwe_d, n1 = (size_t)n + 1;
+    host::ChaCha8 r(seed);
+
+    // ---- host sampling, in stream order ----
+    std::vector<u64> ha(npairs * N), he(npairs * N);
+    for (size_t pr = 0; pr < npairs; pr++) {
+        for (u32 i = 0; i < N; i++) ha[pr * N + i] = r.uniform(q);
+        for (u32 i = 0; i < N; i++) he[pr * N + i] = host::map_i64(q, host::gauss_round(r));
+    }
+    std::vector<u64> hksk(ksk_rows * n1);
+    std::vector<int64_t> herr(ksk_rows);
+    for (size_t row = 0; row < ksk_rows; row++) {
+        for (u32 k = 0; k < n; k++) hksk[row * n1 + 1 + k] = r.uniform(p.lwe_q);
+        herr[row] = (int64_t)host::map_i64(p.lwe_q, host::gauss_round(r));
+    }
+    // auto-key messages -s(X^k) (seeded_auto_key_gen, src/rgsw/keygen.rs:403-415)
+    std::vector<u64> hmsg((size_t)(p.w + 1) * N);
+    for (u32 k = 0; k <= p.w; k++) {
+        uint64_t kk = dc.auto_el[k];
+        for (u32 i = 0; i < N; i++) {
+            uint64_t to = ((uint64_t)i * kk) % (2ull * N);
+            bool pos = to < N;
+            if (!pos) to -= N;
+            u64 el = host::map_i64(q, h_s[i]);
+            hmsg[(size_t)k * N + to] = pos ? q - el : el;
+        }
+    }
+    std::vector<u64> hs_mod(N);
+    for (u32 i = 0; i < N; i++) hs_mod[i] = host::map_i64(q, h_s[i]);
+
+    bool ok = true;
+    u64 *d_a = nullptr, *d_e = nullptr, *d_as = nullptr, *d_s = nullptr, *d_msg = nullptr;
+    u64 *d_rgsw = nullptr, *d_auto = nullptr, *d_ksk = nullptr;
+    int32_t *d_si = nullptr, *d_zi = nullptr;
+    int64_t *d_err = nullptr;
+    RowPair *d_pairs = nullptr;
+    std::vector<RowPair> pairs(npairs);
+    const u32 lq_log = host::ref_log2(p.lwe_q);
+    const u32 ib_lwe = lq_log - p.lwe_logb * p.lwe_d;
+    KG_CHK(hipMalloc(&d_a, 8 * npairs * N));
+    KG_CHK(hipMalloc(&d_e, 8 * npairs * N));
+    KG_CHK(hipMalloc(&d_as, 8 * npairs * N));
+    KG_CHK(hipMalloc(&d_s, 8 * (size_t)N));
+    KG_CHK(hipMalloc(&d_msg, 8 * hmsg.size()));
+    KG_CHK(hipMalloc(&d_rgsw, 8 * (size_t)n * rgsw_rows * N));
+    KG_CHK(hipMalloc(&d_auto, 8 * (size_t)(p.w + 1) * auto_rows * N));
+    KG_CHK(hipMalloc(&d_ksk, 8 * ksk_rows * n1));
+    KG_CHK(hipMalloc(&d_si, 4 * (size_t)N));
+    KG_CHK(hipMalloc(&d_zi, 4 * (size_t)n));
+    KG_CHK(hipMalloc(&d_err, 8 * ksk_rows));
+    KG_CHK(hipMalloc(&d_pairs, sizeof(RowPair) * npairs));
+    KG_CHK(hipMemcpyAsync(d_a, ha.data(), 8 * ha.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_e, he.data(), 8 * he.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_as, ha.data(), 8 * ha.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_s, hs_mod.data(), 8 * (size_t)N, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_msg, hmsg.data(), 8 * hmsg.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_ksk, hksk.data(), 8 * hksk.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_si, h_s, 4 * (size_t)N, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_zi, h_z, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_err, herr.data(), 8 * ksk_rows, hipMemcpyHostToDevice, st));
+
+    // s_eval, then a*s = backward(forward(a) . s_eval) for every row pair
+    launch_ntt(logn, 1, d_s, 1, dc.tb, st);
+    launch_ntt(logn, 1, d_as, npairs, dc.tb, st);
+    hipLaunchKernelGGL(k_barrett_rows, grid_of(npairs * N), 256, 0, st, d_as, d_s, npairs, N,
+                       dc.br);
+    launch_ntt(logn, 3, d_as, npairs, dc.tb, st);
+
+    // row-pair descriptors
+    for (u32 i = 0; i < n; i++) {
+        int64_t si = (int64_t)emb * h_z[i];
+        u32 pos = si < 0 ? (u32)(N - (u32)(-si)) : (u32)si;
+        u64 val = si < 0 ? q - 1 : 1;
+        u64 *ct = d_rgsw + (size_t)i * rgsw_rows * N;
+        for (u32 j = 0; j < da + db; j++) {
+            size_t pr = (size_t)i * (da + db) + j;
+            RowPair &rp = pairs[pr];
+            rp.a = d_a + pr * N;
+            rp.e = d_e + pr * N;
+            rp.as = d_as + pr * N;
+            rp.msg = nullptr;
+            rp.mono_pos = pos;
+            rp.mono_val = val;
+            if (j < da) {
+                rp.beta = 1ull << (p.rlrg_logb * j + dc.dec_a.ignore_bits);
+                rp.dst_a = ct + (size_t)j * N;
+                rp.dst_b = ct + (size_t)(da + j) * N;
+                rp.msg_in_a = 1;
+            } else {
+                u32 jj = j - da;
+                rp.beta = 1ull << (p.rlrg_logb * jj + dc.dec_b.ignore_bits);
+                rp.dst_a = ct + (size_t)(2 * da + jj) * N;
+                rp.dst_b = ct + (size_t)(2 * da + db + jj) * N;
+                rp.msg_in_a = 0;
+            }
+        }
+    }
+    for (u32 k = 0; k <= p.w; k++) {
+        u64 *ct = d_auto + (size_t)k * auto_rows * N;
+        for (u32 j = 0; j < dau; j++) {
+            size_t pr = n_rg_pairs + (size_t)k * dau + j;
+            RowPair &rp = pairs[pr];
+            rp.a = d_a + pr * N;
+            rp.e = d_e + pr * N;
+            rp.as = d_as + pr * N;
+            rp.msg = d_msg + (size_t)k * N;
+            rp.mono_pos = 0;
+            rp.mono_val = 0;
+            rp.beta = 1ull << (p.auto_logb * j + dc.dec_auto.ignore_bits);
+            rp.dst_a = ct + (size_t)j * N;
+            rp.dst_b = ct + (size_t)(dau + j) * N;
+            rp.msg_in_a = 0;
+        }
+    }
+    KG_CHK(hipMemcpyAsync(d_pairs, pairs.data(), sizeof(RowPair) * npairs, hipMemcpyHostToDevice,
+                          st));
+    hipLaunchKernelGGL(k_kg_combine, grid_of(npairs * N), 256, 0, st, d_pairs, npairs, N, dc.br);
+    // evaluation domain (Ntt::forward of every row)
+    launch_ntt(logn, 1, d_rgsw, (size_t)n * rgsw_rows, dc.tb, st);
+    launch_ntt(logn, 1, d_auto, (size_t)(p.w + 1) * auto_rows, dc.tb, st);
+    hipLaunchKernelGGL(k_ksk_rows, grid_of(ksk_rows), 256, 0, st, d_ksk, d_si, d_zi, d_err, N,
+                       p.lwe_d, n, p.lwe_logb, ib_lwe, (u64)(p.lwe_q - 1));
+    KG_CHK(hipGetLastError());
+    KG_CHK(hipStreamSynchronize(st));
+    rc = pz_internal_key_adopt(c, d_rgsw, d_auto, d_ksk, out);
+    if (rc == PZ_OK) {
+        d_rgsw = nullptr;  // owned by the key now
+        d_auto = nullptr;
+    }
+done:
+    hipFree(d_a);
+    hipFree(d_e);
+    hipFree(d_as);
+    hipFree(d_s);
+    hipFree(d_msg);
+    hipFree(d_rgsw);
+    hipFree(d_auto);
+    hipFree(d_ksk);
+    hipFree(d_si);
+    hipFree(d_zi);
+    hipFree(d_err);
+    hipFree(d_pairs);
+    if (!ok) return PZ_ERR_HIP;
+    return rc;
+}
+
+// ---- client-side LWE encryption / decryption (encrypt_lwe / decrypt_lwe,
+// src/lwe.rs:104-162; decode, src/bool/evaluator.rs:245-258).  Ciphertext i
+// of a batch uses the seed (seed + i) read as a 256-bit little-endian
+// integer: a = dim uniform draws, then one noise draw.
+static void seed_plus(const uint8_t *seed, uint64_t add, uint8_t out[32])
+{
+    unsigned carry = 0;
+    for (int b = 0; b < 32; b++) {
+        unsigned v = seed[b] + (unsigned)((b < 8 ? (add >> (8 * b)) : 0) & 0xff) + carry;
+        out[b] = (uint8_t)v;
+        carry = v >> 8;
+    }
+}
+
+extern "C" int pz_encrypt_bools(pz_ctx *c, const int32_t *h_s, uint32_t dim, const uint8_t *seed,
+                                const uint8_t *h_bits, size_t count, uint64_t *h_out)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!h_s || !seed || !h_bits || !h_out) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    const u64 q = p.rlwe_q, tru = q >> 3, fal = (q - 1) - tru + 1;
+    for (size_t t = 0; t < count; t++) {
+        uint8_t sd[32];
+        seed_plus(seed, t, sd);
+        host::ChaCha8 r(sd);
+        uint64_t *o = h_out + t * (dim + 1);
+        host::u128 sa = 0;
+        for (uint32_t i = 0; i < dim; i++) o[1 + i] = r.uniform(q);
+        uint64_t acc = 0;
+        for (uint32_t i = 0; i < dim; i++) {
+            acc = (uint64_t)(((host::u128)o[1 + i] * host::map_i64(q, h_s[i]) + acc) % q);
+        }
+        (void)sa;
+        uint64_t e = host::map_i64(q, host::gauss_round(r));
+        uint64_t m = h_bits[t] ? tru : fal;
+        uint64_t b = acc + e;
+        b = b >= q ? b - q : b;
+        b += m;
+        o[0] = b >= q ? b - q : b;
+    }
+    return PZ_OK;
+}
+
+extern "C" int pz_decrypt_bools(pz_ctx *c, const int32_t *h_s, uint32_t dim, const uint64_t *h_cts,
+                                size_t count, uint8_t *h_bits)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!h_s || !h_cts || !h_bits) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    const u64 q = p.rlwe_q, qby8 = q >> 3;
+    int bad = 0;
+    for (size_t t = 0; t < count; t++) {
+        const uint64_t *ct = h_cts + t * (dim + 1);
+        uint64_t acc = 0;
+        for (uint32_t i = 0; i < dim; i++)
+            acc = (uint64_t)(((host::u128)ct[1 + i] * host::map_i64(q, h_s[i]) + acc) % q);
+        uint64_t m = ct[0] >= acc ? ct[0] - acc : q + ct[0] - acc;
+        double x = ((double)(m + qby8) * 4.0) / (double)q;
+        uint64_t v = ((uint64_t)__builtin_round(x)) % 4;
+        if (v > 1) bad = 1;
+        h_bits[t] = (uint8_t)(v == 1);
+    }
+    return bad ? PZ_ERR_STATE : PZ_OK;
+}

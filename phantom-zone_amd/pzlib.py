@@ -56,6 +56,11 @@ SIGNATURES = {
     "pz_key_upload": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "pz_key_free": (C.c_int, [_vp]),
     "pz_key_bytes": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
+    "pz_key_download": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "pz_keygen_secrets": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
+    "pz_keygen_server": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "pz_encrypt_bools": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp, C.c_size_t, _vp]),
+    "pz_decrypt_bools": (C.c_int, [_vp, _vp, C.c_uint32, _vp, C.c_size_t, _vp]),
     "pz_rlwe_by_rgsw": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_rlwe_auto": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_lwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
@@ -227,6 +232,53 @@ class Context:
         _check(self.L.pz_key_upload(self.h, rgsw.ctypes.data, auto.ctypes.data,
                                     ksk.ctypes.data, C.byref(h)))
         return Key(self, h.value)
+
+    # ---- seeded client-side keys and encryption ----
+    @staticmethod
+    def seed32(seed):
+        if isinstance(seed, (bytes, bytearray)):
+            return np.frombuffer(bytes(seed).ljust(32, b"\0")[:32], dtype=np.uint8).copy()
+        return np.frombuffer(int(seed).to_bytes(32, "little"), dtype=np.uint8).copy()
+
+    def keygen_secrets(self, seed, parties):
+        s = np.zeros(self.N, dtype=np.int32)
+        z = np.zeros(self.n, dtype=np.int32)
+        _check(self.L.pz_keygen_secrets(self.h, self.seed32(seed).ctypes.data, parties,
+                                        s.ctypes.data, z.ctypes.data))
+        return s, z
+
+    def keygen_server(self, seed, s, z):
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        z = np.ascontiguousarray(z, dtype=np.int32)
+        h = C.c_void_p()
+        _check(self.L.pz_keygen_server(self.h, self.seed32(seed).ctypes.data, s.ctypes.data,
+                                       z.ctypes.data, C.byref(h)))
+        return Key(self, h.value)
+
+    def key_download(self, key):
+        p = self.p
+        rgsw = np.zeros((p.lwe_n, 2 * p.rlrg_da + 2 * p.rlrg_db, p.rlwe_n), dtype=np.uint64)
+        auto = np.zeros((p.w + 1, 2 * p.auto_d, p.rlwe_n), dtype=np.uint64)
+        ksk = np.zeros((p.rlwe_n * p.lwe_d, p.lwe_n + 1), dtype=np.uint64)
+        _check(self.L.pz_key_download(key.h, rgsw.ctypes.data, auto.ctypes.data,
+                                      ksk.ctypes.data))
+        return rgsw, auto, ksk
+
+    def encrypt_bools(self, s, bits, seed):
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        b = np.ascontiguousarray(bits, dtype=np.uint8)
+        out = np.zeros((len(b), len(s) + 1), dtype=np.uint64)
+        _check(self.L.pz_encrypt_bools(self.h, s.ctypes.data, len(s), self.seed32(seed).ctypes.data,
+                                       b.ctypes.data, len(b), out.ctypes.data))
+        return out
+
+    def decrypt_bools(self, s, cts):
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        c = np.ascontiguousarray(cts, dtype=np.uint64)
+        out = np.zeros(c.shape[0], dtype=np.uint8)
+        _check(self.L.pz_decrypt_bools(self.h, s.ctypes.data, len(s), c.ctypes.data, c.shape[0],
+                                       out.ctypes.data))
+        return out
 
     def rlwe_by_rgsw(self, key, d_rlwe, d_idx, d_triv, B):
         _check(self.L.pz_rlwe_by_rgsw(self.h, key.h, d_rlwe.ptr, d_idx.ptr, d_triv.ptr, B, None))
