@@ -174,6 +174,41 @@ int pz_gates(pz_ctx *ctx, pz_key *key, const uint32_t *d_ops, const uint64_t *d_
 /* not_inplace (src/bool/evaluator.rs:2236-2239) on B ciphertexts. */
 int pz_not(pz_ctx *ctx, uint64_t *d_c, size_t B, void *stream);
 
+/* ---- FheUint8 circuits (src/shortint/ops.rs:5-356, operators of
+ * src/shortint/mod.rs:30-293) built as gate DAGs over the BooleanGates seam
+ * and evaluated level by level: all ready gates of a level, across every
+ * operation in the circuit, are bootstrapped in one batched launch.  Wires
+ * are ciphertext ids; bits are little endian (8 wires per FheUint8). */
+typedef struct pz_circuit pz_circuit;
+enum {
+    PZ_GATE_NOT = 6, /* pz_circuit_gate op for NOT (no bootstrap) */
+    PZ_U8_ADD = 0, PZ_U8_SUB, PZ_U8_MUL, PZ_U8_DIV, PZ_U8_REM, PZ_U8_DIV_REM,
+    PZ_U8_EQ, PZ_U8_NEQ, PZ_U8_LT, PZ_U8_GT, PZ_U8_LE, PZ_U8_GE, PZ_U8_MUX, PZ_U8_MAX,
+    PZ_U8_MIN, PZ_U8_OVERFLOWING_ADD, PZ_U8_OVERFLOWING_SUB
+};
+/* ctx may be NULL for a build-only circuit (no pz_circuit_run). */
+int pz_circuit_create(pz_ctx *ctx, pz_circuit **out);
+int pz_circuit_destroy(pz_circuit *c);
+int pz_circuit_inputs(pz_circuit *c, uint32_t count, uint32_t *wires_out);
+/* one BooleanGates gate (op PZ_NAND..PZ_XNOR, or PZ_GATE_NOT with b unused) */
+int pz_circuit_gate(pz_circuit *c, uint32_t op, uint32_t a, uint32_t b, uint32_t *out);
+/* one FheUint8 operation on a8/b8 (and the selector wire for PZ_U8_MUX);
+ * writes *nout wires to out: 8 for arithmetic and mux/max/min, 16 for
+ * DIV_REM (quotient then remainder), 9 for OVERFLOWING_* (value, flag), 1 for
+ * comparisons.  DIV and DIV_REM also update the circuit's div-by-zero flag
+ * (src/shortint/mod.rs:42-56). */
+int pz_circuit_u8(pz_circuit *c, uint32_t op, const uint32_t *a8, const uint32_t *b8,
+                  const uint32_t *sel, uint32_t *out, uint32_t *nout);
+int pz_circuit_div_zero_flag(pz_circuit *c, uint32_t *wire);
+int pz_circuit_reset_flags(pz_circuit *c);
+int pz_circuit_stats(pz_circuit *c, uint32_t *n_wires, uint32_t *n_pbs, uint32_t *depth);
+int pz_circuit_level_sizes(pz_circuit *c, uint32_t *h_sizes, uint32_t cap);
+int pz_circuit_nodes(pz_circuit *c, uint32_t *h_out, size_t cap);
+/* d_inputs[i] is the ciphertext of the i-th input wire ([N+1] u64 each) */
+int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inputs, void *stream);
+int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t *d_out,
+                    void *stream);
+
 /* Automorphism count of each bootstrap of the last pz_pbs / pz_gates batch
  * (the blind rotation always runs n RLWE x RGSW products; the number of
  * automorphisms depends on the input).  Used for the algorithmic work count. */

@@ -12,6 +12,7 @@
 #include "../../include/pz.h"
 #include "pz_host.hpp"
 #include "pz_internal.hpp"
+#include "pz_exec.hpp"
 
 using namespace pz;
 
@@ -549,6 +550,26 @@ static int ensure_scratch(pz_ctx *c, size_t B)
     return PZ_OK;
 }
 
+// Executor view: grows the pipeline scratch to B ciphertexts and exposes it.
+int pz_internal_exec_view(pz_ctx *c, size_t B, pz_exec_view *v, DevCtx *dc)
+{
+    if (!c) return fail(PZ_ERR_ARG, "null context");
+    int rc = ensure_scratch(c, B);
+    if (rc) return rc;
+    v->p = c->p;
+    v->logn = c->logn;
+    v->digits = c->d_digits;
+    v->ksb = c->d_ksb;
+    v->ks = c->d_ks;
+    v->stats = c->d_stats;
+    size_t resident = (size_t)c->num_cu * c->br_per_cu;
+    v->grid = (int)(resident < 1 ? 1 : resident);
+    *dc = c->dc;
+    return PZ_OK;
+}
+
+const DevKey *pz_internal_devkey(const pz_key *k) { return &k->dk; }
+
 extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint64_t *out,
                                  size_t B, void *s)
 {
@@ -559,10 +580,10 @@ extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint6
     hipStream_t st = S(c, s);
     const pz_params &p = c->p;
     u32 n1 = p.lwe_n + 1;
-    launch_prep(PREP_KSONLY, in, nullptr, nullptr, p.rlwe_n + 1, B, c->d_digits, c->d_ksb,
-                c->d_ks, c->dc, st);
+    launch_prep(PREP_KSONLY, in, nullptr, nullptr, nullptr, nullptr, p.rlwe_n + 1, B,
+                c->d_digits, c->d_ksb, c->d_ks, c->dc, st);
     launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
-              k->dk.ldk, B, c->d_ks, st);
+              k->dk.ldk, 0, B, c->d_ks, st);
     launch_widen(c->d_ks, out, B * n1, (u32)(p.lwe_q - 1), st);
     HIPCHK(hipGetLastError());
     return PZ_OK;
@@ -575,15 +596,17 @@ static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, con
     const pz_params &p = c->p;
     u32 n1 = p.lwe_n + 1;
     if (c->timing) hipEventRecord(c->ev[0], st);
-    launch_prep(mode, c0, c1, ops, p.rlwe_n + 1, B, c->d_digits, c->d_ksb, c->d_ks, c->dc, st);
+    launch_prep(mode, c0, c1, nullptr, nullptr, ops, p.rlwe_n + 1, B, c->d_digits, c->d_ksb,
+                c->d_ks, c->dc, st);
     if (c->timing) hipEventRecord(c->ev[1], st);
     launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
-              k->dk.ldk, B, c->d_ks, st);
+              k->dk.ldk, 0, B, c->d_ks, st);
     if (c->timing) hipEventRecord(c->ev[2], st);
     size_t resident = (size_t)c->num_cu * c->br_per_cu;
     if (resident < 1) resident = 1;
     int grid = (int)(B < resident ? B : resident);
-    launch_blind_rotate(c->logn, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk, st);
+    launch_blind_rotate(c->logn, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk, nullptr,
+                        nullptr, nullptr, st);
     if (c->timing) hipEventRecord(c->ev[3], st);
 }
 

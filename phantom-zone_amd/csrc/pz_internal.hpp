@@ -25,14 +25,16 @@ void launch_decompose(const u64 *in, u64 *digits, size_t count, const Decomp &dc
                       hipStream_t s);
 void launch_ext(int logn, bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv, size_t B,
                 const DevCtx &c, const DevKey &k, hipStream_t s);
-void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *ops, u32 N1, size_t B,
-                 int8_t *digits, u32 *ks_b, u32 *ks_out, const DevCtx &c, hipStream_t s);
+void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const u32 *idx1,
+                 const u32 *ops, u32 N1, size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out,
+                 const DevCtx &c, hipStream_t s);
 void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_bytes, u32 rows,
-               u32 n1, u32 ldk, size_t B, u32 *out, hipStream_t s);
+               u32 n1, u32 ldk, size_t seg_begin, size_t seg_end, u32 *out, hipStream_t s);
 size_t br_lds_bytes(int logn, u32 br_q, u32 n_lwe);
 const void *blind_rotate_kernel(int logn);
 void launch_blind_rotate(int logn, int grid, const u32 *ks, const u32 *gate, size_t B, u64 *out,
-                         u32 *stats, const DevCtx &c, const DevKey &k, hipStream_t s);
+                         u32 *stats, const DevCtx &c, const DevKey &k, const DevKey *keys,
+                         const u32 *key_idx, const u32 *out_idx, hipStream_t s);
 void launch_shoup(const u64 *in, u64 *out, size_t n, u64 q, hipStream_t s);
 void launch_not(u64 *c0, size_t n, u64 q, hipStream_t s);
 void launch_widen(const u32 *in, u64 *out, size_t n, u32 mask, hipStream_t s);

@@ -76,17 +76,20 @@ __global__ void k_decompose(const u64 *in, u64 *digits, size_t count, Decomp dc)
 // the LWE-KS decomposition (src/lwe.rs:31-35) into signed int8 digits.
 // mode: PREP_GATE (c0, c1, ops), PREP_PBS (in), PREP_KSONLY (in is mod q_ks).
 // ---------------------------------------------------------------------------
-__global__ void k_prep(int mode, const u64 *c0, const u64 *c1, const u32 *ops, u32 N1,
-                       size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out, DevCtx c)
+__global__ void k_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const u32 *idx1,
+                       const u32 *ops, u32 N1, size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out,
+                       DevCtx c)
 {
     const size_t bb = blockIdx.x;
     const u32 col = blockIdx.y * blockDim.x + threadIdx.x;
     if (bb >= B || col >= N1) return;
     if (col < c.n_lwe + 1) ks_out[bb * (c.n_lwe + 1) + col] = 0;  // k_ks accumulates
     const u64 q = c.tb.q;
-    u64 v = c0[bb * N1 + col];
+    // inputs are rows bb of c0/c1, or rows idx0[bb]/idx1[bb] of a wire pool
+    const size_t r0 = idx0 ? idx0[bb] : bb, r1 = idx1 ? idx1[bb] : bb;
+    u64 v = c0[r0 * N1 + col];
     if (mode == PREP_GATE) {
-        u64 v1 = c1[bb * N1 + col];
+        u64 v1 = c1[r1 * N1 + col];
         u32 op = ops[bb];
         if (op <= 3) {
             v = add_mod(v, v1, q);
@@ -136,13 +139,16 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 template <bool PACK16>
 __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks_b,
                                              const void *ksk, u32 rows, u32 n1, u32 ldk,
-                                             size_t B, u32 *out)
+                                             size_t seg_begin, size_t seg_end, u32 *out)
 {
     __shared__ __attribute__((aligned(16))) u32 dig[KS_ROWS][KS_CT];
     const u32 unit = threadIdx.x;
     const u32 units = PACK16 ? (n1 + 1) / 2 : n1;
-    const size_t b0 = (size_t)blockIdx.x * KS_CT;
-    const int8_t *tile = digits + (size_t)blockIdx.x * rows * KS_CT;
+    // this group's tile of KS_CT ciphertexts; only those in [seg_begin, seg_end)
+    // (the ciphertexts of this key) are accumulated
+    const size_t tile_id = seg_begin / KS_CT + blockIdx.x;
+    const size_t b0 = tile_id * KS_CT;
+    const int8_t *tile = digits + tile_id * rows * KS_CT;
     const u32 per = (rows + gridDim.y - 1) / gridDim.y;
     const u32 rbeg = blockIdx.y * per;
     const u32 rend = rbeg + per < rows ? rbeg + per : rows;
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks
 #pragma unroll
     for (int t = 0; t < KS_CT; t++) {
         size_t bb = b0 + t;
-        if (bb >= B) continue;
+        if (bb < seg_begin || bb >= seg_end) continue;
         u32 *o = out + bb * n1;
         if (PACK16) {
             u32 c0 = 2 * unit;
@@ -325,27 +331,31 @@ void launch_decompose(const u64 *in, u64 *digits, size_t count, const Decomp &dc
     hipLaunchKernelGGL(k_decompose, grid_for(count, 256), 256, 0, s, in, digits, count, dc);
 }
 
-void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *ops, u32 N1, size_t B,
-                 int8_t *digits, u32 *ks_b, u32 *ks_out, const DevCtx &c, hipStream_t s)
+void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const u32 *idx1,
+                 const u32 *ops, u32 N1, size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out,
+                 const DevCtx &c, hipStream_t s)
 {
     if (B == 0) return;
     dim3 grid((unsigned)B, (N1 + 255) / 256);
-    hipLaunchKernelGGL(k_prep, grid, 256, 0, s, mode, c0, c1, ops, N1, B, digits, ks_b, ks_out,
-                       c);
+    hipLaunchKernelGGL(k_prep, grid, 256, 0, s, mode, c0, c1, idx0, idx1, ops, N1, B, digits,
+                       ks_b, ks_out, c);
 }
 
 void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_bytes, u32 rows,
-               u32 n1, u32 ldk, size_t B, u32 *out, hipStream_t s)
+               u32 n1, u32 ldk, size_t seg_begin, size_t seg_end, u32 *out, hipStream_t s)
 {
-    if (B == 0) return;
+    if (seg_end <= seg_begin) return;
     const bool pack = ksk_bytes == 2;
     u32 units = pack ? (n1 + 1) / 2 : n1;
     u32 tpb = (units + 63) / 64 * 64;
-    dim3 grid((unsigned)((B + KS_CT - 1) / KS_CT), KS_SPLIT);
+    size_t tiles = (seg_end - 1) / KS_CT - seg_begin / KS_CT + 1;
+    dim3 grid((unsigned)tiles, KS_SPLIT);
     if (pack)
-        hipLaunchKernelGGL(k_ks<true>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk, B, out);
+        hipLaunchKernelGGL(k_ks<true>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk,
+                           seg_begin, seg_end, out);
     else
-        hipLaunchKernelGGL(k_ks<false>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk, B, out);
+        hipLaunchKernelGGL(k_ks<false>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk,
+                           seg_begin, seg_end, out);
 }
 
 void launch_shoup(const u64 *in, u64 *out, size_t n, u64 q, hipStream_t s)

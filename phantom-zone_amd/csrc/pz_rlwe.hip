@@ -82,7 +82,10 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *
 template <int LOGN>
 __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(const u32 *ks, const u32 *gate,
                                                                   size_t B, u64 *out, u32 *stats,
-                                                                  DevCtx c, DevKey key)
+                                                                  DevCtx c, DevKey key0,
+                                                                  const DevKey *keys,
+                                                                  const u32 *key_idx,
+                                                                  const u32 *out_idx)
 {
     constexpr int N = 1 << LOGN;
     constexpr int TPB = N / 8;
@@ -99,6 +102,9 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
 
     for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
         const u32 *in = ks + bb * (n + 1);
+        // the bootstrap's server key: one key for the batch, or one per
+        // ciphertext (sessions batched together)
+        const DevKey key = keys ? keys[key_idx[bb]] : key0;
         // (3) odd moddown to br_q and dlog buckets (stable counting sort)
         for (u32 i = j; i <= nb; i += TPB) cnt[i] = 0;
         __syncthreads();
@@ -192,7 +198,7 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
 
         // (6) sample_extract at index 0 (src/pbs.rs:423-449)
         if (stats && j == 0) stats[bb] = n_auto;
-        u64 *o = out + bb * (size_t)(N + 1);
+        u64 *o = out + (out_idx ? (size_t)out_idx[bb] : bb) * (size_t)(N + 1);
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             u32 x = coef_x<LOGN>(j, r);
@@ -258,14 +264,15 @@ const void *blind_rotate_kernel(int logn)
 }
 
 void launch_blind_rotate(int logn, int grid, const u32 *ks, const u32 *gate, size_t B, u64 *out,
-                         u32 *stats, const DevCtx &c, const DevKey &k, hipStream_t s)
+                         u32 *stats, const DevCtx &c, const DevKey &k, const DevKey *keys,
+                         const u32 *key_idx, const u32 *out_idx, hipStream_t s)
 {
     if (B == 0) return;
     size_t lds = br_lds_bytes(logn, c.br_q, c.n_lwe);
     int tpb = 1 << (logn - 3);
     switch (logn) {
-    case 9: hipLaunchKernelGGL(k_blind_rotate<9>, grid, tpb, lds, s, ks, gate, B, out, stats, c, k); break;
-    case 11: hipLaunchKernelGGL(k_blind_rotate<11>, grid, tpb, lds, s, ks, gate, B, out, stats, c, k); break;
+    case 9: hipLaunchKernelGGL(k_blind_rotate<9>, grid, tpb, lds, s, ks, gate, B, out, stats, c, k, keys, key_idx, out_idx); break;
+    case 11: hipLaunchKernelGGL(k_blind_rotate<11>, grid, tpb, lds, s, ks, gate, B, out, stats, c, k, keys, key_idx, out_idx); break;
     }
 }
 

@@ -61,6 +61,18 @@ SIGNATURES = {
     "pz_keygen_server": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "pz_encrypt_bools": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp, C.c_size_t, _vp]),
     "pz_decrypt_bools": (C.c_int, [_vp, _vp, C.c_uint32, _vp, C.c_size_t, _vp]),
+    "pz_circuit_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "pz_circuit_destroy": (C.c_int, [_vp]),
+    "pz_circuit_inputs": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "pz_circuit_gate": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _vp]),
+    "pz_circuit_u8": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
+    "pz_circuit_div_zero_flag": (C.c_int, [_vp, _vp]),
+    "pz_circuit_reset_flags": (C.c_int, [_vp]),
+    "pz_circuit_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "pz_circuit_level_sizes": (C.c_int, [_vp, _vp, C.c_uint32]),
+    "pz_circuit_nodes": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "pz_circuit_run": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "pz_circuit_read": (C.c_int, [_vp, _vp, C.c_size_t, _vp, _vp]),
     "pz_rlwe_by_rgsw": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_rlwe_auto": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_lwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
@@ -140,6 +152,90 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+U8_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "div_rem": 5, "eq": 6, "neq": 7,
+          "lt": 8, "gt": 9, "le": 10, "ge": 11, "mux": 12, "max": 13, "min": 14,
+          "overflowing_add": 15, "overflowing_sub": 16}
+GATE_NOT = 6
+
+
+class Circuit:
+    """A gate DAG of FheUint8 operations (build-only when ctx is None)."""
+
+    def __init__(self, ctx=None):
+        self.L = load()
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(self.L.pz_circuit_create(ctx.h if ctx else None, C.byref(h)))
+        self.h = h.value
+        self.n_inputs = 0
+
+    def close(self):
+        if self.h:
+            self.L.pz_circuit_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def inputs(self, count):
+        w = np.zeros(count, dtype=np.uint32)
+        _check(self.L.pz_circuit_inputs(self.h, count, w.ctypes.data))
+        self.n_inputs += count
+        return w
+
+    def gate(self, op, a, b=0):
+        out = C.c_uint32()
+        op = GATES.get(op, op) if op != "not" else GATE_NOT
+        _check(self.L.pz_circuit_gate(self.h, op, int(a), int(b), C.byref(out)))
+        return out.value
+
+    def u8(self, op, a8, b8, sel=None):
+        a = np.ascontiguousarray(a8, dtype=np.uint32)
+        b = np.ascontiguousarray(b8, dtype=np.uint32)
+        out = np.zeros(16, dtype=np.uint32)
+        n = C.c_uint32()
+        s = C.c_uint32(int(sel)) if sel is not None else None
+        _check(self.L.pz_circuit_u8(self.h, U8_OPS[op], a.ctypes.data, b.ctypes.data,
+                                    C.byref(s) if s is not None else None, out.ctypes.data,
+                                    C.byref(n)))
+        return out[:n.value].copy()
+
+    def div_zero_flag(self):
+        w = C.c_uint32()
+        _check(self.L.pz_circuit_div_zero_flag(self.h, C.byref(w)))
+        return w.value
+
+    def stats(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(self.L.pz_circuit_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"wires": a.value, "pbs": b.value, "depth": c.value}
+
+    def level_sizes(self):
+        d = self.stats()["depth"]
+        out = np.zeros(d + 1, dtype=np.uint32)
+        _check(self.L.pz_circuit_level_sizes(self.h, out.ctypes.data, d + 1))
+        return out
+
+    def nodes(self):
+        n = self.stats()["wires"]
+        out = np.zeros(4 * n, dtype=np.uint32)
+        _check(self.L.pz_circuit_nodes(self.h, out.ctypes.data, n))
+        return out.reshape(n, 4)
+
+    def run(self, key, d_inputs):
+        _check(self.L.pz_circuit_run(self.h, key.h, d_inputs.ptr if d_inputs is not None else None,
+                                     None))
+
+    def read(self, wires):
+        w = np.ascontiguousarray(wires, dtype=np.uint32)
+        d = self.ctx.array((len(w), self.ctx.N + 1))
+        _check(self.L.pz_circuit_read(self.h, w.ctypes.data, len(w), d.ptr, None))
+        return d.download()
 
 
 class Key:

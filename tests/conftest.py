@@ -27,8 +27,16 @@ def oracle_for(name):
     return _oracles[name]
 
 
-def keys_for(name, seed_secret=0, seed_key=1, parties=2):
+# parties each parameter set is designed for (SP_TEST is the reference's
+# single-party test set, src/bool/parameters.rs:706-726)
+PARTIES = {"SP_TEST": 1, "NI_2P": 2, "NI_4P_HB_FR": 4, "NI_4P_LB_SR": 4, "NI_8P": 8,
+           "I_2P_LB_SR": 2, "I_4P": 4, "I_8P": 8}
+
+
+def keys_for(name, seed_secret=0, seed_key=1, parties=None):
     """(oracle, s, z, key) for a parameter set, generated once per session."""
+    if parties is None:
+        parties = PARTIES[name]
     k = (name, seed_secret, seed_key, parties)
     if k not in _keys:
         o = oracle_for(name)

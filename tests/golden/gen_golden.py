@@ -41,7 +41,7 @@ def main():
         out[name + "_ntt_in"] = x
         for which in ("forward", "forward_lazy", "backward", "backward_lazy"):
             out[name + "_ntt_" + which] = o.ntt_rows(which, x)
-        s, z, key = o.keygen(0, 1, 2)
+        s, z, key = o.keygen(0, 1, {"SP_TEST": 1, "NI_2P": 2}[name])
         out[name + "_key_digest"] = digest(key.rgsw, key.auto_key, key.lwe_ksk, s, z)
         rl = g.integers(0, o.Q, size=(2, o.N), dtype=np.uint64)
         rg, rgs = key.rgsw_ct(3)
