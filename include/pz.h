@@ -209,6 +209,39 @@ int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inputs, void *s
 int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t *d_out,
                     void *stream);
 
+/* ---- serving loop: FheUint8 programs from many sessions.  A session's key
+ * (host rows in the pz_key_upload layout, kept alive by the caller) is made
+ * resident when one of its requests is admitted and the resident keys fit
+ * hbm_budget_bytes (idle sessions are evicted LRU).  Each step runs the next
+ * ready gate level of every active request, across sessions and up to
+ * max_batch_pbs bootstraps, as one batched launch; requests arriving while
+ * others run join at the next step.  Arrival times are seconds from the
+ * start of pz_server_run.  All sessions of a server share the context's
+ * parameter set. */
+typedef struct pz_server pz_server;
+typedef struct pz_server_config {
+    size_t hbm_budget_bytes; /* resident key bytes allowed             */
+    uint32_t pool_rows;      /* ciphertext rows of the device wire pool */
+    uint32_t max_batch_pbs;  /* bootstraps per launch (soft cap)        */
+    uint32_t max_active;     /* requests in flight                      */
+} pz_server_config;
+int pz_server_create(pz_ctx *ctx, const pz_server_config *cfg, pz_server **out);
+int pz_server_destroy(pz_server *s);
+int pz_server_add_session(pz_server *s, const uint64_t *h_rgsw, const uint64_t *h_auto,
+                          const uint64_t *h_lwe_ksk, uint32_t *session_id);
+/* prog: a circuit (may be build-only); h_inputs: one [N+1] ciphertext per
+ * input wire in order; out_wires: results to return; n_ops: FheUint8
+ * operations in the program (for per-op latency). */
+int pz_server_submit(pz_server *s, uint32_t session, pz_circuit *prog, const uint64_t *h_inputs,
+                     const uint32_t *out_wires, uint32_t n_out, uint32_t n_ops, double arrival,
+                     uint32_t *request_id);
+int pz_server_run(pz_server *s);
+int pz_server_result(pz_server *s, uint32_t request_id, uint64_t *h_out);
+int pz_server_request_times(pz_server *s, uint32_t request_id, double *arrival, double *admitted,
+                            double *finished, uint32_t *n_pbs, uint32_t *n_ops);
+int pz_server_stats(pz_server *s, uint64_t *total_pbs, uint64_t *batches, uint64_t *key_uploads,
+                    uint64_t *evictions, double *wall_s);
+
 /* Automorphism count of each bootstrap of the last pz_pbs / pz_gates batch
  * (the blind rotation always runs n RLWE x RGSW products; the number of
  * automorphisms depends on the input).  Used for the algorithmic work count. */

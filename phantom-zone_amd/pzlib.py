@@ -73,6 +73,15 @@ SIGNATURES = {
     "pz_circuit_nodes": (C.c_int, [_vp, _vp, C.c_size_t]),
     "pz_circuit_run": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pz_circuit_read": (C.c_int, [_vp, _vp, C.c_size_t, _vp, _vp]),
+    "pz_server_create": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "pz_server_destroy": (C.c_int, [_vp]),
+    "pz_server_add_session": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "pz_server_submit": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32, C.c_uint32,
+                                   C.c_double, _vp]),
+    "pz_server_run": (C.c_int, [_vp]),
+    "pz_server_result": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "pz_server_request_times": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
+    "pz_server_stats": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "pz_rlwe_by_rgsw": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_rlwe_auto": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_lwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
@@ -236,6 +245,78 @@ class Circuit:
         d = self.ctx.array((len(w), self.ctx.N + 1))
         _check(self.L.pz_circuit_read(self.h, w.ctypes.data, len(w), d.ptr, None))
         return d.download()
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("pool_rows", C.c_uint32),
+                ("max_batch_pbs", C.c_uint32), ("max_active", C.c_uint32)]
+
+
+class Server:
+    """Continuous-batching request loop over sessions with resident keys."""
+
+    def __init__(self, ctx, hbm_budget_bytes=64 << 30, pool_rows=1 << 16, max_batch_pbs=8192,
+                 max_active=4096):
+        self.L = load()
+        self.ctx = ctx
+        cfg = ServerConfig(hbm_budget_bytes, pool_rows, max_batch_pbs, max_active)
+        h = C.c_void_p()
+        _check(self.L.pz_server_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h.value
+        self._keep = []
+        self.outs = {}
+
+    def close(self):
+        if self.h:
+            self.L.pz_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_session(self, rgsw, auto, ksk):
+        arrs = [np.ascontiguousarray(x, dtype=np.uint64) for x in (rgsw, auto, ksk)]
+        self._keep.append(arrs)
+        sid = C.c_uint32()
+        _check(self.L.pz_server_add_session(self.h, arrs[0].ctypes.data, arrs[1].ctypes.data,
+                                            arrs[2].ctypes.data, C.byref(sid)))
+        return sid.value
+
+    def submit(self, session, circuit, inputs, out_wires, n_ops, arrival):
+        inp = np.ascontiguousarray(inputs, dtype=np.uint64)
+        ow = np.ascontiguousarray(out_wires, dtype=np.uint32)
+        rid = C.c_uint32()
+        _check(self.L.pz_server_submit(self.h, session, circuit.h, inp.ctypes.data, ow.ctypes.data,
+                                       len(ow), n_ops, float(arrival), C.byref(rid)))
+        self.outs[rid.value] = len(ow)
+        return rid.value
+
+    def run(self):
+        _check(self.L.pz_server_run(self.h))
+
+    def result(self, rid):
+        out = np.zeros((self.outs[rid], self.ctx.N + 1), dtype=np.uint64)
+        _check(self.L.pz_server_result(self.h, rid, out.ctypes.data))
+        return out
+
+    def times(self, rid):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        n, k = C.c_uint32(), C.c_uint32()
+        _check(self.L.pz_server_request_times(self.h, rid, C.byref(a), C.byref(b), C.byref(c),
+                                              C.byref(n), C.byref(k)))
+        return {"arrival": a.value, "admitted": b.value, "finished": c.value, "pbs": n.value,
+                "ops": k.value}
+
+    def stats(self):
+        a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        w = C.c_double()
+        _check(self.L.pz_server_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d),
+                                      C.byref(w)))
+        return {"total_pbs": a.value, "batches": b.value, "key_uploads": c.value,
+                "evictions": d.value, "wall_s": w.value}
 
 
 class Key:
