@@ -1,13 +1,18 @@
-"""Bootstraps/sec of batched gate PBS on one MI355X (NonInteractiveLTE2Party).
+"""Bootstraps/sec of batched gate PBS on one MI355X (NonInteractiveLTE2Party),
+plus plain PBS, FheUint8 circuits and serving latency.
 
-A step is one batched launch of B independent NAND gate bootstraps (gate
-linear step + f64 moddown + LWE key switch + persistent blind rotation +
-sample extract) with keys and ciphertexts resident in HBM, replayed from a
-captured hipGraph.  The plain baseline is the same path with one PBS per
-launch.  The CPU baseline is the oracle (a C restatement of the reference's
-Rust PBS) on the host cores.
+Headline (the JSON line's `value`): a step is one batched launch of B = 4096
+independent NAND gate bootstraps (gate linear step + f64 moddown + LWE key
+switch + persistent blind rotation + sample extract), keys and ciphertexts
+resident in HBM, replayed from a captured hipGraph.  Keys are the seeded
+2-party ideal-secret NI_2P server key; inputs are encryptions of seeded
+random bits; outputs are decrypted and checked after timing.
 
-Prints one JSON line (the driver's contract).
+Extra fields: batch sweep (1024 / 16384), plain one-PBS-per-launch rate,
+FheUint8 at NI_8P (256 mul + 256 div_rem + 256 lt, level by level), serving
+(Poisson programs from 64 sessions; p50/p99 per-op latency; single-multiply
+p99), the roofline of the blind-rotation kernel, and the CPU baseline (the
+oracle -- a C restatement of the reference's Rust PBS -- on the host cores).
 """
 import argparse
 import json
@@ -30,29 +35,158 @@ VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9
 # 1 v_mul_hi_u32, 2 v_add3_u32, 1 v_lshl_add_u64, 2 v_sub(b), 4 v_mov.
 VALU_OPS_PER_MODMUL = 19
 HBM_PEAK_GBS = 8000.0
+PLAIN = {0: lambda a, b: 1 - (a & b)}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def modmul_per_pbs(N, n, autos):
-    """Algorithmic 54-bit modmuls of one blind rotation (NI_2P figures:
-    forward NTT = N/2*log2N, inverse = that + N, ext product = 2F + 2I + 4N,
-    auto = F + 2I + 2N)."""
+    """54-bit modmuls of one blind rotation: forward NTT F = N/2*log2(N),
+    inverse I = F (n^-1 is folded into the key rows), RLWE x RGSW =
+    2F + 2I + 4N, automorphism = F + 2I + 2N."""
     logn = N.bit_length() - 1
     F = N // 2 * logn
-    I = F + N
-    ext = 2 * F + 2 * I + 4 * N
-    aut = F + 2 * I + 2 * N
-    return n * ext + autos * aut
+    I = F
+    return n * (2 * F + 2 * I + 4 * N) + autos * (F + 2 * I + 2 * N)
 
 
-def synthetic_key(p, seed):
-    """Uniform key material of the right shapes (eval-domain RGSW / auto rows,
-    KSK rows mod q_ks): the PBS does the same work for any key values."""
+def keys(ctx, parties, seed=0):
+    s, z = ctx.keygen_secrets(seed, parties)
+    return s, z, ctx.keygen_server(seed + 1, s, z)
+
+
+def batched(ctx, key, s, B, steps, warmup, seed):
+    p = ctx.p
     r = np.random.default_rng(seed)
-    N, n = p.rlwe_n, p.lwe_n
-    rgsw = r.integers(0, p.rlwe_q, size=(n, 2 * p.rlrg_da + 2 * p.rlrg_db, N), dtype=np.uint64)
-    auto = r.integers(0, p.rlwe_q, size=(p.w + 1, 2 * p.auto_d, N), dtype=np.uint64)
-    ksk = r.integers(0, p.lwe_q, size=(N * p.lwe_d, n + 1), dtype=np.uint64)
-    return rgsw, auto, ksk
+    b0 = r.integers(0, 2, size=B).astype(np.uint8)
+    b1 = r.integers(0, 2, size=B).astype(np.uint8)
+    c0 = ctx.encrypt_bools(s, b0, seed * 1000 + 1)
+    c1 = ctx.encrypt_bools(s, b1, seed * 1000 + 1 + B)
+    ops = np.zeros(B, dtype=np.uint32)  # NAND
+    d0, d1, dops = ctx.to_device(c0), ctx.to_device(c1), ctx.to_device(ops)
+    dout = ctx.array((B, p.rlwe_n + 1))
+    for _ in range(warmup):
+        ctx.gates(key, dops, d0, d1, dout, B)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.gates(key, dops, d0, d1, dout, B)
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    got = ctx.decrypt_bools(s, dout.download())
+    ok = bool(np.array_equal(got, (1 - (b0 & b1)).astype(np.uint8)))
+    return B * steps / secs, secs, ok, (dops, d0, d1, dout)
+
+
+def fheuint8_circuits(args):
+    """256 mul + 256 div_rem + 256 lt on uniform u8 at NI_8P, one circuit,
+    every ready gate of a level in one launch."""
+    import pzlib
+    ctx = pzlib.Context("NI_8P")
+    s, z, key = keys(ctx, 8, seed=80)
+    r = np.random.default_rng(81)
+    circ = pzlib.Circuit(ctx)
+    specs, outs, in_bits = [], [], []
+    for op, count in (("mul", args.u8_count), ("div_rem", args.u8_count), ("lt", args.u8_count)):
+        for _ in range(count):
+            a, b = int(r.integers(0, 256)), int(r.integers(0, 256))
+            wa, wb = circ.inputs(8), circ.inputs(8)
+            in_bits += [(a >> i) & 1 for i in range(8)] + [(b >> i) & 1 for i in range(8)]
+            outs.append(circ.u8(op, wa, wb))
+            specs.append((op, a, b))
+    cts = ctx.encrypt_bools(s, np.array(in_bits, dtype=np.uint8), 82)
+    d_in = ctx.to_device(cts)
+    st = circ.stats()
+    sizes = circ.level_sizes()
+    ctx.sync()
+    t0 = time.perf_counter()
+    circ.run(key, d_in)
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    flat = np.concatenate(outs)
+    bits = ctx.decrypt_bools(s, circ.read(flat))
+    pos, ok = 0, True
+    for (op, a, b), w in zip(specs, outs):
+        v = bits[pos:pos + len(w)]
+        pos += len(w)
+        if op == "mul":
+            ok &= sum(int(v[i]) << i for i in range(8)) == (a * b) & 255
+        elif op == "div_rem":
+            q = sum(int(v[i]) << i for i in range(8))
+            rr = sum(int(v[8 + i]) << i for i in range(8))
+            ok &= (q, rr) == ((a // b, a % b) if b else (255, a))
+        else:
+            ok &= int(v[0]) == int(a < b)
+    return {"params": "NI_8P", "ops": {"mul": args.u8_count, "div_rem": args.u8_count,
+                                       "lt": args.u8_count},
+            "pbs": st["pbs"], "levels": st["depth"], "max_level_pbs": int(sizes.max()),
+            "seconds": secs, "bootstraps_per_sec": st["pbs"] / secs,
+            "ops_per_sec": 3 * args.u8_count / secs, "all_correct": bool(ok)}
+
+
+def serving(args, plain_rate):
+    """Poisson arrivals of FheUint8 programs (1..40 ops of add/sub/mul/div/
+    rem/lt/eq) from 64 NI_2P sessions, then single-multiply requests."""
+    import pzlib
+    import pzprog
+    ctx = pzlib.Context("NI_2P")
+    blobs = []
+    tk = time.perf_counter()
+    for i in range(args.distinct_keys):
+        s, z, k = keys(ctx, 2, seed=300 + 2 * i)
+        blobs.append((s, ctx.key_download(k)))
+        k.free()
+    log("serving: %d keys generated in %.1f s" % (len(blobs), time.perf_counter() - tk))
+    out = {}
+    for mode in ("mixed", "mul"):
+        srv = pzlib.Server(ctx, hbm_budget_bytes=args.hbm_budget << 30, pool_rows=1 << 17,
+                           max_batch_pbs=args.max_batch)
+        sids = [srv.add_session(*blobs[i % len(blobs)][1]) for i in range(args.sessions)]
+        r = np.random.default_rng(400 if mode == "mixed" else 401)
+        rate = args.rate if mode == "mixed" else args.mul_rate
+        count = args.programs if mode == "mixed" else args.mul_requests
+        t, rids, progs, metas = 0.0, [], [], []
+        for j in range(count):
+            t += float(r.exponential(1.0 / rate))
+            si = int(r.integers(0, args.sessions))
+            if mode == "mixed":
+                prog = pzprog.Program(r, int(r.integers(1, 41)))
+            else:
+                prog = pzprog.Program(r, 1, ops=("mul",))
+            circ = pzlib.Circuit(None)
+            wires, nu, nb = prog.build(circ)
+            cts = ctx.encrypt_bools(blobs[si % len(blobs)][0], prog.input_bits(), 5000 + 97 * j)
+            rids.append(srv.submit(sids[si], circ, cts, wires, len(prog.steps), t))
+            circ.close()
+            progs.append((prog, si))
+            metas.append((nu, nb))
+        log("serving %s: %d requests submitted" % (mode, count))
+        srv.run()
+        st = srv.stats()
+        lat, per_op, ok = [], [], True
+        for rid, (prog, si), meta in zip(rids, progs, metas):
+            tm = srv.times(rid)
+            lat.append(tm["finished"] - tm["arrival"])
+            per_op.append(lat[-1] / tm["ops"])
+            bits = ctx.decrypt_bools(blobs[si % len(blobs)][0], srv.result(rid))
+            ok &= pzprog.decode(bits, *meta) == prog.plain()
+        res = {"requests": count, "arrival_rate_per_s": rate, "sessions": args.sessions,
+               "distinct_keys": args.distinct_keys, "total_pbs": st["total_pbs"],
+               "launches": st["batches"], "wall_s": st["wall_s"],
+               "bootstraps_per_sec": st["total_pbs"] / st["wall_s"],
+               "vs_serial_plain": st["total_pbs"] / st["wall_s"] / plain_rate,
+               "latency_p50_s": float(np.percentile(lat, 50)),
+               "latency_p99_s": float(np.percentile(lat, 99)),
+               "per_op_latency_p50_s": float(np.percentile(per_op, 50)),
+               "per_op_latency_p99_s": float(np.percentile(per_op, 99)),
+               "key_uploads": st["key_uploads"], "evictions": st["evictions"],
+               "all_correct": bool(ok)}
+        out[mode] = res
+        srv.close()
+    ctx.close()
+    return out
 
 
 def cpu_baseline(params, threads, per_thread):
@@ -66,8 +200,34 @@ def cpu_baseline(params, threads, per_thread):
     _, secs = o.pbs_batch_threads(key, 0, lwes, threads)
     return {"value": count / secs, "unit": "bootstraps/sec", "cores": threads,
             "kind": "port",
-            "sample": "%d NAND PBS at %s, %d threads x %d, %.1f s wall" %
-                      (count, params, threads, per_thread, secs)}
+            "sample": "%d NAND PBS at %s (C restatement of the reference's Rust PBS), "
+                      "%d threads x %d, %.1f s wall, %.1f s CPU" %
+                      (count, params, threads, per_thread, secs, secs * threads)}
+
+
+def copy_probe():
+    import torch
+    x = torch.ones(1 << 25, dtype=torch.int64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(2):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 8 << 25, "copy_probe_bytes_written": 8 << 25,
+                      "dispatches": 2}))
+
+
+def measured_traffic(params, B):
+    """HBM bytes per k_blind_rotate launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy
+    calibration), when it was taken on the same workload."""
+    path = os.path.join(ROOT, "profiles", "k_blind_rotate_%s_b%d_pmc.md" % (params.lower(), B))
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_dispatch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -77,14 +237,31 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--params", default="NI_2P")
-    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--sweep", default="1024,16384")
+    ap.add_argument("--u8-count", type=int, default=256)
+    ap.add_argument("--sessions", type=int, default=64)
+    ap.add_argument("--distinct-keys", type=int, default=64)
+    ap.add_argument("--programs", type=int, default=48)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--mul-requests", type=int, default=128)
+    ap.add_argument("--mul-rate", type=float, default=40.0)
+    ap.add_argument("--max-batch", type=int, default=8192)
+    ap.add_argument("--hbm-budget", type=int, default=64, help="GiB of resident keys")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-per-thread", type=int, default=2)
+    ap.add_argument("--cpu-per-thread", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline, plain and roofline only")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB read + 256 MiB write elementwise kernel twice "
+                         "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box in this build has one MI355X; "
-              "multi-GPU sharding is design-only (replicated keys, split gate batches).",
+        print("bench.py measures one GPU only; multi-GPU runs are not implemented "
+              "(the design shards gate batches over GPUs with replicated keys).",
               file=sys.stderr)
         sys.exit(2)
 
@@ -92,27 +269,13 @@ def main():
     ctx = pzlib.Context(args.params, device=0)
     p = ctx.p
     N, n, B = p.rlwe_n, p.lwe_n, args.batch
-    rgsw, auto, ksk = synthetic_key(p, 7)
-    key = ctx.key_upload(rgsw, auto, ksk)
-    r = np.random.default_rng(3)
-    c0 = r.integers(0, p.rlwe_q, size=(B, N + 1), dtype=np.uint64)
-    c1 = r.integers(0, p.rlwe_q, size=(B, N + 1), dtype=np.uint64)
-    ops = np.zeros(B, dtype=np.uint32)  # NAND
-    d0, d1, dops = ctx.to_device(c0), ctx.to_device(c1), ctx.to_device(ops)
-    dout = ctx.array((B, N + 1))
-
-    for _ in range(args.warmup):
-        ctx.gates(key, dops, d0, d1, dout, B)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.gates(key, dops, d0, d1, dout, B)
-    ctx.sync()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    value = B * args.steps / secs
+    s, z, key = keys(ctx, 2, seed=0)
+    log("keys ready")
+    value, secs, ok, bufs = batched(ctx, key, s, B, args.steps, args.warmup, seed=3)
+    log("batched B=%d: %.0f bootstraps/s" % (B, value))
 
     # per-kernel durations (HIP events on the launch stream) of one more step
+    dops, d0, d1, dout = bufs
     ctx.set_timing(True)
     ctx.gates(key, dops, d0, d1, dout, B)
     tim = ctx.last_timing()
@@ -122,12 +285,15 @@ def main():
     br_s = tim["blind_rotate_ms"] / 1e3
     achieved = mm / br_s / 1e9  # Gmodmul/s
     peak = VALU_PEAK_OPS / VALU_OPS_PER_MODMUL / 1e9
-    key_bytes = 8 * (rgsw.size + auto.size)
+    key_bytes = key.nbytes()
     ct_bytes = B * (4 * (n + 1) + 8 * (N + 1) + 4)
     roof_hbm = (key_bytes + ct_bytes) / br_s / 1e9
 
     # plain: one PBS per launch
-    d0p, d1p, dopsp = ctx.to_device(c0[:1]), ctx.to_device(c1[:1]), ctx.to_device(ops[:1])
+    c0 = ctx.encrypt_bools(s, np.array([1], dtype=np.uint8), 9)
+    c1 = ctx.encrypt_bools(s, np.array([1], dtype=np.uint8), 10)
+    d0p, d1p = ctx.to_device(c0), ctx.to_device(c1)
+    dopsp = ctx.to_device(np.zeros(1, dtype=np.uint32))
     doutp = ctx.array((1, N + 1))
     ctx.gates(key, dopsp, d0p, d1p, doutp, 1)
     ctx.sync()
@@ -136,11 +302,30 @@ def main():
         ctx.gates(key, dopsp, d0p, d1p, doutp, 1)
         ctx.sync()
     plain = args.plain_steps / (time.perf_counter() - tp0)
+    ok &= int(ctx.decrypt_bools(s, doutp.download())[0]) == 0
+    log("plain: %.1f bootstraps/s" % plain)
+
+    extras = {}
+    if not args.no_extras:
+        sweep = {}
+        for bs in [int(x) for x in args.sweep.split(",") if x]:
+            v, _, okb, _ = batched(ctx, key, s, bs, 2 if bs <= 4096 else 1, 1, seed=bs)
+            sweep[str(bs)] = {"bootstraps_per_sec": v, "outputs_correct": okb}
+            log("batched B=%d: %.0f bootstraps/s" % (bs, v))
+        sweep[str(B)] = {"bootstraps_per_sec": value, "outputs_correct": ok}
+        extras["batch_sweep"] = sweep
+        key.free()
+        ctx.close()
+        extras["fheuint8_ni8p"] = fheuint8_circuits(args)
+        log("fheuint8: %s" % extras["fheuint8_ni8p"])
+        extras["serving_ni2p"] = serving(args, plain)
+        log("serving: %s" % extras["serving_ni2p"])
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(args.params, threads, args.cpu_per_thread)
+        log("cpu: %s" % cpu)
 
     out = {
         "metric": "bootstraps/sec (batched vs plain PBS, 1 MI355X) + p99 FheUint8 op latency",
@@ -154,16 +339,18 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
-        "data": "synthetic: uniform eval-domain keys and uniform input ciphertexts, seeded",
+        "data": "synthetic: seeded 2-party NI_2P keys, encryptions of seeded random bits; "
+                "outputs decrypted and checked",
         "config": {"workload": "batched NAND PBS, %s (N=%d, n=%d), B=%d per launch" %
-                               (args.params, N, n, B),
-                   "batch": B, "params": args.params},
+                               (args.params, N, n, B), "batch": B, "params": args.params},
+        "outputs_correct": bool(ok),
         "plain_bootstraps_per_sec": plain,
         "batched_over_plain": value / plain,
         "kernel_ms": tim,
         "autos_mean": float(np.mean(autos)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak,
-                     "unit": "Gmodmul/s", "frac": achieved / peak, "traffic": None,
+                     "unit": "Gmodmul/s", "frac": achieved / peak,
+                     "traffic": measured_traffic(args.params, B),
                      "kernel": "k_blind_rotate",
                      "note": "54-bit Shoup modmuls/s of the blind rotation; peak = "
                              "VALU lane-op peak / %d ops per Shoup multiply" %
@@ -172,6 +359,7 @@ def main():
                          "unit": "GB/s", "frac": roof_hbm / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out))
 
 

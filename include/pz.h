@@ -120,7 +120,7 @@ int pz_key_bytes(pz_key *key, size_t *bytes);
  * rows [N*d_ks][n+1]).  Any pointer may be NULL to skip that part. */
 int pz_key_download(pz_key *key, uint64_t *h_rgsw, uint64_t *h_auto, uint64_t *h_lwe_ksk);
 
-/* ---- seeded client-side key generation (this build's own seeded
+/* ---- seeded client-side key generation (the library's own seeded
  * procedure; structure of ClientKey::sk_rlwe / sk_lwe, src/bool/keys.rs:
  * 75-154, secret_key_encrypt_rgsw, seeded_auto_key_gen, seeded_lwe_ksk_keygen,
  * src/rgsw/keygen.rs:68-430, src/lwe.rs:46-101).  The ideal secret is the sum
@@ -210,9 +210,10 @@ int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t
                     void *stream);
 
 /* ---- serving loop: FheUint8 programs from many sessions.  A session's key
- * (host rows in the pz_key_upload layout, kept alive by the caller) is made
- * resident when one of its requests is admitted and the resident keys fit
- * hbm_budget_bytes (idle sessions are evicted LRU).  Each step runs the next
+ * (host rows in the pz_key_upload layout, kept alive by the caller) is
+ * uploaded by pz_server_add_session while the resident keys fit
+ * hbm_budget_bytes, otherwise when one of its requests is admitted (idle
+ * sessions are then evicted LRU to make room).  Each step runs the next
  * ready gate level of every active request, across sessions and up to
  * max_batch_pbs bootstraps, as one batched launch; requests arriving while
  * others run join at the next step.  Arrival times are seconds from the

@@ -9,7 +9,7 @@ Two tables, each T[k] = floor(2^64 * P(|X| <= k)), k = 0..41, X ~ N(0, 3.19^2):
     with f64::round, src/backend/mod.rs:60-69);
   * TRUNC: |X| truncated toward zero (the reference's i32::from_f64 for the
     Gaussian LWE secret, src/random.rs:145-160).
-The draws themselves are this build's own (ChaCha8 words against the table),
+The draws themselves are this project's own (ChaCha8 words against the table),
 not rand_distr's ziggurat, so noise streams are not bit-identical to Rust's.
 """
 import math

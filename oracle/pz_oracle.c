@@ -31,7 +31,7 @@
  *   PBS / gates          gate truth tables after decryption for all 4 inputs
  *                        of all 6 gates.
  *   RNG streams (uniform / Gaussian / ternary sampling) and therefore the key
- *   material:            PARITY UNPINNED -- this is the build's own seeded
+ *   material:            PARITY UNPINNED -- this is the project's own seeded
  *                        procedure (rand / rand_distr sources are not
  *                        available), not the Rust crate's stream.
  */
@@ -1253,7 +1253,7 @@ double pzo_pbs_batch_threads(const pzo_eval *e, const pzo_key *key, uint32_t gat
 }
 
 /* ------------------------------------------------------------------------- */
-/* Seeded key generation -- this build's own procedure (RNG parity with the  */
+/* Seeded key generation -- the project's own procedure (RNG parity with the */
 /* Rust crates is unpinned).  Structure follows the reference:               */
 /*   secrets  ClientKey::sk_rlwe/sk_lwe (src/bool/keys.rs:75-154), summed     */
 /*            over parties into the ideal secret;                            */

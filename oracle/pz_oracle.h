@@ -121,7 +121,7 @@ typedef struct pzo_key {
     uint64_t *lwe_ksk;
 } pzo_key;
 
-/* seeded key generation (this build's own procedure, see pz_oracle.c) */
+/* seeded key generation (the oracle's own procedure, see pz_oracle.c) */
 void pzo_keygen_secrets(const pzo_eval *e, const uint8_t seed[32], uint32_t parties,
                         int32_t *s_rlwe, int32_t *z_lwe);
 void pzo_keygen_server(const pzo_eval *e, const uint8_t seed[32],

@@ -397,6 +397,9 @@ int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows
         return fail(PZ_ERR_HIP, "key allocation failed");
     }
     hipStream_t s = c->stream;
+    // fold n^-1 into the RGSW and automorphism rows (DevKey)
+    launch_scale(rgsw, rgsw_words, c->dc.tb.n_inv, c->dc.tb.n_inv_s, p.rlwe_q, s);
+    launch_scale(auto_k, auto_words, c->dc.tb.n_inv, c->dc.tb.n_inv_s, p.rlwe_q, s);
     launch_shoup(rgsw, k->rgsw_s, rgsw_words, p.rlwe_q, s);
     launch_shoup(auto_k, k->auto_s, auto_words, p.rlwe_q, s);
     launch_pack_ksk(ksk_rows, k->ksk, (size_t)N * p.lwe_d, n1, ldk, kb, s);
@@ -481,8 +484,20 @@ extern "C" int pz_key_download(pz_key *k, uint64_t *h_rgsw, uint64_t *h_auto, ui
     size_t auto_words = (size_t)(p.w + 1) * k->dk.auto_rows * N;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (h_rgsw) HIPCHK(hipMemcpy(h_rgsw, k->rgsw, 8 * rgsw_words, hipMemcpyDeviceToHost));
-    if (h_auto) HIPCHK(hipMemcpy(h_auto, k->auto_k, 8 * auto_words, hipMemcpyDeviceToHost));
+    // device rows carry n^-1: multiply by N to return the key as generated
+    auto unscale = [&](uint64_t *h, size_t words) {
+        const uint64_t q = p.rlwe_q;
+        for (size_t i = 0; i < words; i++)
+            h[i] = (uint64_t)(((unsigned __int128)h[i] * N) % q);
+    };
+    if (h_rgsw) {
+        HIPCHK(hipMemcpy(h_rgsw, k->rgsw, 8 * rgsw_words, hipMemcpyDeviceToHost));
+        unscale(h_rgsw, rgsw_words);
+    }
+    if (h_auto) {
+        HIPCHK(hipMemcpy(h_auto, k->auto_k, 8 * auto_words, hipMemcpyDeviceToHost));
+        unscale(h_auto, auto_words);
+    }
     if (h_ksk) {
         size_t rows = (size_t)N * p.lwe_d, n1 = p.lwe_n + 1, ld = k->dk.ldk;
         std::vector<uint8_t> packed(rows * ld * k->dk.ksk_bytes);

@@ -36,6 +36,8 @@ struct DevCtx {
 };
 
 // Device key (reference row layout, evaluation domain, Shoup twins apart).
+// RGSW and automorphism rows are stored times n^-1 mod q, so the inverse NTT
+// of a key product needs no final scaling.
 struct DevKey {
     const u64 *rgsw, *rgsw_s;  // [n][rgsw_rows][N]
     const u64 *auto_k, *auto_s;  // [w+1][auto_rows][N]
@@ -119,7 +121,7 @@ PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
             fma_row(acc[1], K + o1, Ks + o1, D[0], j, q, q2);
         }
     }
-    ntt_inv<LOGN, 2, true>(acc, j, lds, c.tb);
+    ntt_inv<LOGN, 2, true, false>(acc, j, lds, c.tb);  // n^-1 is in the key rows
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         a[r] = acc[0][r];
@@ -183,7 +185,7 @@ PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
         fma_row(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
         fma_row(acc[1], K + (size_t)(d + rr) * N, Ks + (size_t)(d + rr) * N, D[0], j, q, q2);
     }
-    ntt_inv<LOGN, 2, true>(acc, j, lds, c.tb);
+    ntt_inv<LOGN, 2, true, false>(acc, j, lds, c.tb);  // n^-1 is in the key rows
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         u64 v = bp[r] & ~SIGN;

@@ -16,7 +16,8 @@ enum { PREP_GATE = 0, PREP_PBS = 1, PREP_KSONLY = 2 };
 // ciphertexts per group, KSK rows staged per LDS round.
 constexpr int KS_CT = 32;     // ciphertexts per work group
 constexpr int KS_ROWS = 256;  // KSK rows per LDS round
-constexpr int KS_SPLIT = 2;   // row splits (summed with atomics)
+constexpr int KS_SPLIT = 2;   // minimum row splits (summed with atomics)
+constexpr int KS_GROUPS = 2048;  // work groups a key-switch launch aims for
 
 void launch_ntt(int logn, int mode, u64 *rows, size_t nrows, const Tables &tb, hipStream_t s);
 void launch_fma(u64 *out, const u64 *a, const u64 *as, const u64 *b, u32 d, size_t n, u64 q,
@@ -35,6 +36,7 @@ const void *blind_rotate_kernel(int logn);
 void launch_blind_rotate(int logn, int grid, const u32 *ks, const u32 *gate, size_t B, u64 *out,
                          u32 *stats, const DevCtx &c, const DevKey &k, const DevKey *keys,
                          const u32 *key_idx, const u32 *out_idx, hipStream_t s);
+void launch_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q, hipStream_t s);
 void launch_shoup(const u64 *in, u64 *out, size_t n, u64 q, hipStream_t s);
 void launch_not(u64 *c0, size_t n, u64 q, hipStream_t s);
 void launch_widen(const u32 *in, u64 *out, size_t n, u32 mask, hipStream_t s);

@@ -126,7 +126,7 @@ __global__ void k_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, 
 // ---------------------------------------------------------------------------
 // k_ks: LWE key switch as a digits x KSK product, mod q_ks (src/lwe.rs:14-44).
 //   out[b][col] += sum_{rows in this split} digit[b][row] * ksk[row][col]
-// grid (ceil(B/KS_CT), KS_SPLIT); each thread owns one output unit for KS_CT
+// grid (ciphertext tiles, row splits); each thread owns one output unit for KS_CT
 // ciphertexts: for q_ks <= 2^16 a unit is two adjacent columns held as 16-bit
 // lanes of one register (v_pk_mad_u16: both lanes wrap mod 2^16, which is
 // the reference's mask arithmetic); otherwise one column in 32 bits.  Digits
@@ -238,6 +238,16 @@ __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks
 
 // Shoup twins: out[i] = floor(in[i] * 2^64 / q) (ToShoup, src/utils.rs:49-73)
 // by restoring long division of (v << 64) by q, v < q.
+// x[i] = x[i] * c mod q in [0,q) (c < q with Shoup twin c_s; x[i] < q)
+__global__ void k_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        u64 v = shoup_mul(x[i], c, c_s, q);
+        x[i] = umin(v, v - q);
+    }
+}
+
 __global__ void k_shoup(const u64 *in, u64 *out, size_t n, u64 q)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -349,13 +359,24 @@ void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_b
     u32 units = pack ? (n1 + 1) / 2 : n1;
     u32 tpb = (units + 63) / 64 * 64;
     size_t tiles = (seg_end - 1) / KS_CT - seg_begin / KS_CT + 1;
-    dim3 grid((unsigned)tiles, KS_SPLIT);
+    // split the rows so that even one tile fills the chip: about KS_GROUPS
+    // work groups, at least KS_SPLIT row ranges, each at least KS_ROWS rows
+    size_t split = (KS_GROUPS + tiles - 1) / tiles;
+    size_t max_split = rows / KS_ROWS > 0 ? rows / KS_ROWS : 1;
+    if (split > max_split) split = max_split;
+    if (split < KS_SPLIT) split = KS_SPLIT;
+    dim3 grid((unsigned)tiles, (unsigned)split);
     if (pack)
         hipLaunchKernelGGL(k_ks<true>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk,
                            seg_begin, seg_end, out);
     else
         hipLaunchKernelGGL(k_ks<false>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk,
                            seg_begin, seg_end, out);
+}
+
+void launch_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale, grid_for(n, 256), 256, 0, s, x, n, c, c_s, q);
 }
 
 void launch_shoup(const u64 *in, u64 *out, size_t n, u64 q, hipStream_t s)

@@ -1,7 +1,7 @@
 // pz_keygen.hip -- seeded client-side key generation and LWE encryption.
 //
 // Secrets and all random polynomials are drawn on the host from ChaCha8
-// streams (the stream order is this build's own, fixed below); the products
+// streams (the stream order is the project's own, fixed below); the products
 // a*s, the message terms and the transforms to the evaluation domain run on
 // the device with the same exact arithmetic as the reference's keygen:
 //   RGSW(X^{emb*z_i})   secret_key_encrypt_rgsw (src/rgsw/keygen.rs:68-160),

@@ -104,8 +104,10 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 }
 
 // One inverse stage S (m = 2^S, t = N >> (S+1)); S == 0 is the last stage,
-// which also scales by n^-1 (and reduces to [0,q) when STRICT).
-template <int LOGN, int S, int ST, int NP, bool STRICT>
+// which also scales by n^-1 when SCALE (and reduces to [0,q) when STRICT).
+// SCALE = false is for products whose key rows already carry the n^-1
+// factor: the result is the same residue, and with STRICT the same word.
+template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE>
 PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 {
     constexpr int N = 1 << LOGN;
@@ -122,7 +124,10 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 #pragma unroll
         for (int p = 0; p < NP; p++) {
             ib_2q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
-            if (S == 0) {
+            if (S == 0 && !SCALE && STRICT) {
+                v[p][r] = umin(v[p][r], v[p][r] - tb.q);
+                v[p][r + LD] = umin(v[p][r + LD], v[p][r + LD] - tb.q);
+            } else if (S == 0 && SCALE) {
                 u64 ox = shoup_mul(v[p][r], tb.n_inv, tb.n_inv_s, tb.q);
                 u64 oy = shoup_mul(v[p][r + LD], tb.n_inv, tb.n_inv_s, tb.q);
                 if (STRICT) {
@@ -153,14 +158,14 @@ PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
 }
 
 // Inverse phase covering stages S0+NS-1 .. S0 (descending).
-template <int LOGN, int S0, int NP, bool STRICT>
+template <int LOGN, int S0, int NP, bool STRICT, bool SCALE>
 PZ_DEV void inv_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
 {
     using P = Phase<LOGN, S0>;
     const u32 base = P::base(j);
-    if constexpr (P::NS > 2) inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT>(v, base, tb);
-    if constexpr (P::NS > 1) inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT>(v, base, tb);
-    inv_stage<LOGN, S0, P::ST, NP, STRICT>(v, base, tb);
+    if constexpr (P::NS > 2) inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE>(v, base, tb);
+    if constexpr (P::NS > 1) inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE>(v, base, tb);
+    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE>(v, base, tb);
 }
 
 // Store the thread's coefficients (layout of phase SA) into LDS, barrier,
@@ -217,19 +222,20 @@ PZ_DEV void ntt_fwd(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
 }
 
 // Inverse NTT of NP polynomials: contiguous layout in, stride layout out.
-// STRICT = Ntt::backward ([0,q)), else backward_lazy ([0,2q)).
-template <int LOGN, int NP, bool STRICT>
+// STRICT = Ntt::backward ([0,q)), else backward_lazy ([0,2q)); SCALE = false
+// leaves out the n^-1 factor (see inv_stage).
+template <int LOGN, int NP, bool STRICT, bool SCALE = true>
 PZ_DEV void ntt_inv(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
 {
     if constexpr (LOGN > 9) {
-        inv_phase<LOGN, 9, NP, STRICT>(v, j, fresh(tb));
+        inv_phase<LOGN, 9, NP, STRICT, SCALE>(v, j, fresh(tb));
         exchange<LOGN, 9, 6, NP>(v, j, lds);
     }
-    inv_phase<LOGN, 6, NP, STRICT>(v, j, fresh(tb));
+    inv_phase<LOGN, 6, NP, STRICT, SCALE>(v, j, fresh(tb));
     exchange<LOGN, 6, 3, NP>(v, j, lds);
-    inv_phase<LOGN, 3, NP, STRICT>(v, j, fresh(tb));
+    inv_phase<LOGN, 3, NP, STRICT, SCALE>(v, j, fresh(tb));
     exchange<LOGN, 3, 0, NP>(v, j, lds);
-    inv_phase<LOGN, 0, NP, STRICT>(v, j, fresh(tb));
+    inv_phase<LOGN, 0, NP, STRICT, SCALE>(v, j, fresh(tb));
 }
 
 // Coefficient index of register r in the stride (coefficient-domain) layout

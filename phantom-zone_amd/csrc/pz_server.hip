@@ -1,8 +1,9 @@
 // pz_server.hip -- serving loop for FheUint8 programs from many sessions.
 //
-// Sessions own server keys; a session's key is made resident in HBM when one
-// of its requests is admitted and the keys' total footprint fits the budget
-// (idle resident sessions are evicted, least recently used first).  Requests
+// Sessions own server keys; a session's key is uploaded to HBM when the
+// session is created if the keys' total footprint fits the budget, otherwise
+// when one of its requests is admitted (idle resident sessions are then
+// evicted, least recently used first).  Requests
 // are gate DAGs (pz_circuit) with input ciphertexts and arrival times.  Each
 // loop step admits arrived requests as HBM and pool rows allow, then takes the
 // next ready level of every active request -- across sessions, up to a
@@ -144,6 +145,29 @@ extern "C" int pz_server_add_session(pz_server *s, const uint64_t *h_rgsw, const
     se.h_ksk = h_ksk;
     s->sessions.push_back(se);
     *sid = (uint32_t)(s->sessions.size() - 1);
+    // keys go resident at session creation while the budget holds them
+    // without evicting anyone; the rest are uploaded on admission
+    size_t need = 0;
+    for (auto &o : s->sessions)
+        if (o.key) {
+            need = o.bytes;
+            break;
+        }
+    if (s->resident_bytes + need <= s->cfg.hbm_budget_bytes) {
+        Session &ns = s->sessions.back();
+        if (pz_key_upload(s->ctx, h_rgsw, h_auto, h_ksk, &ns.key) != PZ_OK) {
+            ns.key = nullptr;
+            return PZ_OK;  // retried on admission
+        }
+        pz_key_bytes(ns.key, &ns.bytes);
+        if (s->resident_bytes + ns.bytes > s->cfg.hbm_budget_bytes) {
+            pz_key_free(ns.key);
+            ns.key = nullptr;
+            return PZ_OK;
+        }
+        s->resident_bytes += ns.bytes;
+        s->key_uploads++;
+    }
     return PZ_OK;
 }
 

@@ -99,3 +99,65 @@ def test_test_vectors(name):
         val = t if i < 3 * (o.p.br_q // 8) else f
         got = int(tv[to]) if to < qby2 else (q - int(tv[to - qby2])) % q
         assert got == val
+
+
+def py_br_schedule(ks, p):
+    """Independent restatement of the blind-rotation op order (src/pbs.rs:
+    146-161 buckets, :240-414 loop; dlog map src/bool/evaluator.rs:664-673;
+    mod_switch_odd :416-420 in Python floats).  Returns (kind, arg, trivial)
+    with kind 0 = RLWE x RGSW of LWE index arg, 1 = automorphism key arg."""
+    import math
+    br_q, lwe_q, g, w = int(p.br_q), int(p.lwe_q), int(p.g), int(p.w)
+    qb4 = br_q // 4
+    dlog = [0] * br_q
+    for i in range(qb4):
+        v = pow(g, i, br_q)
+        dlog[v] = i
+        dlog[br_q - v] = i + qb4
+    buckets = [[] for _ in range(br_q // 2)]
+    for index, v in enumerate(int(x) for x in ks[1:]):
+        odd = math.floor((float(v) * float(br_q)) / float(lwe_q))
+        odd += (odd & 1) ^ 1
+        buckets[dlog[odd]].append(index)
+    ops, state = [], {"trivial": True}
+
+    def ext(i):
+        ops.append((0, i, state["trivial"]))
+        state["trivial"] = False
+
+    def auto(k):
+        ops.append((1, k, state["trivial"]))
+
+    for base in (qb4, 0):  # -(g^k) buckets, then +(g^k)
+        v = 0
+        for i in range(qb4 - 1, 0, -1):
+            for si in buckets[base + i]:
+                ext(si)
+            v += 1
+            if buckets[base + i - 1] or v == w or i == 1:
+                auto(v)
+                v = 0
+        for si in buckets[base]:
+            ext(si)
+        if base == qb4:
+            auto(0)  # -(g^0) closes with the -g automorphism
+    return ops
+
+
+@pytest.mark.parametrize("name", ["NI_2P", "NI_4P_HB_FR", "NI_8P", "SP_TEST"])
+def test_schedule_matches_python_restatement(name):
+    """The oracle's recorded op order equals an independent restatement of
+    the reference's loop, including the trivial-RLWE flags, for uniform key
+    switch outputs and for ones concentrated in few buckets."""
+    from conftest import oracle_for
+    o = oracle_for(name)
+    r = rng(72)
+    lq = int(o.p.lwe_q)
+    cases = [r.integers(0, lq, size=o.n + 1, dtype=np.uint64) for _ in range(6)]
+    cases.append(np.full(o.n + 1, 1, dtype=np.uint64))             # one bucket
+    cases.append(np.zeros(o.n + 1, dtype=np.uint64))               # +g^0 only
+    cases.append(r.integers(0, 4, size=o.n + 1, dtype=np.uint64) * (lq // 4))
+    for ks in cases:
+        ops = o.br_schedule(ks)
+        got = [(int(x) & 0xFF, (int(x) >> 8) & 0x7FFFFF, bool(int(x) >> 31)) for x in ops]
+        assert got == py_br_schedule(ks, o.p)
