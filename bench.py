@@ -189,6 +189,58 @@ def serving(args, plain_rate):
     return out
 
 
+def kernel_batch():
+    """4096 forward + inverse NTTs of uniform rows and 4096 RLWE x RGSW
+    products with a uniform evaluation-domain RGSW key at NI_2P."""
+    import pzlib
+    ctx = pzlib.Context("NI_2P")
+    p = ctx.p
+    N, n, B = p.rlwe_n, p.lwe_n, 4096
+    r = np.random.default_rng(7)
+    x = r.integers(0, ctx.Q, size=(B, N), dtype=np.uint64)
+    d = ctx.to_device(x)
+    reps = 20
+    ctx.ntt("forward", d, B)
+    ctx.ntt("backward", d, B)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.ntt("forward", d, B)
+        ctx.ntt("backward", d, B)
+    ctx.sync()
+    pair_s = (time.perf_counter() - t0) / reps
+    round_trip = bool(np.array_equal(d.download(), x))
+    logn = N.bit_length() - 1
+    F = N // 2 * logn
+    rows = 2 * (p.rlrg_da + p.rlrg_db)
+    rgsw = r.integers(0, ctx.Q, size=(n, rows, N), dtype=np.uint64)
+    auto = np.zeros((p.w + 1, 2 * p.auto_d, N), dtype=np.uint64)
+    ksk = np.zeros((N * p.lwe_d, n + 1), dtype=np.uint64)
+    key = ctx.key_upload(rgsw, auto, ksk)
+    rl = ctx.to_device(r.integers(0, ctx.Q, size=(B, 2, N), dtype=np.uint64))
+    idx = ctx.to_device(r.integers(0, n, size=B, dtype=np.uint32))
+    triv = ctx.to_device(np.zeros(B, dtype=np.uint8))
+    ctx.rlwe_by_rgsw(key, rl, idx, triv, B)
+    ctx.sync()
+    reps_e = 5
+    t0 = time.perf_counter()
+    for _ in range(reps_e):
+        ctx.rlwe_by_rgsw(key, rl, idx, triv, B)
+    ctx.sync()
+    ext_s = (time.perf_counter() - t0) / reps_e
+    ext_mm = B * (2 * F + 2 * F + 4 * N)  # n^-1 is folded into the stored key rows
+    key.free()
+    ctx.close()
+    return {"params": "NI_2P", "rows": B,
+            "ntt_fwd_plus_inv_ms": pair_s * 1e3,
+            "ntt_gbs": 4 * B * N * 8 / pair_s / 1e9,
+            "ntt_round_trip_exact": round_trip,
+            "ext_products_ms": ext_s * 1e3,
+            "ext_products_per_sec": B / ext_s,
+            "ext_gmodmul_per_sec": ext_mm / ext_s / 1e9,
+            "ext_rlwe_gbs": 2 * B * 2 * N * 8 / ext_s / 1e9}
+
+
 def cpu_baseline(params, threads, per_thread):
     """Oracle PBS throughput on the host cores (one PBS stream per thread)."""
     from oracle import Oracle
@@ -198,8 +250,9 @@ def cpu_baseline(params, threads, per_thread):
     r = np.random.default_rng(5)
     lwes = r.integers(0, o.Q, size=(count, o.N + 1), dtype=np.uint64)
     _, secs = o.pbs_batch_threads(key, 0, lwes, threads)
+    _, secs1 = o.pbs_batch_threads(key, 0, lwes[:4], 1)
     return {"value": count / secs, "unit": "bootstraps/sec", "cores": threads,
-            "kind": "port",
+            "kind": "port", "single_thread_pbs_ms": secs1 / 4 * 1e3,
             "sample": "%d NAND PBS at %s (C restatement of the reference's Rust PBS), "
                       "%d threads x %d, %.1f s wall, %.1f s CPU" %
                       (count, params, threads, per_thread, secs, secs * threads)}
@@ -316,6 +369,8 @@ def main():
         extras["batch_sweep"] = sweep
         key.free()
         ctx.close()
+        extras["kernel_batch"] = kernel_batch()
+        log("kernel batch: %s" % extras["kernel_batch"])
         extras["fheuint8_ni8p"] = fheuint8_circuits(args)
         log("fheuint8: %s" % extras["fheuint8_ni8p"])
         extras["serving_ni2p"] = serving(args, plain)

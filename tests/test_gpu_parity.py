@@ -13,7 +13,8 @@ from conftest import keys_for, oracle_for, rng, uniform_mod
 
 pytestmark = pytest.mark.gpu
 
-PARAMS = ["NI_2P", "SP_TEST"]
+# NI_4P_HB_FR: br_q = 2048 (embedding factor 2); NI_8P: q_ks = 2^17 (32-bit KSK)
+PARAMS = ["NI_2P", "SP_TEST", "NI_4P_HB_FR", "NI_8P"]
 _ctx = {}
 _dkeys = {}
 
@@ -52,7 +53,7 @@ def test_ntt_tables_match(pname):
 def test_ntt_rows(pname, which):
     o = oracle_for(pname)
     c = ctx_for(pname)
-    rows = 512 if pname == "NI_2P" else 2048
+    rows = 512 if o.N == 2048 else 2048
     log("ntt %s %s rows=%d" % (pname, which, rows))
     r = rng(11)
     # inputs in the reference's accepted ranges: [0,2q) for lazy entry points
@@ -193,7 +194,7 @@ def test_pbs(pname):
     o, s, z, key = keys_for(pname)
     c = ctx_for(pname)
     k = dkey_for(pname)
-    B = 6 if pname == "NI_2P" else 48
+    B = 6 if o.N == 2048 else 48
     log("pbs %s B=%d" % (pname, B))
     r = rng(51)
     x = uniform_mod(r, o.Q, (B, o.N + 1))
@@ -210,7 +211,7 @@ def test_gates_truth_and_parity(pname):
     o, s, z, key = keys_for(pname)
     c = ctx_for(pname)
     k = dkey_for(pname)
-    B = 12 if pname == "NI_2P" else 96
+    B = 12 if o.N == 2048 else 96
     log("gates %s B=%d" % (pname, B))
     bits0, bits1, c0, c1 = _gate_inputs(o, s, B, 61)
     ops = (np.arange(B) % 6).astype(np.uint32)
