@@ -358,6 +358,13 @@ def main():
     ok &= int(ctx.decrypt_bools(s, doutp.download())[0]) == 0
     log("plain: %.1f bootstraps/s" % plain)
 
+    # measured VALU rates: the instruction forms of the Shoup multiply, and
+    # the Shoup multiply itself over independent chains (its achievable peak)
+    probe = {name: ctx.valu_probe(kind) for kind, name in
+             enumerate(("add_u32", "mul_lo_u32", "mul_hi_u32", "mad_u64_u32"))}
+    probe["shoup_mul_per_s"] = ctx.valu_probe(4, 1024)
+    log("valu probe: %s" % probe)
+
     extras = {}
     if not args.no_extras:
         sweep = {}
@@ -410,6 +417,9 @@ def main():
                      "note": "54-bit Shoup modmuls/s of the blind rotation; peak = "
                              "VALU lane-op peak / %d ops per Shoup multiply" %
                              VALU_OPS_PER_MODMUL},
+        "valu_probe": {"lane_ops_per_s": {k: v for k, v in probe.items() if k != "shoup_mul_per_s"},
+                       "shoup_mul_per_s": probe["shoup_mul_per_s"],
+                       "blind_rotate_frac_of_probe": mm / br_s / probe["shoup_mul_per_s"]},
         "roofline_hbm": {"bound": "hbm", "achieved": roof_hbm, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": roof_hbm / HBM_PEAK_GBS},
         "cpu_baseline": cpu,

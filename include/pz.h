@@ -243,6 +243,15 @@ int pz_server_request_times(pz_server *s, uint32_t request_id, double *arrival, 
 int pz_server_stats(pz_server *s, uint64_t *total_pbs, uint64_t *batches, uint64_t *key_uploads,
                     uint64_t *evictions, double *wall_s);
 
+/* Blind-rotation kernel choice.  No counterpart in the reference (its
+ * evaluator is single-threaded): 0 = automatic (two half work groups per
+ * bootstrap while every bootstrap of the batch has a CU of its own, for the
+ * lower latency; one work group per bootstrap beyond that, for the higher
+ * throughput), 1 = always one work group, 2 = always two (only where
+ * d_a = d_b = d_auto = 1; PZ_ERR_UNSUPPORTED otherwise).  Outputs are the
+ * same words in every mode. */
+int pz_set_blind_rotate_mode(pz_ctx *ctx, int mode);
+
 /* Automorphism count of each bootstrap of the last pz_pbs / pz_gates batch
  * (the blind rotation always runs n RLWE x RGSW products; the number of
  * automorphisms depends on the input).  Used for the algorithmic work count. */
@@ -252,6 +261,13 @@ int pz_last_auto_counts(pz_ctx *ctx, uint32_t *h_counts, size_t B);
  * enabled (milliseconds from HIP events on the launch stream). */
 int pz_set_timing(pz_ctx *ctx, int enabled);
 int pz_last_timing(pz_ctx *ctx, float *ms_prep, float *ms_ks, float *ms_blind_rotate);
+
+/* VALU issue-rate probe.  No counterpart in the reference: it measures the
+ * integer VALU rates the PBS kernels' roofline rests on.  kind: 0 v_add_u32,
+ * 1 v_mul_lo_u32, 2 v_mul_hi_u32, 3 v_mad_u64_u32 (lane operations per
+ * second), 4 one 54-bit Shoup multiply (multiplies per second), each over
+ * 8 independent chains per thread on every CU. */
+int pz_valu_probe(pz_ctx *ctx, int kind, uint32_t iters, double *ops_per_s);
 
 #ifdef __cplusplus
 }

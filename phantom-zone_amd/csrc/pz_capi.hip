@@ -58,6 +58,8 @@ struct pz_ctx {
     u32 *d_ksb = nullptr, *d_ks = nullptr, *d_stats = nullptr;
     size_t cap = 0;
     int num_cu = 0, br_per_cu = 0;
+    int br_split_per_cu = 0;  // resident split work groups per CU (0: not available)
+    int br_mode = 0;          // 0 auto, 1 one-half kernel, 2 split kernel
     // captured gate batches
     std::map<GraphKey, hipGraphExec_t> graphs;
     const pz_key *graph_key = nullptr;
@@ -212,11 +214,19 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
     int per = 0;
     size_t lds = br_lds_bytes(logn, p->br_q, p->lwe_n);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, blind_rotate_kernel(logn),
-                                                     1 << (logn - 3), lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, blind_rotate_kernel(logn, false),
+                                                     br_threads(logn, false), lds) != hipSuccess ||
         per < 1)
         per = 1;
     c->br_per_cu = per;
+    if (br_split(d)) {
+        int ps = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ps, blind_rotate_kernel(logn, true),
+                                                         br_threads(logn, true), lds) != hipSuccess ||
+            ps < 1)
+            ps = 1;
+        c->br_split_per_cu = ps;
+    }
     for (int i = 0; i < 4; i++) hipEventCreate(&c->ev[i]);
     *out = c;
     return PZ_OK;
@@ -604,6 +614,43 @@ extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint6
     return PZ_OK;
 }
 
+// Blind-rotation kernel and grid for a batch of B bootstraps.  The split
+// kernel (two half work groups per bootstrap) has the lower latency while
+// every bootstrap gets a CU of its own; beyond that the one-half kernel,
+// twice as many resident bootstraps per CU, has the higher throughput.
+static void br_plan(const pz_ctx *c, size_t B, bool *split, int *grid)
+{
+    const size_t res1 = (size_t)c->num_cu * (c->br_per_cu > 0 ? c->br_per_cu : 1);
+    const size_t res2 = (size_t)c->num_cu * (c->br_split_per_cu > 0 ? c->br_split_per_cu : 1);
+    bool sp = c->br_split_per_cu > 0 &&
+              (c->br_mode == 2 || (c->br_mode == 0 && B <= res2));
+    const size_t res = sp ? res2 : res1;
+    *split = sp;
+    *grid = (int)(B < res ? (B > 0 ? B : 1) : res);
+}
+
+int pz_internal_br_plan(pz_ctx *c, size_t B, int *split, int *grid)
+{
+    bool sp = false;
+    br_plan(c, B, &sp, grid);
+    *split = sp ? 1 : 0;
+    return PZ_OK;
+}
+
+extern "C" int pz_set_blind_rotate_mode(pz_ctx *c, int mode)
+{
+    if (!c || mode < 0 || mode > 2) return fail(PZ_ERR_ARG, "mode must be 0, 1 or 2");
+    if (mode == 2 && c->br_split_per_cu == 0)
+        return fail(PZ_ERR_UNSUPPORTED, "the split kernel needs d_a = d_b = d_auto = 1");
+    if (mode != c->br_mode) {
+        HIPCHK(hipSetDevice(c->device));
+        hipStreamSynchronize(c->stream);
+        drop_graphs(c);  // captured graphs hold the previous kernel choice
+        c->br_mode = mode;
+    }
+    return PZ_OK;
+}
+
 // moddown/KS/blind-rotate sequence on stream st
 static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, const uint64_t *c1,
                          const uint32_t *ops, uint64_t *out, size_t B, hipStream_t st)
@@ -617,11 +664,11 @@ static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, con
     launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
               k->dk.ldk, 0, B, c->d_ks, st);
     if (c->timing) hipEventRecord(c->ev[2], st);
-    size_t resident = (size_t)c->num_cu * c->br_per_cu;
-    if (resident < 1) resident = 1;
-    int grid = (int)(B < resident ? B : resident);
-    launch_blind_rotate(c->logn, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk, nullptr,
-                        nullptr, nullptr, st);
+    bool split = false;
+    int grid = 1;
+    br_plan(c, B, &split, &grid);
+    launch_blind_rotate(c->logn, split, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk,
+                        nullptr, nullptr, nullptr, st);
     if (c->timing) hipEventRecord(c->ev[3], st);
 }
 
@@ -717,5 +764,36 @@ extern "C" int pz_last_timing(pz_ctx *c, float *a, float *b, float *d)
     if (a) *a = c->t_prep;
     if (b) *b = c->t_ks;
     if (d) *d = c->t_br;
+    return PZ_OK;
+}
+
+// VALU issue-rate probe (no counterpart in the reference; measures the chip's
+// integer VALU rates for the roofline of the PBS kernels).
+extern "C" int pz_valu_probe(pz_ctx *c, int kind, uint32_t iters, double *ops_per_s)
+{
+    if (!c || !ops_per_s || kind < 0 || kind > 4 || iters == 0)
+        return fail(PZ_ERR_ARG, "bad probe arguments");
+    HIPCHK(hipSetDevice(c->device));
+    const int grid = c->num_cu * 16;  // 16 work groups of 4 waves per CU: 2 rounds at 8 waves/SIMD
+    const size_t threads = (size_t)grid * 256;
+    u64 *out = nullptr;
+    HIPCHK(hipMalloc(&out, 8 * threads));
+    const u64 q = c->p.rlwe_q, w = (q >> 1) | 1;
+    const u64 ws = (u64)(((unsigned __int128)w << 64) / q);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    launch_valu_probe(kind, grid, out, iters / 8 + 1, 0x9e3779b1u, w, ws, q, c->stream);  // warm-up
+    hipEventRecord(e0, c->stream);
+    launch_valu_probe(kind, grid, out, iters, 0x9e3779b1u, w, ws, q, c->stream);
+    hipEventRecord(e1, c->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(out);
+    if (e != hipSuccess) return fail(PZ_ERR_HIP, hipGetErrorString(e));
+    *ops_per_s = (double)threads * iters * PROBE_CHAINS / (ms * 1e-3);
     return PZ_OK;
 }

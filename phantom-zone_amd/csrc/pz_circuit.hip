@@ -147,9 +147,10 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
                       s0, e0, v.ks, st);
             s0 = e0;
         }
-        int grid = (int)std::min<size_t>(B, (size_t)v.grid);
+        int split = 0, grid = 1;
+        pz_internal_br_plan(c, B, &split, &grid);
         const DevKey *k0 = pz_internal_devkey(keys[pbs[0].key]);
-        launch_blind_rotate(v.logn, grid, v.ks, d, B, pool, v.stats, dc, *k0,
+        launch_blind_rotate(v.logn, split != 0, grid, v.ks, d, B, pool, v.stats, dc, *k0,
                             keys.size() > 1 ? g_stage.dk : nullptr,
                             keys.size() > 1 ? d + 4 * B : nullptr, d + 3 * B, st);
     }

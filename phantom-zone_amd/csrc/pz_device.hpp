@@ -195,4 +195,130 @@ PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split work group (2 x N/8 threads) for d_a = d_b = d_auto = 1 (every NI
+// parameter set): half h (wave-uniform) holds polynomial h of the RLWE
+// (0 = a, 1 = b), 8 coefficients per thread in the stride layout, and
+// exchanges through its own LDS region (lds + h*W).  Half the registers of
+// the one-half operators above, twice the waves per bootstrap.
+// ---------------------------------------------------------------------------
+
+// The barriers of one ntt_fwd, for a half that has no transform to run.
+template <int LOGN>
+PZ_DEV void ntt_fwd_idle()
+{
+#pragma unroll
+    for (int e = 0; e < (LOGN > 9 ? 6 : 4); e++) __syncthreads();
+}
+
+// rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) with d_a = d_b = 1:
+// each half decomposes and transforms its own polynomial, the halves swap
+// their digit transforms through LDS, half 0 accumulates a' over rows
+// A(-sm), A(m) and half 1 accumulates b' over rows B(-sm), B(m).  A trivial
+// RLWE (a = 0) needs no special case: its digit transform is 0 mod q and the
+// strict inverse returns the same words as the reference's skip.
+template <int LOGN>
+PZ_DEV void ext_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+                      const DevCtx &c, u32 j, u32 h, u64 *lds)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int W = Geom<LOGN>::LDS_WORDS;
+    const u64 q = c.tb.q, q2 = c.tb.q2;
+    u64 *mine = lds + h * W;
+    const u64 *theirs = lds + (h ^ 1) * W;
+    u64 D[1][8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(h ? c.dec_b : c.dec_a, p[r], 0);
+    ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, mine, c.tb);
+#pragma unroll
+    for (int r = 0; r < 8; r++) mine[lds_ix(eval_x<LOGN>(j, r))] = D[0][r];
+    __syncthreads();
+    u64 E[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) E[r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
+    __syncthreads();
+    u64 acc[1][8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[0][r] = 0;
+    const size_t ra = (size_t)h * N, rb = (size_t)(2 + h) * N;  // rows for D_a, D_b
+    if (h == 0) {
+        fma_row(acc[0], K + ra, Ks + ra, D[0], j, q, q2);
+        fma_row(acc[0], K + rb, Ks + rb, E, j, q, q2);
+    } else {
+        fma_row(acc[0], K + ra, Ks + ra, E, j, q, q2);
+        fma_row(acc[0], K + rb, Ks + rb, D[0], j, q, q2);
+    }
+    ntt_inv<LOGN, 1, true, false>(acc, j, mine, c.tb);  // n^-1 is in the key rows
+#pragma unroll
+    for (int r = 0; r < 8; r++) p[r] = acc[0][r];
+}
+
+// rlwe_auto_shoup (src/rgsw/runtime.rs:530-643) with d_auto = 1: each half
+// permutes its polynomial; half 0 decomposes and transforms auto(a) while
+// half 1 waits at the matching barriers, then both halves take the digit
+// transform, half 0 accumulates a' (row A) and half 1 accumulates b' (row B)
+// and adds the permuted b.  Trivial: only b is permuted (q - v where the map
+// negates, the reference's neg), a is left as it is.
+template <int LOGN>
+PZ_DEV void auto_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+                       u64 kexp, bool trivial, const DevCtx &c, u32 j, u32 h, u64 *lds)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int W = Geom<LOGN>::LDS_WORDS;
+    const u64 q = c.tb.q, q2 = c.tb.q2;
+    const u64 SIGN = 1ull << 63;
+    u64 *mine = lds + h * W;
+    const bool move = !(trivial && h == 0);
+    if (move) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            u32 x = coef_x<LOGN>(j, r);
+            u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
+            bool pos = to < (u32)N;
+            to = pos ? to : to - N;
+            const u64 v = p[r];
+            mine[lds_ix(to)] = pos ? v : ((h == 0 || trivial) ? neg_mod(v, q) : (v | SIGN));
+        }
+    }
+    __syncthreads();
+    if (move) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) p[r] = mine[lds_ix(coef_x<LOGN>(j, r))];
+    }
+    __syncthreads();
+    if (trivial) return;  // uniform over the work group
+    u64 D[1][8];
+    if (h == 0) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_auto, p[r], 0);
+        ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, mine, c.tb);
+#pragma unroll
+        for (int r = 0; r < 8; r++) mine[lds_ix(eval_x<LOGN>(j, r))] = D[0][r];
+    } else {
+        ntt_fwd_idle<LOGN>();
+    }
+    __syncthreads();
+    if (h == 1) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) D[0][r] = lds[lds_ix(eval_x<LOGN>(j, r))];
+    }
+    __syncthreads();
+    u64 acc[1][8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[0][r] = 0;
+    fma_row(acc[0], K + (size_t)h * N, Ks + (size_t)h * N, D[0], j, q, q2);
+    ntt_inv<LOGN, 1, true, false>(acc, j, mine, c.tb);  // n^-1 is in the key rows
+    if (h == 0) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) p[r] = acc[0][r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            u64 v = p[r] & ~SIGN;
+            bool neg = (p[r] & SIGN) != 0;
+            p[r] = neg ? sub_mod(acc[0][r], v, q) : add_mod(acc[0][r], v, q);
+        }
+    }
+}
+
 }  // namespace pz

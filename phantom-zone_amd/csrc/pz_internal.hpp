@@ -32,10 +32,16 @@ void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const 
 void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_bytes, u32 rows,
                u32 n1, u32 ldk, size_t seg_begin, size_t seg_end, u32 *out, hipStream_t s);
 size_t br_lds_bytes(int logn, u32 br_q, u32 n_lwe);
-const void *blind_rotate_kernel(int logn);
-void launch_blind_rotate(int logn, int grid, const u32 *ks, const u32 *gate, size_t B, u64 *out,
-                         u32 *stats, const DevCtx &c, const DevKey &k, const DevKey *keys,
-                         const u32 *key_idx, const u32 *out_idx, hipStream_t s);
+const void *blind_rotate_kernel(int logn, bool split);
+bool br_split(const DevCtx &c);
+int br_threads(int logn, bool split);
+void launch_blind_rotate(int logn, bool split, int grid, const u32 *ks, const u32 *gate, size_t B,
+                         u64 *out, u32 *stats, const DevCtx &c, const DevKey &k,
+                         const DevKey *keys, const u32 *key_idx, const u32 *out_idx,
+                         hipStream_t s);
+constexpr int PROBE_CHAINS = 8;  // independent chains per thread in k_valu_probe
+void launch_valu_probe(int kind, int grid, u64 *out, u32 iters, u32 c, u64 w, u64 ws, u64 q,
+                       hipStream_t s);
 void launch_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q, hipStream_t s);
 void launch_shoup(const u64 *in, u64 *out, size_t n, u64 q, hipStream_t s);
 void launch_not(u64 *c0, size_t n, u64 q, hipStream_t s);

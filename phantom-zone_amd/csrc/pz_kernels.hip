@@ -274,6 +274,66 @@ __global__ void k_not(u64 *c0, size_t n, u64 q)
         c0[i] = neg_mod(c0[i], q);
 }
 
+// ---------------------------------------------------------------------------
+// k_valu_probe: issue-rate microbenchmark of the VALU instruction forms the
+// PBS kernels are built from.  Each thread runs PROBE_CH independent chains
+// for `iters` steps (an empty asm keeps every step live) and stores the xor
+// of its chains.  kind: 0 v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32,
+// 3 v_mad_u64_u32, 4 one 54-bit Shoup multiply (shoup_mul).
+// ---------------------------------------------------------------------------
+constexpr int PROBE_CH = 8;
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu_probe(u64 *out, u32 iters, u32 c, u64 w, u64 ws,
+                                                    u64 q)
+{
+    const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    u64 acc = 0;
+    if (KIND >= 3) {
+        u64 x[PROBE_CH];
+#pragma unroll
+        for (int k = 0; k < PROBE_CH; k++) x[k] = gid * 0x9e3779b97f4a7c15ull + (u64)k;
+        for (u32 i = 0; i < iters; i++) {
+#pragma unroll
+            for (int k = 0; k < PROBE_CH; k++) {
+                if (KIND == 3) x[k] = (u64)(u32)x[k] * c + x[k];
+                else x[k] = shoup_mul(x[k], w, ws, q);
+                asm volatile("" : "+v"(x[k]));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PROBE_CH; k++) acc ^= x[k];
+    } else {
+        u32 y[PROBE_CH];
+#pragma unroll
+        for (int k = 0; k < PROBE_CH; k++) y[k] = (u32)gid * 2654435761u + (u32)k;
+        for (u32 i = 0; i < iters; i++) {
+#pragma unroll
+            for (int k = 0; k < PROBE_CH; k++) {
+                if (KIND == 0) y[k] = y[k] + c;
+                else if (KIND == 1) y[k] = y[k] * c;
+                else y[k] = __umulhi(y[k], c);
+                asm volatile("" : "+v"(y[k]));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PROBE_CH; k++) acc ^= y[k];
+    }
+    out[gid] = acc;
+}
+
+void launch_valu_probe(int kind, int grid, u64 *out, u32 iters, u32 c, u64 w, u64 ws, u64 q,
+                       hipStream_t s)
+{
+    switch (kind) {
+    case 0: hipLaunchKernelGGL(k_valu_probe<0>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
+    case 1: hipLaunchKernelGGL(k_valu_probe<1>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
+    case 2: hipLaunchKernelGGL(k_valu_probe<2>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
+    case 3: hipLaunchKernelGGL(k_valu_probe<3>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
+    default: hipLaunchKernelGGL(k_valu_probe<4>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
+    }
+}
+
 // u32 LWE words -> u64
 __global__ void k_widen(const u32 *in, u64 *out, size_t n, u32 mask)
 {

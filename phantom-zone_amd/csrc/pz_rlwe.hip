@@ -10,6 +10,12 @@
 #ifndef PZ_BR_WAVES
 #define PZ_BR_WAVES 2
 #endif
+#ifndef PZ_BR_SPLIT
+#define PZ_BR_SPLIT 1
+#endif
+#ifndef PZ_BR_SPLIT_WAVES
+#define PZ_BR_SPLIT_WAVES 2
+#endif
 #define PZ_BR_ATTR __attribute__((amdgpu_waves_per_eu(PZ_BR_WAVES, PZ_BR_WAVES)))
 
 namespace pz {
@@ -77,18 +83,20 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *
 //   ks:   [B][n+1] key-switched LWE (mod q_ks)
 //   gate: [B] test-vector index
 //   out:  [B][N+1] output LWE (mod Q)
+// SPLIT = false: N/8 threads hold a and b (16 coefficients each, any d).
+// SPLIT = true:  2 x N/8 threads, half h holds polynomial h (d = 1 only).
 // LDS: 2 polynomials (exchange / permutation) + bucket starts + index list.
 // ---------------------------------------------------------------------------
-template <int LOGN>
-__global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(const u32 *ks, const u32 *gate,
-                                                                  size_t B, u64 *out, u32 *stats,
-                                                                  DevCtx c, DevKey key0,
-                                                                  const DevKey *keys,
-                                                                  const u32 *key_idx,
-                                                                  const u32 *out_idx)
+template <int LOGN, bool SPLIT>
+__global__ __launch_bounds__((SPLIT ? 2 : 1) << (LOGN - 3))
+__attribute__((amdgpu_waves_per_eu(SPLIT ? PZ_BR_SPLIT_WAVES : PZ_BR_WAVES,
+                                   SPLIT ? PZ_BR_SPLIT_WAVES : PZ_BR_WAVES))) void k_blind_rotate(
+    const u32 *ks, const u32 *gate, size_t B, u64 *out, u32 *stats, DevCtx c, DevKey key0,
+    const DevKey *keys, const u32 *key_idx, const u32 *out_idx)
 {
     constexpr int N = 1 << LOGN;
     constexpr int TPB = N / 8;
+    constexpr int THREADS = SPLIT ? 2 * TPB : TPB;
     constexpr int W = Geom<LOGN>::LDS_WORDS;
     extern __shared__ u64 lds[];
     const u32 nb = c.br_q >> 1;
@@ -97,7 +105,10 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
     u16 *bidx = bstart + nb + 1;                          // [n]
     u32 *cnt = reinterpret_cast<u32 *>(lds);               // scratch in poly area
     u16 *kbin = reinterpret_cast<u16 *>(cnt + nb + 1);     // scratch in poly area
-    const u32 j = threadIdx.x;
+    const u32 tid = threadIdx.x;
+    const u32 j = SPLIT ? tid % TPB : tid;
+    // half index, uniform over each wave (TPB is a multiple of 64)
+    const u32 h = SPLIT ? (u32)__builtin_amdgcn_readfirstlane(tid / TPB) : 0u;
     const u64 q = c.tb.q;
 
     for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
@@ -106,9 +117,9 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
         // ciphertext (sessions batched together)
         const DevKey key = keys ? keys[key_idx[bb]] : key0;
         // (3) odd moddown to br_q and dlog buckets (stable counting sort)
-        for (u32 i = j; i <= nb; i += TPB) cnt[i] = 0;
+        for (u32 i = tid; i <= nb; i += THREADS) cnt[i] = 0;
         __syncthreads();
-        for (u32 i = j; i < n; i += TPB) {
+        for (u32 i = tid; i < n; i += THREADS) {
             u32 v = in[1 + i] & (u32)c.lwe_mask;
             u32 odd = ((u32)(((u64)v * c.br_q) >> c.lwe_q_log)) | 1u;
             u32 k = c.dlog[odd];
@@ -116,7 +127,7 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
             atomicAdd(&cnt[k + 1], 1u);
         }
         __syncthreads();
-        if (j == 0) {
+        if (tid == 0) {
             u32 s = 0;
             for (u32 i = 0; i <= nb; i++) {
                 s += cnt[i];
@@ -138,26 +149,32 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
         // gate ids outside 0..5 are invalid input; clamp so memory stays in bounds
         const u32 gid = gate[bb] < 6 ? gate[bb] : 0;
         const u64 *tv = c.test_vec + (size_t)gid * nb;
-        u64 a[8], b[8];
+        u64 tvr[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             u32 x = coef_x<LOGN>(j, r);
-            a[r] = 0;
             u64 val = 0;
             if (x % c.embedding == 0) {
                 u32 y = x / c.embedding;
                 u32 src;
-                bool s;
+                bool sg;
                 if (y >= exp) {
                     src = y - exp;
-                    s = msign;
+                    sg = msign;
                 } else {
                     src = y + nb - exp;
-                    s = !msign;
+                    sg = !msign;
                 }
-                val = s ? tv[src] : neg_mod(tv[src], q);
+                val = sg ? tv[src] : neg_mod(tv[src], q);
             }
-            b[r] = val;
+            tvr[r] = val;
+        }
+        // accumulator RLWE (0, test vector): split halves hold a or b in p
+        u64 a[8], b[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            a[r] = SPLIT ? (h ? tvr[r] : 0) : 0;  // SPLIT: a[] is this half's polynomial
+            b[r] = tvr[r];
         }
 
         // (5) blind rotation, src/pbs.rs:240-414
@@ -165,48 +182,52 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_blind_rotate(con
         bool trivial = true;
         u32 n_auto = 0;
         const u32 qb4 = c.br_q >> 2;
+        auto ext = [&](size_t si) {
+            if constexpr (SPLIT)
+                ext_split<LOGN>(a, key.rgsw + si * rs, key.rgsw_s + si * rs, c, j, h, lds);
+            else
+                ext_product_blk<LOGN>(a, b, key.rgsw + si * rs, key.rgsw_s + si * rs, trivial, c,
+                                      j, lds);
+            trivial = false;
+        };
+        auto autom = [&](u32 v) {
+            if constexpr (SPLIT)
+                auto_split<LOGN>(a, key.auto_k + v * as, key.auto_s + v * as, c.auto_el[v],
+                                 trivial, c, j, h, lds);
+            else
+                auto_blk<LOGN>(a, b, key.auto_k + v * as, key.auto_s + v * as, c.auto_el[v],
+                               trivial, c, j, lds);
+            n_auto++;
+        };
         for (int half = 0; half < 2; half++) {
             const u32 off = half == 0 ? qb4 : 0;  // -(g^k) buckets first
             u32 v = 0;
             for (u32 i = qb4 - 1; i >= 1; i--) {
-                for (u32 t = bstart[off + i]; t < bstart[off + i + 1]; t++) {
-                    size_t si = bidx[t];
-                    ext_product_blk<LOGN>(a, b, key.rgsw + si * rs, key.rgsw_s + si * rs, trivial,
-                                          c, j, lds);
-                    trivial = false;
-                }
+                for (u32 t = bstart[off + i]; t < bstart[off + i + 1]; t++) ext(bidx[t]);
                 v += 1;
                 if (bstart[off + i] != bstart[off + i - 1] || v == c.w || i == 1) {
-                    auto_blk<LOGN>(a, b, key.auto_k + v * as, key.auto_s + v * as, c.auto_el[v],
-                                   trivial, c, j, lds);
-                    n_auto++;
+                    autom(v);
                     v = 0;
                 }
             }
             // -(g^0) then the -g automorphism / +(g^0) (no automorphism)
-            for (u32 t = bstart[off]; t < bstart[off + 1]; t++) {
-                size_t si = bidx[t];
-                ext_product_blk<LOGN>(a, b, key.rgsw + si * rs, key.rgsw_s + si * rs, trivial, c,
-                                      j, lds);
-                trivial = false;
-            }
-            if (half == 0) {
-                auto_blk<LOGN>(a, b, key.auto_k, key.auto_s, c.auto_el[0], trivial, c, j, lds);
-                n_auto++;
-            }
+            for (u32 t = bstart[off]; t < bstart[off + 1]; t++) ext(bidx[t]);
+            if (half == 0) autom(0);
         }
 
         // (6) sample_extract at index 0 (src/pbs.rs:423-449)
-        if (stats && j == 0) stats[bb] = n_auto;
+        if (stats && tid == 0) stats[bb] = n_auto;
         u64 *o = out + (out_idx ? (size_t)out_idx[bb] : bb) * (size_t)(N + 1);
+        const bool has_a = !SPLIT || h == 0, has_b = !SPLIT || h == 1;
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             u32 x = coef_x<LOGN>(j, r);
+            const u64 av = a[r], bv = SPLIT ? a[r] : b[r];
             if (x == 0) {
-                o[1] = a[r];
-                o[0] = b[r];
-            } else {
-                o[1 + N - x] = neg_mod(a[r], q);
+                if (has_a) o[1] = av;
+                if (has_b) o[0] = bv;
+            } else if (has_a) {
+                o[1 + N - x] = neg_mod(av, q);
             }
         }
         __syncthreads();
@@ -248,32 +269,49 @@ size_t br_lds_bytes(int logn, u32 br_q, u32 n_lwe)
     return poly + 2 * (size_t)(br_q / 2 + 1) + 2 * (size_t)n_lwe + 16;
 }
 
-template <int LOGN>
+// The split work group serves d_a = d_b = d_auto = 1 (every NI set).
+bool br_split(const DevCtx &c) { return PZ_BR_SPLIT && c.da == 1 && c.db == 1 && c.dauto == 1; }
+
+int br_threads(int logn, bool split) { return (split ? 2 : 1) << (logn - 3); }
+
+template <int LOGN, bool SPLIT>
 static const void *br_kernel_ptr()
 {
-    return reinterpret_cast<const void *>(&k_blind_rotate<LOGN>);
+    return reinterpret_cast<const void *>(&k_blind_rotate<LOGN, SPLIT>);
 }
 
-const void *blind_rotate_kernel(int logn)
+const void *blind_rotate_kernel(int logn, bool split)
 {
     switch (logn) {
-    case 9: return br_kernel_ptr<9>();
-    case 11: return br_kernel_ptr<11>();
+    case 9: return split ? br_kernel_ptr<9, true>() : br_kernel_ptr<9, false>();
+    case 11: return split ? br_kernel_ptr<11, true>() : br_kernel_ptr<11, false>();
     }
     return nullptr;
 }
 
-void launch_blind_rotate(int logn, int grid, const u32 *ks, const u32 *gate, size_t B, u64 *out,
-                         u32 *stats, const DevCtx &c, const DevKey &k, const DevKey *keys,
-                         const u32 *key_idx, const u32 *out_idx, hipStream_t s)
+void launch_blind_rotate(int logn, bool split, int grid, const u32 *ks, const u32 *gate, size_t B,
+                         u64 *out, u32 *stats, const DevCtx &c, const DevKey &k,
+                         const DevKey *keys, const u32 *key_idx, const u32 *out_idx,
+                         hipStream_t s)
 {
     if (B == 0) return;
     size_t lds = br_lds_bytes(logn, c.br_q, c.n_lwe);
-    int tpb = 1 << (logn - 3);
+    split = split && br_split(c);
+    const int tpb = br_threads(logn, split);
+#define PZ_BR_LAUNCH(L, S)                                                                       \
+    hipLaunchKernelGGL((k_blind_rotate<L, S>), grid, tpb, lds, s, ks, gate, B, out, stats, c, k, \
+                       keys, key_idx, out_idx)
     switch (logn) {
-    case 9: hipLaunchKernelGGL(k_blind_rotate<9>, grid, tpb, lds, s, ks, gate, B, out, stats, c, k, keys, key_idx, out_idx); break;
-    case 11: hipLaunchKernelGGL(k_blind_rotate<11>, grid, tpb, lds, s, ks, gate, B, out, stats, c, k, keys, key_idx, out_idx); break;
+    case 9:
+        if (split) PZ_BR_LAUNCH(9, true);
+        else PZ_BR_LAUNCH(9, false);
+        break;
+    case 11:
+        if (split) PZ_BR_LAUNCH(11, true);
+        else PZ_BR_LAUNCH(11, false);
+        break;
     }
+#undef PZ_BR_LAUNCH
 }
 
 }  // namespace pz

@@ -92,6 +92,8 @@ SIGNATURES = {
     "pz_set_timing": (C.c_int, [_vp, C.c_int]),
     "pz_last_timing": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                  C.POINTER(C.c_float)]),
+    "pz_valu_probe": (C.c_int, [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_double)]),
+    "pz_set_blind_rotate_mode": (C.c_int, [_vp, C.c_int]),
 }
 
 _lib = None
@@ -482,6 +484,17 @@ class Context:
 
     def set_timing(self, on):
         _check(self.L.pz_set_timing(self.h, 1 if on else 0))
+
+    def set_blind_rotate_mode(self, mode):
+        """0 automatic, 1 one work group per bootstrap, 2 two half groups."""
+        _check(self.L.pz_set_blind_rotate_mode(self.h, mode))
+
+    def valu_probe(self, kind, iters=4096):
+        """Measured rate of one VALU instruction form (kind 0-3, lane ops/s)
+        or of the Shoup multiply (kind 4, multiplies/s)."""
+        v = C.c_double()
+        _check(self.L.pz_valu_probe(self.h, kind, iters, C.byref(v)))
+        return v.value
 
     def last_timing(self):
         a, b, c = C.c_float(), C.c_float(), C.c_float()

@@ -3,7 +3,8 @@
 - The kernel-level batch: 4096 forward + inverse NTTs of uniform rows, and
   4096 RLWE x RGSW products of uniform RLWEs with a uniform evaluation-domain
   RGSW key (logB = 17, d_a = d_b = 1), every output word against the oracle.
-- Batched gate bootstraps at B = 1024 and 4096 (NAND / XOR mix, NI_2P):
+- Batched gate bootstraps at B = 256, 1024 and 4096 (NAND / XOR mix, NI_2P),
+  with the automatic kernel choice and with the split kernel forced:
   every output decrypts to the gate of the input bits, a sample of outputs
   equals the oracle word for word, and replays of the captured graph repeat
   the first launch exactly.
@@ -76,11 +77,12 @@ def test_ext_product_batch_4096_uniform_rgsw(ctx):
 PLAIN = {0: lambda a, b: 1 - (a & b), 4: lambda a, b: a ^ b}
 
 
-@pytest.mark.parametrize("B", [1024, 4096])
-def test_gates_full_batch(ctx, B):
+@pytest.mark.parametrize("B,mode", [(1024, 0), (4096, 0), (1024, 2), (256, 0)])
+def test_gates_full_batch(ctx, B, mode):
     o, s, z, key = keys_for("NI_2P")
     k = ctx.key_upload(key.rgsw, key.auto_key, key.lwe_ksk)
-    log("gates full batch B=%d" % B)
+    log("gates full batch B=%d mode=%d" % (B, mode))
+    ctx.set_blind_rotate_mode(mode)
     r = rng(103 + B)
     b0 = r.integers(0, 2, size=B).astype(np.uint8)
     b1 = r.integers(0, 2, size=B).astype(np.uint8)
@@ -99,4 +101,5 @@ def test_gates_full_batch(ctx, B):
     for _ in range(2):  # graph replays
         ctx.gates(k, dops, d0, d1, dout, B)
         assert np.array_equal(dout.download(), first)
+    ctx.set_blind_rotate_mode(0)
     k.free()

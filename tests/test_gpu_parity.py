@@ -15,6 +15,9 @@ pytestmark = pytest.mark.gpu
 
 # NI_4P_HB_FR: br_q = 2048 (embedding factor 2); NI_8P: q_ks = 2^17 (32-bit KSK)
 PARAMS = ["NI_2P", "SP_TEST", "NI_4P_HB_FR", "NI_8P"]
+# blind-rotation kernels: 1 = one work group per bootstrap, 2 = two half
+# groups (d = 1 sets only); both must give the same words
+MODES = [(p, m) for p in PARAMS for m in ((1,) if p == "SP_TEST" else (1, 2))]
 _ctx = {}
 _dkeys = {}
 
@@ -189,13 +192,14 @@ PLAIN = {0: lambda a, b: 1 - (a & b), 1: lambda a, b: a & b, 2: lambda a, b: a |
          3: lambda a, b: 1 - (a | b), 4: lambda a, b: a ^ b, 5: lambda a, b: 1 - (a ^ b)}
 
 
-@pytest.mark.parametrize("pname", PARAMS)
-def test_pbs(pname):
+@pytest.mark.parametrize("pname,mode", MODES)
+def test_pbs(pname, mode):
     o, s, z, key = keys_for(pname)
     c = ctx_for(pname)
     k = dkey_for(pname)
     B = 6 if o.N == 2048 else 48
-    log("pbs %s B=%d" % (pname, B))
+    log("pbs %s mode=%d B=%d" % (pname, mode, B))
+    c.set_blind_rotate_mode(mode)
     r = rng(51)
     x = uniform_mod(r, o.Q, (B, o.N + 1))
     x[0, :8] = o.Q - 1  # coefficients whose f64 moddown rounds to q_ks
@@ -203,16 +207,18 @@ def test_pbs(pname):
     want = np.stack([o.pbs(key, int(gate[b]), x[b])[0] for b in range(B)])
     d = c.to_device(x)
     c.pbs(k, c.to_device(gate), d, B)
+    c.set_blind_rotate_mode(0)
     assert np.array_equal(d.download(), want)
 
 
-@pytest.mark.parametrize("pname", PARAMS)
-def test_gates_truth_and_parity(pname):
+@pytest.mark.parametrize("pname,mode", MODES)
+def test_gates_truth_and_parity(pname, mode):
     o, s, z, key = keys_for(pname)
     c = ctx_for(pname)
     k = dkey_for(pname)
     B = 12 if o.N == 2048 else 96
-    log("gates %s B=%d" % (pname, B))
+    log("gates %s mode=%d B=%d" % (pname, mode, B))
+    c.set_blind_rotate_mode(mode)
     bits0, bits1, c0, c1 = _gate_inputs(o, s, B, 61)
     ops = (np.arange(B) % 6).astype(np.uint32)
     want = np.stack([o.gate(key, int(ops[b]), c0[b], c1[b]) for b in range(B)])
@@ -222,5 +228,14 @@ def test_gates_truth_and_parity(pname):
         c.gates(k, dops, d0, d1, dout, B)
         got = dout.download()
         assert np.array_equal(got, want), "replay %d" % rep
+    c.set_blind_rotate_mode(0)
     for b in range(B):
         assert o.decrypt_bool(s, got[b]) == PLAIN[int(ops[b])](int(bits0[b]), int(bits1[b]))
+
+
+def test_split_mode_needs_single_digit_decompositions():
+    import pzlib
+    c = ctx_for("SP_TEST")
+    with pytest.raises(pzlib.PzError):
+        c.set_blind_rotate_mode(2)
+    c.set_blind_rotate_mode(0)
