@@ -109,12 +109,31 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
     int rc = pz_internal_exec_view(c, B ? B : 1, &v, &dc);
     if (rc) return rc;
     const u32 N1 = v.p.rlwe_n + 1, n1 = v.p.lwe_n + 1;
-    // staging: ops | in0 | in1 | out | key | not_in | not_out
+    // staging: ops | in0 | in1 | out | key | not_in | not_out | KS work list
     const size_t nn = nots.size();
+    // key-switch work list for batches that mix keys: one entry per
+    // (ciphertext tile, key segment) -- a tile that straddles two keys'
+    // segments appears once for each
+    std::vector<u32> work;
+    if (keys.size() > 1) {
+        size_t s0 = 0;
+        while (s0 < B) {
+            size_t e0 = s0;
+            while (e0 < B && pbs[e0].key == pbs[s0].key) e0++;
+            for (size_t t = s0 / KS_CT; t <= (e0 - 1) / KS_CT; t++) {
+                work.push_back((u32)t);
+                work.push_back((u32)s0);
+                work.push_back((u32)e0);
+                work.push_back(pbs[s0].key);
+            }
+            s0 = e0;
+        }
+    }
+    const size_t nw = work.size();
     // the previous batch's staging copies read the pinned buffer
     // asynchronously: wait for them before rewriting it
     if (hipStreamSynchronize(st) != hipSuccess) return PZ_ERR_HIP;
-    if (!g_stage.reserve(5 * B + 2 * nn + 8, keys.size())) return PZ_ERR_HIP;
+    if (!g_stage.reserve(5 * B + 2 * nn + nw + 8, keys.size())) return PZ_ERR_HIP;
     u32 *h = g_stage.h;
     for (size_t i = 0; i < B; i++) {
         h[i] = pbs[i].op;
@@ -127,9 +146,10 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
         h[5 * B + i] = nots[i].in0;
         h[5 * B + nn + i] = nots[i].out;
     }
+    if (nw) memcpy(h + 5 * B + 2 * nn, work.data(), 4 * nw);
     for (size_t k = 0; k < keys.size(); k++) g_stage.hk[k] = *pz_internal_devkey(keys[k]);
     u32 *d = g_stage.d;
-    if (hipMemcpyAsync(d, h, 4 * (5 * B + 2 * nn), hipMemcpyHostToDevice, st) != hipSuccess)
+    if (hipMemcpyAsync(d, h, 4 * (5 * B + 2 * nn + nw), hipMemcpyHostToDevice, st) != hipSuccess)
         return PZ_ERR_HIP;
     if (keys.size() > 1 &&
         hipMemcpyAsync(g_stage.dk, g_stage.hk, sizeof(DevKey) * keys.size(),
@@ -138,18 +158,15 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
     if (B) {
         launch_prep(PREP_GATE, pool, pool, d + B, d + 2 * B, d, N1, B, v.digits, v.ksb, v.ks, dc,
                     st);
-        size_t s0 = 0;
-        while (s0 < B) {
-            size_t e0 = s0;
-            while (e0 < B && pbs[e0].key == pbs[s0].key) e0++;
-            const DevKey *k = pz_internal_devkey(keys[pbs[s0].key]);
-            launch_ks(v.digits, v.ksb, k->ksk, k->ksk_bytes, v.p.rlwe_n * v.p.lwe_d, n1, k->ldk,
-                      s0, e0, v.ks, st);
-            s0 = e0;
-        }
+        const DevKey *k0 = pz_internal_devkey(keys[pbs[0].key]);
+        if (keys.size() > 1)
+            launch_ks_multi(v.digits, v.ksb, g_stage.dk, d + 5 * B + 2 * nn, nw / 4,
+                            k0->ksk_bytes, v.p.rlwe_n * v.p.lwe_d, n1, k0->ldk, v.ks, st);
+        else
+            launch_ks(v.digits, v.ksb, k0->ksk, k0->ksk_bytes, v.p.rlwe_n * v.p.lwe_d, n1,
+                      k0->ldk, 0, B, v.ks, st);
         int split = 0, grid = 1;
         pz_internal_br_plan(c, B, &split, &grid);
-        const DevKey *k0 = pz_internal_devkey(keys[pbs[0].key]);
         launch_blind_rotate(v.logn, split != 0, grid, v.ks, d, B, pool, v.stats, dc, *k0,
                             keys.size() > 1 ? g_stage.dk : nullptr,
                             keys.size() > 1 ? d + 4 * B : nullptr, d + 3 * B, st);

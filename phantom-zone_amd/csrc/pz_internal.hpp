@@ -35,6 +35,9 @@ size_t br_lds_bytes(int logn, u32 br_q, u32 n_lwe);
 const void *blind_rotate_kernel(int logn, bool split);
 bool br_split(const DevCtx &c);
 int br_threads(int logn, bool split);
+void launch_ks_multi(const int8_t *digits, const u32 *ks_b, const DevKey *keys, const u32 *work,
+                     size_t n_work, u32 ksk_bytes, u32 rows, u32 n1, u32 ldk, u32 *out,
+                     hipStream_t s);
 void launch_blind_rotate(int logn, bool split, int grid, const u32 *ks, const u32 *gate, size_t B,
                          u64 *out, u32 *stats, const DevCtx &c, const DevKey &k,
                          const DevKey *keys, const u32 *key_idx, const u32 *out_idx,

@@ -139,14 +139,24 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 template <bool PACK16>
 __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks_b,
                                              const void *ksk, u32 rows, u32 n1, u32 ldk,
-                                             size_t seg_begin, size_t seg_end, u32 *out)
+                                             size_t seg_begin, size_t seg_end, u32 *out,
+                                             const u32 *work, const DevKey *keys)
 {
     __shared__ __attribute__((aligned(16))) u32 dig[KS_ROWS][KS_CT];
     const u32 unit = threadIdx.x;
     const u32 units = PACK16 ? (n1 + 1) / 2 : n1;
     // this group's tile of KS_CT ciphertexts; only those in [seg_begin, seg_end)
-    // (the ciphertexts of this key) are accumulated
-    const size_t tile_id = seg_begin / KS_CT + blockIdx.x;
+    // (the ciphertexts of this key) are accumulated.  With a work list, entry
+    // blockIdx.x names the tile, the segment and the key (batches that mix
+    // server keys: one launch for every key).
+    size_t tile_id = seg_begin / KS_CT + blockIdx.x;
+    if (work) {
+        const u32 *e = work + 4 * (size_t)blockIdx.x;
+        tile_id = e[0];
+        seg_begin = e[1];
+        seg_end = e[2];
+        ksk = keys[e[3]].ksk;
+    }
     const size_t b0 = tile_id * KS_CT;
     const int8_t *tile = digits + tile_id * rows * KS_CT;
     const u32 per = (rows + gridDim.y - 1) / gridDim.y;
@@ -428,10 +438,33 @@ void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_b
     dim3 grid((unsigned)tiles, (unsigned)split);
     if (pack)
         hipLaunchKernelGGL(k_ks<true>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk,
-                           seg_begin, seg_end, out);
+                           seg_begin, seg_end, out, nullptr, nullptr);
     else
         hipLaunchKernelGGL(k_ks<false>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk,
-                           seg_begin, seg_end, out);
+                           seg_begin, seg_end, out, nullptr, nullptr);
+}
+
+// One key-switch launch over a work list of n_work (tile, segment begin,
+// segment end, key index) entries; keys[i].ksk are KSKs of one layout.
+void launch_ks_multi(const int8_t *digits, const u32 *ks_b, const DevKey *keys, const u32 *work,
+                     size_t n_work, u32 ksk_bytes, u32 rows, u32 n1, u32 ldk, u32 *out,
+                     hipStream_t s)
+{
+    if (n_work == 0) return;
+    const bool pack = ksk_bytes == 2;
+    u32 units = pack ? (n1 + 1) / 2 : n1;
+    u32 tpb = (units + 63) / 64 * 64;
+    size_t split = (KS_GROUPS + n_work - 1) / n_work;
+    size_t max_split = rows / KS_ROWS > 0 ? rows / KS_ROWS : 1;
+    if (split > max_split) split = max_split;
+    if (split < KS_SPLIT) split = KS_SPLIT;
+    dim3 grid((unsigned)n_work, (unsigned)split);
+    if (pack)
+        hipLaunchKernelGGL(k_ks<true>, grid, tpb, 0, s, digits, ks_b, nullptr, rows, n1, ldk,
+                           (size_t)0, (size_t)0, out, work, keys);
+    else
+        hipLaunchKernelGGL(k_ks<false>, grid, tpb, 0, s, digits, ks_b, nullptr, rows, n1, ldk,
+                           (size_t)0, (size_t)0, out, work, keys);
 }
 
 void launch_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q, hipStream_t s)
