@@ -17,18 +17,44 @@ typedef uint32_t u32;
 // v_mad_u64_u32 plus one v_mul_hi_u32 on CDNA.
 PZ_DEV u64 mulhi64(u64 a, u64 b) { return __umul64hi(a, b); }
 
+// The two halves of the Shoup step, as 32-bit products for v_mad_u64_u32.
+// shoup_quot: floor(ws*a / 2^64) (three middle/high partial products and the
+// high half of the low one, carries kept exact).
+PZ_DEV u64 shoup_quot(u64 a, u64 ws)
+{
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), s0 = (u32)ws, s1 = (u32)(ws >> 32);
+    const u64 t = (u64)s0 * a1 + __umulhi(s0, a0);
+    const u64 u = (u64)s1 * a0 + (u32)t;
+    return (u64)s1 * a1 + (t >> 32) + (u >> 32);
+}
+// c + w*a - q*h (mod 2^64), written as c + w*a + (2^64 - q)*h: the same word,
+// with the high cross products folded into one 32-bit sum.
+PZ_DEV u64 shoup_rem(u64 c, u64 a, u64 w, u64 h, u64 q)
+{
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32);
+    const u64 nq = 0 - q;
+    const u32 h0 = (u32)h, h1 = (u32)(h >> 32);
+    const u32 m = (u32)w * a1 + (u32)(w >> 32) * a0 + (u32)nq * h1 + (u32)(nq >> 32) * h0;
+    return (u64)(u32)w * a0 + (u64)(u32)nq * h0 + c + ((u64)m << 32);
+}
+
 // ShoupMul::mul, src/utils.rs:43-46: b*a - q*floor(b_shoup*a/2^64) (wrapping),
 // result in [0, 2q) for any a < 2^64.
+// SPLIT selects the two-step form above (two fewer VALU instructions per
+// multiply, more registers live); both return the same word.
+template <bool SPLIT = true>
 PZ_DEV u64 shoup_mul(u64 a, u64 w, u64 ws, u64 q)
 {
-    return w * a - q * mulhi64(ws, a);
+    if constexpr (SPLIT) return shoup_rem(0, a, w, shoup_quot(a, ws), q);
+    else return w * a - q * mulhi64(ws, a);
 }
 
 // forward_butterly_0_to_4q, src/ntt.rs:31-49
+template <bool SPLIT = true>
 PZ_DEV void fb_4q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)
 {
     u64 xx = x >= q2 ? x - q2 : x;
-    u64 t = shoup_mul(y, w, ws, q);
+    u64 t = shoup_mul<SPLIT>(y, w, ws, q);
     x = xx + t;
     y = xx + q2 - t;
 }
@@ -36,23 +62,25 @@ PZ_DEV void fb_4q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)
 PZ_DEV u64 umin(u64 a, u64 b) { return a < b ? a : b; }
 
 // forward_butterly_0_to_2q, src/ntt.rs:51-75
+template <bool SPLIT = true>
 PZ_DEV void fb_2q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)
 {
     u64 xx = x >= q2 ? x - q2 : x;
-    u64 t = shoup_mul(y, w, ws, q);
+    u64 t = shoup_mul<SPLIT>(y, w, ws, q);
     u64 ox = xx + t, oy = xx - t;
     x = umin(ox, ox - q2);
     y = umin(oy, oy + q2);
 }
 
 // inverse_butterfly_0_to_2q, src/ntt.rs:84-104
+template <bool SPLIT = true>
 PZ_DEV void ib_2q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)
 {
     u64 xd = x + y;
     xd = xd >= q2 ? xd - q2 : xd;
     u64 t = x + q2 - y;
     x = xd;
-    y = shoup_mul(t, w, ws, q);
+    y = shoup_mul<SPLIT>(t, w, ws, q);
 }
 
 // ModularOpsU64 add_mod_fast / sub_mod_fast / neg (src/backend/modulus_u64.rs:46-77,
@@ -67,11 +95,12 @@ PZ_DEV u64 neg_mod(u64 a, u64 q) { return q - a; }
 
 // One Shoup multiply-accumulate step of shoup_matrix_fma
 // (src/backend/modulus_u64.rs:252-260): out in [0,2q) -> [0,2q).
+template <bool SPLIT = true>
 PZ_DEV u64 shoup_fma(u64 out, u64 a, u64 as, u64 b, u64 q, u64 q2)
 {
-    u64 quot = mulhi64(as, b);
-    u64 v = a * b + out;
-    v = v - q * quot;
+    u64 v;
+    if constexpr (SPLIT) v = shoup_rem(out, b, a, shoup_quot(b, as), q);  // a*b + out - q*quot
+    else v = a * b + out - q * mulhi64(as, b);
     return v >= q2 ? v - q2 : v;
 }
 

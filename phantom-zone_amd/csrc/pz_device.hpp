@@ -58,6 +58,7 @@ PZ_DEV void load8(const u64 *__restrict__ p, u64 (&v)[8])
 }
 
 // out[r] = shoup_fma(out[r], K[8j+r], Ks[8j+r], D[r]) for the thread's slice
+template <bool SPLIT>
 PZ_DEV void fma_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
                     const u64 (&D)[8], u32 j, u64 q, u64 q2)
 {
@@ -70,7 +71,7 @@ PZ_DEV void fma_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restr
     load8(K + 8 * j, kv);
     load8(Ks + 8 * j, ks);
 #pragma unroll
-    for (int r = 0; r < 8; r++) acc[r] = shoup_fma(acc[r], kv[r], ks[r], D[r], q, q2);
+    for (int r = 0; r < 8; r++) acc[r] = shoup_fma<SPLIT>(acc[r], kv[r], ks[r], D[r], q, q2);
 }
 
 // rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) on the RLWE held in
@@ -94,20 +95,20 @@ PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
             D[0][r] = decomp_digit(c.dec_a, a[r], 0);
             D[1][r] = decomp_digit(c.dec_b, b[r], 0);
         }
-        ntt_fwd<LOGN, 2, PZ_FWD_MODE>(D, j, lds, c.tb);
-        fma_row(acc[0], K + 0 * N, Ks + 0 * N, D[0], j, q, q2);
-        fma_row(acc[0], K + 2 * N, Ks + 2 * N, D[1], j, q, q2);
-        fma_row(acc[1], K + 1 * N, Ks + 1 * N, D[0], j, q, q2);
-        fma_row(acc[1], K + 3 * N, Ks + 3 * N, D[1], j, q, q2);
+        ntt_fwd<LOGN, 2, PZ_FWD_MODE, false>(D, j, lds, c.tb);
+        fma_row<false>(acc[0], K + 0 * N, Ks + 0 * N, D[0], j, q, q2);
+        fma_row<false>(acc[0], K + 2 * N, Ks + 2 * N, D[1], j, q, q2);
+        fma_row<false>(acc[1], K + 1 * N, Ks + 1 * N, D[0], j, q, q2);
+        fma_row<false>(acc[1], K + 3 * N, Ks + 3 * N, D[1], j, q, q2);
     } else {
         if (!trivial) {
             for (u32 rr = 0; rr < da; rr++) {
                 u64 D[1][8];
 #pragma unroll
                 for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_a, a[r], rr);
-                ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, lds, c.tb);
-                fma_row(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
-                fma_row(acc[1], K + (size_t)(da + rr) * N, Ks + (size_t)(da + rr) * N, D[0], j,
+                ntt_fwd<LOGN, 1, PZ_FWD_MODE, false>(D, j, lds, c.tb);
+                fma_row<false>(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
+                fma_row<false>(acc[1], K + (size_t)(da + rr) * N, Ks + (size_t)(da + rr) * N, D[0], j,
                         q, q2);
             }
         }
@@ -115,13 +116,13 @@ PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
             u64 D[1][8];
 #pragma unroll
             for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_b, b[r], rr);
-            ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, lds, c.tb);
+            ntt_fwd<LOGN, 1, PZ_FWD_MODE, false>(D, j, lds, c.tb);
             size_t o0 = (size_t)(2 * da + rr) * N, o1 = (size_t)(2 * da + db + rr) * N;
-            fma_row(acc[0], K + o0, Ks + o0, D[0], j, q, q2);
-            fma_row(acc[1], K + o1, Ks + o1, D[0], j, q, q2);
+            fma_row<false>(acc[0], K + o0, Ks + o0, D[0], j, q, q2);
+            fma_row<false>(acc[1], K + o1, Ks + o1, D[0], j, q, q2);
         }
     }
-    ntt_inv<LOGN, 2, true, false>(acc, j, lds, c.tb);  // n^-1 is in the key rows
+    ntt_inv<LOGN, 2, true, false, false>(acc, j, lds, c.tb);  // n^-1 is in the key rows
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         a[r] = acc[0][r];
@@ -181,11 +182,11 @@ PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
         u64 D[1][8];
 #pragma unroll
         for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_auto, a[r], rr);
-        ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, lds, c.tb);
-        fma_row(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
-        fma_row(acc[1], K + (size_t)(d + rr) * N, Ks + (size_t)(d + rr) * N, D[0], j, q, q2);
+        ntt_fwd<LOGN, 1, PZ_FWD_MODE, false>(D, j, lds, c.tb);
+        fma_row<false>(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
+        fma_row<false>(acc[1], K + (size_t)(d + rr) * N, Ks + (size_t)(d + rr) * N, D[0], j, q, q2);
     }
-    ntt_inv<LOGN, 2, true, false>(acc, j, lds, c.tb);  // n^-1 is in the key rows
+    ntt_inv<LOGN, 2, true, false, false>(acc, j, lds, c.tb);  // n^-1 is in the key rows
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         u64 v = bp[r] & ~SIGN;
@@ -242,11 +243,11 @@ PZ_DEV void ext_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__restr
     for (int r = 0; r < 8; r++) acc[0][r] = 0;
     const size_t ra = (size_t)h * N, rb = (size_t)(2 + h) * N;  // rows for D_a, D_b
     if (h == 0) {
-        fma_row(acc[0], K + ra, Ks + ra, D[0], j, q, q2);
-        fma_row(acc[0], K + rb, Ks + rb, E, j, q, q2);
+        fma_row<true>(acc[0], K + ra, Ks + ra, D[0], j, q, q2);
+        fma_row<true>(acc[0], K + rb, Ks + rb, E, j, q, q2);
     } else {
-        fma_row(acc[0], K + ra, Ks + ra, E, j, q, q2);
-        fma_row(acc[0], K + rb, Ks + rb, D[0], j, q, q2);
+        fma_row<true>(acc[0], K + ra, Ks + ra, E, j, q, q2);
+        fma_row<true>(acc[0], K + rb, Ks + rb, D[0], j, q, q2);
     }
     ntt_inv<LOGN, 1, true, false>(acc, j, mine, c.tb);  // n^-1 is in the key rows
 #pragma unroll
@@ -306,7 +307,7 @@ PZ_DEV void auto_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     u64 acc[1][8];
 #pragma unroll
     for (int r = 0; r < 8; r++) acc[0][r] = 0;
-    fma_row(acc[0], K + (size_t)h * N, Ks + (size_t)h * N, D[0], j, q, q2);
+    fma_row<true>(acc[0], K + (size_t)h * N, Ks + (size_t)h * N, D[0], j, q, q2);
     ntt_inv<LOGN, 1, true, false>(acc, j, mine, c.tb);  // n^-1 is in the key rows
     if (h == 0) {
 #pragma unroll

@@ -68,8 +68,10 @@ enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2 };
 
 // One forward stage S inside a phase of stride ST. KIND: 0 = 0_to_4q
 // butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q),
-// 3 = unreduced (FWD_FAST).
-template <int LOGN, int S, int ST, int NP, int KIND>
+// 3 = unreduced (FWD_FAST).  SM selects the two-step Shoup form
+// (shoup_mul<true>, fewer instructions); the one-step form has fewer live
+// registers, which keeps the one-work-group blind rotation within 256 VGPRs.
+template <int LOGN, int S, int ST, int NP, int KIND, bool SM>
 PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 {
     constexpr int N = 1 << LOGN;
@@ -86,14 +88,14 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 #pragma unroll
         for (int p = 0; p < NP; p++) {
             if (KIND == 0) {
-                fb_4q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
+                fb_4q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
             } else if (KIND == 3) {
                 const u64 x = v[p][r];
-                const u64 t = shoup_mul(v[p][r + LD], w, ws, tb.q);
+                const u64 t = shoup_mul<SM>(v[p][r + LD], w, ws, tb.q);
                 v[p][r] = x + t;
                 v[p][r + LD] = x + tb.q2 - t;
             } else {
-                fb_2q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
+                fb_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
                 if (KIND == 2) {
                     v[p][r] = umin(v[p][r], v[p][r] - tb.q);
                     v[p][r + LD] = umin(v[p][r + LD], v[p][r + LD] - tb.q);
@@ -107,7 +109,7 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 // which also scales by n^-1 when SCALE (and reduces to [0,q) when STRICT).
 // SCALE = false is for products whose key rows already carry the n^-1
 // factor: the result is the same residue, and with STRICT the same word.
-template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE>
+template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE, bool SM>
 PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 {
     constexpr int N = 1 << LOGN;
@@ -123,13 +125,13 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
         const u64 w = tw.x, ws = tw.y;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
-            ib_2q(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
+            ib_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
             if (S == 0 && !SCALE && STRICT) {
                 v[p][r] = umin(v[p][r], v[p][r] - tb.q);
                 v[p][r + LD] = umin(v[p][r + LD], v[p][r + LD] - tb.q);
             } else if (S == 0 && SCALE) {
-                u64 ox = shoup_mul(v[p][r], tb.n_inv, tb.n_inv_s, tb.q);
-                u64 oy = shoup_mul(v[p][r + LD], tb.n_inv, tb.n_inv_s, tb.q);
+                u64 ox = shoup_mul<SM>(v[p][r], tb.n_inv, tb.n_inv_s, tb.q);
+                u64 oy = shoup_mul<SM>(v[p][r + LD], tb.n_inv, tb.n_inv_s, tb.q);
                 if (STRICT) {
                     ox = umin(ox, ox - tb.q);
                     oy = umin(oy, oy - tb.q);
@@ -142,7 +144,7 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 }
 
 // Forward phase starting at stage S0 (up to three stages).
-template <int LOGN, int S0, int NP, int MODE>
+template <int LOGN, int S0, int NP, int MODE, bool SM>
 PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
 {
     using P = Phase<LOGN, S0>;
@@ -150,22 +152,22 @@ PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
     constexpr int LAST = LOGN - 1;
     constexpr int K_LAST = MODE == FWD_STRICT ? 2 : (MODE == FWD_LAZY ? 1 : 3);
     constexpr int K_MID = MODE == FWD_FAST ? 3 : 0;
-    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID>(v, base, tb);
+    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM>(v, base, tb);
     if constexpr (P::NS > 1)
-        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID>(v, base, tb);
+        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID, SM>(v, base, tb);
     if constexpr (P::NS > 2)
-        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID>(v, base, tb);
+        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID, SM>(v, base, tb);
 }
 
 // Inverse phase covering stages S0+NS-1 .. S0 (descending).
-template <int LOGN, int S0, int NP, bool STRICT, bool SCALE>
+template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM>
 PZ_DEV void inv_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
 {
     using P = Phase<LOGN, S0>;
     const u32 base = P::base(j);
-    if constexpr (P::NS > 2) inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE>(v, base, tb);
-    if constexpr (P::NS > 1) inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE>(v, base, tb);
-    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE>(v, base, tb);
+    if constexpr (P::NS > 2) inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM>(v, base, tb);
+    if constexpr (P::NS > 1) inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM>(v, base, tb);
+    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM>(v, base, tb);
 }
 
 // Store the thread's coefficients (layout of phase SA) into LDS, barrier,
@@ -193,10 +195,21 @@ PZ_DEV void exchange(u64 (&v)[NP][8], u32 j, u64 *lds)
 #ifndef PZ_TWIDDLE_RELOAD
 #define PZ_TWIDDLE_RELOAD 1
 #endif
+// The pointers are uniform; readfirstlane states it where the compiler cannot
+// prove it, so the asm's scalar-register operands always hold.
+PZ_DEV const u64x2 *uniform_ptr(const u64x2 *p)
+{
+    const u64 v = (u64)p;
+    const u32 lo = (u32)__builtin_amdgcn_readfirstlane((int)(u32)v);
+    const u32 hi = (u32)__builtin_amdgcn_readfirstlane((int)(u32)(v >> 32));
+    return (const u64x2 *)(((u64)hi << 32) | lo);
+}
 PZ_DEV Tables fresh(const Tables &tb)
 {
     Tables t = tb;
 #if PZ_TWIDDLE_RELOAD
+    t.fw = uniform_ptr(t.fw);
+    t.iw = uniform_ptr(t.iw);
     asm volatile("" : "+s"(t.fw), "+s"(t.iw));
 #endif
     return t;
@@ -205,37 +218,37 @@ PZ_DEV Tables fresh(const Tables &tb)
 // Forward NTT of NP polynomials: stride layout in, contiguous layout out.
 // MODE FWD_STRICT = Ntt::forward (output in [0,q)), FWD_LAZY = forward_lazy
 // ([0,2q)), FWD_FAST = unreduced (see above).
-template <int LOGN, int NP, int MODE>
+template <int LOGN, int NP, int MODE, bool SM = true>
 PZ_DEV void ntt_fwd(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
     static_assert(MODE != FWD_FAST || LOGN <= 11, "unreduced values must stay below 2^64");
-    fwd_phase<LOGN, 0, NP, MODE>(v, j, fresh(tb));
+    fwd_phase<LOGN, 0, NP, MODE, SM>(v, j, fresh(tb));
     exchange<LOGN, 0, 3, NP>(v, j, lds);
-    fwd_phase<LOGN, 3, NP, MODE>(v, j, fresh(tb));
+    fwd_phase<LOGN, 3, NP, MODE, SM>(v, j, fresh(tb));
     exchange<LOGN, 3, 6, NP>(v, j, lds);
-    fwd_phase<LOGN, 6, NP, MODE>(v, j, fresh(tb));
+    fwd_phase<LOGN, 6, NP, MODE, SM>(v, j, fresh(tb));
     if constexpr (LOGN > 9) {
         exchange<LOGN, 6, 9, NP>(v, j, lds);
-        fwd_phase<LOGN, 9, NP, MODE>(v, j, fresh(tb));
+        fwd_phase<LOGN, 9, NP, MODE, SM>(v, j, fresh(tb));
     }
 }
 
 // Inverse NTT of NP polynomials: contiguous layout in, stride layout out.
 // STRICT = Ntt::backward ([0,q)), else backward_lazy ([0,2q)); SCALE = false
 // leaves out the n^-1 factor (see inv_stage).
-template <int LOGN, int NP, bool STRICT, bool SCALE = true>
+template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true>
 PZ_DEV void ntt_inv(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
 {
     if constexpr (LOGN > 9) {
-        inv_phase<LOGN, 9, NP, STRICT, SCALE>(v, j, fresh(tb));
+        inv_phase<LOGN, 9, NP, STRICT, SCALE, SM>(v, j, fresh(tb));
         exchange<LOGN, 9, 6, NP>(v, j, lds);
     }
-    inv_phase<LOGN, 6, NP, STRICT, SCALE>(v, j, fresh(tb));
+    inv_phase<LOGN, 6, NP, STRICT, SCALE, SM>(v, j, fresh(tb));
     exchange<LOGN, 6, 3, NP>(v, j, lds);
-    inv_phase<LOGN, 3, NP, STRICT, SCALE>(v, j, fresh(tb));
+    inv_phase<LOGN, 3, NP, STRICT, SCALE, SM>(v, j, fresh(tb));
     exchange<LOGN, 3, 0, NP>(v, j, lds);
-    inv_phase<LOGN, 0, NP, STRICT, SCALE>(v, j, fresh(tb));
+    inv_phase<LOGN, 0, NP, STRICT, SCALE, SM>(v, j, fresh(tb));
 }
 
 // Coefficient index of register r in the stride (coefficient-domain) layout
