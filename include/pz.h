@@ -244,12 +244,13 @@ int pz_server_stats(pz_server *s, uint64_t *total_pbs, uint64_t *batches, uint64
                     uint64_t *evictions, double *wall_s);
 
 /* Blind-rotation kernel choice.  No counterpart in the reference (its
- * evaluator is single-threaded): 0 = automatic (two half work groups per
- * bootstrap while every bootstrap of the batch has a CU of its own, for the
- * lower latency; one work group per bootstrap beyond that, for the higher
- * throughput), 1 = always one work group, 2 = always two (only where
- * d_a = d_b = d_auto = 1; PZ_ERR_UNSUPPORTED otherwise).  Outputs are the
- * same words in every mode. */
+ * evaluator is single-threaded): 0 = automatic, 1 = one work group per
+ * bootstrap, 2 = two half work groups per bootstrap (only where
+ * d_a = d_b = d_auto = 1; PZ_ERR_UNSUPPORTED otherwise).  Automatic is 2
+ * where available, else 1; mode 2 runs its latency variant while every
+ * bootstrap of a batch has a CU of its own and its throughput variant (two
+ * bootstraps per CU) beyond that.  Outputs are the same words in every
+ * mode. */
 int pz_set_blind_rotate_mode(pz_ctx *ctx, int mode);
 
 /* Automorphism count of each bootstrap of the last pz_pbs / pz_gates batch

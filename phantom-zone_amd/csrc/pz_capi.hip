@@ -57,8 +57,9 @@ struct pz_ctx {
     int8_t *d_digits = nullptr;
     u32 *d_ksb = nullptr, *d_ks = nullptr, *d_stats = nullptr;
     size_t cap = 0;
-    int num_cu = 0, br_per_cu = 0;
-    int br_split_per_cu = 0;  // resident split work groups per CU (0: not available)
+    int num_cu = 0;
+    int br_per_cu[3] = {0, 0, 0};  // resident work groups per CU of each kernel variant
+                                   // (BR_ONE, BR_SPLIT_LAT, BR_SPLIT; 0: not available)
     int br_mode = 0;          // 0 auto, 1 one-half kernel, 2 split kernel
     // captured gate batches
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -212,20 +213,15 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
-    int per = 0;
-    size_t lds = br_lds_bytes(logn, p->br_q, p->lwe_n);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, blind_rotate_kernel(logn, false),
-                                                     br_threads(logn, false), lds) != hipSuccess ||
-        per < 1)
-        per = 1;
-    c->br_per_cu = per;
-    if (br_split(d)) {
-        int ps = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ps, blind_rotate_kernel(logn, true),
-                                                         br_threads(logn, true), lds) != hipSuccess ||
-            ps < 1)
-            ps = 1;
-        c->br_split_per_cu = ps;
+    for (int var = BR_ONE; var <= BR_SPLIT; var++) {
+        if (var != BR_ONE && !br_split(d)) continue;
+        int per = 0;
+        size_t lds = br_lds_bytes(logn, var != BR_ONE, p->br_q, p->lwe_n);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, blind_rotate_kernel(logn, var),
+                                                         br_threads(logn, var), lds) != hipSuccess ||
+            per < 1)
+            per = 1;
+        c->br_per_cu[var] = per;
     }
     for (int i = 0; i < 4; i++) hipEventCreate(&c->ev[i]);
     *out = c;
@@ -263,7 +259,7 @@ extern "C" int pz_ctx_info(pz_ctx *c, uint64_t *psi, uint32_t *num_cu, uint32_t 
     if (!c) return fail(PZ_ERR_ARG, "null context");
     if (psi) *psi = c->ht.psi;
     if (num_cu) *num_cu = (uint32_t)c->num_cu;
-    if (per_cu) *per_cu = (uint32_t)c->br_per_cu;
+    if (per_cu) *per_cu = (uint32_t)c->br_per_cu[c->br_per_cu[BR_SPLIT] ? BR_SPLIT : BR_ONE];
     return PZ_OK;
 }
 
@@ -587,7 +583,7 @@ int pz_internal_exec_view(pz_ctx *c, size_t B, pz_exec_view *v, DevCtx *dc)
     v->ksb = c->d_ksb;
     v->ks = c->d_ks;
     v->stats = c->d_stats;
-    size_t resident = (size_t)c->num_cu * c->br_per_cu;
+    size_t resident = (size_t)c->num_cu * c->br_per_cu[BR_ONE];
     v->grid = (int)(resident < 1 ? 1 : resident);
     *dc = c->dc;
     return PZ_OK;
@@ -614,33 +610,31 @@ extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint6
     return PZ_OK;
 }
 
-// Blind-rotation kernel and grid for a batch of B bootstraps.  The split
-// kernel (two half work groups per bootstrap) has the lower latency while
-// every bootstrap gets a CU of its own; beyond that the one-half kernel,
-// twice as many resident bootstraps per CU, has the higher throughput.
-static void br_plan(const pz_ctx *c, size_t B, bool *split, int *grid)
+// Blind-rotation kernel variant and grid for a batch of B bootstraps.  With
+// d = 1 the split kernel (two half work groups per bootstrap) is used: its
+// latency variant while every bootstrap has a CU of its own, its throughput
+// variant (two bootstraps per CU) beyond that.  Otherwise, or in mode 1, one
+// work group per bootstrap.
+static void br_plan(const pz_ctx *c, size_t B, int *var, int *grid)
 {
-    const size_t res1 = (size_t)c->num_cu * (c->br_per_cu > 0 ? c->br_per_cu : 1);
-    const size_t res2 = (size_t)c->num_cu * (c->br_split_per_cu > 0 ? c->br_split_per_cu : 1);
-    bool sp = c->br_split_per_cu > 0 &&
-              (c->br_mode == 2 || (c->br_mode == 0 && B <= res2));
-    const size_t res = sp ? res2 : res1;
-    *split = sp;
+    int v = BR_ONE;
+    if (c->br_per_cu[BR_SPLIT] > 0 && c->br_mode != 1)
+        v = B <= (size_t)c->num_cu * c->br_per_cu[BR_SPLIT_LAT] ? BR_SPLIT_LAT : BR_SPLIT;
+    const size_t res = (size_t)c->num_cu * (c->br_per_cu[v] > 0 ? c->br_per_cu[v] : 1);
+    *var = v;
     *grid = (int)(B < res ? (B > 0 ? B : 1) : res);
 }
 
-int pz_internal_br_plan(pz_ctx *c, size_t B, int *split, int *grid)
+int pz_internal_br_plan(pz_ctx *c, size_t B, int *var, int *grid)
 {
-    bool sp = false;
-    br_plan(c, B, &sp, grid);
-    *split = sp ? 1 : 0;
+    br_plan(c, B, var, grid);
     return PZ_OK;
 }
 
 extern "C" int pz_set_blind_rotate_mode(pz_ctx *c, int mode)
 {
     if (!c || mode < 0 || mode > 2) return fail(PZ_ERR_ARG, "mode must be 0, 1 or 2");
-    if (mode == 2 && c->br_split_per_cu == 0)
+    if (mode == 2 && c->br_per_cu[BR_SPLIT] == 0)
         return fail(PZ_ERR_UNSUPPORTED, "the split kernel needs d_a = d_b = d_auto = 1");
     if (mode != c->br_mode) {
         HIPCHK(hipSetDevice(c->device));
@@ -664,10 +658,9 @@ static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, con
     launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
               k->dk.ldk, 0, B, c->d_ks, st);
     if (c->timing) hipEventRecord(c->ev[2], st);
-    bool split = false;
-    int grid = 1;
-    br_plan(c, B, &split, &grid);
-    launch_blind_rotate(c->logn, split, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk,
+    int var = BR_ONE, grid = 1;
+    br_plan(c, B, &var, &grid);
+    launch_blind_rotate(c->logn, var, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk,
                         nullptr, nullptr, nullptr, st);
     if (c->timing) hipEventRecord(c->ev[3], st);
 }

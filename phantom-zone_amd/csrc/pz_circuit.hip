@@ -165,9 +165,9 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
         else
             launch_ks(v.digits, v.ksb, k0->ksk, k0->ksk_bytes, v.p.rlwe_n * v.p.lwe_d, n1,
                       k0->ldk, 0, B, v.ks, st);
-        int split = 0, grid = 1;
-        pz_internal_br_plan(c, B, &split, &grid);
-        launch_blind_rotate(v.logn, split != 0, grid, v.ks, d, B, pool, v.stats, dc, *k0,
+        int var = BR_ONE, grid = 1;
+        pz_internal_br_plan(c, B, &var, &grid);
+        launch_blind_rotate(v.logn, var, grid, v.ks, d, B, pool, v.stats, dc, *k0,
                             keys.size() > 1 ? g_stage.dk : nullptr,
                             keys.size() > 1 ? d + 4 * B : nullptr, d + 3 * B, st);
     }

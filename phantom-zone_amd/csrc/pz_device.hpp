@@ -8,11 +8,21 @@ namespace pz {
 
 constexpr int MAX_AUTO = 32;
 
+// Blind-rotation op list entries: an LWE index (RLWE x RGSW), or SCHED_AUTO
+// plus an automorphism key index.  Length bound: n products plus at most one
+// automorphism per bucket (2 (br_q/4 - 1) + 1 < br_q/2).
+constexpr u32 SCHED_AUTO = 0x8000;
+__host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q / 2 + 4; }
+
 // Forward NTT mode of the fused RLWE operators: FWD_FAST (unreduced) or
 // FWD_LAZY (the reference's forward_lazy values).  Both give identical
 // ciphertexts; only the evaluation-domain scratch values differ.
 #ifndef PZ_FWD_MODE
 #define PZ_FWD_MODE FWD_FAST
+#endif
+
+#ifndef PZ_SPLIT_SM
+#define PZ_SPLIT_SM true
 #endif
 
 #ifndef PZ_MAC_FENCE
@@ -45,6 +55,30 @@ struct DevKey {
     u32 rgsw_rows, auto_rows, ksk_bytes, ldk;
 };
 
+// A DevKey read from memory is the same for the whole work group: keep its
+// fields in scalar registers.
+PZ_DEV u32 ufl(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+template <class T>
+PZ_DEV T *ufl_ptr(T *p)
+{
+    const u64 v = (u64)p;
+    return (T *)(((u64)ufl((u32)(v >> 32)) << 32) | ufl((u32)v));
+}
+PZ_DEV DevKey uniform_key(const DevKey &k)
+{
+    DevKey u;
+    u.rgsw = ufl_ptr(k.rgsw);
+    u.rgsw_s = ufl_ptr(k.rgsw_s);
+    u.auto_k = ufl_ptr(k.auto_k);
+    u.auto_s = ufl_ptr(k.auto_s);
+    u.ksk = ufl_ptr(k.ksk);
+    u.rgsw_rows = ufl(k.rgsw_rows);
+    u.auto_rows = ufl(k.auto_rows);
+    u.ksk_bytes = ufl(k.ksk_bytes);
+    u.ldk = ufl(k.ldk);
+    return u;
+}
+
 // 8 consecutive words (evaluation-domain slice of one thread)
 PZ_DEV void load8(const u64 *__restrict__ p, u64 (&v)[8])
 {
@@ -72,6 +106,26 @@ PZ_DEV void fma_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restr
     load8(Ks + 8 * j, ks);
 #pragma unroll
     for (int r = 0; r < 8; r++) acc[r] = shoup_fma<SPLIT>(acc[r], kv[r], ks[r], D[r], q, q2);
+}
+
+// fma_row in two halves of four words, a scheduling fence between them:
+// sixteen fewer registers hold key words at any time.
+template <bool SPLIT>
+PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+                           const u64 (&D)[8], u32 j, u64 q, u64 q2)
+{
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        const ulonglong2 *k2 = reinterpret_cast<const ulonglong2 *>(K + 8 * j + 4 * half);
+        const ulonglong2 *s2 = reinterpret_cast<const ulonglong2 *>(Ks + 8 * j + 4 * half);
+        const ulonglong2 k0 = k2[0], k1 = k2[1], s0 = s2[0], s1 = s2[1];
+        const u64 kv[4] = {k0.x, k0.y, k1.x, k1.y}, ks[4] = {s0.x, s0.y, s1.x, s1.y};
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            acc[4 * half + r] =
+                shoup_fma<SPLIT>(acc[4 * half + r], kv[r], ks[r], D[4 * half + r], q, q2);
+        __builtin_amdgcn_sched_barrier(0);
+    }
 }
 
 // rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) on the RLWE held in
@@ -212,114 +266,108 @@ PZ_DEV void ntt_fwd_idle()
     for (int e = 0; e < (LOGN > 9 ? 6 : 4); e++) __syncthreads();
 }
 
-// rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) with d_a = d_b = 1:
-// each half decomposes and transforms its own polynomial, the halves swap
-// their digit transforms through LDS, half 0 accumulates a' over rows
-// A(-sm), A(m) and half 1 accumulates b' over rows B(-sm), B(m).  A trivial
-// RLWE (a = 0) needs no special case: its digit transform is 0 mod q and the
-// strict inverse returns the same words as the reference's skip.
+// One blind-rotation step on a split work group, either
+//  - rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) with d_a = d_b = 1:
+//    each half decomposes and transforms its own polynomial, the halves swap
+//    their digit transforms through LDS, half 0 accumulates a' over rows
+//    A(-sm), A(m) and half 1 accumulates b' over rows B(-sm), B(m); or
+//  - rlwe_auto_shoup (src/rgsw/runtime.rs:530-643) with d_auto = 1: each half
+//    permutes its polynomial (half 1 into `stash`, W words of LDS nothing
+//    else touches, where the permuted b waits for the final add); half 0
+//    decomposes and transforms auto(a) while half 1 waits at the matching
+//    barriers; half 0 accumulates a' (row A), half 1 takes the digit
+//    transform, accumulates b' (row B) and adds the permuted b.  Trivial:
+//    only b is permuted (q - v where the map negates, the reference's neg).
+// Both steps share one instruction stream (the transforms and the row
+// products appear once), which keeps the step within 128 VGPRs.  A trivial
+// RLWE needs no special case in the product: its digit transform is 0 mod q.
+// The accumulation order of the product differs from the reference's; the
+// strict inverse returns the same words.
 template <int LOGN>
-PZ_DEV void ext_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
-                      const DevCtx &c, u32 j, u32 h, u64 *lds)
-{
-    constexpr int N = 1 << LOGN;
-    constexpr int W = Geom<LOGN>::LDS_WORDS;
-    const u64 q = c.tb.q, q2 = c.tb.q2;
-    u64 *mine = lds + h * W;
-    const u64 *theirs = lds + (h ^ 1) * W;
-    u64 D[1][8];
-#pragma unroll
-    for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(h ? c.dec_b : c.dec_a, p[r], 0);
-    ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, mine, c.tb);
-#pragma unroll
-    for (int r = 0; r < 8; r++) mine[lds_ix(eval_x<LOGN>(j, r))] = D[0][r];
-    __syncthreads();
-    u64 E[8];
-#pragma unroll
-    for (int r = 0; r < 8; r++) E[r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
-    __syncthreads();
-    u64 acc[1][8];
-#pragma unroll
-    for (int r = 0; r < 8; r++) acc[0][r] = 0;
-    const size_t ra = (size_t)h * N, rb = (size_t)(2 + h) * N;  // rows for D_a, D_b
-    if (h == 0) {
-        fma_row<true>(acc[0], K + ra, Ks + ra, D[0], j, q, q2);
-        fma_row<true>(acc[0], K + rb, Ks + rb, E, j, q, q2);
-    } else {
-        fma_row<true>(acc[0], K + ra, Ks + ra, E, j, q, q2);
-        fma_row<true>(acc[0], K + rb, Ks + rb, D[0], j, q, q2);
-    }
-    ntt_inv<LOGN, 1, true, false>(acc, j, mine, c.tb);  // n^-1 is in the key rows
-#pragma unroll
-    for (int r = 0; r < 8; r++) p[r] = acc[0][r];
-}
-
-// rlwe_auto_shoup (src/rgsw/runtime.rs:530-643) with d_auto = 1: each half
-// permutes its polynomial; half 0 decomposes and transforms auto(a) while
-// half 1 waits at the matching barriers, then both halves take the digit
-// transform, half 0 accumulates a' (row A) and half 1 accumulates b' (row B)
-// and adds the permuted b.  Trivial: only b is permuted (q - v where the map
-// negates, the reference's neg), a is left as it is.
-template <int LOGN>
-PZ_DEV void auto_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
-                       u64 kexp, bool trivial, const DevCtx &c, u32 j, u32 h, u64 *lds)
+PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+                       bool is_auto, u64 kexp, bool trivial, const DevCtx &c, u32 j, u32 h,
+                       u64 *lds, u64 *stash)
 {
     constexpr int N = 1 << LOGN;
     constexpr int W = Geom<LOGN>::LDS_WORDS;
     const u64 q = c.tb.q, q2 = c.tb.q2;
     const u64 SIGN = 1ull << 63;
     u64 *mine = lds + h * W;
-    const bool move = !(trivial && h == 0);
-    if (move) {
+    const u64 *theirs = lds + (h ^ 1) * W;
+    if (is_auto) {
+        u64 *perm = h ? stash : mine;
+        const bool move = !(trivial && h == 0);
+        if (move) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            u32 x = coef_x<LOGN>(j, r);
-            u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
-            bool pos = to < (u32)N;
-            to = pos ? to : to - N;
-            const u64 v = p[r];
-            mine[lds_ix(to)] = pos ? v : ((h == 0 || trivial) ? neg_mod(v, q) : (v | SIGN));
+            for (int r = 0; r < 8; r++) {
+                u32 x = coef_x<LOGN>(j, r);
+                u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
+                bool pos = to < (u32)N;
+                to = pos ? to : to - N;
+                const u64 v = p[r];
+                perm[lds_ix(to)] = pos ? v : ((h == 0 || trivial) ? neg_mod(v, q) : (v | SIGN));
+            }
         }
-    }
-    __syncthreads();
-    if (move) {
+        __syncthreads();
+        if (trivial) {  // uniform over the work group
+            if (move) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) p[r] = mine[lds_ix(coef_x<LOGN>(j, r))];
+                for (int r = 0; r < 8; r++) p[r] = perm[lds_ix(coef_x<LOGN>(j, r))];
+            }
+            __syncthreads();
+            return;
+        }
+        if (h == 0) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) p[r] = mine[lds_ix(coef_x<LOGN>(j, r))];
+        }
+        __syncthreads();  // read before the transform rewrites the region
     }
-    __syncthreads();
-    if (trivial) return;  // uniform over the work group
+    // half 1 of an automorphism has no transform (its digits are not used)
+    const bool work = !is_auto || h == 0;
+    const Decomp dc = is_auto ? c.dec_auto : (h ? c.dec_b : c.dec_a);
     u64 D[1][8];
-    if (h == 0) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(c.dec_auto, p[r], 0);
-        ntt_fwd<LOGN, 1, PZ_FWD_MODE>(D, j, mine, c.tb);
+    for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(dc, p[r], 0);
+    if (work) {
+        ntt_fwd<LOGN, 1, PZ_FWD_MODE, PZ_SPLIT_SM>(D, j, mine, c.tb);
 #pragma unroll
         for (int r = 0; r < 8; r++) mine[lds_ix(eval_x<LOGN>(j, r))] = D[0][r];
     } else {
         ntt_fwd_idle<LOGN>();
     }
     __syncthreads();
-    if (h == 1) {
-#pragma unroll
-        for (int r = 0; r < 8; r++) D[0][r] = lds[lds_ix(eval_x<LOGN>(j, r))];
-    }
-    __syncthreads();
     u64 acc[1][8];
 #pragma unroll
     for (int r = 0; r < 8; r++) acc[0][r] = 0;
-    fma_row<true>(acc[0], K + (size_t)h * N, Ks + (size_t)h * N, D[0], j, q, q2);
-    ntt_inv<LOGN, 1, true, false>(acc, j, mine, c.tb);  // n^-1 is in the key rows
-    if (h == 0) {
+    // product rows: RLWE x RGSW -- own digits with the own-part row (half 0:
+    // A(-sm), row 0; half 1: B(m), row 3), then the other half's digits with
+    // the other row (A(m), row 2; B(-sm), row 1); automorphism -- half 0 row
+    // A (0) with its digits, half 1 row B (1) with half 0's digits
+    const size_t r1 = (size_t)(is_auto ? 0 : (h ? 3 : 0)) * N;
+    const size_t r2 = (size_t)(is_auto ? 1 : (h ? 1 : 2)) * N;
+    if (work) fma_row_halves<PZ_SPLIT_SM>(acc[0], K + r1, Ks + r1, D[0], j, q, q2);
+    __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
+    const bool second = !is_auto || h == 1;
+    if (second) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) p[r] = acc[0][r];
-    } else {
+        for (int r = 0; r < 8; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
+    }
+    __syncthreads();  // both halves have read: the regions may be rewritten
+    if (second) fma_row_halves<PZ_SPLIT_SM>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
+    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM>(acc, j, mine, c.tb);  // n^-1 is in the key rows
+    if (is_auto && h == 1) {
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            u64 v = p[r] & ~SIGN;
-            bool neg = (p[r] & SIGN) != 0;
-            p[r] = neg ? sub_mod(acc[0][r], v, q) : add_mod(acc[0][r], v, q);
+            const u64 bp = stash[lds_ix(coef_x<LOGN>(j, r))];
+            const u64 v = bp & ~SIGN;
+            p[r] = (bp & SIGN) ? sub_mod(acc[0][r], v, q) : add_mod(acc[0][r], v, q);
         }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; r++) p[r] = acc[0][r];
     }
+    if (is_auto) __syncthreads();  // the stash is read; a following automorphism may rewrite it
 }
 
 }  // namespace pz

@@ -37,7 +37,7 @@ struct pz_exec_view {
     int grid;  // resident blind-rotation work groups (one-half kernel)
 };
 int pz_internal_exec_view(pz_ctx *c, size_t B, pz_exec_view *v, pz::DevCtx *dc);
-// blind-rotation kernel choice (split: 1 = two half work groups per
-// bootstrap) and grid for a batch of B bootstraps
-int pz_internal_br_plan(pz_ctx *c, size_t B, int *split, int *grid);
+// blind-rotation kernel variant (BR_ONE, BR_SPLIT_LAT, BR_SPLIT) and grid
+// for a batch of B bootstraps
+int pz_internal_br_plan(pz_ctx *c, size_t B, int *var, int *grid);
 const pz::DevKey *pz_internal_devkey(const pz_key *k);

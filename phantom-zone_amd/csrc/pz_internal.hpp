@@ -31,14 +31,17 @@ void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const 
                  const DevCtx &c, hipStream_t s);
 void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_bytes, u32 rows,
                u32 n1, u32 ldk, size_t seg_begin, size_t seg_end, u32 *out, hipStream_t s);
-size_t br_lds_bytes(int logn, u32 br_q, u32 n_lwe);
-const void *blind_rotate_kernel(int logn, bool split);
+size_t br_lds_bytes(int logn, bool split, u32 br_q, u32 n_lwe);
+// blind-rotation kernel variants: one work group per bootstrap, or two half
+// groups (d = 1) with the register budget of 2 (latency) or 4 waves per SIMD
+enum { BR_ONE = 0, BR_SPLIT_LAT = 1, BR_SPLIT = 2 };
+const void *blind_rotate_kernel(int logn, int var);
 bool br_split(const DevCtx &c);
-int br_threads(int logn, bool split);
+int br_threads(int logn, int var);
 void launch_ks_multi(const int8_t *digits, const u32 *ks_b, const DevKey *keys, const u32 *work,
                      size_t n_work, u32 ksk_bytes, u32 rows, u32 n1, u32 ldk, u32 *out,
                      hipStream_t s);
-void launch_blind_rotate(int logn, bool split, int grid, const u32 *ks, const u32 *gate, size_t B,
+void launch_blind_rotate(int logn, int var, int grid, const u32 *ks, const u32 *gate, size_t B,
                          u64 *out, u32 *stats, const DevCtx &c, const DevKey &k,
                          const DevKey *keys, const u32 *key_idx, const u32 *out_idx,
                          hipStream_t s);

@@ -143,6 +143,16 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
     }
 }
 
+PZ_DEV struct Tables fresh(const Tables &tb);
+#ifndef PZ_TWIDDLE_PER_STAGE
+#define PZ_TWIDDLE_PER_STAGE 0
+#endif
+#if PZ_TWIDDLE_PER_STAGE
+#define STAGE_TB(t) fresh(t)
+#else
+#define STAGE_TB(t) (t)
+#endif
+
 // Forward phase starting at stage S0 (up to three stages).
 template <int LOGN, int S0, int NP, int MODE, bool SM>
 PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
@@ -152,11 +162,13 @@ PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
     constexpr int LAST = LOGN - 1;
     constexpr int K_LAST = MODE == FWD_STRICT ? 2 : (MODE == FWD_LAZY ? 1 : 3);
     constexpr int K_MID = MODE == FWD_FAST ? 3 : 0;
-    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM>(v, base, tb);
+    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM>(v, base, STAGE_TB(tb));
     if constexpr (P::NS > 1)
-        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID, SM>(v, base, tb);
+        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID, SM>(v, base,
+                                                                                STAGE_TB(tb));
     if constexpr (P::NS > 2)
-        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID, SM>(v, base, tb);
+        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID, SM>(v, base,
+                                                                                STAGE_TB(tb));
 }
 
 // Inverse phase covering stages S0+NS-1 .. S0 (descending).
@@ -165,9 +177,11 @@ PZ_DEV void inv_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
 {
     using P = Phase<LOGN, S0>;
     const u32 base = P::base(j);
-    if constexpr (P::NS > 2) inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM>(v, base, tb);
-    if constexpr (P::NS > 1) inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM>(v, base, tb);
-    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM>(v, base, tb);
+    if constexpr (P::NS > 2)
+        inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM>(v, base, STAGE_TB(tb));
+    if constexpr (P::NS > 1)
+        inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM>(v, base, STAGE_TB(tb));
+    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM>(v, base, STAGE_TB(tb));
 }
 
 // Store the thread's coefficients (layout of phase SA) into LDS, barrier,

@@ -13,8 +13,13 @@
 #ifndef PZ_BR_SPLIT
 #define PZ_BR_SPLIT 1
 #endif
+// waves per SIMD of the split kernel's two variants: latency (a bootstrap per
+// CU) and throughput (two per CU, 128 VGPRs)
+#ifndef PZ_BR_SPLIT_WAVES_LAT
+#define PZ_BR_SPLIT_WAVES_LAT 2
+#endif
 #ifndef PZ_BR_SPLIT_WAVES
-#define PZ_BR_SPLIT_WAVES 2
+#define PZ_BR_SPLIT_WAVES 4
 #endif
 #define PZ_BR_ATTR __attribute__((amdgpu_waves_per_eu(PZ_BR_WAVES, PZ_BR_WAVES)))
 
@@ -83,17 +88,21 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *
 //   ks:   [B][n+1] key-switched LWE (mod q_ks)
 //   gate: [B] test-vector index
 //   out:  [B][N+1] output LWE (mod Q)
-// SPLIT = false: N/8 threads hold a and b (16 coefficients each, any d).
-// SPLIT = true:  2 x N/8 threads, half h holds polynomial h (d = 1 only).
-// LDS: 2 polynomials (exchange / permutation) + bucket starts + index list.
+// VAR 0 (BR_ONE): N/8 threads hold a and b (16 coefficients each, any d).
+// VAR 1, 2 (BR_SPLIT_LAT, BR_SPLIT): 2 x N/8 threads, half h holds
+// polynomial h (d = 1 only); register budgets for 2 and 4 waves per SIMD.
+// LDS: 2 polynomials (exchange / permutation; SPLIT: one per half), SPLIT: a
+// third for the permuted b of an automorphism, bucket starts + index list.
 // ---------------------------------------------------------------------------
-template <int LOGN, bool SPLIT>
-__global__ __launch_bounds__((SPLIT ? 2 : 1) << (LOGN - 3))
-__attribute__((amdgpu_waves_per_eu(SPLIT ? PZ_BR_SPLIT_WAVES : PZ_BR_WAVES,
-                                   SPLIT ? PZ_BR_SPLIT_WAVES : PZ_BR_WAVES))) void k_blind_rotate(
+template <int LOGN, int VAR>
+__global__ __launch_bounds__((VAR ? 2 : 1) << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu(
+    VAR == 2 ? PZ_BR_SPLIT_WAVES : (VAR == 1 ? PZ_BR_SPLIT_WAVES_LAT : PZ_BR_WAVES),
+    VAR == 2 ? PZ_BR_SPLIT_WAVES : (VAR == 1 ? PZ_BR_SPLIT_WAVES_LAT : PZ_BR_WAVES)))) void
+k_blind_rotate(
     const u32 *ks, const u32 *gate, size_t B, u64 *out, u32 *stats, DevCtx c, DevKey key0,
     const DevKey *keys, const u32 *key_idx, const u32 *out_idx)
 {
+    constexpr bool SPLIT = VAR != BR_ONE;
     constexpr int N = 1 << LOGN;
     constexpr int TPB = N / 8;
     constexpr int THREADS = SPLIT ? 2 * TPB : TPB;
@@ -101,8 +110,11 @@ __attribute__((amdgpu_waves_per_eu(SPLIT ? PZ_BR_SPLIT_WAVES : PZ_BR_WAVES,
     extern __shared__ u64 lds[];
     const u32 nb = c.br_q >> 1;
     const u32 n = c.n_lwe;
-    u16 *bstart = reinterpret_cast<u16 *>(lds + 2 * W);  // [nb + 1]
-    u16 *bidx = bstart + nb + 1;                          // [n]
+    u64 *stash = lds + 2 * W;                               // SPLIT: permuted b
+    u32 *nsched = reinterpret_cast<u32 *>(lds + (SPLIT ? 3 : 2) * W);  // op count
+    u16 *bstart = reinterpret_cast<u16 *>(nsched + 2);      // [nb + 1]
+    u16 *bidx = bstart + nb + 1;                            // [n]
+    u16 *sched = bidx + n;                                  // [br_sched_len] op list
     u32 *cnt = reinterpret_cast<u32 *>(lds);               // scratch in poly area
     u16 *kbin = reinterpret_cast<u16 *>(cnt + nb + 1);     // scratch in poly area
     const u32 tid = threadIdx.x;
@@ -115,7 +127,7 @@ __attribute__((amdgpu_waves_per_eu(SPLIT ? PZ_BR_SPLIT_WAVES : PZ_BR_WAVES,
         const u32 *in = ks + bb * (n + 1);
         // the bootstrap's server key: one key for the batch, or one per
         // ciphertext (sessions batched together)
-        const DevKey key = keys ? keys[key_idx[bb]] : key0;
+        const DevKey key = keys ? uniform_key(keys[key_idx[bb]]) : key0;
         // (3) odd moddown to br_q and dlog buckets (stable counting sort)
         for (u32 i = tid; i <= nb; i += THREADS) cnt[i] = 0;
         __syncthreads();
@@ -135,6 +147,28 @@ __attribute__((amdgpu_waves_per_eu(SPLIT ? PZ_BR_SPLIT_WAVES : PZ_BR_WAVES,
                 bstart[i] = (u16)s;
             }
             for (u32 i = 0; i < n; i++) bidx[cnt[kbin[i]]++] = (u16)i;
+            // (5) the blind rotation's op order, src/pbs.rs:240-414: RLWE x RGSW
+            // of LWE index s (< 2^15), or SCHED_AUTO | automorphism key v.
+            // Buckets -(g^k) first, then -(g^0) and the -g automorphism, then
+            // +(g^k), then +(g^0); a pending automorphism fires when the next
+            // bucket is non-empty, after w steps, or at the last bucket.
+            const u32 qb4 = c.br_q >> 2;
+            u32 m = 0;
+            for (int half = 0; half < 2; half++) {
+                const u32 off = half == 0 ? qb4 : 0;
+                u32 v = 0;
+                for (u32 i = qb4 - 1; i >= 1; i--) {
+                    for (u32 t = bstart[off + i]; t < bstart[off + i + 1]; t++) sched[m++] = bidx[t];
+                    v += 1;
+                    if (bstart[off + i] != bstart[off + i - 1] || v == c.w || i == 1) {
+                        sched[m++] = (u16)(SCHED_AUTO | v);
+                        v = 0;
+                    }
+                }
+                for (u32 t = bstart[off]; t < bstart[off + 1]; t++) sched[m++] = bidx[t];
+                if (half == 0) sched[m++] = (u16)SCHED_AUTO;
+            }
+            *nsched = m;
         }
         __syncthreads();
 
@@ -177,42 +211,29 @@ __attribute__((amdgpu_waves_per_eu(SPLIT ? PZ_BR_SPLIT_WAVES : PZ_BR_WAVES,
             b[r] = tvr[r];
         }
 
-        // (5) blind rotation, src/pbs.rs:240-414
+        // (5) blind rotation: the op list built above, one step at a time
         const size_t rs = (size_t)key.rgsw_rows * N, as = (size_t)key.auto_rows * N;
         bool trivial = true;
         u32 n_auto = 0;
-        const u32 qb4 = c.br_q >> 2;
-        auto ext = [&](size_t si) {
-            if constexpr (SPLIT)
-                ext_split<LOGN>(a, key.rgsw + si * rs, key.rgsw_s + si * rs, c, j, h, lds);
-            else
-                ext_product_blk<LOGN>(a, b, key.rgsw + si * rs, key.rgsw_s + si * rs, trivial, c,
-                                      j, lds);
-            trivial = false;
-        };
-        auto autom = [&](u32 v) {
-            if constexpr (SPLIT)
-                auto_split<LOGN>(a, key.auto_k + v * as, key.auto_s + v * as, c.auto_el[v],
-                                 trivial, c, j, h, lds);
-            else
-                auto_blk<LOGN>(a, b, key.auto_k + v * as, key.auto_s + v * as, c.auto_el[v],
+        const u32 nops = (u32)__builtin_amdgcn_readfirstlane((int)*nsched);
+        for (u32 k = 0; k < nops; k++) {
+            const u32 op = (u32)__builtin_amdgcn_readfirstlane((int)sched[k]);
+            const bool is_auto = (op & SCHED_AUTO) != 0;
+            const u32 arg = op & (SCHED_AUTO - 1);
+            if constexpr (SPLIT) {
+                const size_t off = is_auto ? arg * as : arg * rs;
+                step_split<LOGN>(a, (is_auto ? key.auto_k : key.rgsw) + off,
+                                 (is_auto ? key.auto_s : key.rgsw_s) + off, is_auto,
+                                 c.auto_el[is_auto ? arg : 0], trivial, c, j, h, lds, stash);
+            } else if (is_auto) {
+                auto_blk<LOGN>(a, b, key.auto_k + arg * as, key.auto_s + arg * as, c.auto_el[arg],
                                trivial, c, j, lds);
-            n_auto++;
-        };
-        for (int half = 0; half < 2; half++) {
-            const u32 off = half == 0 ? qb4 : 0;  // -(g^k) buckets first
-            u32 v = 0;
-            for (u32 i = qb4 - 1; i >= 1; i--) {
-                for (u32 t = bstart[off + i]; t < bstart[off + i + 1]; t++) ext(bidx[t]);
-                v += 1;
-                if (bstart[off + i] != bstart[off + i - 1] || v == c.w || i == 1) {
-                    autom(v);
-                    v = 0;
-                }
+            } else {
+                ext_product_blk<LOGN>(a, b, key.rgsw + arg * rs, key.rgsw_s + arg * rs, trivial, c,
+                                      j, lds);
             }
-            // -(g^0) then the -g automorphism / +(g^0) (no automorphism)
-            for (u32 t = bstart[off]; t < bstart[off + 1]; t++) ext(bidx[t]);
-            if (half == 0) autom(0);
+            n_auto += is_auto;
+            trivial = trivial && is_auto;
         }
 
         // (6) sample_extract at index 0 (src/pbs.rs:423-449)
@@ -260,55 +281,64 @@ void launch_ext(int logn, bool is_auto, u64 *rlwe, const u32 *idx, const u8 *tri
     }
 }
 
-size_t br_lds_bytes(int logn, u32 br_q, u32 n_lwe)
+size_t br_lds_bytes(int logn, bool split, u32 br_q, u32 n_lwe)
 {
     size_t words = (size_t)(1 << logn) + (size_t)(1 << logn) / 8;
-    size_t poly = 2 * 8 * words;
+    size_t poly = (split ? 3 : 2) * 8 * words;
     size_t scratch_need = 4 * (size_t)(br_q / 2 + 1) + 2 * (size_t)n_lwe;
     if (scratch_need > poly) poly = scratch_need;
-    return poly + 2 * (size_t)(br_q / 2 + 1) + 2 * (size_t)n_lwe + 16;
+    // op count (8 bytes), bucket starts, index list, op list (u16 each)
+    return poly + 8 + 2 * ((size_t)(br_q / 2 + 1) + n_lwe + br_sched_len(br_q, n_lwe)) + 16;
 }
 
 // The split work group serves d_a = d_b = d_auto = 1 (every NI set).
 bool br_split(const DevCtx &c) { return PZ_BR_SPLIT && c.da == 1 && c.db == 1 && c.dauto == 1; }
 
-int br_threads(int logn, bool split) { return (split ? 2 : 1) << (logn - 3); }
+int br_threads(int logn, int var) { return (var ? 2 : 1) << (logn - 3); }
 
-template <int LOGN, bool SPLIT>
+template <int LOGN, int VAR>
 static const void *br_kernel_ptr()
 {
-    return reinterpret_cast<const void *>(&k_blind_rotate<LOGN, SPLIT>);
+    return reinterpret_cast<const void *>(&k_blind_rotate<LOGN, VAR>);
 }
 
-const void *blind_rotate_kernel(int logn, bool split)
+const void *blind_rotate_kernel(int logn, int var)
 {
     switch (logn) {
-    case 9: return split ? br_kernel_ptr<9, true>() : br_kernel_ptr<9, false>();
-    case 11: return split ? br_kernel_ptr<11, true>() : br_kernel_ptr<11, false>();
+    case 9:
+        return var == BR_SPLIT ? br_kernel_ptr<9, BR_SPLIT>()
+                               : (var == BR_SPLIT_LAT ? br_kernel_ptr<9, BR_SPLIT_LAT>()
+                                                      : br_kernel_ptr<9, BR_ONE>());
+    case 11:
+        return var == BR_SPLIT ? br_kernel_ptr<11, BR_SPLIT>()
+                               : (var == BR_SPLIT_LAT ? br_kernel_ptr<11, BR_SPLIT_LAT>()
+                                                      : br_kernel_ptr<11, BR_ONE>());
     }
     return nullptr;
 }
 
-void launch_blind_rotate(int logn, bool split, int grid, const u32 *ks, const u32 *gate, size_t B,
+void launch_blind_rotate(int logn, int var, int grid, const u32 *ks, const u32 *gate, size_t B,
                          u64 *out, u32 *stats, const DevCtx &c, const DevKey &k,
                          const DevKey *keys, const u32 *key_idx, const u32 *out_idx,
                          hipStream_t s)
 {
     if (B == 0) return;
-    size_t lds = br_lds_bytes(logn, c.br_q, c.n_lwe);
-    split = split && br_split(c);
-    const int tpb = br_threads(logn, split);
-#define PZ_BR_LAUNCH(L, S)                                                                       \
-    hipLaunchKernelGGL((k_blind_rotate<L, S>), grid, tpb, lds, s, ks, gate, B, out, stats, c, k, \
+    if (!br_split(c)) var = BR_ONE;
+    size_t lds = br_lds_bytes(logn, var != BR_ONE, c.br_q, c.n_lwe);
+    const int tpb = br_threads(logn, var);
+#define PZ_BR_LAUNCH(L, V)                                                                       \
+    hipLaunchKernelGGL((k_blind_rotate<L, V>), grid, tpb, lds, s, ks, gate, B, out, stats, c, k, \
                        keys, key_idx, out_idx)
     switch (logn) {
     case 9:
-        if (split) PZ_BR_LAUNCH(9, true);
-        else PZ_BR_LAUNCH(9, false);
+        if (var == BR_SPLIT) PZ_BR_LAUNCH(9, BR_SPLIT);
+        else if (var == BR_SPLIT_LAT) PZ_BR_LAUNCH(9, BR_SPLIT_LAT);
+        else PZ_BR_LAUNCH(9, BR_ONE);
         break;
     case 11:
-        if (split) PZ_BR_LAUNCH(11, true);
-        else PZ_BR_LAUNCH(11, false);
+        if (var == BR_SPLIT) PZ_BR_LAUNCH(11, BR_SPLIT);
+        else if (var == BR_SPLIT_LAT) PZ_BR_LAUNCH(11, BR_SPLIT_LAT);
+        else PZ_BR_LAUNCH(11, BR_ONE);
         break;
     }
 #undef PZ_BR_LAUNCH
