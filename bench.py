@@ -30,10 +30,11 @@ for _p in (ROOT, PKG):
 
 # MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.32e12 32-bit VALU lane-ops/s.
 VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9
-# VALU instructions of one 54-bit Shoup multiply (w*a - q*hi(w'*a)) in the
-# gfx950 ISA of pz_arith.hpp's shoup_mul: 5 v_mad_u64_u32, 4 v_mul_lo_u32,
-# 1 v_mul_hi_u32, 2 v_add3_u32, 1 v_lshl_add_u64, 2 v_sub(b), 4 v_mov.
-VALU_OPS_PER_MODMUL = 19
+# VALU instructions of one 54-bit Shoup multiply in the gfx950 ISA of
+# pz_arith.hpp's two-step form (shoup_quot + shoup_rem), the form the batched
+# blind rotation uses: 6 v_mad_u64_u32, 3 v_mul_lo_u32, 1 v_mul_hi_u32,
+# 2 v_lshl_add_u64, 1 v_add3_u32, 4 v_mov_b32.
+VALU_OPS_PER_MODMUL = 17
 HBM_PEAK_GBS = 8000.0
 PLAIN = {0: lambda a, b: 1 - (a & b)}
 
@@ -273,7 +274,8 @@ def measured_traffic(params, B):
     """HBM bytes per k_blind_rotate launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy
     calibration), when it was taken on the same workload."""
-    path = os.path.join(ROOT, "profiles", "k_blind_rotate_%s_b%d_pmc.md" % (params.lower(), B))
+    path = os.path.join(ROOT, "profiles",
+                        "k_blind_rotate_%s_b%d_pmc.md" % (params.lower().replace("_", ""), B))
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_dispatch:"):
