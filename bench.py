@@ -242,6 +242,50 @@ def kernel_batch():
             "ext_rlwe_gbs": 2 * B * 2 * N * 8 / ext_s / 1e9}
 
 
+def input_path():
+    """Input path at NI_2P: 64 RLWEs of encrypted bits (131072 bits) under a
+    party's secret, switched to the ideal secret, every bit extracted."""
+    import pzlib
+    ctx = pzlib.Context("NI_2P")
+    N = ctx.N
+    s, z = ctx.keygen_secrets(500, 2)
+    u = ctx.keygen_user_secret(501)
+    ksk = ctx.keygen_rlwe_ksk(502, u, s)
+    R = 64
+    bits = np.random.default_rng(503).integers(0, 2, size=R * N).astype(np.uint8)
+    cts = ctx.encrypt_rlwe_bools(u, bits, 504)
+    d_in = ctx.to_device(cts)
+    d_sw = ctx.array((R, 2, N))
+    count = R * N
+    src = np.stack([np.arange(count) // N, np.arange(count) % N], axis=1).astype(np.uint32)
+    d_src = ctx.to_device(src)
+    d_lwe = ctx.array((count, N + 1))
+    ctx.rlwe_key_switch(ksk, d_in, d_sw, R)
+    ctx.sync()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.rlwe_key_switch(ksk, d_in, d_sw, R)
+    ctx.sync()
+    ks_s = (time.perf_counter() - t0) / reps
+    ctx.sample_extract(d_sw, d_src, count, d_lwe)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.sample_extract(d_sw, d_src, count, d_lwe)
+    ctx.sync()
+    ex_s = time.perf_counter() - t0
+    check = 4096
+    lw = ctx.array((check, N + 1))
+    ctx.sample_extract(d_sw, ctx.to_device(src[:check]), check, lw)
+    ok = bool(np.array_equal(ctx.decrypt_bools(s, lw.download()), bits[:check]))
+    ksk.free()
+    ctx.close()
+    return {"params": "NI_2P", "rlwes": R, "bits": count,
+            "key_switch_ms": ks_s * 1e3, "rlwes_per_sec": R / ks_s,
+            "extract_ms": ex_s * 1e3, "extract_gbs": count * (N + 1) * 8 / ex_s / 1e9,
+            "bits_per_sec": count / (ks_s + ex_s), "sample_correct": ok}
+
+
 def cpu_baseline(params, threads, per_thread):
     """Oracle PBS throughput on the host cores (one PBS stream per thread)."""
     from oracle import Oracle
@@ -380,6 +424,8 @@ def main():
         ctx.close()
         extras["kernel_batch"] = kernel_batch()
         log("kernel batch: %s" % extras["kernel_batch"])
+        extras["input_path"] = input_path()
+        log("input path: %s" % extras["input_path"])
         extras["fheuint8_ni8p"] = fheuint8_circuits(args)
         log("fheuint8: %s" % extras["fheuint8_ni8p"])
         extras["serving_ni2p"] = serving(args, plain)

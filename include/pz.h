@@ -44,6 +44,7 @@ typedef struct pz_params {
     uint32_t auto_logb, auto_d;
     uint32_t g, w;
     uint32_t lwe_secret_gaussian;
+    uint32_t uitos_logb, uitos_d; /* u_j -> s RLWE key switch (NI sets; 0 = none) */
 } pz_params;
 
 typedef struct pz_ctx pz_ctx;
@@ -242,6 +243,42 @@ int pz_server_request_times(pz_server *s, uint32_t request_id, double *arrival, 
                             double *finished, uint32_t *n_pbs, uint32_t *n_ops);
 int pz_server_stats(pz_server *s, uint64_t *total_pbs, uint64_t *batches, uint64_t *key_uploads,
                     uint64_t *evictions, double *wall_s);
+
+/* ---- input path of the non-interactive protocol: a party encrypts bits
+ * under its own RLWE secret u_j as batched RLWE ciphertexts
+ * (src/bool/ni_mp_api.rs:294-352); the server switches them to the ideal
+ * secret s (KeySwitchWithId, src/bool/ni_mp_api.rs:395-440, rlwe_key_switch
+ * src/rgsw/runtime.rs:1001-1063) and extracts one LWE per bit
+ * (sample_extract, src/pbs.rs:423-449; SampleExtractor,
+ * src/shortint/enc_dec.rs:100-135).  Parameter sets with uitos_d = 0 have
+ * no such path (PZ_ERR_UNSUPPORTED). */
+typedef struct pz_rlwe_ksk pz_rlwe_ksk;
+/* party secret u_j[N] (ternary, Hamming weight N/2; the library's own seeded
+ * draw) */
+int pz_keygen_user_secret(pz_ctx *ctx, const uint8_t *seed32, int32_t *h_u);
+/* Ksk(u -> s) (ui_to_s_ksk, src/bool/keys.rs:965-995): the library's own
+ * seeded generation of rows A_j uniform, B_j = A_j*s + e + 2^(logb*j + ib)*u,
+ * sent to the evaluation domain */
+int pz_keygen_rlwe_ksk(pz_ctx *ctx, const uint8_t *seed32, const int32_t *h_u, const int32_t *h_s,
+                       pz_rlwe_ksk **out);
+/* upload a Ksk given as host rows [2*uitos_d][N] in the evaluation domain
+ * (A rows then B rows); Shoup twins are computed on the device */
+int pz_rlwe_ksk_upload(pz_ctx *ctx, const uint64_t *h_rows, pz_rlwe_ksk **out);
+int pz_rlwe_ksk_download(pz_rlwe_ksk *ksk, uint64_t *h_rows);
+int pz_rlwe_ksk_free(pz_rlwe_ksk *ksk);
+/* seeded RLWE encryption of a bool batch under h_u: ceil(count/N)
+ * ciphertexts [.][2][N], bit t in coefficient t % N of ciphertext t / N
+ * (+-Q/8, unused coefficients 0); a from one ChaCha8 stream and the noise from
+ * another, both derived from seed32 */
+int pz_encrypt_rlwe_bools(pz_ctx *ctx, const int32_t *h_u, const uint8_t *seed32,
+                          const uint8_t *h_bits, size_t count, uint64_t *h_out);
+/* d_out[b] = rlwe_key_switch(d_in[b]) for B ciphertexts [B][2][N]
+ * (d_out may equal d_in) */
+int pz_rlwe_key_switch(pz_ctx *ctx, pz_rlwe_ksk *ksk, const uint64_t *d_in, uint64_t *d_out,
+                       size_t B, void *stream);
+/* d_lwe[t] = sample_extract(d_rlwe[d_src[2t]], index d_src[2t+1]) ([N+1]) */
+int pz_sample_extract(pz_ctx *ctx, const uint64_t *d_rlwe, const uint32_t *d_src, size_t count,
+                      uint64_t *d_lwe, void *stream);
 
 /* Blind-rotation kernel choice.  No counterpart in the reference (its
  * evaluator is single-threaded): 0 = automatic, 1 = one work group per

@@ -32,6 +32,7 @@ class Params(C.Structure):
         ("rlrg_logb", C.c_uint32), ("rlrg_da", C.c_uint32), ("rlrg_db", C.c_uint32),
         ("auto_logb", C.c_uint32), ("auto_d", C.c_uint32),
         ("g", C.c_uint32), ("w", C.c_uint32), ("lwe_secret_gaussian", C.c_uint32),
+        ("uitos_logb", C.c_uint32), ("uitos_d", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -128,6 +129,10 @@ def lib():
             "pzo_pbs_batch_threads": (C.c_double, [vp, C.POINTER(Key), C.c_uint32, u64p,
                                                    C.c_uint32, C.c_uint32]),
             "pzo_encrypt_lwe": (None, [vp, u8p, i32p, C.c_uint32, C.c_uint64, u64p]),
+            "pzo_keygen_user_secret": (None, [vp, u8p, i32p]),
+            "pzo_keygen_rlwe_ksk": (None, [vp, u8p, i32p, i32p, u64p]),
+            "pzo_rlwe_key_switch": (None, [vp, u64p, u64p, u64p, u64p]),
+            "pzo_encrypt_rlwe_bools": (None, [vp, u8p, i32p, u8p, C.c_size_t, u64p]),
             "pzo_decrypt_lwe": (C.c_uint64, [vp, u64p, i32p, C.c_uint32]),
             "pzo_decode_bool": (C.c_int, [vp, C.c_uint64]),
             "pzo_true_el": (C.c_uint64, [vp]),
@@ -311,6 +316,46 @@ class Oracle:
         secs = self.L.pzo_pbs_batch_threads(self.ev, C.byref(key.cstruct()),
                                             GATES.get(gate, gate), arr.reshape(-1), count, threads)
         return arr, secs
+
+    # ---- input path (non-interactive sets) ----
+    def keygen_user_secret(self, seed):
+        u = np.zeros(self.N, dtype=np.int32)
+        self.L.pzo_keygen_user_secret(self.ev, seed_bytes(seed), u)
+        return u
+
+    def keygen_rlwe_ksk(self, seed, u, s):
+        ksk = np.zeros((2 * self.p.uitos_d, self.N), dtype=np.uint64)
+        self.L.pzo_keygen_rlwe_ksk(self.ev, seed_bytes(seed), np.ascontiguousarray(u, dtype=np.int32),
+                                   np.ascontiguousarray(s, dtype=np.int32), ksk.reshape(-1))
+        return ksk
+
+    def shoup_rows(self, rows):
+        """Shoup twins of a matrix of residues (ToShoup, src/utils.rs:49-73)."""
+        q = self.Q
+        return np.array([[(int(v) << 64) // q for v in row] for row in np.asarray(rows)],
+                        dtype=np.uint64)
+
+    def rlwe_key_switch(self, rlwe, ksk, ksk_shoup):
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        self.L.pzo_rlwe_key_switch(self.ev, np.ascontiguousarray(rlwe, dtype=np.uint64).reshape(-1),
+                                   np.ascontiguousarray(ksk, dtype=np.uint64).reshape(-1),
+                                   np.ascontiguousarray(ksk_shoup, dtype=np.uint64).reshape(-1),
+                                   out.reshape(-1))
+        return out
+
+    def encrypt_rlwe_bools(self, u, bits, seed):
+        b = np.ascontiguousarray(bits, dtype=np.uint8)
+        nr = (len(b) + self.N - 1) // self.N
+        out = np.zeros((nr, 2, self.N), dtype=np.uint64)
+        self.L.pzo_encrypt_rlwe_bools(self.ev, seed_bytes(seed), np.ascontiguousarray(u, dtype=np.int32),
+                                      b, len(b), out.reshape(-1))
+        return out
+
+    def sample_extract(self, rlwe, index):
+        out = np.zeros(self.N + 1, dtype=np.uint64)
+        self.L.pzo_sample_extract(self.ev, out, np.ascontiguousarray(rlwe, dtype=np.uint64).reshape(-1),
+                                  index)
+        return out
 
     # ---- encryption ----
     def encrypt_bool(self, s, bit, seed):

@@ -61,13 +61,13 @@ int pzo_params_named(int which, pzo_params *o)
     switch (which) {
     case 0: /* NI_2P :598-623 */
         o->lwe_q = 1ULL << 16; o->br_q = 1u << 12; o->lwe_n = 520; o->lwe_d = 13;
-        o->lwe_secret_gaussian = 1; return 0;
+        o->lwe_secret_gaussian = 1; o->uitos_logb = 1; o->uitos_d = 50; return 0;
     case 1: /* NI_4P_HB_FR :625-650 */
         o->lwe_q = 1ULL << 16; o->br_q = 1u << 11; o->lwe_n = 620; o->lwe_d = 13;
-        return 0;
+        o->uitos_logb = 1; o->uitos_d = 50; return 0;
     case 2: /* NI_8P :679-704 */
         o->lwe_q = 1ULL << 17; o->br_q = 1u << 12; o->lwe_n = 660; o->lwe_d = 14;
-        return 0;
+        o->uitos_logb = 1; o->uitos_d = 50; return 0;
     case 3: /* SP_TEST_BOOL_PARAMS :706-726 */
         o->rlwe_q = 268369921ULL; o->rlwe_n = 512; o->lwe_q = 1ULL << 16;
         o->br_q = 1u << 9; o->lwe_n = 100; o->lwe_logb = 4; o->lwe_d = 4;
@@ -85,7 +85,7 @@ int pzo_params_named(int which, pzo_params *o)
         return 0;
     case 7: /* NI_4P_LB_SR :652-677 */
         o->lwe_q = 1ULL << 16; o->br_q = 1u << 12; o->lwe_n = 620; o->lwe_d = 13;
-        return 0;
+        o->uitos_logb = 1; o->uitos_d = 50; return 0;
     default:
         return -1;
     }
@@ -687,7 +687,7 @@ struct pzo_eval {
     pzo_ntt *ntt;
     modq mq;                 /* RLWE modulus Q */
     uint64_t lwe_mask;
-    decomp dec_lwe, dec_a, dec_b, dec_auto;
+    decomp dec_lwe, dec_a, dec_b, dec_auto, dec_uitos;
     uint32_t *dlog;          /* [br_q] */
     uint32_t embedding;
     uint64_t qby4, true_el, false_el;
@@ -731,6 +731,7 @@ pzo_eval *pzo_eval_new(const pzo_params *p)
     decomp_init(&e->dec_a, p->rlwe_q, p->rlrg_logb, p->rlrg_da);
     decomp_init(&e->dec_b, p->rlwe_q, p->rlrg_logb, p->rlrg_db);
     decomp_init(&e->dec_auto, p->rlwe_q, p->auto_logb, p->auto_d);
+    if (p->uitos_d) decomp_init(&e->dec_uitos, p->rlwe_q, p->uitos_logb, p->uitos_d);
 
     /* dlog map :664-673 */
     e->dlog = (uint32_t *)calloc(brq, sizeof(uint32_t));
@@ -1408,6 +1409,98 @@ void pzo_key_shoup(const pzo_eval *e, pzo_key *key)
     uint64_t q = e->p.rlwe_q;
     for (size_t i = 0; i < ks.rgsw_words; i++) key->rgsw_shoup[i] = pzo_shoup_repr(key->rgsw[i], q);
     for (size_t i = 0; i < ks.auto_words; i++) key->auto_shoup[i] = pzo_shoup_repr(key->auto_key[i], q);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Input path of the non-interactive protocol.                               */
+/* ------------------------------------------------------------------------- */
+
+void pzo_keygen_user_secret(const pzo_eval *e, const uint8_t seed[32], int32_t *u)
+{
+    pzo_rng *r = pzo_rng_new(seed);
+    pzo_ternary_hw(r, u, e->p.rlwe_n, e->p.rlwe_n >> 1);
+    pzo_rng_free(r);
+}
+
+void pzo_keygen_rlwe_ksk(const pzo_eval *e, const uint8_t seed[32], const int32_t *u,
+                         const int32_t *s, uint64_t *ksk)
+{
+    uint32_t N = e->p.rlwe_n, d = e->p.uitos_d;
+    uint64_t q = e->p.rlwe_q;
+    pzo_rng *r = pzo_rng_new(seed);
+    uint64_t *s_eval = (uint64_t *)malloc(8 * (size_t)N), *tmp = (uint64_t *)malloc(8 * (size_t)N);
+    uint64_t *msg = (uint64_t *)malloc(8 * (size_t)N);
+    for (uint32_t i = 0; i < N; i++) s_eval[i] = map_i64(q, s[i]);
+    ntt_fwd(e->ntt, s_eval, 1);
+    for (uint32_t i = 0; i < N; i++) msg[i] = map_i64(q, u[i]);
+    for (uint32_t j = 0; j < d; j++) {
+        uint64_t beta = 1ULL << (e->p.uitos_logb * j + e->dec_uitos.ignore_bits);
+        rlwe_pair(e, r, s_eval, NULL, msg, beta, ksk + (size_t)j * N, ksk + (size_t)(d + j) * N,
+                  tmp);
+    }
+    for (uint32_t row = 0; row < 2 * d; row++) ntt_fwd(e->ntt, ksk + (size_t)row * N, 1);
+    free(s_eval); free(tmp); free(msg);
+    pzo_rng_free(r);
+}
+
+void pzo_rlwe_key_switch(const pzo_eval *e, const uint64_t *rlwe_in, const uint64_t *ksk,
+                         const uint64_t *ksk_shoup, uint64_t *out)
+{
+    uint32_t N = e->p.rlwe_n, d = e->p.uitos_d;
+    uint64_t q = e->p.rlwe_q;
+    const modq *m = &e->mq;
+    uint64_t *tmp_row = (uint64_t *)calloc(N, 8);
+    uint64_t *tmp = (uint64_t *)calloc((size_t)d * N, 8);
+    /* -A, decomposed (decompose_r) and forward_lazy per row */
+    for (uint32_t i = 0; i < N; i++) tmp_row[i] = mm_neg(m, rlwe_in[i]);
+    decompose_r(&e->dec_uitos, tmp_row, tmp, N);
+    for (uint32_t row = 0; row < d; row++) ntt_fwd(e->ntt, tmp + (size_t)row * N, 0);
+    /* A' = decomp(-A) . Ksk_A, B' = decomp(-A) . Ksk_B (shoup_matrix_fma) */
+    memset(out, 0, 16 * (size_t)N);
+    pzo_shoup_matrix_fma(q, out, ksk, ksk_shoup, tmp, d, N);
+    pzo_shoup_matrix_fma(q, out + N, ksk + (size_t)d * N, ksk_shoup + (size_t)d * N, tmp, d, N);
+    ntt_inv(e->ntt, out, 1);
+    ntt_inv(e->ntt, out + N, 1);
+    /* B' + B */
+    for (uint32_t i = 0; i < N; i++) out[N + i] = mm_add(m, out[N + i], rlwe_in[N + i]);
+    free(tmp_row); free(tmp);
+}
+
+void pzo_encrypt_rlwe_bools(const pzo_eval *e, const uint8_t seed[32], const int32_t *u,
+                            const uint8_t *bits, size_t count, uint64_t *out)
+{
+    uint32_t N = e->p.rlwe_n;
+    uint64_t q = e->p.rlwe_q;
+    const modq *m = &e->mq;
+    uint8_t sa[32], se[32];
+    pzo_rng *master = pzo_rng_new(seed);
+    pzo_rng_fill_bytes(master, sa, 32);
+    pzo_rng_fill_bytes(master, se, 32);
+    pzo_rng_free(master);
+    pzo_rng *ra = pzo_rng_new(sa), *re = pzo_rng_new(se);
+    uint64_t *u_eval = (uint64_t *)malloc(8 * (size_t)N), *tmp = (uint64_t *)malloc(8 * (size_t)N);
+    for (uint32_t i = 0; i < N; i++) u_eval[i] = map_i64(q, u[i]);
+    ntt_fwd(e->ntt, u_eval, 1);
+    size_t nr = (count + N - 1) / N;
+    for (size_t k = 0; k < nr; k++) {
+        uint64_t *a = out + k * 2 * N, *b = a + N;
+        for (uint32_t i = 0; i < N; i++) a[i] = pzo_rng_uniform(ra, q);
+        /* a * u: forward, Barrett pointwise (elwise_mul_mut), backward */
+        memcpy(tmp, a, 8 * (size_t)N);
+        ntt_fwd(e->ntt, tmp, 1);
+        for (uint32_t i = 0; i < N; i++) tmp[i] = mm_mul(m, tmp[i], u_eval[i]);
+        ntt_inv(e->ntt, tmp, 1);
+        /* b = e + m + a*u, in that order of additions */
+        for (uint32_t i = 0; i < N; i++) {
+            size_t t = k * N + i;
+            uint64_t msg = t < count ? (bits[t] ? e->true_el : e->false_el) : 0;
+            uint64_t v = map_i64(q, pzo_rng_gauss_round(re));
+            v = mm_add(m, v, msg);
+            b[i] = mm_add(m, v, tmp[i]);
+        }
+    }
+    free(u_eval); free(tmp);
+    pzo_rng_free(ra); pzo_rng_free(re);
 }
 
 /* encrypt_lwe, src/lwe.rs:104-137 (operator = RLWE modop, secret of dim) */

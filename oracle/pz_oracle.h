@@ -34,6 +34,7 @@ typedef struct pzo_params {
     uint32_t auto_logb, auto_d;             /* automorphism decomposer        */
     uint32_t g, w;                          /* generator, window              */
     uint32_t lwe_secret_gaussian;           /* 1: error dist, 0: ternary      */
+    uint32_t uitos_logb, uitos_d;           /* u_j -> s RLWE key switch (NI)  */
 } pzo_params;
 
 /* Named parameter sets: 0 NI_2P, 1 NI_4P_HB_FR, 2 NI_8P, 3 SP_TEST_BOOL_PARAMS,
@@ -169,6 +170,26 @@ double pzo_pbs_batch_threads(const pzo_eval *e, const pzo_key *key, uint32_t gat
                              uint64_t *lwes, uint32_t count, uint32_t threads);
 
 /* encryption helpers (tests) */
+/* ---- input path of the non-interactive protocol ---- */
+/* party RLWE secret u_j (ternary, Hamming weight N/2), the oracle's own
+ * seeded draw */
+void pzo_keygen_user_secret(const pzo_eval *e, const uint8_t seed[32], int32_t *u);
+/* Ksk(u -> s), [2d][N] evaluation domain: rows A_j uniform, then
+ * B_j = A_j*s + e_j + 2^(logb*j + ignore_bits) * u (the oracle's own stream:
+ * per j, N uniform then N noise draws) */
+void pzo_keygen_rlwe_ksk(const pzo_eval *e, const uint8_t seed[32], const int32_t *u,
+                         const int32_t *s, uint64_t *ksk);
+/* rlwe_key_switch, src/rgsw/runtime.rs:1001-1063: rlwe_in [2][N] under u to
+ * out [2][N] under s with ksk [2d][N] (evaluation domain) and its Shoup rows */
+void pzo_rlwe_key_switch(const pzo_eval *e, const uint64_t *rlwe_in, const uint64_t *ksk,
+                         const uint64_t *ksk_shoup, uint64_t *out);
+/* bool batch under u as ceil(count/N) RLWEs [.][2][N]
+ * (src/bool/ni_mp_api.rs:294-352, seeded_secret_key_encrypt_rlwe
+ * src/rgsw/keygen.rs:439-481); a from the stream of seed_a, noise from the
+ * stream of seed_e, both derived from seed (first and second 32 bytes) */
+void pzo_encrypt_rlwe_bools(const pzo_eval *e, const uint8_t seed[32], const int32_t *u,
+                            const uint8_t *bits, size_t count, uint64_t *out);
+
 void pzo_encrypt_lwe(const pzo_eval *e, const uint8_t seed[32], const int32_t *s,
                      uint32_t dim, uint64_t m, uint64_t *out);
 uint64_t pzo_decrypt_lwe(const pzo_eval *e, const uint64_t *lwe, const int32_t *s,

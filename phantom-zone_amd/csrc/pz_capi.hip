@@ -109,6 +109,8 @@ static int check_params(const pz_params *p, int *logn)
     if (p->rlrg_logb * p->rlrg_da > lq || p->rlrg_logb * p->rlrg_db > lq ||
         p->auto_logb * p->auto_d > lq || p->lwe_logb * p->lwe_d > host::ref_log2(p->lwe_q))
         return fail(PZ_ERR_ARG, "decomposer wants logq >= logb*d");
+    if (p->uitos_d && (p->uitos_logb == 0 || p->uitos_logb * p->uitos_d > lq || p->uitos_d > 64))
+        return fail(PZ_ERR_ARG, "u -> s decomposer wants 0 < logb*d <= logq, d <= 64");
     *logn = l;
     return PZ_OK;
 }
@@ -175,6 +177,7 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
     d.tb.n_inv_s = c->ht.n_inv_s;
     d.tb.q = p->rlwe_q;
     d.tb.q2 = 2 * p->rlwe_q;
+    if (p->uitos_d) d.dec_uitos = make_decomp(p->rlwe_q, p->uitos_logb, p->uitos_d);
     d.dec_a = make_decomp(p->rlwe_q, p->rlrg_logb, p->rlrg_da);
     d.dec_b = make_decomp(p->rlwe_q, p->rlrg_logb, p->rlrg_db);
     d.dec_auto = make_decomp(p->rlwe_q, p->auto_logb, p->auto_d);

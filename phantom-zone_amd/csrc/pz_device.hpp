@@ -33,6 +33,7 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 struct DevCtx {
     Tables tb;
     Decomp dec_a, dec_b, dec_auto, dec_lwe;
+    Decomp dec_uitos;  // u_j -> s RLWE key switch (non-interactive sets; d = 0: none)
     u32 da, db, dauto, lwe_d;
     u32 n_lwe, br_q, w, g, embedding;
     u32 lwe_q_log;

@@ -31,11 +31,13 @@ inline int params_named(const char *name, pz_params *o)
     std::string s(name);
     if (s == "NI_2P") {
         o->lwe_q = 1ULL << 16; o->br_q = 1u << 12; o->lwe_n = 520; o->lwe_d = 13;
-        o->lwe_secret_gaussian = 1;
+        o->lwe_secret_gaussian = 1; o->uitos_logb = 1; o->uitos_d = 50;
     } else if (s == "NI_4P_HB_FR") {
         o->lwe_q = 1ULL << 16; o->br_q = 1u << 11; o->lwe_n = 620; o->lwe_d = 13;
+        o->uitos_logb = 1; o->uitos_d = 50;
     } else if (s == "NI_8P") {
         o->lwe_q = 1ULL << 17; o->br_q = 1u << 12; o->lwe_n = 660; o->lwe_d = 14;
+        o->uitos_logb = 1; o->uitos_d = 50;
     } else if (s == "SP_TEST") {
         o->rlwe_q = 268369921ULL; o->rlwe_n = 512; o->lwe_q = 1ULL << 16;
         o->br_q = 1u << 9; o->lwe_n = 100; o->lwe_logb = 4; o->lwe_d = 4;
@@ -49,6 +51,7 @@ inline int params_named(const char *name, pz_params *o)
         o->lwe_q = 1ULL << 17; o->br_q = 1u << 12; o->lwe_n = 660; o->lwe_d = 14;
     } else if (s == "NI_4P_LB_SR") {
         o->lwe_q = 1ULL << 16; o->br_q = 1u << 12; o->lwe_n = 620; o->lwe_d = 13;
+        o->uitos_logb = 1; o->uitos_d = 50;
     } else {
         return -1;
     }

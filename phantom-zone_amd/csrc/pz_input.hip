@@ -1,0 +1,208 @@
+// pz_input.hip -- input path of the non-interactive protocol on the device:
+// RLWE key switching from a party's secret u_j to the ideal secret s, and
+// sample extraction of one LWE per bit.
+//
+//   k_rlwe_ksw         rlwe_key_switch (src/rgsw/runtime.rs:1001-1063), one
+//                      RLWE per work group: -a decomposed into d digit rows
+//                      (DecomposerIter order), each row transformed and
+//                      multiply-accumulated into A' and B' (rows of Ksk), one
+//                      strict inverse per output, then B' + b
+//   k_sample_extract   sample_extract (src/pbs.rs:423-449) at any index
+#include "pz_internal.hpp"
+
+namespace pz {
+
+template <int LOGN>
+__global__ __launch_bounds__(1 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+k_rlwe_ksw(const u64 *in, u64 *out, size_t B, DevCtx c, const u64 *K, const u64 *Ks, u32 d)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ u64 lds[];
+    const u32 j = threadIdx.x;
+    const u64 q = c.tb.q, q2 = c.tb.q2;
+    for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
+        const u64 *p = in + bb * 2 * N;
+        u64 val[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) val[r] = decomp_prepare(c.dec_uitos, neg_mod(p[coef_x<LOGN>(j, r)], q));
+        u64 acc[2][8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) acc[0][r] = acc[1][r] = 0;
+        for (u32 row = 0; row < d; row++) {
+            __builtin_amdgcn_sched_barrier(0);  // no key loads hoisted across rows
+            u64 D[1][8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) D[0][r] = decomp_next(c.dec_uitos, val[r]);
+            ntt_fwd<LOGN, 1, PZ_FWD_MODE, false>(D, j, lds, c.tb);
+            fma_row<false>(acc[0], K + (size_t)row * N, Ks + (size_t)row * N, D[0], j, q, q2);
+            fma_row<false>(acc[1], K + (size_t)(d + row) * N, Ks + (size_t)(d + row) * N, D[0], j,
+                          q, q2);
+        }
+        ntt_inv<LOGN, 2, true, true, false>(acc, j, lds, c.tb);
+        u64 *o = out + bb * 2 * N;
+        u64 bv[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) bv[r] = p[N + coef_x<LOGN>(j, r)];
+        __syncthreads();  // out may alias in: every read of this RLWE is done
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            o[coef_x<LOGN>(j, r)] = acc[0][r];
+            o[N + coef_x<LOGN>(j, r)] = add_mod(acc[1][r], bv[r], q);
+        }
+    }
+}
+
+__global__ void k_sample_extract(const u64 *rlwe, const u32 *src, size_t count, u32 N, u64 q,
+                                 u64 *lwe)
+{
+    const size_t N1 = (size_t)N + 1;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < count * N1;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const size_t o = t / N1;
+        const u32 col = (u32)(t % N1);
+        const u64 *r = rlwe + (size_t)src[2 * o] * 2 * N;
+        const u32 k = src[2 * o + 1];
+        u64 v;
+        if (col == 0) {
+            v = r[N + k];                             // b at the index
+        } else {
+            const u32 i = col - 1;                    // a_i of the LWE
+            v = i <= k ? r[k - i] : neg_mod(r[N + k - i], q);
+        }
+        lwe[t] = v;
+    }
+}
+
+}  // namespace pz
+
+using namespace pz;
+
+extern int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStream_t *s);
+
+int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    const size_t words = 2 * (size_t)d * p.rlwe_n;
+    pz_rlwe_ksk *k = new pz_rlwe_ksk();
+    k->owner = c;
+    k->d = d;
+    if (hipMalloc(&k->rows_s, 8 * words) != hipSuccess) {
+        delete k;
+        return PZ_ERR_HIP;
+    }
+    launch_shoup(d_rows, k->rows_s, words, p.rlwe_q, st);
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        hipFree(k->rows_s);
+        delete k;
+        return PZ_ERR_HIP;
+    }
+    k->rows = d_rows;
+    *out = k;
+    return PZ_OK;
+}
+
+extern "C" int pz_rlwe_ksk_upload(pz_ctx *c, const uint64_t *h_rows, pz_rlwe_ksk **out)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!h_rows || !out) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (p.uitos_d == 0) return PZ_ERR_UNSUPPORTED;
+    const size_t words = 2 * (size_t)p.uitos_d * p.rlwe_n;
+    u64 *rows = nullptr;
+    if (hipMalloc(&rows, 8 * words) != hipSuccess) return PZ_ERR_HIP;
+    if (hipMemcpy(rows, h_rows, 8 * words, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(rows);
+        return PZ_ERR_HIP;
+    }
+    rc = pz_internal_rlwe_ksk_adopt(c, rows, p.uitos_d, out);
+    if (rc) hipFree(rows);
+    return rc;
+}
+
+extern "C" int pz_rlwe_ksk_download(pz_rlwe_ksk *k, uint64_t *h_rows)
+{
+    if (!k || !h_rows) return PZ_ERR_ARG;
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    int rc = pz_internal_ctx_view(k->owner, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (hipStreamSynchronize(st) != hipSuccess) return PZ_ERR_HIP;
+    if (hipMemcpy(h_rows, k->rows, 16 * (size_t)k->d * p.rlwe_n, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return PZ_ERR_HIP;
+    return PZ_OK;
+}
+
+extern "C" int pz_rlwe_ksk_free(pz_rlwe_ksk *k)
+{
+    if (!k) return PZ_OK;
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (pz_internal_ctx_view(k->owner, &p, &dc, &logn, &st) == PZ_OK) hipStreamSynchronize(st);
+    hipFree(k->rows);
+    hipFree(k->rows_s);
+    delete k;
+    return PZ_OK;
+}
+
+extern "C" int pz_rlwe_key_switch(pz_ctx *c, pz_rlwe_ksk *k, const uint64_t *d_in, uint64_t *d_out,
+                                  size_t B, void *stream)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!k || !d_in || !d_out) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (p.uitos_d == 0 || k->d != p.uitos_d) return PZ_ERR_UNSUPPORTED;
+    if (B == 0) return PZ_OK;
+    if (stream) st = (hipStream_t)stream;
+    const int grid = (int)(B < 65535 ? B : 65535);
+    const size_t lds = 8 * (size_t)(2 * ((1 << logn) + (1 << logn) / 8));
+    switch (logn) {
+    case 9:
+        hipLaunchKernelGGL(k_rlwe_ksw<9>, grid, 1 << 6, lds, st, d_in, d_out, B, dc, k->rows,
+                           k->rows_s, k->d);
+        break;
+    case 11:
+        hipLaunchKernelGGL(k_rlwe_ksw<11>, grid, 1 << 8, lds, st, d_in, d_out, B, dc, k->rows,
+                           k->rows_s, k->d);
+        break;
+    default:
+        return PZ_ERR_UNSUPPORTED;
+    }
+    return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
+}
+
+extern "C" int pz_sample_extract(pz_ctx *c, const uint64_t *d_rlwe, const uint32_t *d_src,
+                                 size_t count, uint64_t *d_lwe, void *stream)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!d_rlwe || !d_src || !d_lwe) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (count == 0) return PZ_OK;
+    if (stream) st = (hipStream_t)stream;
+    const size_t total = count * ((size_t)p.rlwe_n + 1);
+    const int grid = (int)((total + 255) / 256 < 65535 * 4 ? (total + 255) / 256 : 65535 * 4);
+    hipLaunchKernelGGL(k_sample_extract, grid, 256, 0, st, d_rlwe, d_src, count, p.rlwe_n,
+                       p.rlwe_q, d_lwe);
+    return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
+}

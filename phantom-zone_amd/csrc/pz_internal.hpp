@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/pz.h"
 #include "pz_device.hpp"
 
 typedef uint8_t u8;
@@ -56,3 +57,14 @@ void launch_pack_ksk(const u64 *in, void *out, size_t rows, u32 n1, u32 ldk, u32
                      hipStream_t s);
 
 }  // namespace pz
+
+// Key-switching key for RLWE ciphertexts (Ksk(u -> s)): [2d][N] rows in the
+// evaluation domain, A_0..A_{d-1} then B_0..B_{d-1}, with Shoup twins.
+struct pz_rlwe_ksk {
+    pz_ctx *owner = nullptr;
+    u64 *rows = nullptr, *rows_s = nullptr;
+    u32 d = 0;
+};
+// takes ownership of device rows d_rows (evaluation domain)
+int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out);
+

@@ -356,3 +356,170 @@ extern "C" int pz_decrypt_bools(pz_ctx *c, const int32_t *h_s, uint32_t dim, con
     }
     return bad ? PZ_ERR_STATE : PZ_OK;
 }
+
+// ---- input path of the non-interactive protocol (the library's own seeded
+// procedures; structure of ClientKey::sk_u_rlwe, ui_to_s_ksk and the batched
+// bool encryption, src/bool/keys.rs:75-154, src/bool/ni_mp_api.rs:294-352,
+// seeded_secret_key_encrypt_rlwe src/rgsw/keygen.rs:439-481).
+extern int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out);
+
+extern "C" int pz_keygen_user_secret(pz_ctx *c, const uint8_t *seed, int32_t *h_u)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!seed || !h_u) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    host::ChaCha8 r(seed);
+    host::ternary_hw(r, h_u, p.rlwe_n, p.rlwe_n >> 1);
+    return PZ_OK;
+}
+
+// Ksk(u -> s): per row j, N uniform a then N noise draws (stream order);
+// B_j = (e + a*s) + 2^(logb*j + ib)*u, rows [A_0..A_{d-1}, B_0..B_{d-1}],
+// then Ntt::forward of every row
+extern "C" int pz_keygen_rlwe_ksk(pz_ctx *c, const uint8_t *seed, const int32_t *h_u,
+                                  const int32_t *h_s, pz_rlwe_ksk **out)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!seed || !h_u || !h_s || !out) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (p.uitos_d == 0) return PZ_ERR_UNSUPPORTED;
+    const u64 q = p.rlwe_q;
+    const u32 N = p.rlwe_n, d = p.uitos_d;
+    host::ChaCha8 r(seed);
+    std::vector<u64> ha((size_t)d * N), he((size_t)d * N), hs(N), hu(N);
+    for (u32 j = 0; j < d; j++) {
+        for (u32 i = 0; i < N; i++) ha[(size_t)j * N + i] = r.uniform(q);
+        for (u32 i = 0; i < N; i++) he[(size_t)j * N + i] = host::map_i64(q, host::gauss_round(r));
+    }
+    for (u32 i = 0; i < N; i++) {
+        hs[i] = host::map_i64(q, h_s[i]);
+        hu[i] = host::map_i64(q, h_u[i]);
+    }
+    bool ok = true;
+    u64 *d_a = nullptr, *d_e = nullptr, *d_as = nullptr, *d_s = nullptr, *d_u = nullptr;
+    u64 *d_rows = nullptr;
+    RowPair *d_pairs = nullptr;
+    std::vector<RowPair> pairs(d);
+    KG_CHK(hipMalloc(&d_a, 8 * (size_t)d * N));
+    KG_CHK(hipMalloc(&d_e, 8 * (size_t)d * N));
+    KG_CHK(hipMalloc(&d_as, 8 * (size_t)d * N));
+    KG_CHK(hipMalloc(&d_s, 8 * (size_t)N));
+    KG_CHK(hipMalloc(&d_u, 8 * (size_t)N));
+    KG_CHK(hipMalloc(&d_rows, 16 * (size_t)d * N));
+    KG_CHK(hipMalloc(&d_pairs, sizeof(RowPair) * d));
+    KG_CHK(hipMemcpyAsync(d_a, ha.data(), 8 * ha.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_as, ha.data(), 8 * ha.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_e, he.data(), 8 * he.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_s, hs.data(), 8 * (size_t)N, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_u, hu.data(), 8 * (size_t)N, hipMemcpyHostToDevice, st));
+    launch_ntt(logn, 1, d_s, 1, dc.tb, st);
+    launch_ntt(logn, 1, d_as, d, dc.tb, st);
+    hipLaunchKernelGGL(k_barrett_rows, grid_of((size_t)d * N), 256, 0, st, d_as, d_s, (size_t)d,
+                       N, dc.br);
+    launch_ntt(logn, 3, d_as, d, dc.tb, st);
+    for (u32 j = 0; j < d; j++) {
+        RowPair &rp = pairs[j];
+        rp.a = d_a + (size_t)j * N;
+        rp.e = d_e + (size_t)j * N;
+        rp.as = d_as + (size_t)j * N;
+        rp.msg = d_u;
+        rp.mono_pos = 0;
+        rp.mono_val = 0;
+        rp.beta = 1ull << (p.uitos_logb * j + dc.dec_uitos.ignore_bits);
+        rp.dst_a = d_rows + (size_t)j * N;
+        rp.dst_b = d_rows + (size_t)(d + j) * N;
+        rp.msg_in_a = 0;
+    }
+    KG_CHK(hipMemcpyAsync(d_pairs, pairs.data(), sizeof(RowPair) * d, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_kg_combine, grid_of((size_t)d * N), 256, 0, st, d_pairs, (size_t)d, N,
+                       dc.br);
+    launch_ntt(logn, 1, d_rows, 2 * (size_t)d, dc.tb, st);
+    KG_CHK(hipGetLastError());
+    KG_CHK(hipStreamSynchronize(st));
+    rc = pz_internal_rlwe_ksk_adopt(c, d_rows, d, out);
+    if (rc == PZ_OK) d_rows = nullptr;  // owned by the ksk now
+done:
+    hipFree(d_a);
+    hipFree(d_e);
+    hipFree(d_as);
+    hipFree(d_s);
+    hipFree(d_u);
+    hipFree(d_rows);
+    hipFree(d_pairs);
+    if (!ok) return PZ_ERR_HIP;
+    return rc;
+}
+
+// Bool batch under u: sub-seeds seed_a, seed_e are the first and second 32
+// bytes of ChaCha8(seed); per ciphertext N uniform a from seed_a's stream and
+// N noise draws from seed_e's; b = (e + m) + a*u with a*u through
+// forward / Barrett pointwise / backward
+extern "C" int pz_encrypt_rlwe_bools(pz_ctx *c, const int32_t *h_u, const uint8_t *seed,
+                                     const uint8_t *h_bits, size_t count, uint64_t *h_out)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!h_u || !seed || !h_bits || !h_out) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (count == 0) return PZ_OK;
+    const u64 q = p.rlwe_q, tru = q >> 3, fal = (q - 1) - tru + 1;
+    const u32 N = p.rlwe_n;
+    const size_t nr = (count + N - 1) / N;
+    uint8_t sa[32], se[32];
+    {
+        host::ChaCha8 master(seed);
+        master.fill_bytes(sa, 32);
+        master.fill_bytes(se, 32);
+    }
+    host::ChaCha8 ra(sa), re(se);
+    std::vector<u64> ha(nr * N), hm(nr * N), hu(N);
+    for (size_t k = 0; k < nr; k++) {
+        for (u32 i = 0; i < N; i++) ha[k * N + i] = ra.uniform(q);
+        for (u32 i = 0; i < N; i++) {
+            size_t t = k * N + i;
+            u64 e = host::map_i64(q, host::gauss_round(re));
+            u64 m = t < count ? (h_bits[t] ? tru : fal) : 0;
+            u64 v = e + m;
+            hm[t] = v >= q ? v - q : v;  // e + m (add_mod_fast)
+        }
+    }
+    for (u32 i = 0; i < N; i++) hu[i] = host::map_i64(q, h_u[i]);
+    bool ok = true;
+    u64 *d_as = nullptr, *d_u = nullptr;
+    KG_CHK(hipMalloc(&d_as, 8 * nr * N));
+    KG_CHK(hipMalloc(&d_u, 8 * (size_t)N));
+    KG_CHK(hipMemcpyAsync(d_as, ha.data(), 8 * ha.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_u, hu.data(), 8 * (size_t)N, hipMemcpyHostToDevice, st));
+    launch_ntt(logn, 1, d_u, 1, dc.tb, st);
+    launch_ntt(logn, 1, d_as, nr, dc.tb, st);
+    hipLaunchKernelGGL(k_barrett_rows, grid_of(nr * N), 256, 0, st, d_as, d_u, nr, N, dc.br);
+    launch_ntt(logn, 3, d_as, nr, dc.tb, st);
+    {
+        std::vector<u64> has(nr * N);
+        KG_CHK(hipMemcpyAsync(has.data(), d_as, 8 * has.size(), hipMemcpyDeviceToHost, st));
+        KG_CHK(hipStreamSynchronize(st));
+        for (size_t k = 0; k < nr; k++) {
+            uint64_t *o = h_out + k * 2 * N;
+            for (u32 i = 0; i < N; i++) {
+                o[i] = ha[k * N + i];
+                u64 v = hm[k * N + i] + has[k * N + i];
+                o[N + i] = v >= q ? v - q : v;
+            }
+        }
+    }
+done:
+    hipFree(d_as);
+    hipFree(d_u);
+    return ok ? PZ_OK : PZ_ERR_HIP;
+}

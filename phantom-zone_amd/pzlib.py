@@ -28,6 +28,7 @@ class Params(C.Structure):
         ("rlrg_logb", C.c_uint32), ("rlrg_da", C.c_uint32), ("rlrg_db", C.c_uint32),
         ("auto_logb", C.c_uint32), ("auto_d", C.c_uint32),
         ("g", C.c_uint32), ("w", C.c_uint32), ("lwe_secret_gaussian", C.c_uint32),
+        ("uitos_logb", C.c_uint32), ("uitos_d", C.c_uint32),
     ]
 
 
@@ -94,6 +95,14 @@ SIGNATURES = {
                                  C.POINTER(C.c_float)]),
     "pz_valu_probe": (C.c_int, [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_double)]),
     "pz_set_blind_rotate_mode": (C.c_int, [_vp, C.c_int]),
+    "pz_keygen_user_secret": (C.c_int, [_vp, _vp, _vp]),
+    "pz_keygen_rlwe_ksk": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "pz_rlwe_ksk_upload": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "pz_rlwe_ksk_download": (C.c_int, [_vp, _vp]),
+    "pz_rlwe_ksk_free": (C.c_int, [_vp]),
+    "pz_encrypt_rlwe_bools": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_rlwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_sample_extract": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp]),
 }
 
 _lib = None
@@ -345,6 +354,31 @@ class Key:
             pass
 
 
+class RlweKsk:
+    """Ksk(u_j -> s) resident in HBM (ui_to_s_ksk of the server key)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    def download(self):
+        p = self.ctx.p
+        out = np.zeros((2 * p.uitos_d, p.rlwe_n), dtype=np.uint64)
+        _check(self.ctx.L.pz_rlwe_ksk_download(self.h, out.ctypes.data))
+        return out
+
+    def free(self):
+        if self.h:
+            self.ctx.L.pz_rlwe_ksk_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """One evaluator on one device (BoolEvaluator over the GPU engine)."""
 
@@ -458,6 +492,41 @@ class Context:
         _check(self.L.pz_decrypt_bools(self.h, s.ctypes.data, len(s), c.ctypes.data, c.shape[0],
                                        out.ctypes.data))
         return out
+
+    # ---- input path (non-interactive sets) ----
+    def keygen_user_secret(self, seed):
+        u = np.zeros(self.N, dtype=np.int32)
+        _check(self.L.pz_keygen_user_secret(self.h, self.seed32(seed).ctypes.data, u.ctypes.data))
+        return u
+
+    def keygen_rlwe_ksk(self, seed, u, s):
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        h = C.c_void_p()
+        _check(self.L.pz_keygen_rlwe_ksk(self.h, self.seed32(seed).ctypes.data, u.ctypes.data,
+                                         s.ctypes.data, C.byref(h)))
+        return RlweKsk(self, h.value)
+
+    def rlwe_ksk_upload(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        h = C.c_void_p()
+        _check(self.L.pz_rlwe_ksk_upload(self.h, rows.ctypes.data, C.byref(h)))
+        return RlweKsk(self, h.value)
+
+    def encrypt_rlwe_bools(self, u, bits, seed):
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        b = np.ascontiguousarray(bits, dtype=np.uint8)
+        nr = (len(b) + self.N - 1) // self.N
+        out = np.zeros((nr, 2, self.N), dtype=np.uint64)
+        _check(self.L.pz_encrypt_rlwe_bools(self.h, u.ctypes.data, self.seed32(seed).ctypes.data,
+                                            b.ctypes.data, len(b), out.ctypes.data))
+        return out
+
+    def rlwe_key_switch(self, ksk, d_in, d_out, B):
+        _check(self.L.pz_rlwe_key_switch(self.h, ksk.h, d_in.ptr, d_out.ptr, B, None))
+
+    def sample_extract(self, d_rlwe, d_src, count, d_lwe):
+        _check(self.L.pz_sample_extract(self.h, d_rlwe.ptr, d_src.ptr, count, d_lwe.ptr, None))
 
     def rlwe_by_rgsw(self, key, d_rlwe, d_idx, d_triv, B):
         _check(self.L.pz_rlwe_by_rgsw(self.h, key.h, d_rlwe.ptr, d_idx.ptr, d_triv.ptr, B, None))
