@@ -45,6 +45,7 @@ typedef struct pz_params {
     uint32_t g, w;
     uint32_t lwe_secret_gaussian;
     uint32_t uitos_logb, uitos_d; /* u_j -> s RLWE key switch (NI sets; 0 = none) */
+    uint32_t rgrg_logb, rgrg_da, rgrg_db; /* RGSW x RGSW (multi-party sets; 0 = none) */
 } pz_params;
 
 typedef struct pz_ctx pz_ctx;
@@ -279,6 +280,16 @@ int pz_rlwe_key_switch(pz_ctx *ctx, pz_rlwe_ksk *ksk, const uint64_t *d_in, uint
 /* d_lwe[t] = sample_extract(d_rlwe[d_src[2t]], index d_src[2t+1]) ([N+1]) */
 int pz_sample_extract(pz_ctx *ctx, const uint64_t *d_rlwe, const uint32_t *d_src, size_t count,
                       uint64_t *d_lwe, void *stream);
+
+/* ---- RGSW x RGSW product of the non-interactive server-key aggregation:
+ * rgsw_by_rgsw_inplace (src/rgsw/runtime.rs:883-985) on B pairs.
+ * d_rgsw0[B][2 rlrg_da + 2 rlrg_db][N] (coefficient domain) is overwritten
+ * by its product with d_rgsw1[B][2 rgrg_da + 2 rgrg_db][N] (evaluation
+ * domain), as the aggregation does per LWE index and party
+ * (src/bool/evaluator.rs:1505-1535).  PZ_ERR_UNSUPPORTED where the
+ * parameter set has no RGSW x RGSW decomposer. */
+int pz_rgsw_by_rgsw(pz_ctx *ctx, uint64_t *d_rgsw0, const uint64_t *d_rgsw1, size_t B,
+                    void *stream);
 
 /* Blind-rotation kernel choice.  No counterpart in the reference (its
  * evaluator is single-threaded): 0 = automatic, 1 = one work group per

@@ -33,6 +33,7 @@ class Params(C.Structure):
         ("auto_logb", C.c_uint32), ("auto_d", C.c_uint32),
         ("g", C.c_uint32), ("w", C.c_uint32), ("lwe_secret_gaussian", C.c_uint32),
         ("uitos_logb", C.c_uint32), ("uitos_d", C.c_uint32),
+        ("rgrg_logb", C.c_uint32), ("rgrg_da", C.c_uint32), ("rgrg_db", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -133,6 +134,7 @@ def lib():
             "pzo_keygen_rlwe_ksk": (None, [vp, u8p, i32p, i32p, u64p]),
             "pzo_rlwe_key_switch": (None, [vp, u64p, u64p, u64p, u64p]),
             "pzo_encrypt_rlwe_bools": (None, [vp, u8p, i32p, u8p, C.c_size_t, u64p]),
+            "pzo_rgsw_by_rgsw": (None, [vp, u64p, u64p]),
             "pzo_decrypt_lwe": (C.c_uint64, [vp, u64p, i32p, C.c_uint32]),
             "pzo_decode_bool": (C.c_int, [vp, C.c_uint64]),
             "pzo_true_el": (C.c_uint64, [vp]),
@@ -355,6 +357,12 @@ class Oracle:
         out = np.zeros(self.N + 1, dtype=np.uint64)
         self.L.pzo_sample_extract(self.ev, out, np.ascontiguousarray(rlwe, dtype=np.uint64).reshape(-1),
                                   index)
+        return out
+
+    def rgsw_by_rgsw(self, rgsw0, rgsw1):
+        out = np.ascontiguousarray(rgsw0, dtype=np.uint64).copy()
+        self.L.pzo_rgsw_by_rgsw(self.ev, out.reshape(-1),
+                                np.ascontiguousarray(rgsw1, dtype=np.uint64).reshape(-1))
         return out
 
     # ---- encryption ----

@@ -49,7 +49,24 @@ typedef unsigned __int128 u128;
 /* ------------------------------------------------------------------------- */
 /* Parameter sets: src/bool/parameters.rs:526-726                            */
 /* ------------------------------------------------------------------------- */
+/* rgrg_decomposer_params, src/bool/parameters.rs:526-726 (SP_TEST: none) */
+static const uint32_t PZO_RGRG[8][3] = {{4, 10, 9}, {3, 13, 12}, {2, 20, 18}, {0, 0, 0},
+                                        {7, 6, 5},  {6, 7, 6},   {5, 9, 8},   {4, 10, 9}};
+
+static int params_base(int which, pzo_params *o);
+
 int pzo_params_named(int which, pzo_params *o)
+{
+    int rc = params_base(which, o);
+    if (rc == 0) {
+        o->rgrg_logb = PZO_RGRG[which][0];
+        o->rgrg_da = PZO_RGRG[which][1];
+        o->rgrg_db = PZO_RGRG[which][2];
+    }
+    return rc;
+}
+
+static int params_base(int which, pzo_params *o)
 {
     memset(o, 0, sizeof(*o));
     o->rlwe_q = 18014398509404161ULL;
@@ -1501,6 +1518,51 @@ void pzo_encrypt_rlwe_bools(const pzo_eval *e, const uint8_t seed[32], const int
     }
     free(u_eval); free(tmp);
     pzo_rng_free(ra); pzo_rng_free(re);
+}
+
+/* poly_fma_routine, src/rgsw/runtime.rs:363-372: out += sum_r a[r] . b[r]
+ * (elwise_fma_mut: add_mod_fast(out, mul_mod_fast(a, b))) */
+static void poly_fma_rows(const modq *m, uint64_t *out, const uint64_t *a, const uint64_t *b,
+                          uint32_t rows, uint32_t N)
+{
+    for (uint32_t r = 0; r < rows; r++)
+        for (uint32_t i = 0; i < N; i++)
+            out[i] = mm_add(m, out[i], mm_mul(m, a[(size_t)r * N + i], b[(size_t)r * N + i]));
+}
+
+void pzo_rgsw_by_rgsw(const pzo_eval *e, uint64_t *rgsw0, const uint64_t *rgsw1)
+{
+    const pzo_params *p = &e->p;
+    uint32_t N = p->rlwe_n, da0 = p->rlrg_da, db0 = p->rlrg_db;
+    uint32_t da1 = p->rgrg_da, db1 = p->rgrg_db;
+    decomp dca, dcb;
+    decomp_init(&dca, p->rlwe_q, p->rgrg_logb, da1);
+    decomp_init(&dcb, p->rlwe_q, p->rgrg_logb, db1);
+    const uint32_t r0 = 2 * da0 + 2 * db0;
+    uint32_t dmax = da1 > db1 ? da1 : db1;
+    uint64_t *space = (uint64_t *)calloc((size_t)r0 * N, 8);
+    uint64_t *dec = (uint64_t *)calloc((size_t)dmax * N, 8);
+    /* rgsw1 parts: nsm A, nsm B (da1 rows each), m A, m B (db1 rows each) */
+    const uint64_t *nsm_a = rgsw1, *nsm_b = rgsw1 + (size_t)da1 * N;
+    const uint64_t *m_a = rgsw1 + (size_t)2 * da1 * N, *m_b = m_a + (size_t)db1 * N;
+    /* RLWE pairs of rgsw0: nsm rows (i, da0 + i), then m rows (2da0 + i, 2da0 + db0 + i) */
+    for (uint32_t k = 0; k < da0 + db0; k++) {
+        uint32_t ra = k < da0 ? k : 2 * da0 + (k - da0);
+        uint32_t rb = k < da0 ? da0 + k : 2 * da0 + db0 + (k - da0);
+        const uint64_t *rlwe_a = rgsw0 + (size_t)ra * N, *rlwe_b = rgsw0 + (size_t)rb * N;
+        uint64_t *out_a = space + (size_t)ra * N, *out_b = space + (size_t)rb * N;
+        decompose_r(&dca, rlwe_a, dec, N);
+        for (uint32_t r = 0; r < da1; r++) ntt_fwd(e->ntt, dec + (size_t)r * N, 1);
+        poly_fma_rows(&e->mq, out_a, dec, nsm_a, da1, N);
+        poly_fma_rows(&e->mq, out_b, dec, nsm_b, da1, N);
+        decompose_r(&dcb, rlwe_b, dec, N);
+        for (uint32_t r = 0; r < db1; r++) ntt_fwd(e->ntt, dec + (size_t)r * N, 1);
+        poly_fma_rows(&e->mq, out_a, dec, m_a, db1, N);
+        poly_fma_rows(&e->mq, out_b, dec, m_b, db1, N);
+    }
+    memcpy(rgsw0, space, 8 * (size_t)r0 * N);
+    for (uint32_t r = 0; r < r0; r++) ntt_inv(e->ntt, rgsw0 + (size_t)r * N, 1);
+    free(space); free(dec);
 }
 
 /* encrypt_lwe, src/lwe.rs:104-137 (operator = RLWE modop, secret of dim) */

@@ -35,6 +35,7 @@ typedef struct pzo_params {
     uint32_t g, w;                          /* generator, window              */
     uint32_t lwe_secret_gaussian;           /* 1: error dist, 0: ternary      */
     uint32_t uitos_logb, uitos_d;           /* u_j -> s RLWE key switch (NI)  */
+    uint32_t rgrg_logb, rgrg_da, rgrg_db;   /* RGSW x RGSW (multi-party)      */
 } pzo_params;
 
 /* Named parameter sets: 0 NI_2P, 1 NI_4P_HB_FR, 2 NI_8P, 3 SP_TEST_BOOL_PARAMS,
@@ -189,6 +190,11 @@ void pzo_rlwe_key_switch(const pzo_eval *e, const uint64_t *rlwe_in, const uint6
  * stream of seed_e, both derived from seed (first and second 32 bytes) */
 void pzo_encrypt_rlwe_bools(const pzo_eval *e, const uint8_t seed[32], const int32_t *u,
                             const uint8_t *bits, size_t count, uint64_t *out);
+
+/* rgsw_by_rgsw_inplace, src/rgsw/runtime.rs:883-985: rgsw0 [2da+2db][N]
+ * (RLWE x RGSW decomposer, coefficient domain, overwritten) times rgsw1
+ * [2da'+2db'][N] (RGSW x RGSW decomposer, evaluation domain) */
+void pzo_rgsw_by_rgsw(const pzo_eval *e, uint64_t *rgsw0, const uint64_t *rgsw1);
 
 void pzo_encrypt_lwe(const pzo_eval *e, const uint8_t seed[32], const int32_t *s,
                      uint32_t dim, uint64_t m, uint64_t *out);

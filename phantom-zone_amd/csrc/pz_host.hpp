@@ -55,6 +55,16 @@ inline int params_named(const char *name, pz_params *o)
     } else {
         return -1;
     }
+    // RGSW x RGSW decomposers (rgrg_decomposer_params, src/bool/parameters.rs:526-726)
+    struct { const char *name; uint32_t logb, da, db; } rg[] = {
+        {"NI_2P", 4, 10, 9}, {"NI_4P_HB_FR", 3, 13, 12}, {"NI_8P", 2, 20, 18},
+        {"I_2P_LB_SR", 7, 6, 5}, {"I_4P", 6, 7, 6}, {"I_8P", 5, 9, 8}, {"NI_4P_LB_SR", 4, 10, 9}};
+    for (auto &e : rg)
+        if (s == e.name) {
+            o->rgrg_logb = e.logb;
+            o->rgrg_da = e.da;
+            o->rgrg_db = e.db;
+        }
     return 0;
 }
 

@@ -1,6 +1,7 @@
 // pz_input.hip -- input path of the non-interactive protocol on the device:
 // RLWE key switching from a party's secret u_j to the ideal secret s, and
-// sample extraction of one LWE per bit.
+// sample extraction of one LWE per bit; and the RGSW x RGSW product of the
+// server-key aggregation.
 //
 //   k_rlwe_ksw         rlwe_key_switch (src/rgsw/runtime.rs:1001-1063), one
 //                      RLWE per work group: -a decomposed into d digit rows
@@ -8,6 +9,7 @@
 //                      multiply-accumulated into A' and B' (rows of Ksk), one
 //                      strict inverse per output, then B' + b
 //   k_sample_extract   sample_extract (src/pbs.rs:423-449) at any index
+#include "pz_host.hpp"
 #include "pz_internal.hpp"
 
 namespace pz {
@@ -206,3 +208,51 @@ extern "C" int pz_sample_extract(pz_ctx *c, const uint64_t *d_rlwe, const uint32
                        p.rlwe_q, d_lwe);
     return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
 }
+
+// RGSW x RGSW product of the non-interactive server-key aggregation
+// (rgsw_by_rgsw_inplace, src/rgsw/runtime.rs:883-985, applied per LWE index
+// and party at src/bool/evaluator.rs:1505-1535).
+extern "C" int pz_rgsw_by_rgsw(pz_ctx *c, uint64_t *d_rgsw0, const uint64_t *d_rgsw1, size_t B,
+                               void *stream)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!d_rgsw0 || !d_rgsw1) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (p.rgrg_da == 0 || p.rgrg_db == 0) return PZ_ERR_UNSUPPORTED;
+    if (B == 0) return PZ_OK;
+    if (stream) st = (hipStream_t)stream;
+    const u32 N = p.rlwe_n, r0 = 2 * (p.rlrg_da + p.rlrg_db), r1 = 2 * (p.rgrg_da + p.rgrg_db);
+    const size_t words = B * (size_t)r1 * N;
+    // RGSW1 rows times n^-1 (the step's inverse transforms leave it out) and
+    // their Shoup twins, in scratch
+    u64 *k1 = nullptr, *k1s = nullptr;
+    if (hipMalloc(&k1, 8 * words) != hipSuccess) return PZ_ERR_HIP;
+    if (hipMalloc(&k1s, 8 * words) != hipSuccess) {
+        hipFree(k1);
+        return PZ_ERR_HIP;
+    }
+    hipMemcpyAsync(k1, d_rgsw1, 8 * words, hipMemcpyDeviceToDevice, st);
+    launch_scale(k1, words, dc.tb.n_inv, dc.tb.n_inv_s, p.rlwe_q, st);
+    launch_shoup(k1, k1s, words, p.rlwe_q, st);
+    DevCtx cc = dc;
+    const u32 lq = host::ref_log2(p.rlwe_q);
+    cc.dec_a.q = cc.dec_b.q = p.rlwe_q;
+    cc.dec_a.logb = cc.dec_b.logb = p.rgrg_logb;
+    cc.dec_a.d = p.rgrg_da;
+    cc.dec_b.d = p.rgrg_db;
+    cc.dec_a.ignore_bits = lq - p.rgrg_logb * p.rgrg_da;
+    cc.dec_b.ignore_bits = lq - p.rgrg_logb * p.rgrg_db;
+    cc.da = p.rgrg_da;
+    cc.db = p.rgrg_db;
+    launch_rgsw_rows(logn, d_rgsw0, r0, p.rlrg_da, p.rlrg_db, B, cc, k1, k1s, r1, st);
+    hipError_t e = hipGetLastError();
+    hipStreamSynchronize(st);  // the scratch is read by the launch above
+    hipFree(k1);
+    hipFree(k1s);
+    return e == hipSuccess ? PZ_OK : PZ_ERR_HIP;
+}
+

@@ -84,6 +84,63 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *
 }
 
 // ---------------------------------------------------------------------------
+// RGSW x RGSW (rgsw_by_rgsw_inplace, src/rgsw/runtime.rs:883-985): each work
+// group takes one RLWE row pair of one RGSW0 (coefficient domain; rows (i,
+// da0 + i) of the -sm part, then (2 da0 + i, 2 da0 + db0 + i) of the m part)
+// and multiplies it by that RGSW0's RGSW1 (evaluation domain, rows scaled by
+// n^-1, Shoup twins) with the RGSW x RGSW decomposers in c.dec_a / c.dec_b.
+// ---------------------------------------------------------------------------
+template <int LOGN>
+__global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_rgsw_rows(
+    u64 *rgsw0, u32 r0, u32 da0, u32 db0, size_t B, DevCtx c, const u64 *K1, const u64 *K1s,
+    u32 r1)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ u64 lds[];
+    const u32 j = threadIdx.x;
+    const u32 npairs = da0 + db0;
+    for (size_t w = blockIdx.x; w < B * npairs; w += gridDim.x) {
+        const size_t bb = w / npairs;
+        const u32 k = (u32)(w % npairs);
+        const u32 ra = k < da0 ? k : 2 * da0 + (k - da0);
+        const u32 rb = k < da0 ? da0 + k : 2 * da0 + db0 + (k - da0);
+        u64 *pa = rgsw0 + (bb * r0 + ra) * N, *pb = rgsw0 + (bb * r0 + rb) * N;
+        u64 a[8], b[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            a[r] = pa[coef_x<LOGN>(j, r)];
+            b[r] = pb[coef_x<LOGN>(j, r)];
+        }
+        const size_t off = bb * r1 * N;
+        ext_product_blk<LOGN>(a, b, K1 + off, K1s + off, false, c, j, lds);
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            pa[coef_x<LOGN>(j, r)] = a[r];
+            pb[coef_x<LOGN>(j, r)] = b[r];
+        }
+    }
+}
+
+void launch_rgsw_rows(int logn, u64 *rgsw0, u32 r0, u32 da0, u32 db0, size_t B, const DevCtx &c,
+                      const u64 *K1, const u64 *K1s, u32 r1, hipStream_t s)
+{
+    const size_t work = B * (size_t)(da0 + db0);
+    if (work == 0) return;
+    const int grid = (int)(work < 65535 ? work : 65535);
+    const size_t lds = 2 * 8 * (size_t)(((size_t)1 << logn) + ((size_t)1 << logn) / 8);
+    switch (logn) {
+    case 9:
+        hipLaunchKernelGGL(k_rgsw_rows<9>, grid, 1 << 6, lds, s, rgsw0, r0, da0, db0, B, c, K1,
+                           K1s, r1);
+        break;
+    case 11:
+        hipLaunchKernelGGL(k_rgsw_rows<11>, grid, 1 << 8, lds, s, rgsw0, r0, da0, db0, B, c, K1,
+                           K1s, r1);
+        break;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_blind_rotate: persistent PBS tail (src/pbs.rs:146-232).
 //   ks:   [B][n+1] key-switched LWE (mod q_ks)
 //   gate: [B] test-vector index

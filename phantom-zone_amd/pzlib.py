@@ -29,6 +29,7 @@ class Params(C.Structure):
         ("auto_logb", C.c_uint32), ("auto_d", C.c_uint32),
         ("g", C.c_uint32), ("w", C.c_uint32), ("lwe_secret_gaussian", C.c_uint32),
         ("uitos_logb", C.c_uint32), ("uitos_d", C.c_uint32),
+        ("rgrg_logb", C.c_uint32), ("rgrg_da", C.c_uint32), ("rgrg_db", C.c_uint32),
     ]
 
 
@@ -103,6 +104,7 @@ SIGNATURES = {
     "pz_encrypt_rlwe_bools": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_rlwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_sample_extract": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp]),
+    "pz_rgsw_by_rgsw": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp]),
 }
 
 _lib = None
@@ -527,6 +529,9 @@ class Context:
 
     def sample_extract(self, d_rlwe, d_src, count, d_lwe):
         _check(self.L.pz_sample_extract(self.h, d_rlwe.ptr, d_src.ptr, count, d_lwe.ptr, None))
+
+    def rgsw_by_rgsw(self, d_rgsw0, d_rgsw1, B):
+        _check(self.L.pz_rgsw_by_rgsw(self.h, d_rgsw0.ptr, d_rgsw1.ptr, B, None))
 
     def rlwe_by_rgsw(self, key, d_rlwe, d_idx, d_triv, B):
         _check(self.L.pz_rlwe_by_rgsw(self.h, key.h, d_rlwe.ptr, d_idx.ptr, d_triv.ptr, B, None))
