@@ -24,6 +24,14 @@ i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
+def _u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64).reshape(-1)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+
+
 class Params(C.Structure):
     _fields_ = [
         ("rlwe_q", C.c_uint64), ("rlwe_n", C.c_uint32), ("lwe_n", C.c_uint32),
@@ -135,6 +143,17 @@ def lib():
             "pzo_rlwe_key_switch": (None, [vp, u64p, u64p, u64p, u64p]),
             "pzo_encrypt_rlwe_bools": (None, [vp, u8p, i32p, u8p, C.c_size_t, u64p]),
             "pzo_rgsw_by_rgsw": (None, [vp, u64p, u64p]),
+            "pzo_puncture": (None, [u8p, C.c_uint32, u8p]),
+            "pzo_ni_leader": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "pzo_ni_share_uitos": (None, [vp, u8p, u8p, C.c_uint32, C.c_uint32, i32p, i32p,
+                                          u64p, u64p]),
+            "pzo_ni_share_rgsw": (None, [vp, u8p, u8p, C.c_uint32, C.c_uint32, i32p, i32p, i32p,
+                                         C.c_uint32, C.c_uint32, u64p, u64p]),
+            "pzo_ni_uitos_ksk": (None, [vp, u8p, C.c_uint32, C.c_uint32, u64p, u64p, u64p]),
+            "pzo_ni_produce": (None, [vp, u8p, C.c_uint32, C.c_uint32, u64p, u64p, u64p,
+                                      C.c_int, u64p]),
+            "pzo_ni_aggregate_rgsw": (None, [vp, u8p, C.c_uint32, u64p, C.c_uint32, C.c_uint32,
+                                             u64p, u64p, u64p]),
             "pzo_decrypt_lwe": (C.c_uint64, [vp, u64p, i32p, C.c_uint32]),
             "pzo_decode_bool": (C.c_int, [vp, C.c_uint64]),
             "pzo_true_el": (C.c_uint64, [vp]),
@@ -150,6 +169,8 @@ def lib():
 
 def seed_bytes(seed):
     """32-byte seed array from an int or bytes."""
+    if isinstance(seed, np.ndarray):
+        seed = seed.astype(np.uint8).tobytes()
     if isinstance(seed, (bytes, bytearray)):
         b = bytes(seed).ljust(32, b"\0")[:32]
     else:
@@ -363,6 +384,72 @@ class Oracle:
         out = np.ascontiguousarray(rgsw0, dtype=np.uint64).copy()
         self.L.pzo_rgsw_by_rgsw(self.ev, out.reshape(-1),
                                 np.ascontiguousarray(rgsw1, dtype=np.uint64).reshape(-1))
+        return out
+
+    # ---- non-interactive server key: shares and aggregation ----
+    @property
+    def embedding(self):
+        return 2 * self.N // self.p.br_q
+
+    @property
+    def ni_dmax(self):
+        return max(self.p.rgrg_da, self.p.rgrg_db)
+
+    def puncture(self, seed, times):
+        out = np.zeros(32, dtype=np.uint8)
+        self.L.pzo_puncture(seed_bytes(seed), times, out)
+        return out
+
+    def keygen_party(self, seed):
+        """One party's (s_j, z_j): the secrets of a single-party draw."""
+        s = np.zeros(self.N, dtype=np.int32)
+        z = np.zeros(self.p.lwe_n, dtype=np.int32)
+        self.L.pzo_keygen_secrets(self.ev, seed_bytes(seed), 1, s, z)
+        return s, z
+
+    def ni_leader(self, l, total):
+        return int(self.L.pzo_ni_leader(l, total, self.p.lwe_n))
+
+    def ni_share_uitos(self, crs, noise_seed, me, total, s, u):
+        d, N = self.p.uitos_d, self.N
+        ksk_b = np.zeros((d, N), dtype=np.uint64)
+        zo = np.zeros((max(total - 1, 1), d, N), dtype=np.uint64)
+        self.L.pzo_ni_share_uitos(self.ev, seed_bytes(crs), seed_bytes(noise_seed), me, total,
+                                  _i32(s), _i32(u), ksk_b.reshape(-1), zo.reshape(-1))
+        return ksk_b, zo[:total - 1]
+
+    def ni_share_rgsw(self, crs, noise_seed, me, total, s, u, z, l0, count):
+        N = self.N
+        zero = np.zeros((count, self.p.rgrg_da, N), dtype=np.uint64)
+        cts = np.zeros((count, self.ni_dmax, N), dtype=np.uint64)
+        self.L.pzo_ni_share_rgsw(self.ev, seed_bytes(crs), seed_bytes(noise_seed), me, total,
+                                 _i32(s), _i32(u), _i32(z), l0, count, zero.reshape(-1),
+                                 cts.reshape(-1))
+        return zero, cts
+
+    def ni_uitos_ksk(self, crs, total, user, ksk_b, zero_for_user):
+        d, N = self.p.uitos_d, self.N
+        out = np.zeros((2 * d, N), dtype=np.uint64)
+        zf = np.ascontiguousarray(zero_for_user, dtype=np.uint64).reshape(-1)
+        if zf.size == 0:
+            zf = np.zeros(1, dtype=np.uint64)
+        self.L.pzo_ni_uitos_ksk(self.ev, seed_bytes(crs), total, user, _u64(ksk_b), zf,
+                                out.reshape(-1))
+        return out
+
+    def ni_produce(self, crs, total, l, ksk, zero_l, ct, leader):
+        p = self.p
+        rows = 2 * (p.rlrg_da + p.rlrg_db) if leader else 2 * (p.rgrg_da + p.rgrg_db)
+        out = np.zeros((rows, self.N), dtype=np.uint64)
+        self.L.pzo_ni_produce(self.ev, seed_bytes(crs), total, l, _u64(ksk), _u64(zero_l),
+                              _u64(ct), 1 if leader else 0, out.reshape(-1))
+        return out
+
+    def ni_aggregate_rgsw(self, crs, total, ksks, l0, count, zero, cts):
+        p = self.p
+        out = np.zeros((count, 2 * (p.rlrg_da + p.rlrg_db), self.N), dtype=np.uint64)
+        self.L.pzo_ni_aggregate_rgsw(self.ev, seed_bytes(crs), total, _u64(ksks), l0, count,
+                                     _u64(zero), _u64(cts), out.reshape(-1))
         return out
 
     # ---- encryption ----

@@ -1565,6 +1565,269 @@ void pzo_rgsw_by_rgsw(const pzo_eval *e, uint64_t *rgsw0, const uint64_t *rgsw1)
     free(space); free(dec);
 }
 
+/* ---- non-interactive multi-party server key: shares and aggregation ----
+ * The seed tree follows NonInteractiveMultiPartyCrs
+ * (src/bool/evaluator.rs:97-160): puncture(seed, t) draws t 32-byte seeds
+ * from ChaCha8(seed) and keeps the last (puncture_p_rng, src/utils.rs:219-228);
+ *   index l:  puncture(puncture(puncture(crs, 1), 1), l + 1)
+ *   user j:   puncture(puncture(crs, 2), j + 1)
+ * Noise streams are the oracle's own: ChaCha8(puncture(noise_seed, 1)) for
+ * the u->s parts, ChaCha8(puncture(puncture(noise_seed, 2), l + 1)) for LWE
+ * index l. */
+void pzo_puncture(const uint8_t seed[32], uint32_t times, uint8_t out[32])
+{
+    pzo_rng *r = pzo_rng_new(seed);
+    memset(out, 0, 32);
+    for (uint32_t t = 0; t < times; t++) pzo_rng_fill_bytes(r, out, 32);
+    pzo_rng_free(r);
+}
+
+static void ni_seed_index(const uint8_t crs[32], uint32_t l, uint8_t out[32])
+{
+    uint8_t k[32], m[32];
+    pzo_puncture(crs, 1, k);
+    pzo_puncture(k, 1, m);
+    pzo_puncture(m, l + 1, out);
+}
+
+static void ni_seed_user(const uint8_t crs[32], uint32_t j, uint8_t out[32])
+{
+    uint8_t k[32];
+    pzo_puncture(crs, 2, k);
+    pzo_puncture(k, j + 1, out);
+}
+
+/* multi_party_user_id_lwe_segment, src/bool/evaluator.rs:613-626: the
+ * leader of LWE index l (ceil(lwe_n / total) indices per party) */
+uint32_t pzo_ni_leader(uint32_t l, uint32_t total, uint32_t lwe_n)
+{
+    uint32_t per = (lwe_n + total - 1) / total;
+    return l / per;
+}
+
+/* out = a*x + e + beta*msg (msg may be NULL): the row recipe of
+ * non_interactive_ksk_gen / zero encryptions (src/multi_party.rs:186-291)
+ * and of the NI RGSW cts (src/bool/evaluator.rs:1744-1937); e drawn N times
+ * from `noise` */
+static void ni_row(const pzo_eval *e, const uint64_t *a, const uint64_t *x_eval, pzo_rng *noise,
+                   const uint64_t *msg, uint64_t beta, uint64_t *out)
+{
+    uint32_t N = e->p.rlwe_n;
+    const modq *m = &e->mq;
+    mul_by_s(e, a, x_eval, out);
+    for (uint32_t i = 0; i < N; i++)
+        out[i] = mm_add(m, out[i], map_i64(e->p.rlwe_q, pzo_rng_gauss_round(noise)));
+    if (msg)
+        for (uint32_t i = 0; i < N; i++) out[i] = mm_add(m, out[i], mm_mul(m, msg[i], beta));
+}
+
+static uint64_t *secret_eval(const pzo_eval *e, const int32_t *x)
+{
+    uint32_t N = e->p.rlwe_n;
+    uint64_t *v = (uint64_t *)malloc(8 * (size_t)N);
+    for (uint32_t i = 0; i < N; i++) v[i] = map_i64(e->p.rlwe_q, x[i]);
+    ntt_fwd(e->ntt, v, 1);
+    return v;
+}
+
+void pzo_ni_share_uitos(const pzo_eval *e, const uint8_t crs[32], const uint8_t noise_seed[32],
+                        uint32_t self, uint32_t total, const int32_t *s, const int32_t *u,
+                        uint64_t *ksk_b, uint64_t *zero_others)
+{
+    uint32_t N = e->p.rlwe_n, d = e->p.uitos_d;
+    uint64_t q = e->p.rlwe_q;
+    uint8_t ns[32], sd[32];
+    pzo_puncture(noise_seed, 1, ns);
+    pzo_rng *noise = pzo_rng_new(ns);
+    uint64_t *s_eval = secret_eval(e, s), *msg = (uint64_t *)malloc(8 * (size_t)N);
+    uint64_t *a = (uint64_t *)malloc(8 * (size_t)N);
+    for (uint32_t i = 0; i < N; i++) msg[i] = map_i64(q, u[i]);
+    ni_seed_user(crs, self, sd);
+    pzo_rng *pr = pzo_rng_new(sd);
+    for (uint32_t j = 0; j < d; j++) {
+        for (uint32_t i = 0; i < N; i++) a[i] = pzo_rng_uniform(pr, q);
+        uint64_t beta = 1ULL << (e->p.uitos_logb * j + e->dec_uitos.ignore_bits);
+        ni_row(e, a, s_eval, noise, msg, beta, ksk_b + (size_t)j * N);
+    }
+    pzo_rng_free(pr);
+    uint32_t slot = 0;
+    for (uint32_t o = 0; o < total; o++) {
+        if (o == self) continue;
+        ni_seed_user(crs, o, sd);
+        pr = pzo_rng_new(sd);
+        for (uint32_t j = 0; j < d; j++) {
+            for (uint32_t i = 0; i < N; i++) a[i] = pzo_rng_uniform(pr, q);
+            ni_row(e, a, s_eval, noise, NULL, 0, zero_others + ((size_t)slot * d + j) * N);
+        }
+        pzo_rng_free(pr);
+        slot++;
+    }
+    pzo_rng_free(noise);
+    free(s_eval); free(msg); free(a);
+}
+
+/* a_{i,l}, i in [0, d_max): the rows of the index-l stream (the RGSW x RGSW
+ * decomposer's larger count) */
+static void ni_index_rows(const pzo_eval *e, const uint8_t crs[32], uint32_t l, uint32_t dmax,
+                          uint64_t *rows)
+{
+    uint32_t N = e->p.rlwe_n;
+    uint8_t sd[32];
+    ni_seed_index(crs, l, sd);
+    pzo_rng *pr = pzo_rng_new(sd);
+    for (size_t i = 0; i < (size_t)dmax * N; i++) rows[i] = pzo_rng_uniform(pr, e->p.rlwe_q);
+    pzo_rng_free(pr);
+}
+
+static uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+void pzo_ni_share_rgsw(const pzo_eval *e, const uint8_t crs[32], const uint8_t noise_seed[32],
+                       uint32_t self, uint32_t total, const int32_t *s, const int32_t *u,
+                       const int32_t *z, uint32_t l0, uint32_t count, uint64_t *zero,
+                       uint64_t *cts)
+{
+    const pzo_params *p = &e->p;
+    uint32_t N = p->rlwe_n, da = p->rgrg_da, dmax = umax(p->rgrg_da, p->rgrg_db);
+    uint32_t dmax1 = umax(p->rlrg_da, p->rlrg_db);
+    uint32_t lq = ref_log2(p->rlwe_q);
+    uint64_t q = p->rlwe_q;
+    uint8_t nr[32], nl[32];
+    pzo_puncture(noise_seed, 2, nr);
+    uint64_t *s_eval = secret_eval(e, s), *u_eval = secret_eval(e, u);
+    uint64_t *a = (uint64_t *)malloc(8 * (size_t)dmax * N), *msg = (uint64_t *)malloc(8 * (size_t)N);
+    for (uint32_t t = 0; t < count; t++) {
+        uint32_t l = l0 + t;
+        ni_index_rows(e, crs, l, dmax, a);
+        pzo_puncture(nr, l + 1, nl);
+        pzo_rng *noise = pzo_rng_new(nl);
+        /* zero encryptions a_i*s_j + e, i in [dmax - da, dmax) */
+        for (uint32_t k = 0; k < da; k++)
+            ni_row(e, a + (size_t)(dmax - da + k) * N, s_eval, noise, NULL, 0,
+                   zero + ((size_t)t * da + k) * N);
+        /* u_j*a_i + e + beta*X^{emb*z_j[l]} (encode_x_pow_si_with_emebedding_factor,
+         * src/utils.rs:195-217) */
+        memset(msg, 0, 8 * (size_t)N);
+        int64_t si = (int64_t)e->embedding * z[l];
+        if (si < 0) msg[N - (uint32_t)(-si)] = q - 1;
+        else msg[si] = 1;
+        uint64_t *ct = cts + (size_t)t * dmax * N;
+        memset(ct, 0, 8 * (size_t)dmax * N);
+        if (pzo_ni_leader(l, total, p->lwe_n) == self) {
+            uint32_t ib = lq - p->rlrg_logb * dmax1;
+            for (uint32_t k = 0; k < dmax1; k++)
+                ni_row(e, a + (size_t)(dmax - dmax1 + k) * N, u_eval, noise, msg,
+                       1ULL << (ib + p->rlrg_logb * k), ct + (size_t)(dmax - dmax1 + k) * N);
+        } else {
+            uint32_t ib = lq - p->rgrg_logb * dmax;
+            for (uint32_t k = 0; k < dmax; k++)
+                ni_row(e, a + (size_t)k * N, u_eval, noise, msg, 1ULL << (ib + p->rgrg_logb * k),
+                       ct + (size_t)k * N);
+        }
+        pzo_rng_free(noise);
+    }
+    free(s_eval); free(u_eval); free(a); free(msg);
+}
+
+void pzo_ni_uitos_ksk(const pzo_eval *e, const uint8_t crs[32], uint32_t total, uint32_t user,
+                      const uint64_t *ksk_b, const uint64_t *zero_for_user, uint64_t *ksk)
+{
+    uint32_t N = e->p.rlwe_n, d = e->p.uitos_d;
+    uint8_t sd[32];
+    ni_seed_user(crs, user, sd);
+    pzo_rng *pr = pzo_rng_new(sd);
+    for (size_t i = 0; i < (size_t)d * N; i++) ksk[i] = pzo_rng_uniform(pr, e->p.rlwe_q);
+    pzo_rng_free(pr);
+    uint64_t *b = ksk + (size_t)d * N;
+    memcpy(b, ksk_b, 8 * (size_t)d * N);
+    for (uint32_t o = 0; o + 1 < total; o++)
+        for (size_t i = 0; i < (size_t)d * N; i++)
+            b[i] = mm_add(&e->mq, b[i], zero_for_user[(size_t)o * d * N + i]);
+    for (uint32_t row = 0; row < 2 * d; row++) ntt_fwd(e->ntt, ksk + (size_t)row * N, 1);
+}
+
+/* produce_rgsw_ciphertext_from_ni_rgsw, src/bool/evaluator.rs:480-608, for
+ * LWE index l and one party: counts (da_o, db_o) of the output decomposer
+ * (leader: RLWE x RGSW, coefficient domain; others: RGSW x RGSW, evaluation
+ * domain).  zero_l [total][da][N] are every party's zero encryptions of
+ * index l; ct [dmax][N] the party's NI RGSW rows, indexed by the a_{i,l} row
+ * i; ksk [2d][N] the party's Ksk(u_j -> s) in the evaluation domain. */
+void pzo_ni_produce(const pzo_eval *e, const uint8_t crs[32], uint32_t total, uint32_t l,
+                    const uint64_t *ksk, const uint64_t *zero_l, const uint64_t *ct, int leader,
+                    uint64_t *out)
+{
+    const pzo_params *p = &e->p;
+    const modq *m = &e->mq;
+    uint32_t N = p->rlwe_n, d = p->uitos_d, da = p->rgrg_da, db = p->rgrg_db;
+    uint32_t dmax = umax(da, db);
+    uint32_t dao = leader ? p->rlrg_da : da, dbo = leader ? p->rlrg_db : db;
+    uint64_t *a = (uint64_t *)malloc(8 * (size_t)dmax * N);
+    uint64_t *c = (uint64_t *)malloc(8 * (size_t)N), *dig = (uint64_t *)malloc(8 * (size_t)d * N);
+    uint64_t *tmp = (uint64_t *)malloc(8 * (size_t)N);
+    ni_index_rows(e, crs, l, dmax, a);
+    for (uint32_t k = 0; k < dao + dbo; k++) {
+        int nsm = k < dao;
+        uint32_t i = nsm ? dmax - dao + k : dmax - dbo + (k - dao);
+        uint64_t *oa = out + (size_t)(nsm ? k : 2 * dao + (k - dao)) * N;
+        uint64_t *ob = out + (size_t)(nsm ? dao + k : 2 * dao + dbo + (k - dao)) * N;
+        if (nsm) {
+            /* sum_k s_k a_{i,l} + e: aggregated zero encryptions */
+            memset(c, 0, 8 * (size_t)N);
+            for (uint32_t o = 0; o < total; o++)
+                for (uint32_t x = 0; x < N; x++)
+                    c[x] = mm_add(m, c[x], zero_l[((size_t)o * da + (i - (dmax - da))) * N + x]);
+        } else {
+            for (uint32_t x = 0; x < N; x++) c[x] = mm_neg(m, a[(size_t)i * N + x]) % p->rlwe_q;
+        }
+        /* key switch with Ksk(u_j -> s): decomposed, forward, elwise_fma_mut */
+        decompose_r(&e->dec_uitos, c, dig, N);
+        for (uint32_t r = 0; r < d; r++) ntt_fwd(e->ntt, dig + (size_t)r * N, 1);
+        memset(oa, 0, 8 * (size_t)N);
+        memset(ob, 0, 8 * (size_t)N);
+        poly_fma_rows(m, oa, dig, ksk, d, N);
+        poly_fma_rows(m, ob, dig, ksk + (size_t)d * N, d, N);
+        uint64_t *dst = nsm ? oa : ob;
+        memcpy(tmp, ct + (size_t)i * N, 8 * (size_t)N);
+        if (leader) {
+            ntt_inv(e->ntt, oa, 1);
+            ntt_inv(e->ntt, ob, 1);
+        } else {
+            ntt_fwd(e->ntt, tmp, 1);
+        }
+        for (uint32_t x = 0; x < N; x++) dst[x] = mm_add(m, dst[x], tmp[x]);
+    }
+    free(a); free(c); free(dig); free(tmp);
+}
+
+/* The RGSW part of aggregate_non_interactive_multi_party_server_key_shares
+ * (src/bool/evaluator.rs:1222-1565) for LWE indices [l0, l0 + count):
+ * the leader's RGSW times, in ascending party order, every other party's. */
+void pzo_ni_aggregate_rgsw(const pzo_eval *e, const uint8_t crs[32], uint32_t total,
+                           const uint64_t *ksks, uint32_t l0, uint32_t count,
+                           const uint64_t *zero, const uint64_t *cts, uint64_t *out)
+{
+    const pzo_params *p = &e->p;
+    uint32_t N = p->rlwe_n, d = p->uitos_d, da = p->rgrg_da, dmax = umax(p->rgrg_da, p->rgrg_db);
+    size_t r0 = e->rgsw_rows, r1 = 2 * (size_t)(p->rgrg_da + p->rgrg_db);
+    uint64_t *zl = (uint64_t *)malloc(8 * (size_t)total * da * N);
+    uint64_t *other = (uint64_t *)malloc(8 * r1 * N);
+    for (uint32_t t = 0; t < count; t++) {
+        uint32_t l = l0 + t, lead = pzo_ni_leader(l, total, p->lwe_n);
+        for (uint32_t o = 0; o < total; o++)
+            memcpy(zl + (size_t)o * da * N, zero + ((size_t)o * count + t) * da * N,
+                   8 * (size_t)da * N);
+        uint64_t *rg = out + (size_t)t * r0 * N;
+        pzo_ni_produce(e, crs, total, l, ksks + (size_t)lead * 2 * d * N, zl,
+                       cts + ((size_t)lead * count + t) * dmax * N, 1, rg);
+        for (uint32_t o = 0; o < total; o++) {
+            if (o == lead) continue;
+            pzo_ni_produce(e, crs, total, l, ksks + (size_t)o * 2 * d * N, zl,
+                           cts + ((size_t)o * count + t) * dmax * N, 0, other);
+            pzo_rgsw_by_rgsw(e, rg, other);
+        }
+    }
+    free(zl); free(other);
+}
+
 /* encrypt_lwe, src/lwe.rs:104-137 (operator = RLWE modop, secret of dim) */
 void pzo_encrypt_lwe(const pzo_eval *e, const uint8_t seed[32], const int32_t *s, uint32_t dim,
                      uint64_t m, uint64_t *out)

@@ -196,6 +196,43 @@ void pzo_encrypt_rlwe_bools(const pzo_eval *e, const uint8_t seed[32], const int
  * [2da'+2db'][N] (RGSW x RGSW decomposer, evaluation domain) */
 void pzo_rgsw_by_rgsw(const pzo_eval *e, uint64_t *rgsw0, const uint64_t *rgsw1);
 
+/* ---- non-interactive server key: shares and aggregation (the seed tree of
+ * NonInteractiveMultiPartyCrs, src/bool/evaluator.rs:97-160; the oracle's own
+ * noise streams) ---- */
+void pzo_puncture(const uint8_t seed[32], uint32_t times, uint8_t out[32]);
+uint32_t pzo_ni_leader(uint32_t l, uint32_t total, uint32_t lwe_n);
+/* party self's u->s parts: ksk_b [d][N] (a_{i,self}*s_j + e + beta_i*u_j) and
+ * zero encryptions a_{i,o}*s_j + e for every other party o, ascending,
+ * [(total-1)][d][N]; coefficient domain (src/multi_party.rs:186-291) */
+void pzo_ni_share_uitos(const pzo_eval *e, const uint8_t crs[32], const uint8_t noise_seed[32],
+                        uint32_t self, uint32_t total, const int32_t *s, const int32_t *u,
+                        uint64_t *ksk_b, uint64_t *zero_others);
+/* party self's RGSW parts for LWE indices [l0, l0 + count)
+ * (src/bool/evaluator.rs:1697-1937): zero [count][da][N] (a_{i,l}*s_j + e for
+ * i in [dmax - da, dmax)) and cts [count][dmax][N] (u_j*a_{i,l} + e +
+ * beta*X^{emb*z_j[l]}; leader indices fill only the last max(da', db') rows,
+ * the rest are zero); RGSW x RGSW counts (da, db), dmax = max(da, db) */
+void pzo_ni_share_rgsw(const pzo_eval *e, const uint8_t crs[32], const uint8_t noise_seed[32],
+                       uint32_t self, uint32_t total, const int32_t *s, const int32_t *u,
+                       const int32_t *z, uint32_t l0, uint32_t count, uint64_t *zero,
+                       uint64_t *cts);
+/* Ksk(u_user -> s) [2d][N] evaluation domain from the user's ksk_b and the
+ * zero encryptions for the user from every other party (ascending)
+ * (src/bool/evaluator.rs:1266-1337) */
+void pzo_ni_uitos_ksk(const pzo_eval *e, const uint8_t crs[32], uint32_t total, uint32_t user,
+                      const uint64_t *ksk_b, const uint64_t *zero_for_user, uint64_t *ksk);
+/* produce_rgsw_ciphertext_from_ni_rgsw (src/bool/evaluator.rs:480-608) for
+ * index l: leader = 1 gives [2da'+2db'][N] coefficient domain, leader = 0
+ * gives [2da+2db][N] evaluation domain */
+void pzo_ni_produce(const pzo_eval *e, const uint8_t crs[32], uint32_t total, uint32_t l,
+                    const uint64_t *ksk, const uint64_t *zero_l, const uint64_t *ct, int leader,
+                    uint64_t *out);
+/* RGSW part of the aggregation: out [count][2da'+2db'][N] coefficient domain;
+ * ksks [total][2d][N], zero [total][count][da][N], cts [total][count][dmax][N] */
+void pzo_ni_aggregate_rgsw(const pzo_eval *e, const uint8_t crs[32], uint32_t total,
+                           const uint64_t *ksks, uint32_t l0, uint32_t count,
+                           const uint64_t *zero, const uint64_t *cts, uint64_t *out);
+
 void pzo_encrypt_lwe(const pzo_eval *e, const uint8_t seed[32], const int32_t *s,
                      uint32_t dim, uint64_t m, uint64_t *out);
 uint64_t pzo_decrypt_lwe(const pzo_eval *e, const uint64_t *lwe, const int32_t *s,
