@@ -291,6 +291,49 @@ int pz_sample_extract(pz_ctx *ctx, const uint64_t *d_rlwe, const uint32_t *d_src
 int pz_rgsw_by_rgsw(pz_ctx *ctx, uint64_t *d_rgsw0, const uint64_t *d_rgsw1, size_t B,
                     void *stream);
 
+/* ---- Non-interactive multi-party server key: shares and their aggregation.
+ * Seeds follow the tree of NonInteractiveMultiPartyCrs
+ * (src/bool/evaluator.rs:97-160) from the 32-byte common reference seed crs;
+ * noise comes from the party's own noise_seed (the library's own streams).
+ * LWE index l is led by party l / ceil(lwe_n / total)
+ * (src/bool/evaluator.rs:613-626).  dmax = max(rgrg_da, rgrg_db).
+ * PZ_ERR_UNSUPPORTED for sets without uitos_d or rgrg counts; at most
+ * PZ_NI_MAX_PARTIES parties. */
+#define PZ_NI_MAX_PARTIES 8
+/* Party `self`'s u -> s parts (non_interactive_ksk_gen and the zero
+ * encryptions for the others, src/multi_party.rs:186-291,
+ * src/bool/evaluator.rs:1650-1690): h_ksk_b[uitos_d][N] and
+ * h_zero_others[total-1][uitos_d][N] (ascending party order), coefficient
+ * domain. */
+int pz_ni_share_uitos(pz_ctx *ctx, const uint8_t *crs, const uint8_t *noise_seed, uint32_t self,
+                      uint32_t total, const int32_t *h_s, const int32_t *h_u, uint64_t *h_ksk_b,
+                      uint64_t *h_zero_others);
+/* Party `self`'s RGSW parts for LWE indices [l0, l0 + count)
+ * (src/bool/evaluator.rs:1697-1937): h_zero[count][rgrg_da][N] and
+ * h_cts[count][dmax][N], row i of an index holding the row built on
+ * a_{i,l}; where `self` leads, only the last max(rlrg_da, rlrg_db) rows are
+ * filled and the others are zero. */
+int pz_ni_share_rgsw(pz_ctx *ctx, const uint8_t *crs, const uint8_t *noise_seed, uint32_t self,
+                     uint32_t total, const int32_t *h_s, const int32_t *h_u, const int32_t *h_z,
+                     uint32_t l0, uint32_t count, uint64_t *h_zero, uint64_t *h_cts);
+/* Server: Ksk(u_user -> s) from the user's ksk_b and the zero encryptions
+ * for the user from the other parties, d_zero_for_user[total-1][uitos_d][N]
+ * in ascending party order (src/bool/evaluator.rs:1266-1337). */
+int pz_ni_uitos_ksk(pz_ctx *ctx, const uint8_t *crs, uint32_t total, uint32_t user,
+                    const uint64_t *d_ksk_b, const uint64_t *d_zero_for_user, pz_rlwe_ksk **out);
+/* Server: the RGSW part of aggregate_non_interactive_multi_party_server_key_
+ * shares (src/bool/evaluator.rs:1338-1565) for LWE indices [l0, l0+count):
+ * every party's produced RGSW (produce_rgsw_ciphertext_from_ni_rgsw,
+ * src/bool/evaluator.rs:480-608) and the leader's times the others' in
+ * ascending party order (rgsw_by_rgsw_inplace).  ksks[total] from
+ * pz_ni_uitos_ksk; d_zero[total][count][rgrg_da][N],
+ * d_cts[total][count][dmax][N] (each party's share arrays);
+ * d_out[count][2 rlrg_da + 2 rlrg_db][N], coefficient domain. */
+int pz_ni_aggregate_rgsw(pz_ctx *ctx, const uint8_t *crs, uint32_t total,
+                         pz_rlwe_ksk *const *ksks, uint32_t l0, uint32_t count,
+                         const uint64_t *d_zero, const uint64_t *d_cts, uint64_t *d_out,
+                         void *stream);
+
 /* Blind-rotation kernel choice.  No counterpart in the reference (its
  * evaluator is single-threaded): 0 = automatic, 1 = one work group per
  * bootstrap, 2 = two half work groups per bootstrap (only where

@@ -8,7 +8,9 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pz.h"
@@ -254,6 +256,55 @@ inline std::vector<uint32_t> dlog_map(const pz_params &p)
         d[q - v] = i + q / 4;
     }
     return d;
+}
+
+// puncture_p_rng (src/utils.rs:219-228): the t-th 32-byte draw of ChaCha8(seed)
+inline void puncture(const uint8_t seed[32], uint32_t times, uint8_t out[32])
+{
+    ChaCha8 r(seed);
+    memset(out, 0, 32);
+    for (uint32_t t = 0; t < times; t++) r.fill_bytes(out, 32);
+}
+
+// Seed tree of NonInteractiveMultiPartyCrs (src/bool/evaluator.rs:97-160):
+// LWE index l's RGSW seed and user j's u -> s seed.
+inline void ni_seed_index(const uint8_t crs[32], uint32_t l, uint8_t out[32])
+{
+    uint8_t k[32], m[32];
+    puncture(crs, 1, k);
+    puncture(k, 1, m);
+    puncture(m, l + 1, out);
+}
+
+inline void ni_seed_user(const uint8_t crs[32], uint32_t j, uint8_t out[32])
+{
+    uint8_t k[32];
+    puncture(crs, 2, k);
+    puncture(k, j + 1, out);
+}
+
+// multi_party_user_id_lwe_segment (src/bool/evaluator.rs:613-626): indices
+// per party; the leader of index l is l / ni_per_party
+inline uint32_t ni_per_party(uint32_t lwe_n, uint32_t total) { return (lwe_n + total - 1) / total; }
+
+// Runs f(i) for i in [0, n) on up to 16 host threads.
+template <class F>
+inline void parallel_for(size_t n, F f)
+{
+    unsigned hw = std::thread::hardware_concurrency();
+    size_t nt = hw == 0 ? 1 : (hw > 16 ? 16 : hw);
+    if (nt > n) nt = n;
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<size_t> next(0);
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nt; t++)
+        pool.emplace_back([&]() {
+            for (size_t i = next++; i < n; i = next++) f(i);
+        });
+    for (auto &th : pool) th.join();
 }
 
 }  // namespace host

@@ -209,6 +209,23 @@ extern "C" int pz_sample_extract(pz_ctx *c, const uint64_t *d_rlwe, const uint32
     return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
 }
 
+// The context with the RGSW x RGSW decomposers in place of the RLWE x RGSW
+// ones (rgsw_x_rgsw_decomposer, src/bool/parameters.rs)
+DevCtx pz::rgrg_devctx(const pz_params &p, const DevCtx &dc)
+{
+    DevCtx cc = dc;
+    const u32 lq = host::ref_log2(p.rlwe_q);
+    cc.dec_a.q = cc.dec_b.q = p.rlwe_q;
+    cc.dec_a.logb = cc.dec_b.logb = p.rgrg_logb;
+    cc.dec_a.d = p.rgrg_da;
+    cc.dec_b.d = p.rgrg_db;
+    cc.dec_a.ignore_bits = lq - p.rgrg_logb * p.rgrg_da;
+    cc.dec_b.ignore_bits = lq - p.rgrg_logb * p.rgrg_db;
+    cc.da = p.rgrg_da;
+    cc.db = p.rgrg_db;
+    return cc;
+}
+
 // RGSW x RGSW product of the non-interactive server-key aggregation
 // (rgsw_by_rgsw_inplace, src/rgsw/runtime.rs:883-985, applied per LWE index
 // and party at src/bool/evaluator.rs:1505-1535).
@@ -238,16 +255,7 @@ extern "C" int pz_rgsw_by_rgsw(pz_ctx *c, uint64_t *d_rgsw0, const uint64_t *d_r
     hipMemcpyAsync(k1, d_rgsw1, 8 * words, hipMemcpyDeviceToDevice, st);
     launch_scale(k1, words, dc.tb.n_inv, dc.tb.n_inv_s, p.rlwe_q, st);
     launch_shoup(k1, k1s, words, p.rlwe_q, st);
-    DevCtx cc = dc;
-    const u32 lq = host::ref_log2(p.rlwe_q);
-    cc.dec_a.q = cc.dec_b.q = p.rlwe_q;
-    cc.dec_a.logb = cc.dec_b.logb = p.rgrg_logb;
-    cc.dec_a.d = p.rgrg_da;
-    cc.dec_b.d = p.rgrg_db;
-    cc.dec_a.ignore_bits = lq - p.rgrg_logb * p.rgrg_da;
-    cc.dec_b.ignore_bits = lq - p.rgrg_logb * p.rgrg_db;
-    cc.da = p.rgrg_da;
-    cc.db = p.rgrg_db;
+    const DevCtx cc = rgrg_devctx(p, dc);
     launch_rgsw_rows(logn, d_rgsw0, r0, p.rlrg_da, p.rlrg_db, B, cc, k1, k1s, r1, st);
     hipError_t e = hipGetLastError();
     hipStreamSynchronize(st);  // the scratch is read by the launch above

@@ -49,6 +49,7 @@ void launch_blind_rotate(int logn, int var, int grid, const u32 *ks, const u32 *
 constexpr int PROBE_CHAINS = 8;  // independent chains per thread in k_valu_probe
 void launch_valu_probe(int kind, int grid, u64 *out, u32 iters, u32 c, u64 w, u64 ws, u64 q,
                        hipStream_t s);
+DevCtx rgrg_devctx(const pz_params &p, const DevCtx &dc);
 void launch_rgsw_rows(int logn, u64 *rgsw0, u32 r0, u32 da0, u32 db0, size_t B, const DevCtx &c,
                       const u64 *K1, const u64 *K1s, u32 r1, hipStream_t s);
 void launch_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q, hipStream_t s);

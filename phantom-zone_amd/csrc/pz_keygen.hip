@@ -87,6 +87,32 @@ __global__ void k_ksk_rows(u64 *ksk, const int32_t *s, const int32_t *z, const i
     }
 }
 
+// NI share rows: out = (prod + e) + beta * msg, msg the party's u (u -> s
+// rows), a monomial X^{emb*z_j[l]} (RGSW rows) or nothing (zero encryptions)
+struct NiRow {
+    const u64 *prod;  // a*s_j or a*u_j
+    const u64 *e;
+    const u64 *msg;   // full message polynomial or null
+    u64 *out;
+    u64 beta;
+    u32 mono_pos;
+    u64 mono_val;     // 1 or q - 1; 0 = no monomial
+};
+
+__global__ void k_ni_combine(const NiRow *rows, size_t nrows, u32 N, Barrett br)
+{
+    const u64 q = br.q;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nrows * N;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const NiRow &r = rows[t / N];
+        const u32 i = (u32)(t % N);
+        u64 m = 0;
+        if (r.msg) m = r.msg[i];
+        else if (r.mono_val && i == r.mono_pos) m = r.mono_val;
+        r.out[i] = add_mod(add_mod(r.prod[i], r.e[i], q), barrett_mul(br, m, r.beta), q);
+    }
+}
+
 static int grid_of(size_t n) { return (int)((n + 255) / 256 < 65535 * 4 ? (n + 255) / 256 : 65535 * 4); }
 
 }  // namespace pz
@@ -523,3 +549,220 @@ done:
     hipFree(d_u);
     return ok ? PZ_OK : PZ_ERR_HIP;
 }
+
+// ---- non-interactive server-key shares (client side).  Uniform rows and
+// noise are drawn on the host (one ChaCha8 stream per seed, several indices
+// at once); products with the party's secrets run on the device.
+
+// rows[r] (coefficient domain, r < nrows) times x (a secret) into prod:
+// forward, Barrett pointwise, backward (the reference's keygen products)
+static bool ni_products(int logn, const DevCtx &dc, hipStream_t st, const u64 *d_rows,
+                        size_t nrows, const int32_t *h_x, u32 N, u64 q, u64 *d_prod)
+{
+    std::vector<u64> hx(N);
+    for (u32 i = 0; i < N; i++) hx[i] = host::map_i64(q, h_x[i]);
+    u64 *d_x = nullptr;
+    if (hipMalloc(&d_x, 8 * (size_t)N) != hipSuccess) return false;
+    bool ok = hipMemcpyAsync(d_x, hx.data(), 8 * (size_t)N, hipMemcpyHostToDevice, st) ==
+                  hipSuccess &&
+              hipMemcpyAsync(d_prod, d_rows, 8 * nrows * N, hipMemcpyDeviceToDevice, st) ==
+                  hipSuccess;
+    if (ok) {
+        launch_ntt(logn, 1, d_x, 1, dc.tb, st);
+        launch_ntt(logn, 1, d_prod, nrows, dc.tb, st);
+        hipLaunchKernelGGL(k_barrett_rows, grid_of(nrows * N), 256, 0, st, d_prod, d_x, nrows, N,
+                           dc.br);
+        launch_ntt(logn, 3, d_prod, nrows, dc.tb, st);
+        ok = hipStreamSynchronize(st) == hipSuccess;
+    }
+    hipFree(d_x);
+    return ok;
+}
+
+static bool ni_combine(hipStream_t st, const DevCtx &dc, const std::vector<NiRow> &rows, u32 N)
+{
+    NiRow *d_rows = nullptr;
+    if (rows.empty()) return true;
+    if (hipMalloc(&d_rows, sizeof(NiRow) * rows.size()) != hipSuccess) return false;
+    bool ok = hipMemcpyAsync(d_rows, rows.data(), sizeof(NiRow) * rows.size(),
+                             hipMemcpyHostToDevice, st) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_ni_combine, grid_of(rows.size() * N), 256, 0, st, d_rows,
+                           rows.size(), N, dc.br);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    }
+    hipFree(d_rows);
+    return ok;
+}
+
+static int ni_check(const pz_params &p, uint32_t self, uint32_t total)
+{
+    if (p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0) return PZ_ERR_UNSUPPORTED;
+    if (total == 0 || total > PZ_NI_MAX_PARTIES || self >= total) return PZ_ERR_ARG;
+    return PZ_OK;
+}
+
+extern "C" int pz_ni_share_uitos(pz_ctx *c, const uint8_t *crs, const uint8_t *noise_seed,
+                                 uint32_t self, uint32_t total, const int32_t *h_s,
+                                 const int32_t *h_u, uint64_t *h_ksk_b, uint64_t *h_zero_others)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!crs || !noise_seed || !h_s || !h_u || !h_ksk_b || (total > 1 && !h_zero_others))
+        return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if ((rc = ni_check(p, self, total))) return rc;
+    const u64 q = p.rlwe_q;
+    const u32 N = p.rlwe_n, d = p.uitos_d;
+    const size_t R = (size_t)total * d;  // own rows, then the others' in ascending order
+    std::vector<u64> ha(R * N), he(R * N), hu(N);
+    {
+        uint8_t ns[32], sd[32];
+        host::puncture(noise_seed, 1, ns);
+        host::ChaCha8 noise(ns);
+        size_t row = 0;
+        for (u32 pass = 0; pass < total; pass++) {
+            const u32 o = pass == 0 ? self : (pass - 1 < self ? pass - 1 : pass);
+            host::ni_seed_user(crs, o, sd);
+            host::ChaCha8 pr(sd);
+            for (u32 k = 0; k < d; k++, row++) {
+                for (u32 i = 0; i < N; i++) ha[row * N + i] = pr.uniform(q);
+                for (u32 i = 0; i < N; i++)
+                    he[row * N + i] = host::map_i64(q, host::gauss_round(noise));
+            }
+        }
+    }
+    for (u32 i = 0; i < N; i++) hu[i] = host::map_i64(q, h_u[i]);
+    bool ok = true;
+    u64 *d_a = nullptr, *d_e = nullptr, *d_prod = nullptr, *d_u = nullptr, *d_out = nullptr;
+    std::vector<NiRow> rows(R);
+    KG_CHK(hipMalloc(&d_a, 8 * R * N));
+    KG_CHK(hipMalloc(&d_e, 8 * R * N));
+    KG_CHK(hipMalloc(&d_prod, 8 * R * N));
+    KG_CHK(hipMalloc(&d_out, 8 * R * N));
+    KG_CHK(hipMalloc(&d_u, 8 * (size_t)N));
+    KG_CHK(hipMemcpyAsync(d_a, ha.data(), 8 * R * N, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_e, he.data(), 8 * R * N, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_u, hu.data(), 8 * (size_t)N, hipMemcpyHostToDevice, st));
+    if (!ni_products(logn, dc, st, d_a, R, h_s, N, q, d_prod)) KG_CHK(hipErrorUnknown);
+    for (size_t r = 0; r < R; r++) {
+        NiRow &x = rows[r];
+        x.prod = d_prod + r * N;
+        x.e = d_e + r * N;
+        x.out = d_out + r * N;
+        x.msg = r < d ? d_u : nullptr;
+        x.beta = r < d ? 1ull << (p.uitos_logb * r + dc.dec_uitos.ignore_bits) : 0;
+        x.mono_pos = 0;
+        x.mono_val = 0;
+    }
+    if (!ni_combine(st, dc, rows, N)) KG_CHK(hipErrorUnknown);
+    KG_CHK(hipMemcpy(h_ksk_b, d_out, 8 * (size_t)d * N, hipMemcpyDeviceToHost));
+    if (total > 1)
+        KG_CHK(hipMemcpy(h_zero_others, d_out + (size_t)d * N, 8 * (R - d) * N,
+                         hipMemcpyDeviceToHost));
+done:
+    hipFree(d_a);
+    hipFree(d_e);
+    hipFree(d_prod);
+    hipFree(d_out);
+    hipFree(d_u);
+    return ok ? PZ_OK : PZ_ERR_HIP;
+}
+
+extern "C" int pz_ni_share_rgsw(pz_ctx *c, const uint8_t *crs, const uint8_t *noise_seed,
+                                uint32_t self, uint32_t total, const int32_t *h_s,
+                                const int32_t *h_u, const int32_t *h_z, uint32_t l0,
+                                uint32_t count, uint64_t *h_zero, uint64_t *h_cts)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!crs || !noise_seed || !h_s || !h_u || !h_z || !h_zero || !h_cts) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if ((rc = ni_check(p, self, total))) return rc;
+    if ((uint64_t)l0 + count > p.lwe_n) return PZ_ERR_ARG;
+    if (count == 0) return PZ_OK;
+    const u64 q = p.rlwe_q;
+    const u32 N = p.rlwe_n, da = p.rgrg_da;
+    const u32 dmax = p.rgrg_da > p.rgrg_db ? p.rgrg_da : p.rgrg_db;
+    const u32 dmax1 = p.rlrg_da > p.rlrg_db ? p.rlrg_da : p.rlrg_db;
+    const u32 lq = host::ref_log2(q), per = host::ni_per_party(p.lwe_n, total);
+    // per index: a_{i,l} rows [dmax][N]; noise of the zero rows [da][N] and of
+    // the ct rows [dmax][N] (rows the party does not fill stay zero)
+    const size_t A = (size_t)count * dmax * N, Z = (size_t)count * da * N;
+    std::vector<u64> ha(A), hez(Z), hec(A, 0);
+    uint8_t nr[32];
+    host::puncture(noise_seed, 2, nr);
+    host::parallel_for(count, [&](size_t t) {
+        const u32 l = l0 + (u32)t;
+        uint8_t sd[32], nl[32];
+        host::ni_seed_index(crs, l, sd);
+        host::ChaCha8 pr(sd);
+        for (size_t i = 0; i < (size_t)dmax * N; i++) ha[t * dmax * N + i] = pr.uniform(q);
+        host::puncture(nr, l + 1, nl);
+        host::ChaCha8 noise(nl);
+        for (size_t i = 0; i < (size_t)da * N; i++)
+            hez[t * da * N + i] = host::map_i64(q, host::gauss_round(noise));
+        const u32 first = l / per == self ? dmax - dmax1 : 0;
+        for (size_t i = (size_t)first * N; i < (size_t)dmax * N; i++)
+            hec[t * dmax * N + i] = host::map_i64(q, host::gauss_round(noise));
+    });
+    bool ok = true;
+    u64 *d_a = nullptr, *d_ez = nullptr, *d_ec = nullptr, *d_ps = nullptr, *d_pu = nullptr;
+    u64 *d_zero = nullptr, *d_cts = nullptr;
+    std::vector<NiRow> rows;
+    KG_CHK(hipMalloc(&d_a, 8 * A));
+    KG_CHK(hipMalloc(&d_ez, 8 * Z));
+    KG_CHK(hipMalloc(&d_ec, 8 * A));
+    KG_CHK(hipMalloc(&d_ps, 8 * A));
+    KG_CHK(hipMalloc(&d_pu, 8 * A));
+    KG_CHK(hipMalloc(&d_zero, 8 * Z));
+    KG_CHK(hipMalloc(&d_cts, 8 * A));
+    KG_CHK(hipMemcpyAsync(d_a, ha.data(), 8 * A, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_ez, hez.data(), 8 * Z, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_ec, hec.data(), 8 * A, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemsetAsync(d_cts, 0, 8 * A, st));
+    if (!ni_products(logn, dc, st, d_a, (size_t)count * dmax, h_s, N, q, d_ps))
+        KG_CHK(hipErrorUnknown);
+    if (!ni_products(logn, dc, st, d_a, (size_t)count * dmax, h_u, N, q, d_pu))
+        KG_CHK(hipErrorUnknown);
+    for (u32 t = 0; t < count; t++) {
+        const u32 l = l0 + t;
+        const bool lead = l / per == self;
+        // X^{emb * z_j[l]} (encode_x_pow_si_with_emebedding_factor, src/utils.rs:195-217)
+        const int64_t si = (int64_t)dc.embedding * h_z[l];
+        const u32 pos = si < 0 ? N - (u32)(-si) : (u32)si;
+        const u64 val = si < 0 ? q - 1 : 1;
+        for (u32 k = 0; k < da; k++) {
+            const size_t ai = ((size_t)t * dmax + dmax - da + k) * N;
+            rows.push_back({d_ps + ai, d_ez + ((size_t)t * da + k) * N, nullptr,
+                            d_zero + ((size_t)t * da + k) * N, 0, 0, 0});
+        }
+        const u32 first = lead ? dmax - dmax1 : 0;
+        const u32 logb = lead ? p.rlrg_logb : p.rgrg_logb;
+        const u32 ib = lq - logb * (lead ? dmax1 : dmax);
+        for (u32 i = first; i < dmax; i++) {
+            const size_t ai = ((size_t)t * dmax + i) * N;
+            rows.push_back({d_pu + ai, d_ec + ai, nullptr, d_cts + ai,
+                            1ull << (ib + logb * (i - first)), pos, val});
+        }
+    }
+    if (!ni_combine(st, dc, rows, N)) KG_CHK(hipErrorUnknown);
+    KG_CHK(hipMemcpy(h_zero, d_zero, 8 * Z, hipMemcpyDeviceToHost));
+    KG_CHK(hipMemcpy(h_cts, d_cts, 8 * A, hipMemcpyDeviceToHost));
+done:
+    hipFree(d_a);
+    hipFree(d_ez);
+    hipFree(d_ec);
+    hipFree(d_ps);
+    hipFree(d_pu);
+    hipFree(d_zero);
+    hipFree(d_cts);
+    return ok ? PZ_OK : PZ_ERR_HIP;
+}
+

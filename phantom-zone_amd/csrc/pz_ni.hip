@@ -1,0 +1,277 @@
+// pz_ni.hip -- server-side aggregation of non-interactive multi-party
+// server-key shares on the device.
+//
+//   pz_ni_uitos_ksk       Ksk(u_j -> s): A rows expanded from user j's seed,
+//                         B = ksk_b + the others' zero encryptions, both sent
+//                         to the evaluation domain (src/bool/evaluator.rs:
+//                         1266-1337)
+//   k_ni_rows             produce_rgsw_ciphertext_from_ni_rgsw
+//                         (src/bool/evaluator.rs:480-608): one work group per
+//                         produced RLWE (LWE index, party, row).  The input
+//                         polynomial (sum of the parties' zero encryptions
+//                         for RLWE'(-sm) rows, -a_{i,l} for RLWE'(m) rows) is
+//                         decomposed in registers under the u -> s decomposer;
+//                         each digit row is transformed and multiply-
+//                         accumulated with the party's Ksk rows.  The index
+//                         leader's rows go back to the coefficient domain
+//                         (plus its NI row); the others' stay in the
+//                         evaluation domain (plus the transformed NI row),
+//                         scaled by n^-1 for the RGSW x RGSW step
+//   pz_ni_aggregate_rgsw  k_ni_rows, Shoup twins of the others' RGSWs, then
+//                         one RGSW x RGSW launch per non-leader slot, in
+//                         ascending party order (src/bool/evaluator.rs:
+//                         1511-1560)
+#include <vector>
+
+#include "pz_host.hpp"
+#include "pz_internal.hpp"
+
+namespace pz {
+
+struct NiArgs {
+    const u64 *zero;   // [P][count][da][N]
+    const u64 *cts;    // [P][count][dmax][N]
+    const u64 *arows;  // [count][db][N]: a_{i,l}, i in [dmax - db, dmax)
+    const u64 *K[PZ_NI_MAX_PARTIES], *Ks[PZ_NI_MAX_PARTIES];
+    u64 *out0;         // [count][2 da0 + 2 db0][N], coefficient domain
+    u64 *out1;         // [P - 1][count][2 da + 2 db][N], evaluation domain * n^-1
+    u32 P, count, l0, per, da, db, dmax, da0, db0, d;
+};
+
+template <int LOGN>
+__global__ __launch_bounds__(1 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+k_ni_rows(NiArgs A, DevCtx c)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ u64 lds[];
+    const u32 j = threadIdx.x;
+    const u64 q = c.tb.q, q2 = c.tb.q2;
+    const u32 n0 = A.da0 + A.db0, n1 = A.da + A.db;
+    const u32 per_l = n0 + (A.P - 1) * n1;
+    for (size_t w = blockIdx.x; w < (size_t)A.count * per_l; w += gridDim.x) {
+        const u32 t = (u32)(w / per_l), k = (u32)(w % per_l);
+        const u32 lead = (A.l0 + t) / A.per;
+        const bool leader = k < n0;
+        u32 party, kk, slot = 0, dao, dbo;
+        if (leader) {
+            party = lead;
+            kk = k;
+            dao = A.da0;
+            dbo = A.db0;
+        } else {
+            slot = (k - n0) / n1;
+            kk = (k - n0) % n1;
+            party = slot < lead ? slot : slot + 1;
+            dao = A.da;
+            dbo = A.db;
+        }
+        const bool nsm = kk < dao;
+        const u32 i = nsm ? A.dmax - dao + kk : A.dmax - dbo + (kk - dao);
+        const u32 ra = nsm ? kk : 2 * dao + (kk - dao);
+        const u32 rb = nsm ? dao + kk : 2 * dao + dbo + (kk - dao);
+        u64 val[8];
+        if (nsm) {
+            // sum over parties of a_{i,l} * s_p + e: a_{i,l} * s + e
+            const u32 zr = i - (A.dmax - A.da);
+#pragma unroll
+            for (int r = 0; r < 8; r++) val[r] = 0;
+            for (u32 p = 0; p < A.P; p++) {
+                const u64 *z = A.zero + (((size_t)p * A.count + t) * A.da + zr) * N;
+#pragma unroll
+                for (int r = 0; r < 8; r++) val[r] = add_mod(val[r], z[coef_x<LOGN>(j, r)], q);
+            }
+        } else {
+            const u64 *a = A.arows + ((size_t)t * A.db + (i - (A.dmax - A.db))) * N;
+#pragma unroll
+            for (int r = 0; r < 8; r++) val[r] = neg_mod(a[coef_x<LOGN>(j, r)], q);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) val[r] = decomp_prepare(c.dec_uitos, val[r]);
+        u64 acc[2][8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) acc[0][r] = acc[1][r] = 0;
+        const u64 *K = A.K[party], *Ks = A.Ks[party];
+        for (u32 row = 0; row < A.d; row++) {
+            __builtin_amdgcn_sched_barrier(0);  // no key loads hoisted across rows
+            u64 D[1][8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) D[0][r] = decomp_next(c.dec_uitos, val[r]);
+            ntt_fwd<LOGN, 1, PZ_FWD_MODE, false>(D, j, lds, c.tb);
+            fma_row<false>(acc[0], K + (size_t)row * N, Ks + (size_t)row * N, D[0], j, q, q2);
+            fma_row<false>(acc[1], K + (size_t)(A.d + row) * N, Ks + (size_t)(A.d + row) * N,
+                          D[0], j, q, q2);
+        }
+        const u64 *ct = A.cts + (((size_t)party * A.count + t) * A.dmax + i) * N;
+        const int part = nsm ? 0 : 1;  // the NI row joins a for RLWE'(-sm), b for RLWE'(m)
+        if (leader) {
+            ntt_inv<LOGN, 2, true, true, false>(acc, j, lds, c.tb);
+            u64 *o = A.out0 + (size_t)t * 2 * n0 * N;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const u64 v = ct[coef_x<LOGN>(j, r)];
+                acc[part][r] = add_mod(acc[part][r], v, q);
+                o[(size_t)ra * N + coef_x<LOGN>(j, r)] = acc[0][r];
+                o[(size_t)rb * N + coef_x<LOGN>(j, r)] = acc[1][r];
+            }
+        } else {
+            u64 cv[1][8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) cv[0][r] = ct[coef_x<LOGN>(j, r)];
+            __syncthreads();  // the LDS of the last digit transform is free again
+            ntt_fwd<LOGN, 1, FWD_STRICT, false>(cv, j, lds, c.tb);
+            u64 *o = A.out1 + (((size_t)slot * A.count + t) * 2 * n1) * N;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                u64 x = umin(acc[0][r], acc[0][r] - q), y = umin(acc[1][r], acc[1][r] - q);
+                if (part == 0) x = add_mod(x, cv[0][r], q);
+                else y = add_mod(y, cv[0][r], q);
+                x = shoup_mul(x, c.tb.n_inv, c.tb.n_inv_s, q);
+                y = shoup_mul(y, c.tb.n_inv, c.tb.n_inv_s, q);
+                o[(size_t)ra * N + eval_x<LOGN>(j, r)] = umin(x, x - q);
+                o[(size_t)rb * N + eval_x<LOGN>(j, r)] = umin(y, y - q);
+            }
+        }
+    }
+}
+
+}  // namespace pz
+
+using namespace pz;
+
+extern int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStream_t *s);
+extern int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out);
+
+__global__ void k_ni_sum_rows(u64 *b, const u64 *zero, u32 others, size_t words, u64 q)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words;
+         i += (size_t)gridDim.x * blockDim.x) {
+        u64 v = b[i];
+        for (u32 o = 0; o < others; o++) v = add_mod(v, zero[(size_t)o * words + i], q);
+        b[i] = v;
+    }
+}
+
+static int ni_grid(size_t n) { return (int)((n + 255) / 256 < 65535 * 4 ? (n + 255) / 256 : 65535 * 4); }
+
+extern "C" int pz_ni_uitos_ksk(pz_ctx *c, const uint8_t *crs, uint32_t total, uint32_t user,
+                               const uint64_t *d_ksk_b, const uint64_t *d_zero_for_user,
+                               pz_rlwe_ksk **out)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!crs || !d_ksk_b || !out || (total > 1 && !d_zero_for_user)) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (p.uitos_d == 0) return PZ_ERR_UNSUPPORTED;
+    if (total == 0 || total > PZ_NI_MAX_PARTIES || user >= total) return PZ_ERR_ARG;
+    const u32 N = p.rlwe_n, d = p.uitos_d;
+    const size_t words = (size_t)d * N;
+    std::vector<u64> ha(words);
+    {
+        uint8_t sd[32];
+        host::ni_seed_user(crs, user, sd);
+        host::ChaCha8 pr(sd);
+        for (size_t i = 0; i < words; i++) ha[i] = pr.uniform(p.rlwe_q);
+    }
+    u64 *rows = nullptr;
+    if (hipMalloc(&rows, 16 * words) != hipSuccess) return PZ_ERR_HIP;
+    bool ok = hipMemcpyAsync(rows, ha.data(), 8 * words, hipMemcpyHostToDevice, st) ==
+                  hipSuccess &&
+              hipMemcpyAsync(rows + words, d_ksk_b, 8 * words, hipMemcpyDeviceToDevice, st) ==
+                  hipSuccess;
+    if (ok) {
+        if (total > 1)
+            hipLaunchKernelGGL(k_ni_sum_rows, ni_grid(words), 256, 0, st, rows + words,
+                               d_zero_for_user, total - 1, words, p.rlwe_q);
+        launch_ntt(logn, 1, rows, 2 * (size_t)d, dc.tb, st);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    }
+    if (ok) rc = pz_internal_rlwe_ksk_adopt(c, rows, d, out);
+    if (!ok || rc) {
+        hipFree(rows);
+        return ok ? rc : PZ_ERR_HIP;
+    }
+    return PZ_OK;
+}
+
+extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t total,
+                                    pz_rlwe_ksk *const *ksks, uint32_t l0, uint32_t count,
+                                    const uint64_t *d_zero, const uint64_t *d_cts, uint64_t *d_out,
+                                    void *stream)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!crs || !ksks || !d_zero || !d_cts || !d_out) return PZ_ERR_ARG;
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0) return PZ_ERR_UNSUPPORTED;
+    if (total == 0 || total > PZ_NI_MAX_PARTIES) return PZ_ERR_ARG;
+    if ((uint64_t)l0 + count > p.lwe_n) return PZ_ERR_ARG;
+    for (u32 o = 0; o < total; o++)
+        if (!ksks[o] || ksks[o]->owner != c || ksks[o]->d != p.uitos_d) return PZ_ERR_ARG;
+    if (count == 0) return PZ_OK;
+    if (stream) st = (hipStream_t)stream;
+    const u32 N = p.rlwe_n, da = p.rgrg_da, db = p.rgrg_db, dmax = da > db ? da : db;
+    const size_t r1 = 2 * (size_t)(da + db);
+    // a_{i,l} for the RLWE'(m) rows: the last db rows of index l's stream
+    std::vector<u64> ha((size_t)count * db * N);
+    host::parallel_for(count, [&](size_t t) {
+        uint8_t sd[32];
+        host::ni_seed_index(crs, l0 + (u32)t, sd);
+        host::ChaCha8 pr(sd);
+        for (size_t i = 0; i < (size_t)(dmax - db) * N; i++) pr.uniform(p.rlwe_q);
+        for (size_t i = 0; i < (size_t)db * N; i++) ha[t * db * N + i] = pr.uniform(p.rlwe_q);
+    });
+    const size_t w1 = (size_t)(total - 1) * count * r1 * N;
+    u64 *d_a = nullptr, *o1 = nullptr, *o1s = nullptr;
+    bool ok = hipMalloc(&d_a, 8 * ha.size()) == hipSuccess;
+    if (ok && w1) ok = hipMalloc(&o1, 8 * w1) == hipSuccess && hipMalloc(&o1s, 8 * w1) == hipSuccess;
+    if (ok) ok = hipMemcpyAsync(d_a, ha.data(), 8 * ha.size(), hipMemcpyHostToDevice, st) ==
+                 hipSuccess;
+    if (ok) {
+        NiArgs A;
+        A.zero = d_zero;
+        A.cts = d_cts;
+        A.arows = d_a;
+        for (u32 o = 0; o < PZ_NI_MAX_PARTIES; o++) {
+            A.K[o] = o < total ? ksks[o]->rows : nullptr;
+            A.Ks[o] = o < total ? ksks[o]->rows_s : nullptr;
+        }
+        A.out0 = d_out;
+        A.out1 = o1;
+        A.P = total;
+        A.count = count;
+        A.l0 = l0;
+        A.per = host::ni_per_party(p.lwe_n, total);
+        A.da = da;
+        A.db = db;
+        A.dmax = dmax;
+        A.da0 = p.rlrg_da;
+        A.db0 = p.rlrg_db;
+        A.d = p.uitos_d;
+        const size_t work = (size_t)count * (p.rlrg_da + p.rlrg_db + (total - 1) * (da + db));
+        const int grid = (int)(work < 65535 ? work : 65535);
+        const size_t lds = 8 * (size_t)(2 * ((1 << logn) + (1 << logn) / 8));
+        if (logn == 11)
+            hipLaunchKernelGGL(k_ni_rows<11>, grid, 1 << 8, lds, st, A, dc);
+        else
+            hipLaunchKernelGGL(k_ni_rows<9>, grid, 1 << 6, lds, st, A, dc);
+        if (w1) {
+            launch_shoup(o1, o1s, w1, p.rlwe_q, st);
+            const DevCtx cc = rgrg_devctx(p, dc);
+            for (u32 s = 0; s + 1 < total; s++)
+                launch_rgsw_rows(logn, d_out, 2 * (p.rlrg_da + p.rlrg_db), p.rlrg_da, p.rlrg_db,
+                                 count, cc, o1 + (size_t)s * count * r1 * N,
+                                 o1s + (size_t)s * count * r1 * N, (u32)r1, st);
+        }
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    }
+    hipFree(d_a);
+    hipFree(o1);
+    hipFree(o1s);
+    return ok ? PZ_OK : PZ_ERR_HIP;
+}

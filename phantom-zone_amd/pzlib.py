@@ -105,6 +105,13 @@ SIGNATURES = {
     "pz_rlwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_sample_extract": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp]),
     "pz_rgsw_by_rgsw": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_ni_share_uitos": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp, _vp, _vp, _vp]),
+    "pz_ni_share_rgsw": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp, _vp, _vp,
+                                   C.c_uint32, C.c_uint32, _vp, _vp]),
+    "pz_ni_uitos_ksk": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _vp, _vp,
+                                  C.POINTER(_vp)]),
+    "pz_ni_aggregate_rgsw": (C.c_int, [_vp, _vp, C.c_uint32, _vp, C.c_uint32, C.c_uint32, _vp,
+                                       _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -451,6 +458,8 @@ class Context:
     # ---- seeded client-side keys and encryption ----
     @staticmethod
     def seed32(seed):
+        """32-byte seed array.  Bind it to a name before passing its address
+        to C: a temporary's buffer is freed before the call reads it."""
         if isinstance(seed, (bytes, bytearray)):
             return np.frombuffer(bytes(seed).ljust(32, b"\0")[:32], dtype=np.uint8).copy()
         return np.frombuffer(int(seed).to_bytes(32, "little"), dtype=np.uint8).copy()
@@ -458,7 +467,8 @@ class Context:
     def keygen_secrets(self, seed, parties):
         s = np.zeros(self.N, dtype=np.int32)
         z = np.zeros(self.n, dtype=np.int32)
-        _check(self.L.pz_keygen_secrets(self.h, self.seed32(seed).ctypes.data, parties,
+        sd_b = self.seed32(seed)
+        _check(self.L.pz_keygen_secrets(self.h, sd_b.ctypes.data, parties,
                                         s.ctypes.data, z.ctypes.data))
         return s, z
 
@@ -466,7 +476,8 @@ class Context:
         s = np.ascontiguousarray(s, dtype=np.int32)
         z = np.ascontiguousarray(z, dtype=np.int32)
         h = C.c_void_p()
-        _check(self.L.pz_keygen_server(self.h, self.seed32(seed).ctypes.data, s.ctypes.data,
+        sd_b = self.seed32(seed)
+        _check(self.L.pz_keygen_server(self.h, sd_b.ctypes.data, s.ctypes.data,
                                        z.ctypes.data, C.byref(h)))
         return Key(self, h.value)
 
@@ -483,7 +494,8 @@ class Context:
         s = np.ascontiguousarray(s, dtype=np.int32)
         b = np.ascontiguousarray(bits, dtype=np.uint8)
         out = np.zeros((len(b), len(s) + 1), dtype=np.uint64)
-        _check(self.L.pz_encrypt_bools(self.h, s.ctypes.data, len(s), self.seed32(seed).ctypes.data,
+        sd_b = self.seed32(seed)
+        _check(self.L.pz_encrypt_bools(self.h, s.ctypes.data, len(s), sd_b.ctypes.data,
                                        b.ctypes.data, len(b), out.ctypes.data))
         return out
 
@@ -498,14 +510,16 @@ class Context:
     # ---- input path (non-interactive sets) ----
     def keygen_user_secret(self, seed):
         u = np.zeros(self.N, dtype=np.int32)
-        _check(self.L.pz_keygen_user_secret(self.h, self.seed32(seed).ctypes.data, u.ctypes.data))
+        sd_b = self.seed32(seed)
+        _check(self.L.pz_keygen_user_secret(self.h, sd_b.ctypes.data, u.ctypes.data))
         return u
 
     def keygen_rlwe_ksk(self, seed, u, s):
         u = np.ascontiguousarray(u, dtype=np.int32)
         s = np.ascontiguousarray(s, dtype=np.int32)
         h = C.c_void_p()
-        _check(self.L.pz_keygen_rlwe_ksk(self.h, self.seed32(seed).ctypes.data, u.ctypes.data,
+        sd_b = self.seed32(seed)
+        _check(self.L.pz_keygen_rlwe_ksk(self.h, sd_b.ctypes.data, u.ctypes.data,
                                          s.ctypes.data, C.byref(h)))
         return RlweKsk(self, h.value)
 
@@ -520,7 +534,8 @@ class Context:
         b = np.ascontiguousarray(bits, dtype=np.uint8)
         nr = (len(b) + self.N - 1) // self.N
         out = np.zeros((nr, 2, self.N), dtype=np.uint64)
-        _check(self.L.pz_encrypt_rlwe_bools(self.h, u.ctypes.data, self.seed32(seed).ctypes.data,
+        sd_b = self.seed32(seed)
+        _check(self.L.pz_encrypt_rlwe_bools(self.h, u.ctypes.data, sd_b.ctypes.data,
                                             b.ctypes.data, len(b), out.ctypes.data))
         return out
 
@@ -529,6 +544,55 @@ class Context:
 
     def sample_extract(self, d_rlwe, d_src, count, d_lwe):
         _check(self.L.pz_sample_extract(self.h, d_rlwe.ptr, d_src.ptr, count, d_lwe.ptr, None))
+
+    # ---- non-interactive server key: shares (client) and aggregation (server)
+    @property
+    def ni_dmax(self):
+        return max(self.p.rgrg_da, self.p.rgrg_db)
+
+    def ni_share_uitos(self, crs, noise_seed, me, total, s, u):
+        d, N = self.p.uitos_d, self.N
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        ksk_b = np.zeros((d, N), dtype=np.uint64)
+        zo = np.zeros((max(total - 1, 1), d, N), dtype=np.uint64)
+        crs_b = self.seed32(crs)
+        ns_b = self.seed32(noise_seed)
+        _check(self.L.pz_ni_share_uitos(self.h, crs_b.ctypes.data,
+                                        ns_b.ctypes.data, me, total,
+                                        s.ctypes.data, u.ctypes.data, ksk_b.ctypes.data,
+                                        zo.ctypes.data))
+        return ksk_b, zo[:total - 1]
+
+    def ni_share_rgsw(self, crs, noise_seed, me, total, s, u, z, l0, count):
+        N = self.N
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        z = np.ascontiguousarray(z, dtype=np.int32)
+        zero = np.zeros((count, self.p.rgrg_da, N), dtype=np.uint64)
+        cts = np.zeros((count, self.ni_dmax, N), dtype=np.uint64)
+        crs_b = self.seed32(crs)
+        ns_b = self.seed32(noise_seed)
+        _check(self.L.pz_ni_share_rgsw(self.h, crs_b.ctypes.data,
+                                       ns_b.ctypes.data, me, total,
+                                       s.ctypes.data, u.ctypes.data, z.ctypes.data, l0, count,
+                                       zero.ctypes.data, cts.ctypes.data))
+        return zero, cts
+
+    def ni_uitos_ksk(self, crs, total, user, d_ksk_b, d_zero_for_user):
+        h = C.c_void_p()
+        crs_b = self.seed32(crs)
+        _check(self.L.pz_ni_uitos_ksk(self.h, crs_b.ctypes.data, total, user,
+                                      d_ksk_b.ptr,
+                                      d_zero_for_user.ptr if d_zero_for_user is not None else None,
+                                      C.byref(h)))
+        return RlweKsk(self, h.value)
+
+    def ni_aggregate_rgsw(self, crs, total, ksks, l0, count, d_zero, d_cts, d_out):
+        arr = (C.c_void_p * len(ksks))(*[k.h for k in ksks])
+        crs_b = self.seed32(crs)
+        _check(self.L.pz_ni_aggregate_rgsw(self.h, crs_b.ctypes.data, total, arr, l0,
+                                           count, d_zero.ptr, d_cts.ptr, d_out.ptr, None))
 
     def rgsw_by_rgsw(self, d_rgsw0, d_rgsw1, B):
         _check(self.L.pz_rgsw_by_rgsw(self.h, d_rgsw0.ptr, d_rgsw1.ptr, B, None))
