@@ -286,6 +286,81 @@ def input_path():
             "bits_per_sec": count / (ks_s + ex_s), "sample_correct": ok}
 
 
+NI_CRS = bytes(range(7, 39))
+
+
+def ni_shares(ctx, total, l0, count):
+    """Party secrets and NI server-key shares (client side) for indices
+    [l0, l0 + count)."""
+    ps, uit, rg = [], [], []
+    for j in range(total):
+        s_j, z_j = ctx.keygen_secrets(1000 + j, 1)
+        u_j = ctx.keygen_user_secret(2000 + j)
+        ps.append((s_j, z_j, u_j))
+        uit.append(ctx.ni_share_uitos(NI_CRS, 3000 + j, j, total, s_j, u_j))
+        rg.append(ctx.ni_share_rgsw(NI_CRS, 4000 + j, j, total, s_j, u_j, z_j, l0, count))
+    return ps, uit, rg
+
+
+def ni_aggregation():
+    """Server-key aggregation at NI_2P, 2 parties, all 520 LWE indices: the
+    u->s key switching keys from the shares and every RGSW (produced RLWEs
+    and the leader x other RGSW products), shares resident in HBM; then a
+    batch of gates on the aggregated key, decrypted with the ideal secret."""
+    import pzlib
+    ctx = pzlib.Context("NI_2P")
+    p, N, total = ctx.p, ctx.N, 2
+    n = p.lwe_n
+    t0 = time.perf_counter()
+    ps, uit, rg = ni_shares(ctx, total, 0, n)
+    share_s = (time.perf_counter() - t0) / total
+    d_kb = [ctx.to_device(uit[j][0]) for j in range(total)]
+    d_zf = [ctx.to_device(np.stack([uit[k][1][j if j < k else j - 1]
+                                    for k in range(total) if k != j])) for j in range(total)]
+    d_zero = ctx.to_device(np.stack([x[0] for x in rg]))
+    d_cts = ctx.to_device(np.stack([x[1] for x in rg]))
+    rows = 2 * (p.rlrg_da + p.rlrg_db)
+    d_out = ctx.array((n, rows, N))
+
+    def run():
+        ksks = [ctx.ni_uitos_ksk(NI_CRS, total, j, d_kb[j], d_zf[j]) for j in range(total)]
+        ctx.ni_aggregate_rgsw(NI_CRS, total, ksks, 0, n, d_zero, d_cts, d_out)
+        ctx.sync()
+        return ksks
+
+    for k in run():
+        k.free()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for k in run():
+            k.free()
+    agg_s = (time.perf_counter() - t0) / reps
+    ctx.ntt("forward", d_out, n * rows)
+    s = sum(x[0].astype(np.int64) for x in ps).astype(np.int32)
+    z = sum(x[1].astype(np.int64) for x in ps).astype(np.int32)
+    base = ctx.keygen_server(5, s, z)
+    _, auto, ksk = ctx.key_download(base)
+    base.free()
+    key = ctx.key_upload(d_out.download(), auto, ksk)
+    B = 256
+    r = np.random.default_rng(600)
+    b0 = r.integers(0, 2, size=B).astype(np.uint8)
+    b1 = r.integers(0, 2, size=B).astype(np.uint8)
+    ops = np.zeros(B, dtype=np.uint32)
+    dout = ctx.array((B, N + 1))
+    ctx.gates(key, ctx.to_device(ops), ctx.to_device(ctx.encrypt_bools(s, b0, 601)),
+              ctx.to_device(ctx.encrypt_bools(s, b1, 602)), dout, B)
+    ok = bool(np.array_equal(ctx.decrypt_bools(s, dout.download()), 1 - (b0 & b1)))
+    key.free()
+    ctx.close()
+    produced = n * (p.rlrg_da + p.rlrg_db + (total - 1) * (p.rgrg_da + p.rgrg_db))
+    return {"params": "NI_2P", "parties": total, "lwe_indices": n,
+            "aggregate_ms": agg_s * 1e3, "produced_rlwes": produced,
+            "uitos_digit_transforms": produced * p.uitos_d,
+            "share_gen_ms_per_party": share_s * 1e3, "gates_correct": ok}
+
+
 def cpu_baseline(params, threads, per_thread):
     """Oracle PBS throughput on the host cores (one PBS stream per thread)."""
     from oracle import Oracle
@@ -296,8 +371,28 @@ def cpu_baseline(params, threads, per_thread):
     lwes = r.integers(0, o.Q, size=(count, o.N + 1), dtype=np.uint64)
     _, secs = o.pbs_batch_threads(key, 0, lwes, threads)
     _, secs1 = o.pbs_batch_threads(key, 0, lwes[:4], 1)
+    agg = None
+    if o.p.uitos_d and o.p.rgrg_da and params == "NI_2P":
+        # server-key aggregation, one thread, on a sample of 4 LWE indices
+        # straddling the leaders' boundary, scaled to all indices
+        total, l0, cnt = 2, o.p.lwe_n // 2 - 2, 4
+        ps = [o.keygen_party(1000 + j) + (o.keygen_user_secret(2000 + j),) for j in range(total)]
+        uit = [o.ni_share_uitos(NI_CRS, 3000 + j, j, total, ps[j][0], ps[j][2])
+               for j in range(total)]
+        rg = [o.ni_share_rgsw(NI_CRS, 4000 + j, j, total, ps[j][0], ps[j][2], ps[j][1], l0, cnt)
+              for j in range(total)]
+        t0 = time.perf_counter()
+        ks = [o.ni_uitos_ksk(NI_CRS, total, j, uit[j][0],
+                             np.stack([uit[k][1][j if j < k else j - 1]
+                                       for k in range(total) if k != j])) for j in range(total)]
+        o.ni_aggregate_rgsw(NI_CRS, total, np.stack(ks), l0, cnt, np.stack([x[0] for x in rg]),
+                            np.stack([x[1] for x in rg]))
+        per_index = (time.perf_counter() - t0) / cnt
+        agg = {"aggregate_ms_est": per_index * o.p.lwe_n * 1e3, "cores": 1,
+               "sample": "%d of %d LWE indices, 2 parties, scaled" % (cnt, o.p.lwe_n)}
     return {"value": count / secs, "unit": "bootstraps/sec", "cores": threads,
             "kind": "port", "single_thread_pbs_ms": secs1 / 4 * 1e3,
+            "ni_aggregation_ni2p": agg,
             "sample": "%d NAND PBS at %s (C restatement of the reference's Rust PBS), "
                       "%d threads x %d, %.1f s wall, %.1f s CPU" %
                       (count, params, threads, per_thread, secs, secs * threads)}
@@ -351,12 +446,19 @@ def main():
     ap.add_argument("--cpu-per-thread", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline, plain and roofline only")
+    ap.add_argument("--leg", choices=["kernel_batch", "input_path", "ni_aggregation"],
+                    help="run one secondary measurement alone (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB read + 256 MiB write elementwise kernel twice "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.leg:
+        legs = {"kernel_batch": kernel_batch, "input_path": input_path,
+                "ni_aggregation": ni_aggregation}
+        print(json.dumps({args.leg: legs[args.leg]()}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only; multi-GPU runs are not implemented "
@@ -426,6 +528,8 @@ def main():
         log("kernel batch: %s" % extras["kernel_batch"])
         extras["input_path"] = input_path()
         log("input path: %s" % extras["input_path"])
+        extras["ni_aggregation"] = ni_aggregation()
+        log("ni aggregation: %s" % extras["ni_aggregation"])
         extras["fheuint8_ni8p"] = fheuint8_circuits(args)
         log("fheuint8: %s" % extras["fheuint8_ni8p"])
         extras["serving_ni2p"] = serving(args, plain)

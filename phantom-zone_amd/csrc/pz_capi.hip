@@ -61,6 +61,10 @@ struct pz_ctx {
     int br_per_cu[3] = {0, 0, 0};  // resident work groups per CU of each kernel variant
                                    // (BR_ONE, BR_SPLIT_LAT, BR_SPLIT; 0: not available)
     int br_mode = 0;          // 0 auto, 1 one-half kernel, 2 split kernel
+    // workspace of the key-preparation operators (grow-only; every user
+    // synchronises the stream before it returns)
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
     // captured gate batches
     std::map<GraphKey, hipGraphExec_t> graphs;
     const pz_key *graph_key = nullptr;
@@ -250,6 +254,7 @@ extern "C" int pz_ctx_destroy(pz_ctx *c)
     hipFree(c->d_ksb);
     hipFree(c->d_ks);
     hipFree(c->d_stats);
+    hipFree(c->ws);
     for (int i = 0; i < 4; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -379,6 +384,22 @@ int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStre
     *dc = c->dc;
     *logn = c->logn;
     *s = c->stream;
+    return PZ_OK;
+}
+
+// The context's workspace, at least `bytes` long (its contents are not kept
+// across growth).
+int pz_internal_workspace(pz_ctx *c, size_t bytes, void **out)
+{
+    if (bytes > c->ws_bytes) {
+        if (c->stream) hipStreamSynchronize(c->stream);
+        hipFree(c->ws);
+        c->ws = nullptr;
+        c->ws_bytes = 0;
+        if (hipMalloc(&c->ws, bytes) != hipSuccess) return fail(PZ_ERR_HIP, "workspace allocation failed");
+        c->ws_bytes = bytes;
+    }
+    *out = c->ws;
     return PZ_OK;
 }
 

@@ -276,6 +276,18 @@ inline void ni_seed_index(const uint8_t crs[32], uint32_t l, uint8_t out[32])
     puncture(m, l + 1, out);
 }
 
+// the seeds of indices [l0, l0 + count) at once: puncture(m, l + 1) is the
+// (l + 1)-th 32-byte chunk of the one stream ChaCha8(m)
+inline void ni_index_seeds(const uint8_t crs[32], uint32_t l0, uint32_t count, uint8_t *out)
+{
+    uint8_t k[32], m[32], skip[32];
+    puncture(crs, 1, k);
+    puncture(k, 1, m);
+    ChaCha8 r(m);
+    for (uint32_t l = 0; l < l0; l++) r.fill_bytes(skip, 32);
+    for (uint32_t t = 0; t < count; t++) r.fill_bytes(out + 32 * (size_t)t, 32);
+}
+
 inline void ni_seed_user(const uint8_t crs[32], uint32_t j, uint8_t out[32])
 {
     uint8_t k[32];

@@ -80,6 +80,7 @@ __global__ void k_sample_extract(const u64 *rlwe, const u32 *src, size_t count, 
 using namespace pz;
 
 extern int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStream_t *s);
+extern int pz_internal_workspace(pz_ctx *c, size_t bytes, void **out);
 
 int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out)
 {
@@ -246,21 +247,16 @@ extern "C" int pz_rgsw_by_rgsw(pz_ctx *c, uint64_t *d_rgsw0, const uint64_t *d_r
     const size_t words = B * (size_t)r1 * N;
     // RGSW1 rows times n^-1 (the step's inverse transforms leave it out) and
     // their Shoup twins, in scratch
-    u64 *k1 = nullptr, *k1s = nullptr;
-    if (hipMalloc(&k1, 8 * words) != hipSuccess) return PZ_ERR_HIP;
-    if (hipMalloc(&k1s, 8 * words) != hipSuccess) {
-        hipFree(k1);
-        return PZ_ERR_HIP;
-    }
+    void *ws = nullptr;
+    if ((rc = pz_internal_workspace(c, 16 * words, &ws))) return rc;
+    u64 *k1 = (u64 *)ws, *k1s = k1 + words;
     hipMemcpyAsync(k1, d_rgsw1, 8 * words, hipMemcpyDeviceToDevice, st);
     launch_scale(k1, words, dc.tb.n_inv, dc.tb.n_inv_s, p.rlwe_q, st);
     launch_shoup(k1, k1s, words, p.rlwe_q, st);
     const DevCtx cc = rgrg_devctx(p, dc);
     launch_rgsw_rows(logn, d_rgsw0, r0, p.rlrg_da, p.rlrg_db, B, cc, k1, k1s, r1, st);
     hipError_t e = hipGetLastError();
-    hipStreamSynchronize(st);  // the scratch is read by the launch above
-    hipFree(k1);
-    hipFree(k1s);
+    if (hipStreamSynchronize(st) != hipSuccess) e = hipErrorUnknown;  // workspace free again
     return e == hipSuccess ? PZ_OK : PZ_ERR_HIP;
 }
 

@@ -50,6 +50,12 @@ constexpr int PROBE_CHAINS = 8;  // independent chains per thread in k_valu_prob
 void launch_valu_probe(int kind, int grid, u64 *out, u32 iters, u32 c, u64 w, u64 ws, u64 q,
                        hipStream_t s);
 DevCtx rgrg_devctx(const pz_params &p, const DevCtx &dc);
+// S ChaCha8 streams (keys [S][8] words) of `draws` UniformInt<u64> draws
+// mod q each; out[s][k - keep_from] for k >= keep_from; reject[s] set when a
+// draw of stream s falls outside the acceptance zone (the positions after it
+// are then not the sequential stream's)
+void launch_expand_uniform(const u32 *keys, u32 S, u64 draws, u64 keep_from, u64 q, u64 *out,
+                           u32 *reject, hipStream_t s);
 void launch_rgsw_rows(int logn, u64 *rgsw0, u32 r0, u32 da0, u32 db0, size_t B, const DevCtx &c,
                       const u64 *K1, const u64 *K1s, u32 r1, hipStream_t s);
 void launch_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q, hipStream_t s);

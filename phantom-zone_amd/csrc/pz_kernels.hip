@@ -499,4 +499,70 @@ void launch_pack_ksk(const u64 *in, void *out, size_t rows, u32 n1, u32 ldk, u32
                            (uint32_t *)out, rows, n1, ldk);
 }
 
+// ---------------------------------------------------------------------------
+// Seed expansion: rand 0.8.5 UniformInt<u64> draws (widening multiply,
+// rejection zone; src/random.rs:93-110) from ChaCha8 streams (rand_chacha's
+// layout: 64-bit block counter in words 12-13, stream 0 in 14-15).  One
+// thread per 64-byte block, i.e. eight draws; a draw outside the zone marks
+// its stream for sequential regeneration on the host.
+// ---------------------------------------------------------------------------
+PZ_DEV u32 rotl32(u32 v, int c) { return (v << c) | (v >> (32 - c)); }
+#define PZ_QR(a, b, c, d)                 \
+    a += b; d ^= a; d = rotl32(d, 16);    \
+    c += d; b ^= c; b = rotl32(b, 12);    \
+    a += b; d ^= a; d = rotl32(d, 8);     \
+    c += d; b ^= c; b = rotl32(b, 7);
+
+__global__ void k_expand_uniform(const u32 *keys, u32 S, u64 draws, u64 keep_from, u64 q,
+                                 u64 zone, u64 *out, u32 *reject)
+{
+    const u64 blocks = (draws + 7) / 8;
+    const u64 kept = draws - keep_from;
+    for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < (u64)S * blocks;
+         t += (u64)gridDim.x * blockDim.x) {
+        const u32 s = (u32)(t / blocks);
+        const u64 b = t % blocks;
+        u32 st[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+        for (int i = 0; i < 8; i++) st[4 + i] = keys[8 * (size_t)s + i];
+        st[12] = (u32)b;
+        st[13] = (u32)(b >> 32);
+        st[14] = 0;
+        st[15] = 0;
+        u32 x[16];
+        for (int i = 0; i < 16; i++) x[i] = st[i];
+        for (int r = 0; r < 8; r += 2) {
+            PZ_QR(x[0], x[4], x[8], x[12]);
+            PZ_QR(x[1], x[5], x[9], x[13]);
+            PZ_QR(x[2], x[6], x[10], x[14]);
+            PZ_QR(x[3], x[7], x[11], x[15]);
+            PZ_QR(x[0], x[5], x[10], x[15]);
+            PZ_QR(x[1], x[6], x[11], x[12]);
+            PZ_QR(x[2], x[7], x[8], x[13]);
+            PZ_QR(x[3], x[4], x[9], x[14]);
+        }
+        bool bad = false;
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            const u64 k = 8 * b + w;
+            if (k >= draws) break;
+            const u64 v = (u64)(x[2 * w] + st[2 * w]) | ((u64)(x[2 * w + 1] + st[2 * w + 1]) << 32);
+            if (v * q > zone) bad = true;
+            if (k >= keep_from) out[(size_t)s * kept + (k - keep_from)] = __umul64hi(v, q);
+        }
+        if (bad) reject[s] = 1;
+    }
+}
+#undef PZ_QR
+
+void launch_expand_uniform(const u32 *keys, u32 S, u64 draws, u64 keep_from, u64 q, u64 *out,
+                           u32 *reject, hipStream_t s)
+{
+    const u64 work = (u64)S * ((draws + 7) / 8);
+    if (work == 0) return;
+    const u64 zone = ~0ull - (0ull - q) % q;  // u64::MAX - (2^64 mod q)
+    const u64 g = (work + 255) / 256;
+    hipLaunchKernelGGL(k_expand_uniform, (int)(g < 65535 * 8 ? g : 65535 * 8), 256, 0, s, keys,
+                       S, draws, keep_from, q, zone, out, reject);
+}
+
 }  // namespace pz

@@ -134,13 +134,6 @@ k_ni_rows(NiArgs A, DevCtx c)
     }
 }
 
-}  // namespace pz
-
-using namespace pz;
-
-extern int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStream_t *s);
-extern int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out);
-
 __global__ void k_ni_sum_rows(u64 *b, const u64 *zero, u32 others, size_t words, u64 q)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words;
@@ -152,6 +145,46 @@ __global__ void k_ni_sum_rows(u64 *b, const u64 *zero, u32 others, size_t words,
 }
 
 static int ni_grid(size_t n) { return (int)((n + 255) / 256 < 65535 * 4 ? (n + 255) / 256 : 65535 * 4); }
+
+
+}  // namespace pz
+
+using namespace pz;
+
+extern int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStream_t *s);
+extern int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out);
+extern int pz_internal_workspace(pz_ctx *c, size_t bytes, void **out);
+
+// Uniform rows of S seeded streams into d_out ([S][draws - keep_from]) on
+// the device; d_kf is workspace for the keys and the reject flags (12*S
+// words).  A stream with a draw outside the acceptance zone (probability
+// about 2^-38 per draw for the 54-bit modulus) is regenerated sequentially on
+// the host, so the rows always equal the sequential streams.
+static bool expand_rows(hipStream_t st, const uint8_t *seeds, u32 S, u64 draws, u64 keep_from,
+                        u64 q, u64 *d_out, u32 *d_kf)
+{
+    u32 *d_keys = d_kf, *d_flags = d_kf + 8 * (size_t)S;
+    std::vector<u32> flags(S);
+    if (hipMemcpyAsync(d_keys, seeds, 32 * (size_t)S, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemsetAsync(d_flags, 0, 4 * (size_t)S, st) != hipSuccess)
+        return false;
+    launch_expand_uniform(d_keys, S, draws, keep_from, q, d_out, d_flags, st);
+    if (hipMemcpyAsync(flags.data(), d_flags, 4 * (size_t)S, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return false;
+    const u64 kept = draws - keep_from;
+    std::vector<u64> h(kept);
+    for (u32 s = 0; s < S; s++) {
+        if (!flags[s]) continue;
+        host::ChaCha8 r(seeds + 32 * (size_t)s);
+        for (u64 k = 0; k < keep_from; k++) r.uniform(q);
+        for (u64 k = 0; k < kept; k++) h[k] = r.uniform(q);
+        if (hipMemcpy(d_out + s * kept, h.data(), 8 * kept, hipMemcpyHostToDevice) != hipSuccess)
+            return false;
+    }
+    return true;
+}
 
 extern "C" int pz_ni_uitos_ksk(pz_ctx *c, const uint8_t *crs, uint32_t total, uint32_t user,
                                const uint64_t *d_ksk_b, const uint64_t *d_zero_for_user,
@@ -168,17 +201,13 @@ extern "C" int pz_ni_uitos_ksk(pz_ctx *c, const uint8_t *crs, uint32_t total, ui
     if (total == 0 || total > PZ_NI_MAX_PARTIES || user >= total) return PZ_ERR_ARG;
     const u32 N = p.rlwe_n, d = p.uitos_d;
     const size_t words = (size_t)d * N;
-    std::vector<u64> ha(words);
-    {
-        uint8_t sd[32];
-        host::ni_seed_user(crs, user, sd);
-        host::ChaCha8 pr(sd);
-        for (size_t i = 0; i < words; i++) ha[i] = pr.uniform(p.rlwe_q);
-    }
+    uint8_t sd[32];
+    host::ni_seed_user(crs, user, sd);
+    void *ws = nullptr;
+    if ((rc = pz_internal_workspace(c, 64, &ws))) return rc;
     u64 *rows = nullptr;
     if (hipMalloc(&rows, 16 * words) != hipSuccess) return PZ_ERR_HIP;
-    bool ok = hipMemcpyAsync(rows, ha.data(), 8 * words, hipMemcpyHostToDevice, st) ==
-                  hipSuccess &&
+    bool ok = expand_rows(st, sd, 1, words, 0, p.rlwe_q, rows, (u32 *)ws) &&
               hipMemcpyAsync(rows + words, d_ksk_b, 8 * words, hipMemcpyDeviceToDevice, st) ==
                   hipSuccess;
     if (ok) {
@@ -217,21 +246,16 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
     if (stream) st = (hipStream_t)stream;
     const u32 N = p.rlwe_n, da = p.rgrg_da, db = p.rgrg_db, dmax = da > db ? da : db;
     const size_t r1 = 2 * (size_t)(da + db);
-    // a_{i,l} for the RLWE'(m) rows: the last db rows of index l's stream
-    std::vector<u64> ha((size_t)count * db * N);
-    host::parallel_for(count, [&](size_t t) {
-        uint8_t sd[32];
-        host::ni_seed_index(crs, l0 + (u32)t, sd);
-        host::ChaCha8 pr(sd);
-        for (size_t i = 0; i < (size_t)(dmax - db) * N; i++) pr.uniform(p.rlwe_q);
-        for (size_t i = 0; i < (size_t)db * N; i++) ha[t * db * N + i] = pr.uniform(p.rlwe_q);
-    });
-    const size_t w1 = (size_t)(total - 1) * count * r1 * N;
-    u64 *d_a = nullptr, *o1 = nullptr, *o1s = nullptr;
-    bool ok = hipMalloc(&d_a, 8 * ha.size()) == hipSuccess;
-    if (ok && w1) ok = hipMalloc(&o1, 8 * w1) == hipSuccess && hipMalloc(&o1s, 8 * w1) == hipSuccess;
-    if (ok) ok = hipMemcpyAsync(d_a, ha.data(), 8 * ha.size(), hipMemcpyHostToDevice, st) ==
-                 hipSuccess;
+    // workspace: the others' RGSWs and their Shoup twins, a_{i,l} of the
+    // RLWE'(m) rows (the last db rows of each index's stream), stream keys
+    const size_t w1 = (size_t)(total - 1) * count * r1 * N, wa = (size_t)count * db * N;
+    void *ws = nullptr;
+    if ((rc = pz_internal_workspace(c, 8 * (2 * w1 + wa) + 48 * (size_t)count, &ws))) return rc;
+    u64 *o1 = (u64 *)ws, *o1s = o1 + w1, *d_a = o1s + w1;
+    std::vector<uint8_t> seeds(32 * (size_t)count);
+    host::ni_index_seeds(crs, l0, count, seeds.data());
+    bool ok = expand_rows(st, seeds.data(), count, (u64)dmax * N, (u64)(dmax - db) * N, p.rlwe_q,
+                          d_a, (u32 *)(d_a + wa));
     if (ok) {
         NiArgs A;
         A.zero = d_zero;
@@ -270,8 +294,5 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
         }
         ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
     }
-    hipFree(d_a);
-    hipFree(o1);
-    hipFree(o1s);
     return ok ? PZ_OK : PZ_ERR_HIP;
 }
