@@ -98,8 +98,10 @@ static int check_params(const pz_params *p, int *logn)
     while ((1u << l) < N) l++;
     if ((1u << l) != N || (l != 9 && l != 11))
         return fail(PZ_ERR_UNSUPPORTED, "rlwe_n must be 512 or 2048");
-    if (p->rlwe_q >= (1ull << 61) || p->rlwe_q % (2ull * N) != 1)
-        return fail(PZ_ERR_ARG, "rlwe_q must be < 2^61 and 1 mod 2N");
+    // unreduced transform stages keep values below 2^10 q (inverse) and 23 q
+    // (forward), so q < 2^54
+    if (p->rlwe_q >= (1ull << 54) || p->rlwe_q % (2ull * N) != 1)
+        return fail(PZ_ERR_ARG, "rlwe_q must be < 2^54 and 1 mod 2N");
     if (p->lwe_q & (p->lwe_q - 1) || p->lwe_q > (1ull << 17) || p->lwe_q < 2)
         return fail(PZ_ERR_UNSUPPORTED, "lwe_q must be a power of two <= 2^17");
     if (p->br_q & (p->br_q - 1) || p->br_q > 2 * N || p->br_q < 16)
@@ -181,6 +183,7 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
     d.tb.n_inv_s = c->ht.n_inv_s;
     d.tb.q = p->rlwe_q;
     d.tb.q2 = 2 * p->rlwe_q;
+    d.tb.one_s = host::shoup_repr(1, p->rlwe_q);
     if (p->uitos_d) d.dec_uitos = make_decomp(p->rlwe_q, p->uitos_logb, p->uitos_d);
     d.dec_a = make_decomp(p->rlwe_q, p->rlrg_logb, p->rlrg_da);
     d.dec_b = make_decomp(p->rlwe_q, p->rlrg_logb, p->rlrg_db);

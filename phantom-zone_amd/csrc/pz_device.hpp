@@ -21,6 +21,11 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #define PZ_FWD_MODE FWD_FAST
 #endif
 
+// Unreduced early inverse stages in the fused operators (inputs in [0,2q),
+// strict outputs; requires q < 2^54).
+#ifndef PZ_INV_FAST
+#define PZ_INV_FAST 1
+#endif
 #ifndef PZ_SPLIT_SM
 #define PZ_SPLIT_SM true
 #endif
@@ -177,7 +182,7 @@ PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
             fma_row<false>(acc[1], K + o1, Ks + o1, D[0], j, q, q2);
         }
     }
-    ntt_inv<LOGN, 2, true, false, false>(acc, j, lds, c.tb);  // n^-1 is in the key rows
+    ntt_inv<LOGN, 2, true, false, false, PZ_INV_FAST>(acc, j, lds, c.tb);  // n^-1: key rows
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         a[r] = acc[0][r];
@@ -241,7 +246,7 @@ PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
         fma_row<false>(acc[0], K + (size_t)rr * N, Ks + (size_t)rr * N, D[0], j, q, q2);
         fma_row<false>(acc[1], K + (size_t)(d + rr) * N, Ks + (size_t)(d + rr) * N, D[0], j, q, q2);
     }
-    ntt_inv<LOGN, 2, true, false, false>(acc, j, lds, c.tb);  // n^-1 is in the key rows
+    ntt_inv<LOGN, 2, true, false, false, PZ_INV_FAST>(acc, j, lds, c.tb);  // n^-1: key rows
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         u64 v = bp[r] & ~SIGN;
@@ -356,7 +361,7 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     }
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (second) fma_row_halves<PZ_SPLIT_SM>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
-    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM>(acc, j, mine, c.tb);  // n^-1 is in the key rows
+    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST>(acc, j, mine, c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
 #pragma unroll
         for (int r = 0; r < 8; r++) {
