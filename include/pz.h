@@ -34,7 +34,7 @@ extern "C" {
 
 /* BoolParameters (src/bool/parameters.rs:96-167). */
 typedef struct pz_params {
-    uint64_t rlwe_q;      /* Q, prime                                 */
+    uint64_t rlwe_q;      /* Q, prime, < 2^54, = 1 mod 2N             */
     uint32_t rlwe_n;      /* N = 512 .. 4096                          */
     uint32_t lwe_n;       /* n                                        */
     uint64_t lwe_q;       /* q_ks, power of two <= 2^17              */
