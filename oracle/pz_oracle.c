@@ -1568,7 +1568,8 @@ void pzo_rgsw_by_rgsw(const pzo_eval *e, uint64_t *rgsw0, const uint64_t *rgsw1)
 /* ---- non-interactive multi-party server key: shares and aggregation ----
  * The seed tree follows NonInteractiveMultiPartyCrs
  * (src/bool/evaluator.rs:97-160): puncture(seed, t) draws t 32-byte seeds
- * from ChaCha8(seed) and keeps the last (puncture_p_rng, src/utils.rs:219-228);
+ * from ChaCha8(seed), one stream word per byte, and keeps the last
+ * (puncture_p_rng, src/utils.rs:219-228);
  *   index l:  puncture(puncture(puncture(crs, 1), 1), l + 1)
  *   user j:   puncture(puncture(crs, 2), j + 1)
  * Noise streams are the oracle's own: ChaCha8(puncture(noise_seed, 1)) for
@@ -1576,9 +1577,13 @@ void pzo_rgsw_by_rgsw(const pzo_eval *e, uint64_t *rgsw0, const uint64_t *rgsw1)
  * index l. */
 void pzo_puncture(const uint8_t seed[32], uint32_t times, uint8_t out[32])
 {
+    /* each of the 32 bytes is Uniform::new_inclusive(0u8, 255u8) (RandomFill<[u8; 32]>,
+     * src/random.rs:162-175): over the full range rand 0.8.5 samples Standard, the low
+     * byte of next_u32 */
     pzo_rng *r = pzo_rng_new(seed);
     memset(out, 0, 32);
-    for (uint32_t t = 0; t < times; t++) pzo_rng_fill_bytes(r, out, 32);
+    for (uint32_t t = 0; t < times; t++)
+        for (int i = 0; i < 32; i++) out[i] = (uint8_t)pzo_rng_next_u32(r);
     pzo_rng_free(r);
 }
 

@@ -258,12 +258,21 @@ inline std::vector<uint32_t> dlog_map(const pz_params &p)
     return d;
 }
 
-// puncture_p_rng (src/utils.rs:219-228): the t-th 32-byte draw of ChaCha8(seed)
+// RandomFill<[u8; 32]> (src/random.rs:162-175): 32 draws of
+// Uniform::new_inclusive(0u8, 255u8).  Over the full u8 range rand 0.8.5's
+// UniformInt stores range 256 as 0 and samples Standard instead, which for u8
+// is the low byte of next_u32: one stream word per byte.
+inline void fill_seed(ChaCha8 &r, uint8_t out[32])
+{
+    for (int i = 0; i < 32; i++) out[i] = (uint8_t)r.next_u32();
+}
+
+// puncture_p_rng (src/utils.rs:219-228): the t-th seed drawn from ChaCha8(seed)
 inline void puncture(const uint8_t seed[32], uint32_t times, uint8_t out[32])
 {
     ChaCha8 r(seed);
     memset(out, 0, 32);
-    for (uint32_t t = 0; t < times; t++) r.fill_bytes(out, 32);
+    for (uint32_t t = 0; t < times; t++) fill_seed(r, out);
 }
 
 // Seed tree of NonInteractiveMultiPartyCrs (src/bool/evaluator.rs:97-160):
@@ -277,15 +286,15 @@ inline void ni_seed_index(const uint8_t crs[32], uint32_t l, uint8_t out[32])
 }
 
 // the seeds of indices [l0, l0 + count) at once: puncture(m, l + 1) is the
-// (l + 1)-th 32-byte chunk of the one stream ChaCha8(m)
+// (l + 1)-th seed of the one stream ChaCha8(m)
 inline void ni_index_seeds(const uint8_t crs[32], uint32_t l0, uint32_t count, uint8_t *out)
 {
     uint8_t k[32], m[32], skip[32];
     puncture(crs, 1, k);
     puncture(k, 1, m);
     ChaCha8 r(m);
-    for (uint32_t l = 0; l < l0; l++) r.fill_bytes(skip, 32);
-    for (uint32_t t = 0; t < count; t++) r.fill_bytes(out + 32 * (size_t)t, 32);
+    for (uint32_t l = 0; l < l0; l++) fill_seed(r, skip);
+    for (uint32_t t = 0; t < count; t++) fill_seed(r, out + 32 * (size_t)t);
 }
 
 inline void ni_seed_user(const uint8_t crs[32], uint32_t j, uint8_t out[32])

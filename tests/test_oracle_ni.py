@@ -5,8 +5,8 @@ src/multi_party.rs:186-291).
 Every check decrypts with the ideal secret over Python ints (Kronecker
 products), so a wrong sign, gadget power, row alignment or party order shows
 up as a phase far from the expected message:
-- puncture(seed, t) is the t-th 32-byte chunk of the ChaCha8 stream (the
-  independent Python block function);
+- puncture(seed, t) is the t-th seed of the ChaCha8 stream, each byte the low
+  byte of one stream word (the independent Python block function);
 - the leader of index l is the party whose segment [per*j, per*(j+1)) holds l,
   per = ceil(lwe_n / total);
 - Ksk(u_j -> s) rows satisfy B_i - A_i*s = beta_i*u_j + e with |e| within the
@@ -70,10 +70,10 @@ def parties(o, total):
 def test_puncture_is_the_stream_chunk():
     o = oracle_for("NI_2P")
     seed = bytes(range(100, 132))
-    words = py_block(seed, 0, 0, 8) + py_block(seed, 1, 0, 8)
-    stream = b"".join(w.to_bytes(4, "little") for w in words)
+    words = sum((py_block(seed, c, 0, 8) for c in range(8)), [])
     for t in (1, 2, 3, 4):
-        assert o.puncture(seed, t).tobytes() == stream[32 * (t - 1):32 * t]
+        want = bytes(w & 0xFF for w in words[32 * (t - 1):32 * t])  # one word per byte
+        assert o.puncture(seed, t).tobytes() == want
 
 
 @pytest.mark.parametrize("total", [1, 2, 3, 4, 7, 8])
