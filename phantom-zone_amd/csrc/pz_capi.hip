@@ -117,6 +117,10 @@ static int check_params(const pz_params *p, int *logn)
         return fail(PZ_ERR_ARG, "decomposer wants logq >= logb*d");
     if (p->uitos_d && (p->uitos_logb == 0 || p->uitos_logb * p->uitos_d > lq || p->uitos_d > 64))
         return fail(PZ_ERR_ARG, "u -> s decomposer wants 0 < logb*d <= logq, d <= 64");
+    if ((p->rgrg_da != 0) != (p->rgrg_db != 0) ||
+        (p->rgrg_da && (p->rgrg_logb == 0 || p->rgrg_logb * p->rgrg_da > lq ||
+                        p->rgrg_logb * p->rgrg_db > lq)))
+        return fail(PZ_ERR_ARG, "RGSW x RGSW decomposer wants 0 < logb*d <= logq for both counts");
     *logn = l;
     return PZ_OK;
 }
