@@ -84,6 +84,13 @@ struct pz_key {
 
 extern "C" const char *pz_last_error(void) { return g_err.c_str(); }
 
+bool pz_internal_key_fits(const pz_ctx *c, const pz_key *k);
+#define KEY_OK(c, k)                                                                \
+    do {                                                                            \
+        if (!pz_internal_key_fits(c, k))                                            \
+            return fail(PZ_ERR_ARG, "key made for another parameter set or device"); \
+    } while (0)
+
 extern "C" int pz_params_named(const char *name, pz_params *out)
 {
     if (!name || !out) return fail(PZ_ERR_ARG, "null argument");
@@ -561,6 +568,7 @@ extern "C" int pz_rlwe_by_rgsw(pz_ctx *c, pz_key *k, uint64_t *rlwe, const uint3
                                const uint8_t *triv, size_t B, void *s)
 {
     if (!c || !k || !rlwe || !idx || !triv) return fail(PZ_ERR_ARG, "null argument");
+    KEY_OK(c, k);
     if (B == 0) return PZ_OK;
     launch_ext(c->logn, false, rlwe, idx, triv, B, c->dc, k->dk, S(c, s));
     HIPCHK(hipGetLastError());
@@ -571,6 +579,7 @@ extern "C" int pz_rlwe_auto(pz_ctx *c, pz_key *k, uint64_t *rlwe, const uint32_t
                             const uint8_t *triv, size_t B, void *s)
 {
     if (!c || !k || !rlwe || !idx || !triv) return fail(PZ_ERR_ARG, "null argument");
+    KEY_OK(c, k);
     if (B == 0) return PZ_OK;
     launch_ext(c->logn, true, rlwe, idx, triv, B, c->dc, k->dk, S(c, s));
     HIPCHK(hipGetLastError());
@@ -622,10 +631,34 @@ int pz_internal_exec_view(pz_ctx *c, size_t B, pz_exec_view *v, DevCtx *dc)
 
 const DevKey *pz_internal_devkey(const pz_key *k) { return &k->dk; }
 
+// Contexts whose keys are interchangeable: same device, same parameter set.
+bool pz_internal_ctx_compatible(const pz_ctx *a, const pz_ctx *b)
+{
+    if (!a || !b) return false;
+    if (a == b) return true;
+    const pz_params &x = a->p, &y = b->p;
+    return a->device == b->device && x.rlwe_q == y.rlwe_q && x.rlwe_n == y.rlwe_n &&
+           x.lwe_n == y.lwe_n && x.lwe_q == y.lwe_q && x.br_q == y.br_q &&
+           x.lwe_logb == y.lwe_logb && x.lwe_d == y.lwe_d && x.rlrg_logb == y.rlrg_logb &&
+           x.rlrg_da == y.rlrg_da && x.rlrg_db == y.rlrg_db && x.auto_logb == y.auto_logb &&
+           x.auto_d == y.auto_d && x.g == y.g && x.w == y.w &&
+           x.lwe_secret_gaussian == y.lwe_secret_gaussian && x.uitos_logb == y.uitos_logb &&
+           x.uitos_d == y.uitos_d && x.rgrg_logb == y.rgrg_logb && x.rgrg_da == y.rgrg_da &&
+           x.rgrg_db == y.rgrg_db;
+}
+
+// A key serves any context of its parameter set on its device.
+bool pz_internal_key_fits(const pz_ctx *c, const pz_key *k)
+{
+    return k && pz_internal_ctx_compatible(c, k->owner);
+}
+
+
 extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint64_t *out,
                                  size_t B, void *s)
 {
     if (!c || !k || !in || !out) return fail(PZ_ERR_ARG, "null argument");
+    KEY_OK(c, k);
     if (B == 0) return PZ_OK;
     int rc = ensure_scratch(c, B);
     if (rc) return rc;
@@ -711,6 +744,7 @@ extern "C" int pz_pbs(pz_ctx *c, pz_key *k, const uint32_t *gate, uint64_t *lwe,
                       void *s)
 {
     if (!c || !k || !gate || !lwe) return fail(PZ_ERR_ARG, "null argument");
+    KEY_OK(c, k);
     if (B == 0) return PZ_OK;
     int rc = ensure_scratch(c, B);
     if (rc) return rc;
@@ -724,6 +758,7 @@ extern "C" int pz_gates(pz_ctx *c, pz_key *k, const uint32_t *ops, const uint64_
                         const uint64_t *c1, uint64_t *out, size_t B, void *s)
 {
     if (!c || !k || !ops || !c0 || !c1 || !out) return fail(PZ_ERR_ARG, "null argument");
+    KEY_OK(c, k);
     if (B == 0) return PZ_OK;
     int rc = ensure_scratch(c, B);
     if (rc) return rc;

@@ -624,6 +624,7 @@ static int ensure_pool(pz_circuit *c)
 extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inputs, void *stream)
 {
     if (!c || !key || (!d_inputs && !c->inputs.empty())) return PZ_ERR_ARG;
+    if (!pz_internal_key_fits(c->ctx, key)) return PZ_ERR_ARG;
     int rc = bind_ctx(c);
     if (rc) return rc;
     schedule(c);

@@ -171,7 +171,8 @@ extern "C" int pz_rlwe_key_switch(pz_ctx *c, pz_rlwe_ksk *k, const uint64_t *d_i
     if (!k || !d_in || !d_out) return PZ_ERR_ARG;
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (p.uitos_d == 0 || k->d != p.uitos_d) return PZ_ERR_UNSUPPORTED;
+    if (p.uitos_d == 0) return PZ_ERR_UNSUPPORTED;
+    if (!pz_internal_ctx_compatible(c, k->owner)) return PZ_ERR_ARG;
     if (B == 0) return PZ_OK;
     if (stream) st = (hipStream_t)stream;
     const int grid = (int)(B < 65535 ? B : 65535);

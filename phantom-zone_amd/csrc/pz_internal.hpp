@@ -76,4 +76,7 @@ struct pz_rlwe_ksk {
 };
 // takes ownership of device rows d_rows (evaluation domain)
 int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out);
+// same device and parameter set: keys made under one serve the other
+bool pz_internal_ctx_compatible(const pz_ctx *a, const pz_ctx *b);
+bool pz_internal_key_fits(const pz_ctx *c, const pz_key *k);
 

@@ -241,7 +241,7 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
     if (total == 0 || total > PZ_NI_MAX_PARTIES) return PZ_ERR_ARG;
     if ((uint64_t)l0 + count > p.lwe_n) return PZ_ERR_ARG;
     for (u32 o = 0; o < total; o++)
-        if (!ksks[o] || ksks[o]->owner != c || ksks[o]->d != p.uitos_d) return PZ_ERR_ARG;
+        if (!ksks[o] || !pz_internal_ctx_compatible(c, ksks[o]->owner)) return PZ_ERR_ARG;
     if (count == 0) return PZ_OK;
     if (stream) st = (hipStream_t)stream;
     const u32 N = p.rlwe_n, da = p.rgrg_da, db = p.rgrg_db, dmax = da > db ? da : db;

@@ -1,0 +1,160 @@
+"""Edge inputs and error paths of the C ABI on the GPU.
+
+- Empty batches (B = 0) of every batched entry point return success and leave
+  the output untouched.
+- Null buffers and out-of-range arguments return PZ_ERR_ARG, parameter sets
+  without the needed decomposer PZ_ERR_UNSUPPORTED; nothing is launched.
+- Keys and Ksk(u -> s) handles serve every context of their parameter set on
+  their device (outputs equal the owner's), and are refused by a context of
+  another parameter set (the kernels would index past the key).
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import keys_for
+
+pytestmark = pytest.mark.gpu
+
+PZ_ERR_ARG, PZ_ERR_UNSUPPORTED = -1, -3
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def env():
+    import pzlib
+    ctx = pzlib.Context("NI_2P")
+    o, s, z, key = keys_for("NI_2P")
+    k = ctx.key_upload(key.rgsw, key.auto_key, key.lwe_ksk)
+    yield pzlib, ctx, k, s
+    k.free()
+    ctx.close()
+
+
+def test_empty_batches(env):
+    pzlib, ctx, k, s = env
+    log("edges: empty batches")
+    N, n = ctx.N, ctx.n
+    sentinel = np.full((2, N + 1), 7, dtype=np.uint64)
+    d = ctx.to_device(sentinel)
+    d_u32 = ctx.to_device(np.zeros(2, dtype=np.uint32))
+    d_u8 = ctx.to_device(np.zeros(2, dtype=np.uint8))
+    L, h = ctx.L, ctx.h
+    assert L.pz_gates(h, k.h, d_u32.ptr, d.ptr, d.ptr, d.ptr, 0, None) == 0
+    assert L.pz_pbs(h, k.h, d_u32.ptr, d.ptr, 0, None) == 0
+    assert L.pz_not(h, d.ptr, 0, None) == 0
+    assert L.pz_lwe_key_switch(h, k.h, d.ptr, d.ptr, 0, None) == 0
+    assert L.pz_rlwe_by_rgsw(h, k.h, d.ptr, d_u32.ptr, d_u8.ptr, 0, None) == 0
+    assert L.pz_rlwe_auto(h, k.h, d.ptr, d_u32.ptr, d_u8.ptr, 0, None) == 0
+    assert L.pz_ntt_forward(h, d.ptr, 0, None) == 0
+    assert L.pz_sample_extract(h, d.ptr, d_u32.ptr, 0, d.ptr, None) == 0
+    assert L.pz_rgsw_by_rgsw(h, d.ptr, d.ptr, 0, None) == 0
+    ctx.sync()
+    assert np.array_equal(d.download(), sentinel)
+
+
+def test_null_and_range_errors(env):
+    pzlib, ctx, k, s = env
+    log("edges: null and range errors")
+    L, h = ctx.L, ctx.h
+    d = ctx.array((4, ctx.N + 1))
+    d_u32 = ctx.to_device(np.zeros(4, dtype=np.uint32))
+    assert L.pz_gates(h, k.h, None, d.ptr, d.ptr, d.ptr, 4, None) == PZ_ERR_ARG
+    assert L.pz_gates(h, None, d_u32.ptr, d.ptr, d.ptr, d.ptr, 4, None) == PZ_ERR_ARG
+    assert L.pz_pbs(h, k.h, d_u32.ptr, None, 4, None) == PZ_ERR_ARG
+    assert L.pz_rgsw_by_rgsw(h, None, d.ptr, 1, None) == PZ_ERR_ARG
+    crs = np.zeros(32, dtype=np.uint8)
+    ks = (C.c_void_p * 2)(None, None)
+    assert L.pz_ni_aggregate_rgsw(h, crs.ctypes.data, 2, ks, 0, 1, d.ptr, d.ptr, d.ptr,
+                                  None) == PZ_ERR_ARG  # null Ksk handles
+    assert L.pz_ni_aggregate_rgsw(h, crs.ctypes.data, 9, ks, 0, 1, d.ptr, d.ptr, d.ptr,
+                                  None) == PZ_ERR_ARG  # more than 8 parties
+    sec = np.zeros(ctx.N, dtype=np.int32)
+    z = np.zeros(ctx.n, dtype=np.int32)
+    out = np.zeros((1, 10, ctx.N), dtype=np.uint64)
+    assert L.pz_ni_share_rgsw(h, crs.ctypes.data, crs.ctypes.data, 2, 2, sec.ctypes.data,
+                              sec.ctypes.data, z.ctypes.data, 0, 1, out.ctypes.data,
+                              out.ctypes.data) == PZ_ERR_ARG  # self >= total
+    assert L.pz_ni_share_rgsw(h, crs.ctypes.data, crs.ctypes.data, 0, 2, sec.ctypes.data,
+                              sec.ctypes.data, z.ctypes.data, ctx.n, 1, out.ctypes.data,
+                              out.ctypes.data) == PZ_ERR_ARG  # index past lwe_n
+    sp = pzlib.Context("SP_TEST")
+    try:
+        assert sp.L.pz_ni_share_uitos(sp.h, crs.ctypes.data, crs.ctypes.data, 0, 2,
+                                      sec.ctypes.data, sec.ctypes.data, out.ctypes.data,
+                                      out.ctypes.data) == PZ_ERR_UNSUPPORTED
+    finally:
+        sp.close()
+
+
+def _gates(ctx, key, s, seed):
+    r = np.random.default_rng(seed)
+    B = 64
+    b0 = r.integers(0, 2, B).astype(np.uint8)
+    b1 = r.integers(0, 2, B).astype(np.uint8)
+    c0, c1 = ctx.encrypt_bools(s, b0, seed), ctx.encrypt_bools(s, b1, seed + 1)
+    out = ctx.array((B, ctx.N + 1))
+    ctx.gates(key, ctx.to_device(np.zeros(B, np.uint32)), ctx.to_device(c0), ctx.to_device(c1),
+              out, B)
+    return out.download(), 1 - (b0 & b1)
+
+
+def test_keys_across_contexts(env):
+    pzlib, ctx, k, s = env
+    log("edges: keys across contexts")
+    first, want = _gates(ctx, k, s, 500)
+    other = pzlib.Context("NI_2P")
+    try:
+        again, _ = _gates(other, k, s, 500)  # same parameter set: the key serves
+        assert np.array_equal(again, first)
+        assert np.array_equal(ctx.decrypt_bools(s, first), want)
+    finally:
+        other.close()
+    wrong = pzlib.Context("NI_8P")
+    try:
+        B = 4
+        d = wrong.array((B, wrong.N + 1))
+        ops = wrong.to_device(np.zeros(B, np.uint32))
+        with pytest.raises(pzlib.PzError):
+            wrong.gates(k, ops, d, d, d, B)
+        with pytest.raises(pzlib.PzError):
+            wrong.pbs(k, ops, d, B)
+        circ = pzlib.Circuit(wrong)
+        circ.gate("nand", *circ.inputs(2))
+        with pytest.raises(pzlib.PzError):
+            circ.run(k, wrong.array((2, wrong.N + 1)))
+        circ.close()
+    finally:
+        wrong.close()
+
+
+def test_ksk_across_contexts():
+    import pzlib
+    log("edges: u->s key switching keys across contexts")
+    a = pzlib.Context("NI_2P")
+    b = pzlib.Context("NI_2P")
+    c = pzlib.Context("NI_8P")
+    try:
+        s, z = a.keygen_secrets(900, 2)
+        u = a.keygen_user_secret(901)
+        ksk = a.keygen_rlwe_ksk(902, u, s)
+        cts = a.encrypt_rlwe_bools(u, np.ones(a.N, dtype=np.uint8), 903)
+        outs = []
+        for ctx in (a, b):
+            d_in, d_out = ctx.to_device(cts), ctx.array(cts.shape)
+            ctx.rlwe_key_switch(ksk, d_in, d_out, 1)
+            outs.append(d_out.download())
+        assert np.array_equal(outs[0], outs[1])
+        with pytest.raises(pzlib.PzError):
+            c.rlwe_key_switch(ksk, c.to_device(cts), c.array(cts.shape), 1)
+        ksk.free()
+    finally:
+        a.close()
+        b.close()
+        c.close()
