@@ -302,14 +302,14 @@ def ni_shares(ctx, total, l0, count):
     return ps, uit, rg
 
 
-def ni_aggregation():
-    """Server-key aggregation at NI_2P, 2 parties, all 520 LWE indices: the
+def ni_aggregation(params="NI_2P", total=2):
+    """Server-key aggregation for `total` parties over all LWE indices: the
     u->s key switching keys from the shares and every RGSW (produced RLWEs
     and the leader x other RGSW products), shares resident in HBM; then a
     batch of gates on the aggregated key, decrypted with the ideal secret."""
     import pzlib
-    ctx = pzlib.Context("NI_2P")
-    p, N, total = ctx.p, ctx.N, 2
+    ctx = pzlib.Context(params)
+    p, N = ctx.p, ctx.N
     n = p.lwe_n
     t0 = time.perf_counter()
     ps, uit, rg = ni_shares(ctx, total, 0, n)
@@ -355,9 +355,8 @@ def ni_aggregation():
     key.free()
     ctx.close()
     produced = n * (p.rlrg_da + p.rlrg_db + (total - 1) * (p.rgrg_da + p.rgrg_db))
-    return {"params": "NI_2P", "parties": total, "lwe_indices": n,
+    return {"params": params, "parties": total, "lwe_indices": n,
             "aggregate_ms": agg_s * 1e3, "produced_rlwes": produced,
-            "uitos_digit_transforms": produced * p.uitos_d,
             "share_gen_ms_per_party": share_s * 1e3, "gates_correct": ok}
 
 
@@ -446,7 +445,8 @@ def main():
     ap.add_argument("--cpu-per-thread", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline, plain and roofline only")
-    ap.add_argument("--leg", choices=["kernel_batch", "input_path", "ni_aggregation"],
+    ap.add_argument("--leg", choices=["kernel_batch", "input_path", "ni_aggregation",
+                                      "ni_aggregation_ni8p"],
                     help="run one secondary measurement alone (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB read + 256 MiB write elementwise kernel twice "
@@ -457,7 +457,8 @@ def main():
         return
     if args.leg:
         legs = {"kernel_batch": kernel_batch, "input_path": input_path,
-                "ni_aggregation": ni_aggregation}
+                "ni_aggregation": ni_aggregation,
+                "ni_aggregation_ni8p": lambda: ni_aggregation("NI_8P", 8)}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
     if args.gpus != 1:
@@ -530,6 +531,8 @@ def main():
         log("input path: %s" % extras["input_path"])
         extras["ni_aggregation"] = ni_aggregation()
         log("ni aggregation: %s" % extras["ni_aggregation"])
+        extras["ni_aggregation_ni8p"] = ni_aggregation("NI_8P", 8)
+        log("ni aggregation NI_8P: %s" % extras["ni_aggregation_ni8p"])
         extras["fheuint8_ni8p"] = fheuint8_circuits(args)
         log("fheuint8: %s" % extras["fheuint8_ni8p"])
         extras["serving_ni2p"] = serving(args, plain)

@@ -26,6 +26,10 @@
 #include "pz_host.hpp"
 #include "pz_internal.hpp"
 
+#ifndef PZ_NI_GROUP
+#define PZ_NI_GROUP 3
+#endif
+
 namespace pz {
 
 struct NiArgs {
@@ -38,7 +42,19 @@ struct NiArgs {
     u32 P, count, l0, per, da, db, dmax, da0, db0, d;
 };
 
-template <int LOGN>
+// Consumers of RGSW x RGSW row r of an index: the non-leaders (each produces
+// every row), then the leader when r is one of its rows (its counts da0 <= da,
+// db0 <= db take the last rows of each part).
+PZ_DEV u32 ni_consumers(const NiArgs &A, u32 r)
+{
+    const bool lead_uses = r < A.da ? r >= A.da - A.da0 : r - A.da >= A.db - A.db0;
+    return A.P - 1 + (lead_uses ? 1 : 0);
+}
+
+// One work group per (index, input polynomial, group of up to G consumers):
+// the 50 digit rows of the polynomial are transformed once for the whole
+// group and multiply-accumulated with each member's Ksk.
+template <int LOGN, int G>
 __global__ __launch_bounds__(1 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 k_ni_rows(NiArgs A, DevCtx c)
 {
@@ -47,88 +63,105 @@ k_ni_rows(NiArgs A, DevCtx c)
     const u32 j = threadIdx.x;
     const u64 q = c.tb.q, q2 = c.tb.q2;
     const u32 n0 = A.da0 + A.db0, n1 = A.da + A.db;
-    const u32 per_l = n0 + (A.P - 1) * n1;
-    for (size_t w = blockIdx.x; w < (size_t)A.count * per_l; w += gridDim.x) {
-        const u32 t = (u32)(w / per_l), k = (u32)(w % per_l);
-        const u32 lead = (A.l0 + t) / A.per;
-        const bool leader = k < n0;
-        u32 party, kk, slot = 0, dao, dbo;
-        if (leader) {
-            party = lead;
-            kk = k;
-            dao = A.da0;
-            dbo = A.db0;
-        } else {
-            slot = (k - n0) / n1;
-            kk = (k - n0) % n1;
-            party = slot < lead ? slot : slot + 1;
-            dao = A.da;
-            dbo = A.db;
+    u32 per_t = 0;
+    for (u32 r = 0; r < n1; r++) per_t += (ni_consumers(A, r) + G - 1) / G;
+    for (size_t w = blockIdx.x; w < (size_t)A.count * per_t; w += gridDim.x) {
+        const u32 t = (u32)(w / per_t);
+        u32 rem = (u32)(w % per_t), r = 0, cons = 0;
+        for (;; r++) {
+            cons = ni_consumers(A, r);
+            const u32 ng = (cons + G - 1) / G;
+            if (rem < ng) break;
+            rem -= ng;
         }
-        const bool nsm = kk < dao;
-        const u32 i = nsm ? A.dmax - dao + kk : A.dmax - dbo + (kk - dao);
-        const u32 ra = nsm ? kk : 2 * dao + (kk - dao);
-        const u32 rb = nsm ? dao + kk : 2 * dao + dbo + (kk - dao);
+        const u32 first = rem * G;
+        const u32 nmem = cons - first < (u32)G ? cons - first : (u32)G;
+        const u32 lead = (A.l0 + t) / A.per;
+        const bool nsm = r < A.da;
+        const u32 i = nsm ? A.dmax - A.da + r : A.dmax - A.db + (r - A.da);
         u64 val[8];
         if (nsm) {
             // sum over parties of a_{i,l} * s_p + e: a_{i,l} * s + e
-            const u32 zr = i - (A.dmax - A.da);
 #pragma unroll
-            for (int r = 0; r < 8; r++) val[r] = 0;
+            for (int x = 0; x < 8; x++) val[x] = 0;
             for (u32 p = 0; p < A.P; p++) {
-                const u64 *z = A.zero + (((size_t)p * A.count + t) * A.da + zr) * N;
+                const u64 *z = A.zero + (((size_t)p * A.count + t) * A.da + r) * N;
 #pragma unroll
-                for (int r = 0; r < 8; r++) val[r] = add_mod(val[r], z[coef_x<LOGN>(j, r)], q);
+                for (int x = 0; x < 8; x++) val[x] = add_mod(val[x], z[coef_x<LOGN>(j, x)], q);
             }
         } else {
-            const u64 *a = A.arows + ((size_t)t * A.db + (i - (A.dmax - A.db))) * N;
+            const u64 *a = A.arows + ((size_t)t * A.db + (r - A.da)) * N;
 #pragma unroll
-            for (int r = 0; r < 8; r++) val[r] = neg_mod(a[coef_x<LOGN>(j, r)], q);
+            for (int x = 0; x < 8; x++) val[x] = neg_mod(a[coef_x<LOGN>(j, x)], q);
         }
 #pragma unroll
-        for (int r = 0; r < 8; r++) val[r] = decomp_prepare(c.dec_uitos, val[r]);
-        u64 acc[2][8];
+        for (int x = 0; x < 8; x++) val[x] = decomp_prepare(c.dec_uitos, val[x]);
+        u32 party[G];
 #pragma unroll
-        for (int r = 0; r < 8; r++) acc[0][r] = acc[1][r] = 0;
-        const u64 *K = A.K[party], *Ks = A.Ks[party];
+        for (int m = 0; m < G; m++) {
+            const u32 ci = first + m;  // consumer index; P - 1 is the leader
+            party[m] = ci >= A.P - 1 ? lead : (ci < lead ? ci : ci + 1);
+        }
+        u64 acc[G][2][8];
+#pragma unroll
+        for (int m = 0; m < G; m++)
+#pragma unroll
+            for (int x = 0; x < 8; x++) acc[m][0][x] = acc[m][1][x] = 0;
         for (u32 row = 0; row < A.d; row++) {
             __builtin_amdgcn_sched_barrier(0);  // no key loads hoisted across rows
             u64 D[1][8];
 #pragma unroll
-            for (int r = 0; r < 8; r++) D[0][r] = decomp_next(c.dec_uitos, val[r]);
+            for (int x = 0; x < 8; x++) D[0][x] = decomp_next(c.dec_uitos, val[x]);
             ntt_fwd<LOGN, 1, PZ_FWD_MODE, false>(D, j, lds, c.tb);
-            fma_row<false>(acc[0], K + (size_t)row * N, Ks + (size_t)row * N, D[0], j, q, q2);
-            fma_row<false>(acc[1], K + (size_t)(A.d + row) * N, Ks + (size_t)(A.d + row) * N,
-                          D[0], j, q, q2);
-        }
-        const u64 *ct = A.cts + (((size_t)party * A.count + t) * A.dmax + i) * N;
-        const int part = nsm ? 0 : 1;  // the NI row joins a for RLWE'(-sm), b for RLWE'(m)
-        if (leader) {
-            ntt_inv<LOGN, 2, true, true, false>(acc, j, lds, c.tb);
-            u64 *o = A.out0 + (size_t)t * 2 * n0 * N;
 #pragma unroll
-            for (int r = 0; r < 8; r++) {
-                const u64 v = ct[coef_x<LOGN>(j, r)];
-                acc[part][r] = add_mod(acc[part][r], v, q);
-                o[(size_t)ra * N + coef_x<LOGN>(j, r)] = acc[0][r];
-                o[(size_t)rb * N + coef_x<LOGN>(j, r)] = acc[1][r];
+            for (int m = 0; m < G; m++) {
+                if ((u32)m >= nmem) break;
+                const u64 *K = A.K[party[m]], *Ks = A.Ks[party[m]];
+                fma_row<false>(acc[m][0], K + (size_t)row * N, Ks + (size_t)row * N, D[0], j, q,
+                               q2);
+                fma_row<false>(acc[m][1], K + (size_t)(A.d + row) * N,
+                               Ks + (size_t)(A.d + row) * N, D[0], j, q, q2);
             }
-        } else {
-            u64 cv[1][8];
+        }
+        const int part = nsm ? 0 : 1;  // the NI row joins a for RLWE'(-sm), b for RLWE'(m)
 #pragma unroll
-            for (int r = 0; r < 8; r++) cv[0][r] = ct[coef_x<LOGN>(j, r)];
-            __syncthreads();  // the LDS of the last digit transform is free again
-            ntt_fwd<LOGN, 1, FWD_STRICT, false>(cv, j, lds, c.tb);
-            u64 *o = A.out1 + (((size_t)slot * A.count + t) * 2 * n1) * N;
+        for (int m = 0; m < G; m++) {
+            if ((u32)m >= nmem) break;
+            const u32 ci = first + m;
+            const u64 *ct = A.cts + (((size_t)party[m] * A.count + t) * A.dmax + i) * N;
+            if (ci == A.P - 1) {
+                // the leader: RLWE x RGSW counts, coefficient domain
+                const u32 k0 = nsm ? r - (A.da - A.da0) : (r - A.da) - (A.db - A.db0);
+                const u32 ra = nsm ? k0 : 2 * A.da0 + k0;
+                const u32 rb = nsm ? A.da0 + k0 : 2 * A.da0 + A.db0 + k0;
+                ntt_inv<LOGN, 2, true, true, false>(acc[m], j, lds, c.tb);
+                u64 *o = A.out0 + (size_t)t * 2 * n0 * N;
 #pragma unroll
-            for (int r = 0; r < 8; r++) {
-                u64 x = umin(acc[0][r], acc[0][r] - q), y = umin(acc[1][r], acc[1][r] - q);
-                if (part == 0) x = add_mod(x, cv[0][r], q);
-                else y = add_mod(y, cv[0][r], q);
-                x = shoup_mul(x, c.tb.n_inv, c.tb.n_inv_s, q);
-                y = shoup_mul(y, c.tb.n_inv, c.tb.n_inv_s, q);
-                o[(size_t)ra * N + eval_x<LOGN>(j, r)] = umin(x, x - q);
-                o[(size_t)rb * N + eval_x<LOGN>(j, r)] = umin(y, y - q);
+                for (int x = 0; x < 8; x++) {
+                    acc[m][part][x] = add_mod(acc[m][part][x], ct[coef_x<LOGN>(j, x)], q);
+                    o[(size_t)ra * N + coef_x<LOGN>(j, x)] = acc[m][0][x];
+                    o[(size_t)rb * N + coef_x<LOGN>(j, x)] = acc[m][1][x];
+                }
+            } else {
+                // non-leader slot ci: RGSW x RGSW counts, evaluation domain * n^-1
+                const u32 ra = nsm ? r : 2 * A.da + (r - A.da);
+                const u32 rb = nsm ? A.da + r : 2 * A.da + A.db + (r - A.da);
+                u64 cv[1][8];
+#pragma unroll
+                for (int x = 0; x < 8; x++) cv[0][x] = ct[coef_x<LOGN>(j, x)];
+                ntt_fwd<LOGN, 1, FWD_STRICT, false>(cv, j, lds, c.tb);
+                u64 *o = A.out1 + (((size_t)ci * A.count + t) * 2 * n1) * N;
+#pragma unroll
+                for (int x = 0; x < 8; x++) {
+                    u64 y0 = umin(acc[m][0][x], acc[m][0][x] - q);
+                    u64 y1 = umin(acc[m][1][x], acc[m][1][x] - q);
+                    if (part == 0) y0 = add_mod(y0, cv[0][x], q);
+                    else y1 = add_mod(y1, cv[0][x], q);
+                    y0 = shoup_mul(y0, c.tb.n_inv, c.tb.n_inv_s, q);
+                    y1 = shoup_mul(y1, c.tb.n_inv, c.tb.n_inv_s, q);
+                    o[(size_t)ra * N + eval_x<LOGN>(j, x)] = umin(y0, y0 - q);
+                    o[(size_t)rb * N + eval_x<LOGN>(j, x)] = umin(y1, y1 - q);
+                }
             }
         }
     }
@@ -238,6 +271,9 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     if (p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0) return PZ_ERR_UNSUPPORTED;
+    // the leader's rows are the last of each part (the reference requires
+    // RLWE x RGSW counts below the RGSW x RGSW ones, src/bool/evaluator.rs:1707-1716)
+    if (p.rlrg_da > p.rgrg_da || p.rlrg_db > p.rgrg_db) return PZ_ERR_UNSUPPORTED;
     if (total == 0 || total > PZ_NI_MAX_PARTIES) return PZ_ERR_ARG;
     if ((uint64_t)l0 + count > p.lwe_n) return PZ_ERR_ARG;
     for (u32 o = 0; o < total; o++)
@@ -277,13 +313,20 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
         A.da0 = p.rlrg_da;
         A.db0 = p.rlrg_db;
         A.d = p.uitos_d;
-        const size_t work = (size_t)count * (p.rlrg_da + p.rlrg_db + (total - 1) * (da + db));
+        // consumers of one input polynomial share its digit transforms, up to
+        // PZ_NI_GROUP per work group
+        u32 per_t = 0;
+        for (u32 r = 0; r < da + db; r++) {
+            const bool lu = r < da ? r >= da - p.rlrg_da : r - da >= db - p.rlrg_db;
+            per_t += (total - 1 + (lu ? 1 : 0) + PZ_NI_GROUP - 1) / PZ_NI_GROUP;
+        }
+        const size_t work = (size_t)count * per_t;
         const int grid = (int)(work < 65535 ? work : 65535);
         const size_t lds = 8 * (size_t)(2 * ((1 << logn) + (1 << logn) / 8));
         if (logn == 11)
-            hipLaunchKernelGGL(k_ni_rows<11>, grid, 1 << 8, lds, st, A, dc);
+            hipLaunchKernelGGL((k_ni_rows<11, PZ_NI_GROUP>), grid, 1 << 8, lds, st, A, dc);
         else
-            hipLaunchKernelGGL(k_ni_rows<9>, grid, 1 << 6, lds, st, A, dc);
+            hipLaunchKernelGGL((k_ni_rows<9, PZ_NI_GROUP>), grid, 1 << 6, lds, st, A, dc);
         if (w1) {
             launch_shoup(o1, o1s, w1, p.rlwe_q, st);
             const DevCtx cc = rgrg_devctx(p, dc);

@@ -2,7 +2,8 @@
 word for word, and end to end.
 
 - Party shares (u -> s parts and RGSW parts) equal the oracle's for the same
-  seeds and secrets (NI_2P with 2 parties, NI_4P_HB_FR with 3).
+  seeds and secrets (NI_2P with 2 parties, NI_4P_HB_FR with 3, NI_4P_LB_SR
+  with 4, NI_8P with 8).
 - Ksk(u_j -> s) from the shares equals the oracle's.
 - The aggregated RGSWs of indices straddling a leader boundary equal the
   oracle's (every produced RLWE and the product chain).
@@ -49,7 +50,8 @@ def gpu_ksks(ctx, total, shares):
     return ksks
 
 
-@pytest.mark.parametrize("name,total", [("NI_2P", 2), ("NI_4P_HB_FR", 3)])
+@pytest.mark.parametrize("name,total", [("NI_2P", 2), ("NI_4P_HB_FR", 3), ("NI_4P_LB_SR", 4),
+                                        ("NI_8P", 8)])
 def test_ni_shares_ksk_and_aggregate_match_oracle(name, total):
     import pzlib
     o = oracle_for(name)
