@@ -473,6 +473,10 @@ def main():
     N, n, B = p.rlwe_n, p.lwe_n, args.batch
     s, z, key = keys(ctx, 2, seed=0)
     log("keys ready")
+    # launch shapes chosen by the library's autotuner for each batch size used
+    tuned = {str(b): ctx.autotune(key, b) for b in
+             [1, B] + [int(x) for x in args.sweep.split(",") if x and not args.no_extras]}
+    log("autotune: %s" % tuned)
     value, secs, ok, bufs = batched(ctx, key, s, B, args.steps, args.warmup, seed=3)
     log("batched B=%d: %.0f bootstraps/s" % (B, value))
 
@@ -561,6 +565,7 @@ def main():
         "config": {"workload": "batched NAND PBS, %s (N=%d, n=%d), B=%d per launch" %
                                (args.params, N, n, B), "batch": B, "params": args.params},
         "outputs_correct": bool(ok),
+        "autotune": tuned,
         "plain_bootstraps_per_sec": plain,
         "batched_over_plain": value / plain,
         "kernel_ms": tim,

@@ -334,6 +334,14 @@ int pz_ni_aggregate_rgsw(pz_ctx *ctx, const uint8_t *crs, uint32_t total,
                          const uint64_t *d_zero, const uint64_t *d_cts, uint64_t *d_out,
                          void *stream);
 
+/* Autotuner: times every blind-rotation launch shape that fits the device
+ * (kernel variant x resident work groups per CU) on B synthetic gates with
+ * this key, and uses the fastest for batches in B's power-of-two bucket
+ * (automatic mode only).  h_choice (may be null) receives the variant, the
+ * work groups per CU and the measured microseconds per batch.  No
+ * counterpart in the reference (its evaluator runs one gate at a time). */
+int pz_autotune(pz_ctx *ctx, pz_key *key, size_t B, uint32_t *h_choice);
+
 /* Blind-rotation kernel choice.  No counterpart in the reference (its
  * evaluator is single-threaded): 0 = automatic, 1 = one work group per
  * bootstrap, 2 = two half work groups per bootstrap (only where

@@ -61,6 +61,11 @@ struct pz_ctx {
     int br_per_cu[3] = {0, 0, 0};  // resident work groups per CU of each kernel variant
                                    // (BR_ONE, BR_SPLIT_LAT, BR_SPLIT; 0: not available)
     int br_mode = 0;          // 0 auto, 1 one-half kernel, 2 split kernel
+    // autotuned blind-rotation shapes per batch bucket (ceil(log2 B)): kernel
+    // variant and resident work groups per CU; var < 0 = not tuned
+    struct Tuned {
+        int var = -1, per_cu = 0;
+    } tuned[40];
     // workspace of the key-preparation operators (grow-only; every user
     // synchronises the stream before it returns)
     void *ws = nullptr;
@@ -679,8 +684,22 @@ extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint6
 // latency variant while every bootstrap has a CU of its own, its throughput
 // variant (two bootstraps per CU) beyond that.  Otherwise, or in mode 1, one
 // work group per bootstrap.
+static int batch_bucket(size_t B)
+{
+    int b = 0;
+    while (((size_t)1 << b) < B && b < 39) b++;
+    return b;
+}
+
 static void br_plan(const pz_ctx *c, size_t B, int *var, int *grid)
 {
+    const pz_ctx::Tuned &t = c->tuned[batch_bucket(B)];
+    if (c->br_mode == 0 && t.var >= 0) {
+        const size_t res = (size_t)c->num_cu * t.per_cu;
+        *var = t.var;
+        *grid = (int)(B < res ? (B > 0 ? B : 1) : res);
+        return;
+    }
     int v = BR_ONE;
     if (c->br_per_cu[BR_SPLIT] > 0 && c->br_mode != 1)
         v = B <= (size_t)c->num_cu * c->br_per_cu[BR_SPLIT_LAT] ? BR_SPLIT_LAT : BR_SPLIT;
@@ -709,9 +728,11 @@ extern "C" int pz_set_blind_rotate_mode(pz_ctx *c, int mode)
     return PZ_OK;
 }
 
-// moddown/KS/blind-rotate sequence on stream st
+// moddown/KS/blind-rotate sequence on stream st; var >= 0 forces the
+// blind-rotation variant and grid (the autotuner's candidates)
 static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, const uint64_t *c1,
-                         const uint32_t *ops, uint64_t *out, size_t B, hipStream_t st)
+                         const uint32_t *ops, uint64_t *out, size_t B, hipStream_t st,
+                         int force_var = -1, int force_grid = 0)
 {
     const pz_params &p = c->p;
     u32 n1 = p.lwe_n + 1;
@@ -723,7 +744,12 @@ static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, con
               k->dk.ldk, 0, B, c->d_ks, st);
     if (c->timing) hipEventRecord(c->ev[2], st);
     int var = BR_ONE, grid = 1;
-    br_plan(c, B, &var, &grid);
+    if (force_var >= 0) {
+        var = force_var;
+        grid = force_grid;
+    } else {
+        br_plan(c, B, &var, &grid);
+    }
     launch_blind_rotate(c->logn, var, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk,
                         nullptr, nullptr, nullptr, st);
     if (c->timing) hipEventRecord(c->ev[3], st);
@@ -737,6 +763,87 @@ static int collect_timing(pz_ctx *c, hipStream_t st)
     hipEventElapsedTime(&c->t_ks, c->ev[1], c->ev[2]);
     hipEventElapsedTime(&c->t_br, c->ev[2], c->ev[3]);
     (void)st;
+    return PZ_OK;
+}
+
+// Autotuner: times every blind-rotation shape that fits the device (kernel
+// variant x resident work groups per CU) on a batch of synthetic gates with
+// this key and keeps the fastest for B's bucket.  Batches larger than four
+// rounds of the largest resident grid are sampled at that size (every shape
+// is persistent there).  Inputs are uniform words mod Q (any word is a valid
+// PBS input); outputs are discarded.
+extern "C" int pz_autotune(pz_ctx *c, pz_key *k, size_t B_req, uint32_t *h_choice)
+{
+    if (!c || !k || B_req == 0) return fail(PZ_ERR_ARG, "null argument or empty batch");
+    KEY_OK(c, k);
+    HIPCHK(hipSetDevice(c->device));
+    int max_per = 1;
+    for (int v = 0; v < 3; v++) max_per = std::max(max_per, c->br_per_cu[v]);
+    const size_t sample = 4 * (size_t)c->num_cu * max_per;
+    const size_t B = B_req < sample ? B_req : sample;
+    int rc = ensure_scratch(c, B);
+    if (rc) return rc;
+    const size_t N1 = (size_t)c->p.rlwe_n + 1;
+    std::vector<u64> h_in(2 * B * N1);
+    std::vector<u32> h_ops(B);
+    u64 x = 0x9e3779b97f4a7c15ull;
+    for (auto &v : h_in) {  // splitmix64 words reduced mod Q
+        x += 0x9e3779b97f4a7c15ull;
+        u64 z = x;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        v = (z ^ (z >> 31)) % c->p.rlwe_q;
+    }
+    for (size_t i = 0; i < B; i++) h_ops[i] = (i & 1) ? 4u : 0u;  // NAND / XOR
+    u64 *d_in = nullptr, *d_out = nullptr;
+    u32 *d_ops = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = hipMalloc(&d_in, 8 * h_in.size()) == hipSuccess &&
+              hipMalloc(&d_out, 8 * B * N1) == hipSuccess &&
+              hipMalloc(&d_ops, 4 * B) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess &&
+              hipMemcpy(d_in, h_in.data(), 8 * h_in.size(), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_ops, h_ops.data(), 4 * B, hipMemcpyHostToDevice) == hipSuccess;
+    int best_var = -1, best_per = 0;
+    float best_ms = 0;
+    for (int v = 0; ok && v < 3; v++) {
+        if (c->br_per_cu[v] <= 0) continue;
+        for (int per = 1; ok && per <= c->br_per_cu[v]; per++) {
+            const size_t res = (size_t)c->num_cu * per;
+            const int grid = (int)(B < res ? B : res);
+            float ms = 0;
+            for (int rep = 0; rep < 3 && ok; rep++) {  // one warm-up, two timed
+                ok = hipEventRecord(e0, c->stream) == hipSuccess;
+                pbs_sequence(c, k, PREP_GATE, d_in, d_in + B * N1, d_ops, d_out, B, c->stream, v,
+                             grid);
+                ok = ok && hipGetLastError() == hipSuccess &&
+                     hipEventRecord(e1, c->stream) == hipSuccess &&
+                     hipEventSynchronize(e1) == hipSuccess;
+                float t = 0;
+                if (ok && rep > 0 && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms += t;
+            }
+            if (ok && (best_var < 0 || ms < best_ms)) {
+                best_var = v;
+                best_per = per;
+                best_ms = ms;
+            }
+        }
+    }
+    hipFree(d_in);
+    hipFree(d_out);
+    hipFree(d_ops);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (!ok || best_var < 0) return fail(PZ_ERR_HIP, "autotune run failed");
+    pz_ctx::Tuned &t = c->tuned[batch_bucket(B_req)];
+    t.var = best_var;
+    t.per_cu = best_per;
+    drop_graphs(c);  // captured graphs hold the previous choice
+    if (h_choice) {
+        h_choice[0] = (uint32_t)best_var;
+        h_choice[1] = (uint32_t)best_per;
+        h_choice[2] = (uint32_t)(best_ms / 2 * 1000);  // microseconds per batch
+    }
     return PZ_OK;
 }
 

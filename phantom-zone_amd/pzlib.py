@@ -105,6 +105,7 @@ SIGNATURES = {
     "pz_rlwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_sample_extract": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp]),
     "pz_rgsw_by_rgsw": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_autotune": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     "pz_ni_share_uitos": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp, _vp, _vp, _vp]),
     "pz_ni_share_rgsw": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp, _vp, _vp,
                                    C.c_uint32, C.c_uint32, _vp, _vp]),
@@ -593,6 +594,14 @@ class Context:
         crs_b = self.seed32(crs)
         _check(self.L.pz_ni_aggregate_rgsw(self.h, crs_b.ctypes.data, total, arr, l0,
                                            count, d_zero.ptr, d_cts.ptr, d_out.ptr, None))
+
+    def autotune(self, key, B):
+        """Pick the fastest blind-rotation launch shape for batches like B."""
+        out = np.zeros(3, dtype=np.uint32)
+        _check(self.L.pz_autotune(self.h, key.h, B, out.ctypes.data))
+        names = {0: "one_half", 1: "split_latency", 2: "split"}
+        return {"variant": names[int(out[0])], "groups_per_cu": int(out[1]),
+                "ms_per_batch": int(out[2]) / 1e3}
 
     def rgsw_by_rgsw(self, d_rgsw0, d_rgsw1, B):
         _check(self.L.pz_rgsw_by_rgsw(self.h, d_rgsw0.ptr, d_rgsw1.ptr, B, None))

@@ -104,3 +104,27 @@ def test_gates_full_batch(ctx, B, mode):
         assert np.array_equal(dout.download(), first)
     ctx.set_blind_rotate_mode(0)
     k.free()
+
+
+@pytest.mark.parametrize("B", [1, 300, 1024])
+def test_autotune_keeps_outputs(ctx, B):
+    """The autotuner changes only the launch shape: outputs before and after
+    it are identical, and its choice is a variant the device can run."""
+    o, s, z, key = keys_for("NI_2P")
+    k = ctx.key_upload(key.rgsw, key.auto_key, key.lwe_ksk)
+    log("autotune B=%d" % B)
+    r = rng(104 + B)
+    b0 = r.integers(0, 2, size=B).astype(np.uint8)
+    b1 = r.integers(0, 2, size=B).astype(np.uint8)
+    ops = np.where(r.integers(0, 2, size=B) == 0, 0, 4).astype(np.uint32)
+    d0 = ctx.to_device(ctx.encrypt_bools(s, b0, 11000 + B))
+    d1 = ctx.to_device(ctx.encrypt_bools(s, b1, 12000 + B))
+    dops, dout = ctx.to_device(ops), ctx.array((B, o.N + 1))
+    ctx.gates(k, dops, d0, d1, dout, B)
+    before = dout.download()
+    choice = ctx.autotune(k, B)
+    assert choice["variant"] in ("one_half", "split_latency", "split")
+    assert choice["groups_per_cu"] >= 1 and choice["ms_per_batch"] > 0
+    ctx.gates(k, dops, d0, d1, dout, B)
+    assert np.array_equal(dout.download(), before)
+    k.free()
