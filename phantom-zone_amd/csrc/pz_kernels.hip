@@ -258,21 +258,18 @@ __global__ void k_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q)
     }
 }
 
-__global__ void k_shoup(const u64 *in, u64 *out, size_t n, u64 q)
+// Shoup twins floor(w*2^64/q) (ToShoup, src/utils.rs:49-73) for w < q.  With
+// 2^64 = Q0*q + R0 the twin is w*Q0 + floor(w*R0/q); the last quotient is the
+// Shoup estimate hi64(w*R0s), R0s = floor(R0*2^64/q), whose remainder
+// w*R0 - q*hi64(w*R0s) lies in [0, 2q): one correction step makes it exact.
+__global__ void k_shoup(const u64 *in, u64 *out, size_t n, u64 q, u64 Q0, u64 R0, u64 R0s)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        u64 rem = in[i], quo = 0;
-        for (int bit = 0; bit < 64; bit++) {
-            // rem < q < 2^63: shifting left cannot overflow
-            rem <<= 1;
-            quo <<= 1;
-            if (rem >= q) {
-                rem -= q;
-                quo |= 1;
-            }
-        }
-        out[i] = quo;
+        const u64 w = in[i];
+        const u64 qt = __umul64hi(w, R0s);
+        const u64 r = w * R0 - q * qt;
+        out[i] = w * Q0 + qt + (r >= q ? 1 : 0);
     }
 }
 
@@ -474,7 +471,11 @@ void launch_scale(u64 *x, size_t n, u64 c, u64 c_s, u64 q, hipStream_t s)
 
 void launch_shoup(const u64 *in, u64 *out, size_t n, u64 q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shoup, grid_for(n, 256), 256, 0, s, in, out, n, q);
+    typedef unsigned __int128 u128;
+    const u128 two64 = (u128)1 << 64;
+    const u64 Q0 = (u64)(two64 / q), R0 = (u64)(two64 % q);
+    const u64 R0s = (u64)(((u128)R0 << 64) / q);
+    hipLaunchKernelGGL(k_shoup, grid_for(n, 256), 256, 0, s, in, out, n, q, Q0, R0, R0s);
 }
 
 void launch_not(u64 *c0, size_t n, u64 q, hipStream_t s)
