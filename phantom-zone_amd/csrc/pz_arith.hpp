@@ -95,12 +95,15 @@ PZ_DEV u64 neg_mod(u64 a, u64 q) { return q - a; }
 
 // One Shoup multiply-accumulate step of shoup_matrix_fma
 // (src/backend/modulus_u64.rs:252-260): out in [0,2q) -> [0,2q).
-template <bool SPLIT = true>
+// REDUCE = false leaves out + [0,2q) unreduced (fused operators whose
+// consumer accepts the wider range).
+template <bool SPLIT = true, bool REDUCE = true>
 PZ_DEV u64 shoup_fma(u64 out, u64 a, u64 as, u64 b, u64 q, u64 q2)
 {
     u64 v;
     if constexpr (SPLIT) v = shoup_rem(out, b, a, shoup_quot(b, as), q);  // a*b + out - q*quot
     else v = a * b + out - q * mulhi64(as, b);
+    if constexpr (!REDUCE) return v;
     return v >= q2 ? v - q2 : v;
 }
 

@@ -26,6 +26,11 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #ifndef PZ_INV_FAST
 #define PZ_INV_FAST 1
 #endif
+// Products of the split blind rotation left unreduced (accumulators below
+// 4q; the fast inverse then takes inputs below 4q).
+#ifndef PZ_LAZY_MAC
+#define PZ_LAZY_MAC 1
+#endif
 #ifndef PZ_SPLIT_SM
 #define PZ_SPLIT_SM true
 #endif
@@ -116,7 +121,7 @@ PZ_DEV void fma_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restr
 
 // fma_row in two halves of four words, a scheduling fence between them:
 // sixteen fewer registers hold key words at any time.
-template <bool SPLIT>
+template <bool SPLIT, bool REDUCE = true>
 PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
                            const u64 (&D)[8], u32 j, u64 q, u64 q2)
 {
@@ -129,7 +134,8 @@ PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *
 #pragma unroll
         for (int r = 0; r < 4; r++)
             acc[4 * half + r] =
-                shoup_fma<SPLIT>(acc[4 * half + r], kv[r], ks[r], D[4 * half + r], q, q2);
+                shoup_fma<SPLIT, REDUCE>(acc[4 * half + r], kv[r], ks[r], D[4 * half + r], q,
+                                         q2);
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -352,7 +358,9 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     // A (0) with its digits, half 1 row B (1) with half 0's digits
     const size_t r1 = (size_t)(is_auto ? 0 : (h ? 3 : 0)) * N;
     const size_t r2 = (size_t)(is_auto ? 1 : (h ? 1 : 2)) * N;
-    if (work) fma_row_halves<PZ_SPLIT_SM>(acc[0], K + r1, Ks + r1, D[0], j, q, q2);
+    // unreduced products: at most two of them, so the accumulator stays below
+    // 4q, the fast inverse's input bound
+    if (work) fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC>(acc[0], K + r1, Ks + r1, D[0], j, q, q2);
     __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
     const bool second = !is_auto || h == 1;
     if (second) {
@@ -360,8 +368,9 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
         for (int r = 0; r < 8; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
     }
     __syncthreads();  // both halves have read: the regions may be rewritten
-    if (second) fma_row_halves<PZ_SPLIT_SM>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
-    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST>(acc, j, mine, c.tb);  // n^-1: key rows
+    if (second) fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
+    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, PZ_LAZY_MAC ? 4 : 2>(acc, j, mine,
+                                                                            c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
 #pragma unroll
         for (int r = 0; r < 8; r++) {

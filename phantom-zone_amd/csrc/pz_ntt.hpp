@@ -111,14 +111,15 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 // SCALE = false is for products whose key rows already carry the n^-1
 // factor: the result is the same residue, and with STRICT the same word.
 //
-// FAST (fused operators, inputs in [0,2q), no n^-1): executed stage k = LOGN - S
-// receives values below 2^k q.  Stages k <= 8 skip the reduction of the sum,
-// x' = x + y < 2^(k+1) q, and take y' = Shoup(x + 2^k q - y) (argument below
-// 2^(k+1) q <= 2^9 q); stage 9 reduces x' = x + y < 2^10 q to [0,2q) with a
-// Shoup multiply by 1 (2^10 q < 2^64 for q < 2^54); later stages are the
-// reference butterfly again.  Residues are the reference's throughout, and
-// the strict last stage returns the same words.
-template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE, bool SM, bool FAST>
+// FAST (fused operators, inputs below IB*q with IB = 2 or 4, no n^-1):
+// executed stage k = LOGN - S receives values below 2^(k-1) IB q.  Early
+// stages skip the reduction of the sum, x' = x + y, and take
+// y' = Shoup(x + 2^(k-1) IB q - y); stage KR (9 for IB = 2, 8 for IB = 4)
+// reduces x' = x + y < 2^KR IB q = 2^10 q to [0,2q) with a Shoup multiply by
+// 1 (2^10 q < 2^64 for q < 2^54); later stages are the reference butterfly
+// again.  Residues are the reference's throughout, and the strict last stage
+// returns the same words.
+template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB>
 PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 {
     constexpr int N = 1 << LOGN;
@@ -126,7 +127,9 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
     constexpr int LD = T / ST;
     constexpr int M = 1 << S;
     constexpr int K = LOGN - S;  // executed stage, 1-based
+    constexpr int KR = IB == 4 ? 8 : 9;
     static_assert(!FAST || !SCALE, "the fast inverse leaves n^-1 to the key rows");
+    static_assert(IB == 2 || (IB == 4 && FAST), "inputs below 4q need the fast inverse");
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         if (r & LD) continue;
@@ -136,11 +139,12 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
         const u64 w = tw.x, ws = tw.y;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
-            if constexpr (FAST && K <= 9) {
+            if constexpr (FAST && K <= KR) {
                 const u64 xv = v[p][r], yv = v[p][r + LD];
                 const u64 sum = xv + yv;
-                v[p][r] = K == 9 ? shoup_mul<SM>(sum, 1, tb.one_s, tb.q) : sum;
-                v[p][r + LD] = shoup_mul<SM>(xv + (tb.q << K) - yv, w, ws, tb.q);
+                v[p][r] = K == KR ? shoup_mul<SM>(sum, 1, tb.one_s, tb.q) : sum;
+                v[p][r + LD] = shoup_mul<SM>(xv + (tb.q << (K - 1 + (IB == 4 ? 2 : 1))) - yv, w, ws,
+                                             tb.q);
             } else {
                 ib_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
             }
@@ -190,16 +194,16 @@ PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
 }
 
 // Inverse phase covering stages S0+NS-1 .. S0 (descending).
-template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST>
+template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB>
 PZ_DEV void inv_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
 {
     using P = Phase<LOGN, S0>;
     const u32 base = P::base(j);
     if constexpr (P::NS > 2)
-        inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM, FAST>(v, base, STAGE_TB(tb));
+        inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM, FAST, IB>(v, base, STAGE_TB(tb));
     if constexpr (P::NS > 1)
-        inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM, FAST>(v, base, STAGE_TB(tb));
-    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM, FAST>(v, base, STAGE_TB(tb));
+        inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM, FAST, IB>(v, base, STAGE_TB(tb));
+    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM, FAST, IB>(v, base, STAGE_TB(tb));
 }
 
 // Store the thread's coefficients (layout of phase SA) into LDS, barrier,
@@ -269,19 +273,20 @@ PZ_DEV void ntt_fwd(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
 // Inverse NTT of NP polynomials: contiguous layout in, stride layout out.
 // STRICT = Ntt::backward ([0,q)), else backward_lazy ([0,2q)); SCALE = false
 // leaves out the n^-1 factor; FAST = unreduced early stages (see inv_stage).
-template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool FAST = false>
+template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool FAST = false,
+          int IB = 2>
 PZ_DEV void ntt_inv(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(!FAST || (STRICT && LOGN <= 12), "fast inverse: strict output");
     if constexpr (LOGN > 9) {
-        inv_phase<LOGN, 9, NP, STRICT, SCALE, SM, FAST>(v, j, fresh(tb));
+        inv_phase<LOGN, 9, NP, STRICT, SCALE, SM, FAST, IB>(v, j, fresh(tb));
         exchange<LOGN, 9, 6, NP>(v, j, lds);
     }
-    inv_phase<LOGN, 6, NP, STRICT, SCALE, SM, FAST>(v, j, fresh(tb));
+    inv_phase<LOGN, 6, NP, STRICT, SCALE, SM, FAST, IB>(v, j, fresh(tb));
     exchange<LOGN, 6, 3, NP>(v, j, lds);
-    inv_phase<LOGN, 3, NP, STRICT, SCALE, SM, FAST>(v, j, fresh(tb));
+    inv_phase<LOGN, 3, NP, STRICT, SCALE, SM, FAST, IB>(v, j, fresh(tb));
     exchange<LOGN, 3, 0, NP>(v, j, lds);
-    inv_phase<LOGN, 0, NP, STRICT, SCALE, SM, FAST>(v, j, fresh(tb));
+    inv_phase<LOGN, 0, NP, STRICT, SCALE, SM, FAST, IB>(v, j, fresh(tb));
 }
 
 // Coefficient index of register r in the stride (coefficient-domain) layout
