@@ -181,6 +181,21 @@ PZ_DEV u64 decomp_next(const Decomp &dc, u64 &value)
     return (neg_carry & dc.q) + k - (carry << dc.logb);
 }
 
+// Digit 0 alone (what a d = 1 decomposition uses), for logb <= 31: only the
+// low 32 bits of the rounded value matter, and the carry into the next digit
+// is not formed.  Same word as decomp_digit(dc, v, 0).
+PZ_DEV u64 decomp_digit0(const Decomp &dc, u64 v)
+{
+    if (v >= (dc.q >> 1)) v -= dc.q;  // -(q - v) in two's complement; v == q gives 0
+    const u32 ib = dc.ignore_bits;
+    u32 value = (u32)v;
+    if (ib != 0) value = (u32)(v >> ib) + ((u32)(v >> (ib - 1)) & 1u);  // round
+    const u32 b = 1u << dc.logb, half = b >> 1;
+    const u32 k = value & (b - 1);
+    const bool carry = k > half || (k == half && ((value >> dc.logb) & 1u));
+    return carry ? dc.q - b + k : (u64)k;  // q + k - B for a negative digit
+}
+
 PZ_DEV u64 decomp_digit(const Decomp &dc, u64 v, u32 r)
 {
     u64 value = decomp_prepare(dc, v);

@@ -158,8 +158,8 @@ PZ_DEV void ext_product_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
         u64 D[2][8];
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            D[0][r] = decomp_digit(c.dec_a, a[r], 0);
-            D[1][r] = decomp_digit(c.dec_b, b[r], 0);
+            D[0][r] = decomp_digit0(c.dec_a, a[r]);
+            D[1][r] = decomp_digit0(c.dec_b, b[r]);
         }
         ntt_fwd<LOGN, 2, PZ_FWD_MODE, false>(D, j, lds, c.tb);
         fma_row<false>(acc[0], K + 0 * N, Ks + 0 * N, D[0], j, q, q2);
@@ -340,7 +340,7 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     const Decomp dc = is_auto ? c.dec_auto : (h ? c.dec_b : c.dec_a);
     u64 D[1][8];
 #pragma unroll
-    for (int r = 0; r < 8; r++) D[0][r] = decomp_digit(dc, p[r], 0);
+    for (int r = 0; r < 8; r++) D[0][r] = decomp_digit0(dc, p[r]);
     if (work) {
         ntt_fwd<LOGN, 1, PZ_FWD_MODE, PZ_SPLIT_SM>(D, j, mine, c.tb);
 #pragma unroll
