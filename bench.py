@@ -444,6 +444,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-per-thread", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch heuristic (profiling runs)")
     ap.add_argument("--no-extras", action="store_true", help="headline, plain and roofline only")
     ap.add_argument("--leg", choices=["kernel_batch", "input_path", "ni_aggregation",
                                       "ni_aggregation_ni8p"],
@@ -474,9 +476,12 @@ def main():
     s, z, key = keys(ctx, 2, seed=0)
     log("keys ready")
     # launch shapes chosen by the library's autotuner for each batch size used
-    tuned = {str(b): ctx.autotune(key, b) for b in
-             [1, B] + [int(x) for x in args.sweep.split(",") if x and not args.no_extras]}
-    log("autotune: %s" % tuned)
+    # (profiling runs skip it so that every blind-rotation dispatch is the batch)
+    tuned = None
+    if not args.no_autotune:
+        tuned = {str(b): ctx.autotune(key, b) for b in
+                 [1, B] + [int(x) for x in args.sweep.split(",") if x and not args.no_extras]}
+        log("autotune: %s" % tuned)
     value, secs, ok, bufs = batched(ctx, key, s, B, args.steps, args.warmup, seed=3)
     log("batched B=%d: %.0f bootstraps/s" % (B, value))
 
