@@ -309,15 +309,28 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     if (is_auto) {
         u64 *perm = h ? stash : mine;
         const bool move = !(trivial && h == 0);
-        if (move) {
+        // a wraps with a negation (and so does b when trivial); a non-trivial b
+        // only records the sign, applied when it is added back.  The choice is
+        // uniform, so each loop body is branch-free.
+        if (move && (h == 0 || trivial)) {
 #pragma unroll
             for (int r = 0; r < 8; r++) {
                 u32 x = coef_x<LOGN>(j, r);
                 u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
-                bool pos = to < (u32)N;
+                const bool pos = to < (u32)N;
                 to = pos ? to : to - N;
                 const u64 v = p[r];
-                perm[lds_ix(to)] = pos ? v : ((h == 0 || trivial) ? neg_mod(v, q) : (v | SIGN));
+                perm[lds_ix(to)] = pos ? v : neg_mod(v, q);
+            }
+        } else if (move) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                u32 x = coef_x<LOGN>(j, r);
+                u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
+                const bool pos = to < (u32)N;
+                to = pos ? to : to - N;
+                const u64 v = p[r];
+                perm[lds_ix(to)] = pos ? v : (v | SIGN);
             }
         }
         __syncthreads();
