@@ -29,13 +29,19 @@ struct Geom {
     static constexpr int LDS_WORDS = N + N / 8;  // per polynomial
 };
 
-// Stride of the thread's 8 coefficients in the phase starting at stage S0.
-template <int LOGN, int S0>
+// Stride of the thread's R coefficients (R = 8: radix-8 phases of three
+// stages; R = 4: radix-4 phases of two) in the phase starting at stage S0.
+template <int R>
+struct RadixLog {
+    static constexpr int value = R == 8 ? 3 : 2;
+};
+template <int LOGN, int S0, int R = 8>
 struct Phase {
     static constexpr int N = 1 << LOGN;
-    static constexpr int ST = ilog_max1(N >> (S0 + 3));
-    static constexpr int BS = 8 * ST;
-    static constexpr int NS = (LOGN - S0) < 3 ? (LOGN - S0) : 3;
+    static constexpr int LR = RadixLog<R>::value;
+    static constexpr int ST = ilog_max1(N >> (S0 + LR));
+    static constexpr int BS = R * ST;
+    static constexpr int NS = (LOGN - S0) < LR ? (LOGN - S0) : LR;
     PZ_DEV static u32 base(u32 j) { return (j / ST) * BS + (j % ST); }
     PZ_DEV static u32 x(u32 j, int r) { return base(j) + ST * r; }
 };
@@ -72,15 +78,15 @@ enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2 };
 // 3 = unreduced (FWD_FAST).  SM selects the two-step Shoup form
 // (shoup_mul<true>, fewer instructions); the one-step form has fewer live
 // registers, which keeps the one-work-group blind rotation within 256 VGPRs.
-template <int LOGN, int S, int ST, int NP, int KIND, bool SM>
-PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
+template <int LOGN, int S, int ST, int NP, int KIND, bool SM, int R>
+PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
 {
     constexpr int N = 1 << LOGN;
     constexpr int T = N >> (S + 1);
     constexpr int LD = T / ST;
     constexpr int M = 1 << S;
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
+    for (int r = 0; r < R; r++) {
         if (r & LD) continue;
         const u32 x = base + ST * r;
         const u32 i = x / (2 * T);
@@ -119,8 +125,9 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
 // 1 (2^10 q < 2^64 for q < 2^54); later stages are the reference butterfly
 // again.  Residues are the reference's throughout, and the strict last stage
 // returns the same words.
-template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB>
-PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
+template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB,
+          int R>
+PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
 {
     constexpr int N = 1 << LOGN;
     constexpr int T = N >> (S + 1);
@@ -131,7 +138,7 @@ PZ_DEV void inv_stage(u64 (&v)[NP][8], u32 base, const Tables &tb)
     static_assert(!FAST || !SCALE, "the fast inverse leaves n^-1 to the key rows");
     static_assert(IB == 2 || (IB == 4 && FAST), "inputs below 4q need the fast inverse");
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
+    for (int r = 0; r < R; r++) {
         if (r & LD) continue;
         const u32 x = base + ST * r;
         const u32 i = x / (2 * T);
@@ -175,52 +182,54 @@ PZ_DEV struct Tables fresh(const Tables &tb);
 #define STAGE_TB(t) (t)
 #endif
 
-// Forward phase starting at stage S0 (up to three stages).
-template <int LOGN, int S0, int NP, int MODE, bool SM>
-PZ_DEV void fwd_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
+// Forward phase starting at stage S0 (up to log2 R stages).
+template <int LOGN, int S0, int NP, int MODE, bool SM, int R>
+PZ_DEV void fwd_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
 {
-    using P = Phase<LOGN, S0>;
+    using P = Phase<LOGN, S0, R>;
     const u32 base = P::base(j);
     constexpr int LAST = LOGN - 1;
     constexpr int K_LAST = MODE == FWD_STRICT ? 2 : (MODE == FWD_LAZY ? 1 : 3);
     constexpr int K_MID = MODE == FWD_FAST ? 3 : 0;
-    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM>(v, base, STAGE_TB(tb));
+    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM, R>(v, base, STAGE_TB(tb));
     if constexpr (P::NS > 1)
-        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID, SM>(v, base,
-                                                                                STAGE_TB(tb));
+        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID, SM, R>(
+            v, base, STAGE_TB(tb));
     if constexpr (P::NS > 2)
-        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID, SM>(v, base,
-                                                                                STAGE_TB(tb));
+        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID, SM, R>(
+            v, base, STAGE_TB(tb));
 }
 
 // Inverse phase covering stages S0+NS-1 .. S0 (descending).
-template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB>
-PZ_DEV void inv_phase(u64 (&v)[NP][8], u32 j, const Tables &tb)
+template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB, int R>
+PZ_DEV void inv_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
 {
-    using P = Phase<LOGN, S0>;
+    using P = Phase<LOGN, S0, R>;
     const u32 base = P::base(j);
     if constexpr (P::NS > 2)
-        inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM, FAST, IB>(v, base, STAGE_TB(tb));
+        inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R>(v, base,
+                                                                           STAGE_TB(tb));
     if constexpr (P::NS > 1)
-        inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM, FAST, IB>(v, base, STAGE_TB(tb));
-    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM, FAST, IB>(v, base, STAGE_TB(tb));
+        inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R>(v, base,
+                                                                           STAGE_TB(tb));
+    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R>(v, base, STAGE_TB(tb));
 }
 
 // Store the thread's coefficients (layout of phase SA) into LDS, barrier,
 // load them back in the layout of phase SB, barrier.
-template <int LOGN, int SA, int SB, int NP>
-PZ_DEV void exchange(u64 (&v)[NP][8], u32 j, u64 *lds)
+template <int LOGN, int SA, int SB, int NP, int R>
+PZ_DEV void exchange(u64 (&v)[NP][R], u32 j, u64 *lds)
 {
     constexpr int W = Geom<LOGN>::LDS_WORDS;
 #pragma unroll
     for (int p = 0; p < NP; p++)
 #pragma unroll
-        for (int r = 0; r < 8; r++) lds[p * W + lds_ix(Phase<LOGN, SA>::x(j, r))] = v[p][r];
+        for (int r = 0; r < R; r++) lds[p * W + lds_ix(Phase<LOGN, SA, R>::x(j, r))] = v[p][r];
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < NP; p++)
 #pragma unroll
-        for (int r = 0; r < 8; r++) v[p][r] = lds[p * W + lds_ix(Phase<LOGN, SB>::x(j, r))];
+        for (int r = 0; r < R; r++) v[p][r] = lds[p * W + lds_ix(Phase<LOGN, SB, R>::x(j, r))];
     __syncthreads();
 }
 
@@ -254,46 +263,57 @@ PZ_DEV Tables fresh(const Tables &tb)
 // Forward NTT of NP polynomials: stride layout in, contiguous layout out.
 // MODE FWD_STRICT = Ntt::forward (output in [0,q)), FWD_LAZY = forward_lazy
 // ([0,2q)), FWD_FAST = unreduced (see above).
-template <int LOGN, int NP, int MODE, bool SM = true>
-PZ_DEV void ntt_fwd(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
+// Phases S0, S0 + log2 R, ... with an LDS exchange between consecutive ones.
+template <int LOGN, int S0, int NP, int MODE, bool SM, int R>
+PZ_DEV void fwd_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
+{
+    constexpr int LR = RadixLog<R>::value;
+    fwd_phase<LOGN, S0, NP, MODE, SM, R>(v, j, fresh(tb));
+    if constexpr (S0 + LR < LOGN) {
+        exchange<LOGN, S0, S0 + LR, NP, R>(v, j, lds);
+        fwd_from<LOGN, S0 + LR, NP, MODE, SM, R>(v, j, lds, tb);
+    }
+}
+
+template <int LOGN, int NP, int MODE, bool SM = true, int R = 8>
+PZ_DEV void ntt_fwd(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
     static_assert(MODE != FWD_FAST || LOGN <= 11, "unreduced values must stay below 2^64");
-    fwd_phase<LOGN, 0, NP, MODE, SM>(v, j, fresh(tb));
-    exchange<LOGN, 0, 3, NP>(v, j, lds);
-    fwd_phase<LOGN, 3, NP, MODE, SM>(v, j, fresh(tb));
-    exchange<LOGN, 3, 6, NP>(v, j, lds);
-    fwd_phase<LOGN, 6, NP, MODE, SM>(v, j, fresh(tb));
-    if constexpr (LOGN > 9) {
-        exchange<LOGN, 6, 9, NP>(v, j, lds);
-        fwd_phase<LOGN, 9, NP, MODE, SM>(v, j, fresh(tb));
-    }
+    static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
+    fwd_from<LOGN, 0, NP, MODE, SM, R>(v, j, lds, tb);
 }
 
 // Inverse NTT of NP polynomials: contiguous layout in, stride layout out.
 // STRICT = Ntt::backward ([0,q)), else backward_lazy ([0,2q)); SCALE = false
 // leaves out the n^-1 factor; FAST = unreduced early stages (see inv_stage).
+// Phases S0, S0 - log2 R, ..., 0 with an LDS exchange between consecutive ones.
+template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB, int R>
+PZ_DEV void inv_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
+{
+    constexpr int LR = RadixLog<R>::value;
+    inv_phase<LOGN, S0, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, fresh(tb));
+    if constexpr (S0 > 0) {
+        exchange<LOGN, S0, S0 - LR, NP, R>(v, j, lds);
+        inv_from<LOGN, S0 - LR, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, lds, tb);
+    }
+}
+
 template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool FAST = false,
-          int IB = 2>
-PZ_DEV void ntt_inv(u64 (&v)[NP][8], u32 j, u64 *lds, const Tables &tb)
+          int IB = 2, int R = 8>
+PZ_DEV void ntt_inv(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(!FAST || (STRICT && LOGN <= 12), "fast inverse: strict output");
-    if constexpr (LOGN > 9) {
-        inv_phase<LOGN, 9, NP, STRICT, SCALE, SM, FAST, IB>(v, j, fresh(tb));
-        exchange<LOGN, 9, 6, NP>(v, j, lds);
-    }
-    inv_phase<LOGN, 6, NP, STRICT, SCALE, SM, FAST, IB>(v, j, fresh(tb));
-    exchange<LOGN, 6, 3, NP>(v, j, lds);
-    inv_phase<LOGN, 3, NP, STRICT, SCALE, SM, FAST, IB>(v, j, fresh(tb));
-    exchange<LOGN, 3, 0, NP>(v, j, lds);
-    inv_phase<LOGN, 0, NP, STRICT, SCALE, SM, FAST, IB>(v, j, fresh(tb));
+    static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
+    constexpr int LR = RadixLog<R>::value;
+    inv_from<LOGN, ((LOGN - 1) / LR) * LR, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, lds, tb);
 }
 
 // Coefficient index of register r in the stride (coefficient-domain) layout
 // and in the contiguous (evaluation-domain) layout.
-template <int LOGN>
-PZ_DEV u32 coef_x(u32 j, int r) { return j + (u32)Geom<LOGN>::TPB * r; }
-template <int LOGN>
-PZ_DEV u32 eval_x(u32 j, int r) { return 8 * j + r; }
+template <int LOGN, int R = 8>
+PZ_DEV u32 coef_x(u32 j, int r) { return j + (u32)((1 << LOGN) / R) * r; }
+template <int LOGN, int R = 8>
+PZ_DEV u32 eval_x(u32 j, int r) { return R * j + r; }
 
 }  // namespace pz
