@@ -35,7 +35,28 @@ PZ_DEV u64 shoup_rem(u64 c, u64 a, u64 w, u64 h, u64 q)
     const u64 nq = 0 - q;
     const u32 h0 = (u32)h, h1 = (u32)(h >> 32);
     const u32 m = (u32)w * a1 + (u32)(w >> 32) * a0 + (u32)nq * h1 + (u32)(nq >> 32) * h0;
-    return (u64)(u32)w * a0 + (u64)(u32)nq * h0 + c + ((u64)m << 32);
+    const u64 lo = (u64)(u32)w * a0 + (u64)(u32)nq * h0 + c;
+    // m << 32 only changes the high word: one 32-bit add (as a 64-bit add the
+    // compiler spends a move on a zero low word as well)
+    u32 hi = (u32)(lo >> 32);
+    asm("v_add_u32 %0, %1, %2" : "=v"(hi) : "v"(hi), "v"(m));
+    return ((u64)hi << 32) | (u32)lo;
+}
+
+// A quotient at most 2 below floor(ws*a / 2^64): the low partial product and
+// the two carries out of the middle column are dropped (each fractional part
+// is below 1).  c + w*a - q*h then lies in c + [0, 4q) instead of c + [0, 2q)
+// and is the same residue; callers whose lazy ranges admit 4q use it.
+PZ_DEV u64 shoup_quot4(u64 a, u64 ws)
+{
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), s0 = (u32)ws, s1 = (u32)(ws >> 32);
+    const u32 x = __umulhi(s0, a1), y = __umulhi(s1, a0);
+    return (u64)s1 * a1 + ((u64)x + y);
+}
+// w*a mod q in [0, 4q) for any a < 2^64 (the two-step form with shoup_quot4)
+PZ_DEV u64 shoup_mul4(u64 a, u64 w, u64 ws, u64 q)
+{
+    return shoup_rem(0, a, w, shoup_quot4(a, ws), q);
 }
 
 // ShoupMul::mul, src/utils.rs:43-46: b*a - q*floor(b_shoup*a/2^64) (wrapping),
@@ -97,11 +118,14 @@ PZ_DEV u64 neg_mod(u64 a, u64 q) { return q - a; }
 // (src/backend/modulus_u64.rs:252-260): out in [0,2q) -> [0,2q).
 // REDUCE = false leaves out + [0,2q) unreduced (fused operators whose
 // consumer accepts the wider range).
-template <bool SPLIT = true, bool REDUCE = true>
+// QUOT4 takes the product with shoup_quot4 (out + [0, 4q), REDUCE = false only).
+template <bool SPLIT = true, bool REDUCE = true, bool QUOT4 = false>
 PZ_DEV u64 shoup_fma(u64 out, u64 a, u64 as, u64 b, u64 q, u64 q2)
 {
+    static_assert(!QUOT4 || (SPLIT && !REDUCE), "the [0,4q) product is left unreduced");
     u64 v;
-    if constexpr (SPLIT) v = shoup_rem(out, b, a, shoup_quot(b, as), q);  // a*b + out - q*quot
+    if constexpr (QUOT4) v = shoup_rem(out, b, a, shoup_quot4(b, as), q);
+    else if constexpr (SPLIT) v = shoup_rem(out, b, a, shoup_quot(b, as), q);  // a*b + out - q*quot
     else v = a * b + out - q * mulhi64(as, b);
     if constexpr (!REDUCE) return v;
     return v >= q2 ? v - q2 : v;
@@ -181,19 +205,23 @@ PZ_DEV u64 decomp_next(const Decomp &dc, u64 &value)
     return (neg_carry & dc.q) + k - (carry << dc.logb);
 }
 
-// Digit 0 alone (what a d = 1 decomposition uses), for logb <= 31: only the
+// Digit 0 alone (what a d = 1 decomposition uses), for logb <= 30: only the
 // low 32 bits of the rounded value matter, and the carry into the next digit
 // is not formed.  Same word as decomp_digit(dc, v, 0).
 PZ_DEV u64 decomp_digit0(const Decomp &dc, u64 v)
 {
     if (v >= (dc.q >> 1)) v -= dc.q;  // -(q - v) in two's complement; v == q gives 0
+    // round(v / 2^ib) = (floor(v / 2^(ib-1)) + 1) / 2, in 32 bits: bits 0..30
+    // of the result are exact, and the digit and its carry use bits 0..logb
     const u32 ib = dc.ignore_bits;
-    u32 value = (u32)v;
-    if (ib != 0) value = (u32)(v >> ib) + ((u32)(v >> (ib - 1)) & 1u);  // round
+    const u32 sh = ib ? ib - 1 : 0, rnd = ib ? 1u : 0u;  // uniform
+    const u32 value = ((u32)(v >> sh) + rnd) >> rnd;
     const u32 b = 1u << dc.logb, half = b >> 1;
     const u32 k = value & (b - 1);
-    const bool carry = k > half || (k == half && ((value >> dc.logb) & 1u));
-    return carry ? dc.q - b + k : (u64)k;  // q + k - B for a negative digit
+    // carry when k > B/2, or k == B/2 and the next digit is odd: one compare
+    // of k + (that bit) against B/2, no divergent branch
+    const bool carry = k + ((value >> dc.logb) & 1u) > half;
+    return (u64)k + (carry ? dc.q - b : 0);  // q + k - B for a negative digit
 }
 
 PZ_DEV u64 decomp_digit(const Decomp &dc, u64 v, u32 r)

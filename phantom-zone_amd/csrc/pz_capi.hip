@@ -123,10 +123,10 @@ static int check_params(const pz_params *p, int *logn)
     if (p->lwe_logb > 7 || p->lwe_logb == 0) return fail(PZ_ERR_UNSUPPORTED, "lwe_logb must be 1..7");
     if (p->rlrg_da == 0 || p->rlrg_db == 0 || p->auto_d == 0 || p->lwe_d == 0)
         return fail(PZ_ERR_ARG, "decomposition counts must be > 0");
-    // digits are formed in 32-bit lanes
-    if (p->rlrg_logb == 0 || p->rlrg_logb > 31 || p->auto_logb == 0 || p->auto_logb > 31 ||
-        p->uitos_logb > 31 || p->rgrg_logb > 31)
-        return fail(PZ_ERR_UNSUPPORTED, "decomposition bases must be 2^1 .. 2^31");
+    // digits are formed in 32-bit lanes (a digit and its carry bit within 31 bits)
+    if (p->rlrg_logb == 0 || p->rlrg_logb > 30 || p->auto_logb == 0 || p->auto_logb > 30 ||
+        p->uitos_logb > 30 || p->rgrg_logb > 30)
+        return fail(PZ_ERR_UNSUPPORTED, "decomposition bases must be 2^1 .. 2^30");
     uint32_t lq = host::ref_log2(p->rlwe_q);
     if (p->rlrg_logb * p->rlrg_da > lq || p->rlrg_logb * p->rlrg_db > lq ||
         p->auto_logb * p->auto_d > lq || p->lwe_logb * p->lwe_d > host::ref_log2(p->lwe_q))

@@ -35,6 +35,13 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #define PZ_SPLIT_SM true
 #endif
 
+// Split blind rotation with the [0,4q) Shoup product (shoup_quot4) in the
+// forward transform, the key products and the inverse's unreduced stages:
+// accumulators below 8q, FWD_FAST4 and the IB = 8 fast inverse.
+#ifndef PZ_QUOT4
+#define PZ_QUOT4 1
+#endif
+
 #ifndef PZ_MAC_FENCE
 #define PZ_MAC_FENCE 0
 #endif
@@ -121,7 +128,7 @@ PZ_DEV void fma_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restr
 
 // fma_row in two halves of four words, a scheduling fence between them:
 // sixteen fewer registers hold key words at any time.
-template <bool SPLIT, bool REDUCE = true>
+template <bool SPLIT, bool REDUCE = true, bool QUOT4 = false>
 PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
                            const u64 (&D)[8], u32 j, u64 q, u64 q2)
 {
@@ -134,8 +141,8 @@ PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *
 #pragma unroll
         for (int r = 0; r < 4; r++)
             acc[4 * half + r] =
-                shoup_fma<SPLIT, REDUCE>(acc[4 * half + r], kv[r], ks[r], D[4 * half + r], q,
-                                         q2);
+                shoup_fma<SPLIT, REDUCE, QUOT4>(acc[4 * half + r], kv[r], ks[r], D[4 * half + r],
+                                                q, q2);
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -355,7 +362,7 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
 #pragma unroll
     for (int r = 0; r < 8; r++) D[0][r] = decomp_digit0(dc, p[r]);
     if (work) {
-        ntt_fwd<LOGN, 1, PZ_FWD_MODE, PZ_SPLIT_SM>(D, j, mine, c.tb);
+        ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM>(D, j, mine, c.tb);
 #pragma unroll
         for (int r = 0; r < 8; r++) mine[lds_ix(eval_x<LOGN>(j, r))] = D[0][r];
     } else {
@@ -372,8 +379,10 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     const size_t r1 = (size_t)(is_auto ? 0 : (h ? 3 : 0)) * N;
     const size_t r2 = (size_t)(is_auto ? 1 : (h ? 1 : 2)) * N;
     // unreduced products: at most two of them, so the accumulator stays below
-    // 4q, the fast inverse's input bound
-    if (work) fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC>(acc[0], K + r1, Ks + r1, D[0], j, q, q2);
+    // 4q (8q with the [0,4q) products), the fast inverse's input bound
+    constexpr bool Q4 = PZ_QUOT4 && PZ_LAZY_MAC && PZ_INV_FAST;
+    if (work)
+        fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], K + r1, Ks + r1, D[0], j, q, q2);
     __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
     const bool second = !is_auto || h == 1;
     if (second) {
@@ -381,9 +390,10 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
         for (int r = 0; r < 8; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
     }
     __syncthreads();  // both halves have read: the regions may be rewritten
-    if (second) fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
-    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, PZ_LAZY_MAC ? 4 : 2>(acc, j, mine,
-                                                                            c.tb);  // n^-1: key rows
+    if (second)
+        fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
+    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2)>(
+        acc, j, mine, c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
 #pragma unroll
         for (int r = 0; r < 8; r++) {

@@ -71,11 +71,14 @@ struct Tables {
 // stage k leaves values below (2k+1)q (< 23q < 2^59 for N = 2048).  FAST is
 // only used where a Shoup product consumes the result (Shoup accepts any
 // 64-bit input), so every ciphertext stays identical to the reference.
-enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2 };
+// FWD_FAST4 is FWD_FAST with the [0,4q) product (shoup_mul4) and y' = x + 4q - t:
+// stage k leaves values below (4k+1)q (< 45q < 2^60 for N = 2048).
+enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3 };
 
 // One forward stage S inside a phase of stride ST. KIND: 0 = 0_to_4q
 // butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q),
-// 3 = unreduced (FWD_FAST).  SM selects the two-step Shoup form
+// 3 = unreduced (FWD_FAST), 4 = unreduced with the [0,4q) product (FWD_FAST4).
+// SM selects the two-step Shoup form
 // (shoup_mul<true>, fewer instructions); the one-step form has fewer live
 // registers, which keeps the one-work-group blind rotation within 256 VGPRs.
 template <int LOGN, int S, int ST, int NP, int KIND, bool SM, int R>
@@ -101,6 +104,11 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
                 const u64 t = shoup_mul<SM>(v[p][r + LD], w, ws, tb.q);
                 v[p][r] = x + t;
                 v[p][r + LD] = x + tb.q2 - t;
+            } else if (KIND == 4) {
+                const u64 x = v[p][r];
+                const u64 t = shoup_mul4(v[p][r + LD], w, ws, tb.q);
+                v[p][r] = x + t;
+                v[p][r + LD] = x + (tb.q2 << 1) - t;
             } else {
                 fb_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
                 if (KIND == 2) {
@@ -117,14 +125,16 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
 // SCALE = false is for products whose key rows already carry the n^-1
 // factor: the result is the same residue, and with STRICT the same word.
 //
-// FAST (fused operators, inputs below IB*q with IB = 2 or 4, no n^-1):
+// FAST (fused operators, inputs below IB*q with IB = 2, 4 or 8, no n^-1):
 // executed stage k = LOGN - S receives values below 2^(k-1) IB q.  Early
 // stages skip the reduction of the sum, x' = x + y, and take
-// y' = Shoup(x + 2^(k-1) IB q - y); stage KR (9 for IB = 2, 8 for IB = 4)
+// y' = Shoup(x + 2^(k-1) IB q - y); stage KR (9, 8, 7 for IB = 2, 4, 8)
 // reduces x' = x + y < 2^KR IB q = 2^10 q to [0,2q) with a Shoup multiply by
 // 1 (2^10 q < 2^64 for q < 2^54); later stages are the reference butterfly
-// again.  Residues are the reference's throughout, and the strict last stage
-// returns the same words.
+// again.  With IB = 8 the products of stages before KR come from shoup_mul4
+// ([0,4q), within the next stage's bound 2^k IB q); stage KR's are exact, so
+// the reference butterflies still see [0,2q).  Residues are the reference's
+// throughout, and the strict last stage returns the same words.
 template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB,
           int R>
 PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
@@ -134,9 +144,11 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
     constexpr int LD = T / ST;
     constexpr int M = 1 << S;
     constexpr int K = LOGN - S;  // executed stage, 1-based
-    constexpr int KR = IB == 4 ? 8 : 9;
+    constexpr int LIB = IB == 8 ? 3 : (IB == 4 ? 2 : 1);  // log2 IB
+    constexpr int KR = 10 - LIB;
     static_assert(!FAST || !SCALE, "the fast inverse leaves n^-1 to the key rows");
-    static_assert(IB == 2 || (IB == 4 && FAST), "inputs below 4q need the fast inverse");
+    static_assert(IB == 2 || ((IB == 4 || IB == 8) && FAST),
+                  "inputs below 4q or 8q need the fast inverse");
 #pragma unroll
     for (int r = 0; r < R; r++) {
         if (r & LD) continue;
@@ -149,9 +161,10 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
             if constexpr (FAST && K <= KR) {
                 const u64 xv = v[p][r], yv = v[p][r + LD];
                 const u64 sum = xv + yv;
+                const u64 dif = xv + (tb.q << (K - 1 + LIB)) - yv;
                 v[p][r] = K == KR ? shoup_mul<SM>(sum, 1, tb.one_s, tb.q) : sum;
-                v[p][r + LD] = shoup_mul<SM>(xv + (tb.q << (K - 1 + (IB == 4 ? 2 : 1))) - yv, w, ws,
-                                             tb.q);
+                v[p][r + LD] = (IB == 8 && K < KR) ? shoup_mul4(dif, w, ws, tb.q)
+                                                   : shoup_mul<SM>(dif, w, ws, tb.q);
             } else {
                 ib_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
             }
@@ -189,8 +202,9 @@ PZ_DEV void fwd_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
     using P = Phase<LOGN, S0, R>;
     const u32 base = P::base(j);
     constexpr int LAST = LOGN - 1;
-    constexpr int K_LAST = MODE == FWD_STRICT ? 2 : (MODE == FWD_LAZY ? 1 : 3);
-    constexpr int K_MID = MODE == FWD_FAST ? 3 : 0;
+    constexpr int K_LAST =
+        MODE == FWD_STRICT ? 2 : (MODE == FWD_LAZY ? 1 : (MODE == FWD_FAST ? 3 : 4));
+    constexpr int K_MID = MODE == FWD_FAST ? 3 : (MODE == FWD_FAST4 ? 4 : 0);
     fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM, R>(v, base, STAGE_TB(tb));
     if constexpr (P::NS > 1)
         fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID, SM, R>(
@@ -279,7 +293,8 @@ template <int LOGN, int NP, int MODE, bool SM = true, int R = 8>
 PZ_DEV void ntt_fwd(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
-    static_assert(MODE != FWD_FAST || LOGN <= 11, "unreduced values must stay below 2^64");
+    static_assert((MODE != FWD_FAST && MODE != FWD_FAST4) || LOGN <= 11,
+                  "unreduced values must stay below 2^64");
     static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
     fwd_from<LOGN, 0, NP, MODE, SM, R>(v, j, lds, tb);
 }

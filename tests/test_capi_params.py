@@ -38,7 +38,7 @@ CASES = [
     ("lwe_n", lambda p: setattr(p, "lwe_n", 0), PZ_ERR_UNSUPPORTED),
     ("lwe_logb", lambda p: setattr(p, "lwe_logb", 8), PZ_ERR_UNSUPPORTED),
     ("rlrg_da", lambda p: setattr(p, "rlrg_da", 0), PZ_ERR_ARG),
-    ("rlrg_logb", lambda p: setattr(p, "rlrg_logb", 32), PZ_ERR_UNSUPPORTED),     # base above 2^31
+    ("rlrg_logb", lambda p: setattr(p, "rlrg_logb", 31), PZ_ERR_UNSUPPORTED),     # base above 2^30
     ("auto_logb", lambda p: setattr(p, "auto_logb", 0), PZ_ERR_UNSUPPORTED),
     ("auto_d", lambda p: setattr(p, "auto_d", 3), PZ_ERR_ARG),                    # 24*3 > 54
     ("lwe_d", lambda p: setattr(p, "lwe_d", 17), PZ_ERR_ARG),                     # 1*17 > 16
