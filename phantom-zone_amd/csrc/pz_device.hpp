@@ -277,12 +277,13 @@ PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
 // the one-half operators above, twice the waves per bootstrap.
 // ---------------------------------------------------------------------------
 
-// The barriers of one ntt_fwd, for a half that has no transform to run.
+// The barriers of one ntt_fwd without exchange tails, for a half that has no
+// transform to run.
 template <int LOGN>
 PZ_DEV void ntt_fwd_idle()
 {
 #pragma unroll
-    for (int e = 0; e < (LOGN > 9 ? 6 : 4); e++) __syncthreads();
+    for (int e = 0; e < ntt_exchanges<LOGN>(); e++) __syncthreads();
 }
 
 // One blind-rotation step on a split work group, either
@@ -302,6 +303,13 @@ PZ_DEV void ntt_fwd_idle()
 // RLWE needs no special case in the product: its digit transform is 0 mod q.
 // The accumulation order of the product differs from the reference's; the
 // strict inverse returns the same words.
+// The transforms run without exchange tails: each thread's first LDS store of
+// a transform goes to the words it loaded last (the inverse ends in the
+// stride layout the forward starts from, the forward ends in the contiguous
+// layout the digit store and the inverse start from), so only the scattered
+// stores of an automorphism's permutation need a barrier before them.  The
+// other half reads this half's digit transform, so a barrier separates that
+// read from the inverse's first store.
 template <int LOGN>
 PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
                        bool is_auto, u64 kexp, bool trivial, const DevCtx &c, u32 j, u32 h,
@@ -314,6 +322,8 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     u64 *mine = lds + h * W;
     const u64 *theirs = lds + (h ^ 1) * W;
     if (is_auto) {
+        // the previous transform's last loads from this region are done
+        __syncthreads();
         u64 *perm = h ? stash : mine;
         const bool move = !(trivial && h == 0);
         // a wraps with a negation (and so does b when trivial); a non-trivial b
@@ -362,7 +372,8 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
 #pragma unroll
     for (int r = 0; r < 8; r++) D[0][r] = decomp_digit0(dc, p[r]);
     if (work) {
-        ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM>(D, j, mine, c.tb);
+        ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, 8, false>(D, j, mine,
+                                                                                  c.tb);
 #pragma unroll
         for (int r = 0; r < 8; r++) mine[lds_ix(eval_x<LOGN>(j, r))] = D[0][r];
     } else {
@@ -392,8 +403,8 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (second)
         fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
-    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2)>(
-        acc, j, mine, c.tb);  // n^-1: key rows
+    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2), 8,
+            false>(acc, j, mine, c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
 #pragma unroll
         for (int r = 0; r < 8; r++) {
@@ -405,7 +416,8 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
 #pragma unroll
         for (int r = 0; r < 8; r++) p[r] = acc[0][r];
     }
-    if (is_auto) __syncthreads();  // the stash is read; a following automorphism may rewrite it
+    // the stash is rewritten only by a following automorphism, after its
+    // leading barrier
 }
 
 }  // namespace pz

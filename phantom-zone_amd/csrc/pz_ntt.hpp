@@ -231,7 +231,12 @@ PZ_DEV void inv_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
 
 // Store the thread's coefficients (layout of phase SA) into LDS, barrier,
 // load them back in the layout of phase SB, barrier.
-template <int LOGN, int SA, int SB, int NP, int R>
+// TAIL = false leaves out the second barrier.  That is safe where the next
+// LDS write of every thread goes only to the words the thread itself read
+// here (the next exchange of the same transform stores the layout SB that was
+// just loaded, and a transform that follows stores its first layout where the
+// last one was loaded); any other writer must synchronise first.
+template <int LOGN, int SA, int SB, int NP, int R, bool TAIL>
 PZ_DEV void exchange(u64 (&v)[NP][R], u32 j, u64 *lds)
 {
     constexpr int W = Geom<LOGN>::LDS_WORDS;
@@ -244,7 +249,7 @@ PZ_DEV void exchange(u64 (&v)[NP][R], u32 j, u64 *lds)
     for (int p = 0; p < NP; p++)
 #pragma unroll
         for (int r = 0; r < R; r++) v[p][r] = lds[p * W + lds_ix(Phase<LOGN, SB, R>::x(j, r))];
-    __syncthreads();
+    if constexpr (TAIL) __syncthreads();
 }
 
 // Twiddle-table pointers passed through an empty asm before each phase.  The
@@ -278,50 +283,57 @@ PZ_DEV Tables fresh(const Tables &tb)
 // MODE FWD_STRICT = Ntt::forward (output in [0,q)), FWD_LAZY = forward_lazy
 // ([0,2q)), FWD_FAST = unreduced (see above).
 // Phases S0, S0 + log2 R, ... with an LDS exchange between consecutive ones.
-template <int LOGN, int S0, int NP, int MODE, bool SM, int R>
+// TAIL = false: exchanges without their second barrier (see exchange).
+template <int LOGN, int S0, int NP, int MODE, bool SM, int R, bool TAIL>
 PZ_DEV void fwd_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
     constexpr int LR = RadixLog<R>::value;
     fwd_phase<LOGN, S0, NP, MODE, SM, R>(v, j, fresh(tb));
     if constexpr (S0 + LR < LOGN) {
-        exchange<LOGN, S0, S0 + LR, NP, R>(v, j, lds);
-        fwd_from<LOGN, S0 + LR, NP, MODE, SM, R>(v, j, lds, tb);
+        exchange<LOGN, S0, S0 + LR, NP, R, TAIL>(v, j, lds);
+        fwd_from<LOGN, S0 + LR, NP, MODE, SM, R, TAIL>(v, j, lds, tb);
     }
 }
 
-template <int LOGN, int NP, int MODE, bool SM = true, int R = 8>
+// Number of LDS exchanges (barriers with TAIL = false) in one transform.
+template <int LOGN, int R = 8>
+constexpr int ntt_exchanges() { return (LOGN - 1) / RadixLog<R>::value; }
+
+template <int LOGN, int NP, int MODE, bool SM = true, int R = 8, bool TAIL = true>
 PZ_DEV void ntt_fwd(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
     static_assert((MODE != FWD_FAST && MODE != FWD_FAST4) || LOGN <= 11,
                   "unreduced values must stay below 2^64");
     static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
-    fwd_from<LOGN, 0, NP, MODE, SM, R>(v, j, lds, tb);
+    fwd_from<LOGN, 0, NP, MODE, SM, R, TAIL>(v, j, lds, tb);
 }
 
 // Inverse NTT of NP polynomials: contiguous layout in, stride layout out.
 // STRICT = Ntt::backward ([0,q)), else backward_lazy ([0,2q)); SCALE = false
 // leaves out the n^-1 factor; FAST = unreduced early stages (see inv_stage).
 // Phases S0, S0 - log2 R, ..., 0 with an LDS exchange between consecutive ones.
-template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB, int R>
+template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB, int R,
+          bool TAIL>
 PZ_DEV void inv_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
     constexpr int LR = RadixLog<R>::value;
     inv_phase<LOGN, S0, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, fresh(tb));
     if constexpr (S0 > 0) {
-        exchange<LOGN, S0, S0 - LR, NP, R>(v, j, lds);
-        inv_from<LOGN, S0 - LR, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, lds, tb);
+        exchange<LOGN, S0, S0 - LR, NP, R, TAIL>(v, j, lds);
+        inv_from<LOGN, S0 - LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL>(v, j, lds, tb);
     }
 }
 
 template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool FAST = false,
-          int IB = 2, int R = 8>
+          int IB = 2, int R = 8, bool TAIL = true>
 PZ_DEV void ntt_inv(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(!FAST || (STRICT && LOGN <= 12), "fast inverse: strict output");
     static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
     constexpr int LR = RadixLog<R>::value;
-    inv_from<LOGN, ((LOGN - 1) / LR) * LR, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, lds, tb);
+    inv_from<LOGN, ((LOGN - 1) / LR) * LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL>(v, j, lds,
+                                                                                    tb);
 }
 
 // Coefficient index of register r in the stride (coefficient-domain) layout
