@@ -42,6 +42,14 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #define PZ_QUOT4 1
 #endif
 
+// Split blind rotation: key rows loaded whole (32 VGPRs), the first before the
+// barrier that precedes its product, the second before the read of the other
+// half's digits and the barrier after it: the load latencies overlap the
+// barrier waits instead of one exposed latency per half row.
+#ifndef PZ_KEY_EARLY
+#define PZ_KEY_EARLY 1
+#endif
+
 #ifndef PZ_MAC_FENCE
 #define PZ_MAC_FENCE 0
 #endif
@@ -144,6 +152,33 @@ PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *
                 shoup_fma<SPLIT, REDUCE, QUOT4>(acc[4 * half + r], kv[r], ks[r], D[4 * half + r],
                                                 q, q2);
         __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// The thread's slice of one key row and its Shoup twins (8 words each), as
+// 16-byte loads.
+struct KeySlice {
+    ulonglong2 k[4], s[4];
+};
+PZ_DEV void load_slice(KeySlice &ks, const u64 *__restrict__ K, const u64 *__restrict__ Ks, u32 j)
+{
+    const ulonglong2 *k2 = reinterpret_cast<const ulonglong2 *>(K + 8 * j);
+    const ulonglong2 *s2 = reinterpret_cast<const ulonglong2 *>(Ks + 8 * j);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        ks.k[i] = k2[i];
+        ks.s[i] = s2[i];
+    }
+}
+// acc[r] += D[r] * K[r] (unreduced, [0,4q) products: shoup_fma<.., QUOT4>)
+template <bool SPLIT, bool REDUCE, bool QUOT4>
+PZ_DEV void fma_slice(u64 (&acc)[8], const KeySlice &ks, const u64 (&D)[8], u64 q, u64 q2)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        acc[2 * i] = shoup_fma<SPLIT, REDUCE, QUOT4>(acc[2 * i], ks.k[i].x, ks.s[i].x, D[2 * i], q, q2);
+        acc[2 * i + 1] =
+            shoup_fma<SPLIT, REDUCE, QUOT4>(acc[2 * i + 1], ks.k[i].y, ks.s[i].y, D[2 * i + 1], q, q2);
     }
 }
 
@@ -371,6 +406,16 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     u64 D[1][8];
 #pragma unroll
     for (int r = 0; r < 8; r++) D[0][r] = decomp_digit0(dc, p[r]);
+    // product rows: RLWE x RGSW -- own digits with the own-part row (half 0:
+    // A(-sm), row 0; half 1: B(m), row 3), then the other half's digits with
+    // the other row (A(m), row 2; B(-sm), row 1); automorphism -- half 0 row
+    // A (0) with its digits, half 1 row B (1) with half 0's digits
+    const size_t r1 = (size_t)(is_auto ? 0 : (h ? 3 : 0)) * N;
+    const size_t r2 = (size_t)(is_auto ? 1 : (h ? 1 : 2)) * N;
+    const bool second = !is_auto || h == 1;
+    // unreduced products: at most two of them, so the accumulator stays below
+    // 4q (8q with the [0,4q) products), the fast inverse's input bound
+    constexpr bool Q4 = PZ_QUOT4 && PZ_LAZY_MAC && PZ_INV_FAST;
     if (work) {
         ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, 8, false>(D, j, mine,
                                                                                   c.tb);
@@ -379,23 +424,27 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     } else {
         ntt_fwd_idle<LOGN>();
     }
-    __syncthreads();
     u64 acc[1][8];
 #pragma unroll
     for (int r = 0; r < 8; r++) acc[0][r] = 0;
-    // product rows: RLWE x RGSW -- own digits with the own-part row (half 0:
-    // A(-sm), row 0; half 1: B(m), row 3), then the other half's digits with
-    // the other row (A(m), row 2; B(-sm), row 1); automorphism -- half 0 row
-    // A (0) with its digits, half 1 row B (1) with half 0's digits
-    const size_t r1 = (size_t)(is_auto ? 0 : (h ? 3 : 0)) * N;
-    const size_t r2 = (size_t)(is_auto ? 1 : (h ? 1 : 2)) * N;
-    // unreduced products: at most two of them, so the accumulator stays below
-    // 4q (8q with the [0,4q) products), the fast inverse's input bound
-    constexpr bool Q4 = PZ_QUOT4 && PZ_LAZY_MAC && PZ_INV_FAST;
+#if PZ_KEY_EARLY
+    KeySlice k1, k2;
+    if (work) load_slice(k1, K + r1, Ks + r1, j);
+    __syncthreads();
+    if (work) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k1, D[0], q, q2);
+    __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
+    if (second) load_slice(k2, K + r2, Ks + r2, j);
+    if (second) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
+    }
+    __syncthreads();  // both halves have read: the regions may be rewritten
+    if (second) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k2, D[0], q, q2);
+#else
+    __syncthreads();
     if (work)
         fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], K + r1, Ks + r1, D[0], j, q, q2);
     __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
-    const bool second = !is_auto || h == 1;
     if (second) {
 #pragma unroll
         for (int r = 0; r < 8; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
@@ -403,6 +452,7 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (second)
         fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
+#endif
     ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2), 8,
             false>(acc, j, mine, c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
