@@ -158,12 +158,13 @@ PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *
 // The thread's slice of one key row and its Shoup twins (8 words each), as
 // 16-byte loads.
 struct KeySlice {
-    ulonglong2 k[4], s[4];
+    u64x2 k[4], s[4];
 };
 PZ_DEV void load_slice(KeySlice &ks, const u64 *__restrict__ K, const u64 *__restrict__ Ks, u32 j)
 {
-    const ulonglong2 *k2 = reinterpret_cast<const ulonglong2 *>(K + 8 * j);
-    const ulonglong2 *s2 = reinterpret_cast<const ulonglong2 *>(Ks + 8 * j);
+    // global (not flat) loads: flat loads also wait on the LDS counter
+    typedef const __attribute__((address_space(1))) u64x2 *gp;
+    const gp k2 = (gp)(K + 8 * j), s2 = (gp)(Ks + 8 * j);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         ks.k[i] = k2[i];
