@@ -313,13 +313,13 @@ PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
 // the one-half operators above, twice the waves per bootstrap.
 // ---------------------------------------------------------------------------
 
-// The barriers of one ntt_fwd without exchange tails, for a half that has no
-// transform to run.
+// The work-group barriers of one ntt_fwd without exchange tails, for a half
+// that has no transform to run.
 template <int LOGN>
 PZ_DEV void ntt_fwd_idle()
 {
 #pragma unroll
-    for (int e = 0; e < ntt_exchanges<LOGN>(); e++) __syncthreads();
+    for (int e = 0; e < ntt_fwd_barriers<LOGN>(); e++) __syncthreads();
 }
 
 // One blind-rotation step on a split work group, either

@@ -236,6 +236,26 @@ PZ_DEV void inv_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
 // here (the next exchange of the same transform stores the layout SB that was
 // just loaded, and a transform that follows stores its first layout where the
 // last one was loaded); any other writer must synchronise first.
+//
+// Wave-local exchanges.  In a phase of stride ST <= 64 (S >= LOGN - 9, eight
+// coefficients per thread), wave w of a polynomial's N/8 threads holds exactly
+// the words [512 w, 512 (w + 1)).  An exchange between two such phases stays
+// within each wave, and LDS operations of one wave execute in order, so with
+// TAIL = false it needs no work-group barrier: a wave barrier keeps the
+// compiler from moving the loads above the stores.
+template <int LOGN, int SA, int SB, int R = 8>
+constexpr bool wave_local_exchange()
+{
+    return R == 8 && (SA < SB ? SA : SB) >= LOGN - 9;
+}
+
+PZ_DEV void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int LOGN, int SA, int SB, int NP, int R, bool TAIL>
 PZ_DEV void exchange(u64 (&v)[NP][R], u32 j, u64 *lds)
 {
@@ -244,7 +264,8 @@ PZ_DEV void exchange(u64 (&v)[NP][R], u32 j, u64 *lds)
     for (int p = 0; p < NP; p++)
 #pragma unroll
         for (int r = 0; r < R; r++) lds[p * W + lds_ix(Phase<LOGN, SA, R>::x(j, r))] = v[p][r];
-    __syncthreads();
+    if constexpr (!TAIL && wave_local_exchange<LOGN, SA, SB, R>()) wave_sync();
+    else __syncthreads();
 #pragma unroll
     for (int p = 0; p < NP; p++)
 #pragma unroll
@@ -295,9 +316,17 @@ PZ_DEV void fwd_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
     }
 }
 
-// Number of LDS exchanges (barriers with TAIL = false) in one transform.
+// Work-group barriers of one forward transform with TAIL = false: its
+// exchanges that are not wave-local.
 template <int LOGN, int R = 8>
-constexpr int ntt_exchanges() { return (LOGN - 1) / RadixLog<R>::value; }
+constexpr int ntt_fwd_barriers()
+{
+    constexpr int LR = RadixLog<R>::value;
+    int n = 0;
+    for (int s0 = 0; s0 + LR < LOGN; s0 += LR)
+        if (!(R == 8 && s0 >= LOGN - 9)) n++;
+    return n;
+}
 
 template <int LOGN, int NP, int MODE, bool SM = true, int R = 8, bool TAIL = true>
 PZ_DEV void ntt_fwd(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
