@@ -344,12 +344,15 @@ int pz_autotune(pz_ctx *ctx, pz_key *key, size_t B, uint32_t *h_choice);
 
 /* Blind-rotation kernel choice.  No counterpart in the reference (its
  * evaluator is single-threaded): 0 = automatic, 1 = one work group per
- * bootstrap, 2 = two half work groups per bootstrap (only where
- * d_a = d_b = d_auto = 1; PZ_ERR_UNSUPPORTED otherwise).  Automatic is 2
- * where available, else 1; mode 2 runs its latency variant while every
- * bootstrap of a batch has a CU of its own and its throughput variant (two
- * bootstraps per CU) beyond that.  Outputs are the same words in every
- * mode. */
+ * bootstrap, 2 = two half work groups per bootstrap, 8 coefficients per
+ * thread, 3 = the same with 4 coefficients per thread (twice the waves;
+ * 2 and 3 only where d_a = d_b = d_auto = 1; PZ_ERR_UNSUPPORTED otherwise).
+ * Modes 2 and 3 run their latency variant while every bootstrap of a batch
+ * has a CU of its own and their throughput variant (two bootstraps per CU)
+ * beyond that.  Automatic (where 2 and 3 are available, else 1) takes the
+ * latency variant of 3 and the throughput variant of 2, unless pz_autotune
+ * chose another shape for the batch size.  Outputs are the same words in
+ * every mode. */
 int pz_set_blind_rotate_mode(pz_ctx *ctx, int mode);
 
 /* Automorphism count of each bootstrap of the last pz_pbs / pz_gates batch

@@ -58,9 +58,11 @@ struct pz_ctx {
     u32 *d_ksb = nullptr, *d_ks = nullptr, *d_stats = nullptr;
     size_t cap = 0;
     int num_cu = 0;
-    int br_per_cu[3] = {0, 0, 0};  // resident work groups per CU of each kernel variant
-                                   // (BR_ONE, BR_SPLIT_LAT, BR_SPLIT; 0: not available)
-    int br_mode = 0;          // 0 auto, 1 one-half kernel, 2 split kernel
+    int br_per_cu[BR_NVAR] = {};  // resident work groups per CU of each kernel variant
+                                  // (BR_ONE, BR_SPLIT_LAT, BR_SPLIT, BR_SPLIT4_LAT,
+                                  // BR_SPLIT4; 0: not available)
+    int br_mode = 0;  // 0 auto, 1 one-half kernel, 2 split kernel, 3 split with 4 coefficients
+                      // per thread
     // autotuned blind-rotation shapes per batch bucket (ceil(log2 B)): kernel
     // variant and resident work groups per CU; var < 0 = not tuned
     struct Tuned {
@@ -243,7 +245,7 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
-    for (int var = BR_ONE; var <= BR_SPLIT; var++) {
+    for (int var = BR_ONE; var < BR_NVAR; var++) {
         if (var != BR_ONE && !br_split(d)) continue;
         int per = 0;
         size_t lds = br_lds_bytes(logn, var != BR_ONE, p->br_q, p->lwe_n);
@@ -684,10 +686,12 @@ extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint6
 }
 
 // Blind-rotation kernel variant and grid for a batch of B bootstraps.  With
-// d = 1 the split kernel (two half work groups per bootstrap) is used: its
-// latency variant while every bootstrap has a CU of its own, its throughput
-// variant (two bootstraps per CU) beyond that.  Otherwise, or in mode 1, one
-// work group per bootstrap.
+// d = 1 the split kernel (two half work groups per bootstrap) is used: a
+// latency variant while every bootstrap has a CU of its own (4 coefficients
+// per thread, or 8 in mode 2), a throughput variant with two bootstraps per
+// CU beyond that (8 coefficients per thread, or 4 in mode 3).  Otherwise, or
+// in mode 1, one work group per bootstrap.  An autotuned shape for the
+// batch's bucket overrides this in mode 0.
 static int batch_bucket(size_t B)
 {
     int b = 0;
@@ -705,8 +709,14 @@ static void br_plan(const pz_ctx *c, size_t B, int *var, int *grid)
         return;
     }
     int v = BR_ONE;
-    if (c->br_per_cu[BR_SPLIT] > 0 && c->br_mode != 1)
-        v = B <= (size_t)c->num_cu * c->br_per_cu[BR_SPLIT_LAT] ? BR_SPLIT_LAT : BR_SPLIT;
+    if (c->br_per_cu[BR_SPLIT] > 0 && c->br_mode != 1) {
+        // automatic: the 4-coefficient latency variant (twice the waves of a
+        // bootstrap, measured faster for one bootstrap per CU), the
+        // 8-coefficient throughput variant (two bootstraps per CU)
+        const int lat = c->br_mode == 2 ? BR_SPLIT_LAT : BR_SPLIT4_LAT;
+        const int thr = c->br_mode == 3 ? BR_SPLIT4 : BR_SPLIT;
+        v = B <= (size_t)c->num_cu * c->br_per_cu[lat] ? lat : thr;
+    }
     const size_t res = (size_t)c->num_cu * (c->br_per_cu[v] > 0 ? c->br_per_cu[v] : 1);
     *var = v;
     *grid = (int)(B < res ? (B > 0 ? B : 1) : res);
@@ -720,8 +730,8 @@ int pz_internal_br_plan(pz_ctx *c, size_t B, int *var, int *grid)
 
 extern "C" int pz_set_blind_rotate_mode(pz_ctx *c, int mode)
 {
-    if (!c || mode < 0 || mode > 2) return fail(PZ_ERR_ARG, "mode must be 0, 1 or 2");
-    if (mode == 2 && c->br_per_cu[BR_SPLIT] == 0)
+    if (!c || mode < 0 || mode > 3) return fail(PZ_ERR_ARG, "mode must be 0, 1, 2 or 3");
+    if (mode >= 2 && c->br_per_cu[BR_SPLIT] == 0)
         return fail(PZ_ERR_UNSUPPORTED, "the split kernel needs d_a = d_b = d_auto = 1");
     if (mode != c->br_mode) {
         HIPCHK(hipSetDevice(c->device));
@@ -782,7 +792,7 @@ extern "C" int pz_autotune(pz_ctx *c, pz_key *k, size_t B_req, uint32_t *h_choic
     KEY_OK(c, k);
     HIPCHK(hipSetDevice(c->device));
     int max_per = 1;
-    for (int v = 0; v < 3; v++) max_per = std::max(max_per, c->br_per_cu[v]);
+    for (int v = 0; v < BR_NVAR; v++) max_per = std::max(max_per, c->br_per_cu[v]);
     const size_t sample = 4 * (size_t)c->num_cu * max_per;
     const size_t B = B_req < sample ? B_req : sample;
     int rc = ensure_scratch(c, B);
@@ -810,7 +820,7 @@ extern "C" int pz_autotune(pz_ctx *c, pz_key *k, size_t B_req, uint32_t *h_choic
               hipMemcpy(d_ops, h_ops.data(), 4 * B, hipMemcpyHostToDevice) == hipSuccess;
     int best_var = -1, best_per = 0;
     float best_ms = 0;
-    for (int v = 0; ok && v < 3; v++) {
+    for (int v = 0; ok && v < BR_NVAR; v++) {
         if (c->br_per_cu[v] <= 0) continue;
         for (int per = 1; ok && per <= c->br_per_cu[v]; per++) {
             const size_t res = (size_t)c->num_cu * per;

@@ -155,28 +155,31 @@ PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *
     }
 }
 
-// The thread's slice of one key row and its Shoup twins (8 words each), as
-// 16-byte loads.
+// The thread's slice of one key row and its Shoup twins (R consecutive words
+// each), as 16-byte loads.
+template <int R>
 struct KeySlice {
-    u64x2 k[4], s[4];
+    u64x2 k[R / 2], s[R / 2];
 };
-PZ_DEV void load_slice(KeySlice &ks, const u64 *__restrict__ K, const u64 *__restrict__ Ks, u32 j)
+template <int R>
+PZ_DEV void load_slice(KeySlice<R> &ks, const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+                       u32 j)
 {
     // global (not flat) loads: flat loads also wait on the LDS counter
     typedef const __attribute__((address_space(1))) u64x2 *gp;
-    const gp k2 = (gp)(K + 8 * j), s2 = (gp)(Ks + 8 * j);
+    const gp k2 = (gp)(K + R * j), s2 = (gp)(Ks + R * j);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < R / 2; i++) {
         ks.k[i] = k2[i];
         ks.s[i] = s2[i];
     }
 }
 // acc[r] += D[r] * K[r] (unreduced, [0,4q) products: shoup_fma<.., QUOT4>)
-template <bool SPLIT, bool REDUCE, bool QUOT4>
-PZ_DEV void fma_slice(u64 (&acc)[8], const KeySlice &ks, const u64 (&D)[8], u64 q, u64 q2)
+template <bool SPLIT, bool REDUCE, bool QUOT4, int R>
+PZ_DEV void fma_slice(u64 (&acc)[R], const KeySlice<R> &ks, const u64 (&D)[R], u64 q, u64 q2)
 {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < R / 2; i++) {
         acc[2 * i] = shoup_fma<SPLIT, REDUCE, QUOT4>(acc[2 * i], ks.k[i].x, ks.s[i].x, D[2 * i], q, q2);
         acc[2 * i + 1] =
             shoup_fma<SPLIT, REDUCE, QUOT4>(acc[2 * i + 1], ks.k[i].y, ks.s[i].y, D[2 * i + 1], q, q2);
@@ -306,20 +309,22 @@ PZ_DEV void auto_blk(u64 (&a)[8], u64 (&b)[8], const u64 *__restrict__ K,
 }
 
 // ---------------------------------------------------------------------------
-// Split work group (2 x N/8 threads) for d_a = d_b = d_auto = 1 (every NI
-// parameter set): half h (wave-uniform) holds polynomial h of the RLWE
-// (0 = a, 1 = b), 8 coefficients per thread in the stride layout, and
-// exchanges through its own LDS region (lds + h*W).  Half the registers of
-// the one-half operators above, twice the waves per bootstrap.
+// Split work group (2 x N/R threads, R = 8 or 4 coefficients per thread) for
+// d_a = d_b = d_auto = 1 (every NI parameter set): half h (wave-uniform)
+// holds polynomial h of the RLWE (0 = a, 1 = b), R coefficients per thread in
+// the stride layout, and exchanges through its own LDS region (lds + h*W).
+// Half the registers of the one-half operators above, twice the waves per
+// bootstrap (R = 4: a quarter of the registers, four times the waves, radix-4
+// register phases).
 // ---------------------------------------------------------------------------
 
 // The work-group barriers of one ntt_fwd without exchange tails, for a half
 // that has no transform to run.
-template <int LOGN>
+template <int LOGN, int R>
 PZ_DEV void ntt_fwd_idle()
 {
 #pragma unroll
-    for (int e = 0; e < ntt_fwd_barriers<LOGN>(); e++) __syncthreads();
+    for (int e = 0; e < ntt_fwd_barriers<LOGN, R>(); e++) __syncthreads();
 }
 
 // One blind-rotation step on a split work group, either
@@ -346,8 +351,8 @@ PZ_DEV void ntt_fwd_idle()
 // stores of an automorphism's permutation need a barrier before them.  The
 // other half reads this half's digit transform, so a barrier separates that
 // read from the inverse's first store.
-template <int LOGN>
-PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+template <int LOGN, int R>
+PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
                        bool is_auto, u64 kexp, bool trivial, const DevCtx &c, u32 j, u32 h,
                        u64 *lds, u64 *stash)
 {
@@ -367,8 +372,8 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
         // uniform, so each loop body is branch-free.
         if (move && (h == 0 || trivial)) {
 #pragma unroll
-            for (int r = 0; r < 8; r++) {
-                u32 x = coef_x<LOGN>(j, r);
+            for (int r = 0; r < R; r++) {
+                u32 x = coef_x<LOGN, R>(j, r);
                 u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
                 const bool pos = to < (u32)N;
                 to = pos ? to : to - N;
@@ -377,8 +382,8 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
             }
         } else if (move) {
 #pragma unroll
-            for (int r = 0; r < 8; r++) {
-                u32 x = coef_x<LOGN>(j, r);
+            for (int r = 0; r < R; r++) {
+                u32 x = coef_x<LOGN, R>(j, r);
                 u32 to = (u32)(((u64)x * kexp) & (2 * N - 1));
                 const bool pos = to < (u32)N;
                 to = pos ? to : to - N;
@@ -390,23 +395,23 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
         if (trivial) {  // uniform over the work group
             if (move) {
 #pragma unroll
-                for (int r = 0; r < 8; r++) p[r] = perm[lds_ix(coef_x<LOGN>(j, r))];
+                for (int r = 0; r < R; r++) p[r] = perm[lds_ix(coef_x<LOGN, R>(j, r))];
             }
             __syncthreads();
             return;
         }
         if (h == 0) {
 #pragma unroll
-            for (int r = 0; r < 8; r++) p[r] = mine[lds_ix(coef_x<LOGN>(j, r))];
+            for (int r = 0; r < R; r++) p[r] = mine[lds_ix(coef_x<LOGN, R>(j, r))];
         }
         __syncthreads();  // read before the transform rewrites the region
     }
     // half 1 of an automorphism has no transform (its digits are not used)
     const bool work = !is_auto || h == 0;
     const Decomp dc = is_auto ? c.dec_auto : (h ? c.dec_b : c.dec_a);
-    u64 D[1][8];
+    u64 D[1][R];
 #pragma unroll
-    for (int r = 0; r < 8; r++) D[0][r] = decomp_digit0(dc, p[r]);
+    for (int r = 0; r < R; r++) D[0][r] = decomp_digit0(dc, p[r]);
     // product rows: RLWE x RGSW -- own digits with the own-part row (half 0:
     // A(-sm), row 0; half 1: B(m), row 3), then the other half's digits with
     // the other row (A(m), row 2; B(-sm), row 1); automorphism -- half 0 row
@@ -418,18 +423,18 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     // 4q (8q with the [0,4q) products), the fast inverse's input bound
     constexpr bool Q4 = PZ_QUOT4 && PZ_LAZY_MAC && PZ_INV_FAST;
     if (work) {
-        ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, 8, false>(D, j, mine,
+        ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(D, j, mine,
                                                                                   c.tb);
 #pragma unroll
-        for (int r = 0; r < 8; r++) mine[lds_ix(eval_x<LOGN>(j, r))] = D[0][r];
+        for (int r = 0; r < R; r++) mine[lds_ix(eval_x<LOGN, R>(j, r))] = D[0][r];
     } else {
-        ntt_fwd_idle<LOGN>();
+        ntt_fwd_idle<LOGN, R>();
     }
-    u64 acc[1][8];
+    u64 acc[1][R];
 #pragma unroll
-    for (int r = 0; r < 8; r++) acc[0][r] = 0;
+    for (int r = 0; r < R; r++) acc[0][r] = 0;
 #if PZ_KEY_EARLY
-    KeySlice k1, k2;
+    KeySlice<R> k1, k2;
     if (work) load_slice(k1, K + r1, Ks + r1, j);
     __syncthreads();
     if (work) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k1, D[0], q, q2);
@@ -437,7 +442,7 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     if (second) load_slice(k2, K + r2, Ks + r2, j);
     if (second) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
+        for (int r = 0; r < R; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN, R>(j, r))];
     }
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (second) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k2, D[0], q, q2);
@@ -448,24 +453,24 @@ PZ_DEV void step_split(u64 (&p)[8], const u64 *__restrict__ K, const u64 *__rest
     __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
     if (second) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN>(j, r))];
+        for (int r = 0; r < R; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN, R>(j, r))];
     }
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (second)
         fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
 #endif
-    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2), 8,
+    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2), R,
             false>(acc, j, mine, c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const u64 bp = stash[lds_ix(coef_x<LOGN>(j, r))];
+        for (int r = 0; r < R; r++) {
+            const u64 bp = stash[lds_ix(coef_x<LOGN, R>(j, r))];
             const u64 v = bp & ~SIGN;
             p[r] = (bp & SIGN) ? sub_mod(acc[0][r], v, q) : add_mod(acc[0][r], v, q);
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < 8; r++) p[r] = acc[0][r];
+        for (int r = 0; r < R; r++) p[r] = acc[0][r];
     }
     // the stash is rewritten only by a following automorphism, after its
     // leading barrier

@@ -237,16 +237,16 @@ PZ_DEV void inv_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
 // just loaded, and a transform that follows stores its first layout where the
 // last one was loaded); any other writer must synchronise first.
 //
-// Wave-local exchanges.  In a phase of stride ST <= 64 (S >= LOGN - 9, eight
-// coefficients per thread), wave w of a polynomial's N/8 threads holds exactly
-// the words [512 w, 512 (w + 1)).  An exchange between two such phases stays
-// within each wave, and LDS operations of one wave execute in order, so with
+// Wave-local exchanges.  In a phase of stride ST = N >> (S + log2 R) <= 64,
+// wave w of a polynomial's N/R threads holds exactly the words
+// [64 R w, 64 R (w + 1)).  An exchange between two such phases stays within
+// each wave, and LDS operations of one wave execute in order, so with
 // TAIL = false it needs no work-group barrier: a wave barrier keeps the
 // compiler from moving the loads above the stores.
 template <int LOGN, int SA, int SB, int R = 8>
 constexpr bool wave_local_exchange()
 {
-    return R == 8 && (SA < SB ? SA : SB) >= LOGN - 9;
+    return (SA < SB ? SA : SB) >= LOGN - 6 - RadixLog<R>::value;
 }
 
 PZ_DEV void wave_sync()
@@ -324,7 +324,7 @@ constexpr int ntt_fwd_barriers()
     constexpr int LR = RadixLog<R>::value;
     int n = 0;
     for (int s0 = 0; s0 + LR < LOGN; s0 += LR)
-        if (!(R == 8 && s0 >= LOGN - 9)) n++;
+        if (s0 < LOGN - 6 - LR) n++;
     return n;
 }
 

@@ -21,6 +21,14 @@
 #ifndef PZ_BR_SPLIT_WAVES
 #define PZ_BR_SPLIT_WAVES 4
 #endif
+// the same for the split kernel with 4 coefficients per thread (twice the
+// waves per bootstrap)
+#ifndef PZ_BR_SPLIT4_WAVES_LAT
+#define PZ_BR_SPLIT4_WAVES_LAT 4
+#endif
+#ifndef PZ_BR_SPLIT4_WAVES
+#define PZ_BR_SPLIT4_WAVES 8
+#endif
 #define PZ_BR_ATTR __attribute__((amdgpu_waves_per_eu(PZ_BR_WAVES, PZ_BR_WAVES)))
 
 namespace pz {
@@ -148,20 +156,33 @@ void launch_rgsw_rows(int logn, u64 *rgsw0, u32 r0, u32 da0, u32 db0, size_t B, 
 // VAR 0 (BR_ONE): N/8 threads hold a and b (16 coefficients each, any d).
 // VAR 1, 2 (BR_SPLIT_LAT, BR_SPLIT): 2 x N/8 threads, half h holds
 // polynomial h (d = 1 only); register budgets for 2 and 4 waves per SIMD.
+// VAR 3, 4 (BR_SPLIT4_LAT, BR_SPLIT4): 2 x N/4 threads, 4 coefficients per
+// thread (radix-4 register phases); budgets for 4 and 8 waves per SIMD.
 // LDS: 2 polynomials (exchange / permutation; SPLIT: one per half), SPLIT: a
 // third for the permuted b of an automorphism, bucket starts + index list.
 // ---------------------------------------------------------------------------
+template <int VAR>
+constexpr int br_waves()
+{
+    return VAR == BR_SPLIT ? PZ_BR_SPLIT_WAVES
+         : VAR == BR_SPLIT_LAT ? PZ_BR_SPLIT_WAVES_LAT
+         : VAR == BR_SPLIT4 ? PZ_BR_SPLIT4_WAVES
+         : VAR == BR_SPLIT4_LAT ? PZ_BR_SPLIT4_WAVES_LAT : PZ_BR_WAVES;
+}
+template <int VAR>
+constexpr int br_coefs() { return VAR == BR_SPLIT4 || VAR == BR_SPLIT4_LAT ? 4 : 8; }
+
 template <int LOGN, int VAR>
-__global__ __launch_bounds__((VAR ? 2 : 1) << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu(
-    VAR == 2 ? PZ_BR_SPLIT_WAVES : (VAR == 1 ? PZ_BR_SPLIT_WAVES_LAT : PZ_BR_WAVES),
-    VAR == 2 ? PZ_BR_SPLIT_WAVES : (VAR == 1 ? PZ_BR_SPLIT_WAVES_LAT : PZ_BR_WAVES)))) void
+__global__ __launch_bounds__((VAR ? 2 : 1) * (1 << LOGN) / br_coefs<VAR>())
+    __attribute__((amdgpu_waves_per_eu(br_waves<VAR>(), br_waves<VAR>()))) void
 k_blind_rotate(
     const u32 *ks, const u32 *gate, size_t B, u64 *out, u32 *stats, DevCtx c, DevKey key0,
     const DevKey *keys, const u32 *key_idx, const u32 *out_idx)
 {
     constexpr bool SPLIT = VAR != BR_ONE;
+    constexpr int R = br_coefs<VAR>();  // coefficients per thread and polynomial
     constexpr int N = 1 << LOGN;
-    constexpr int TPB = N / 8;
+    constexpr int TPB = N / R;
     constexpr int THREADS = SPLIT ? 2 * TPB : TPB;
     constexpr int W = Geom<LOGN>::LDS_WORDS;
     extern __shared__ u64 lds[];
@@ -240,10 +261,10 @@ k_blind_rotate(
         // gate ids outside 0..5 are invalid input; clamp so memory stays in bounds
         const u32 gid = gate[bb] < 6 ? gate[bb] : 0;
         const u64 *tv = c.test_vec + (size_t)gid * nb;
-        u64 tvr[8];
+        u64 tvr[R];
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            u32 x = coef_x<LOGN>(j, r);
+        for (int r = 0; r < R; r++) {
+            u32 x = coef_x<LOGN, R>(j, r);
             u64 val = 0;
             if (x % c.embedding == 0) {
                 u32 y = x / c.embedding;
@@ -261,9 +282,9 @@ k_blind_rotate(
             tvr[r] = val;
         }
         // accumulator RLWE (0, test vector): split halves hold a or b in p
-        u64 a[8], b[8];
+        u64 a[R], b[R];
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
+        for (int r = 0; r < R; r++) {
             a[r] = SPLIT ? (h ? tvr[r] : 0) : 0;  // SPLIT: a[] is this half's polynomial
             b[r] = tvr[r];
         }
@@ -279,15 +300,16 @@ k_blind_rotate(
             const u32 arg = op & (SCHED_AUTO - 1);
             if constexpr (SPLIT) {
                 const size_t off = is_auto ? arg * as : arg * rs;
-                step_split<LOGN>(a, (is_auto ? key.auto_k : key.rgsw) + off,
+                step_split<LOGN, R>(a, (is_auto ? key.auto_k : key.rgsw) + off,
                                  (is_auto ? key.auto_s : key.rgsw_s) + off, is_auto,
                                  c.auto_el[is_auto ? arg : 0], trivial, c, j, h, lds, stash);
-            } else if (is_auto) {
-                auto_blk<LOGN>(a, b, key.auto_k + arg * as, key.auto_s + arg * as, c.auto_el[arg],
-                               trivial, c, j, lds);
-            } else {
-                ext_product_blk<LOGN>(a, b, key.rgsw + arg * rs, key.rgsw_s + arg * rs, trivial, c,
-                                      j, lds);
+            } else if constexpr (R == 8) {
+                if (is_auto)
+                    auto_blk<LOGN>(a, b, key.auto_k + arg * as, key.auto_s + arg * as,
+                                   c.auto_el[arg], trivial, c, j, lds);
+                else
+                    ext_product_blk<LOGN>(a, b, key.rgsw + arg * rs, key.rgsw_s + arg * rs,
+                                          trivial, c, j, lds);
             }
             n_auto += is_auto;
             trivial = trivial && is_auto;
@@ -298,8 +320,8 @@ k_blind_rotate(
         u64 *o = out + (out_idx ? (size_t)out_idx[bb] : bb) * (size_t)(N + 1);
         const bool has_a = !SPLIT || h == 0, has_b = !SPLIT || h == 1;
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            u32 x = coef_x<LOGN>(j, r);
+        for (int r = 0; r < R; r++) {
+            u32 x = coef_x<LOGN, R>(j, r);
             const u64 av = a[r], bv = SPLIT ? a[r] : b[r];
             if (x == 0) {
                 if (has_a) o[1] = av;
@@ -351,7 +373,10 @@ size_t br_lds_bytes(int logn, bool split, u32 br_q, u32 n_lwe)
 // The split work group serves d_a = d_b = d_auto = 1 (every NI set).
 bool br_split(const DevCtx &c) { return PZ_BR_SPLIT && c.da == 1 && c.db == 1 && c.dauto == 1; }
 
-int br_threads(int logn, int var) { return (var ? 2 : 1) << (logn - 3); }
+int br_threads(int logn, int var)
+{
+    return (var ? 2 : 1) << (logn - (var == BR_SPLIT4 || var == BR_SPLIT4_LAT ? 2 : 3));
+}
 
 template <int LOGN, int VAR>
 static const void *br_kernel_ptr()
@@ -359,17 +384,23 @@ static const void *br_kernel_ptr()
     return reinterpret_cast<const void *>(&k_blind_rotate<LOGN, VAR>);
 }
 
+template <int LOGN>
+static const void *br_kernel_of(int var)
+{
+    switch (var) {
+    case BR_SPLIT: return br_kernel_ptr<LOGN, BR_SPLIT>();
+    case BR_SPLIT_LAT: return br_kernel_ptr<LOGN, BR_SPLIT_LAT>();
+    case BR_SPLIT4: return br_kernel_ptr<LOGN, BR_SPLIT4>();
+    case BR_SPLIT4_LAT: return br_kernel_ptr<LOGN, BR_SPLIT4_LAT>();
+    default: return br_kernel_ptr<LOGN, BR_ONE>();
+    }
+}
+
 const void *blind_rotate_kernel(int logn, int var)
 {
     switch (logn) {
-    case 9:
-        return var == BR_SPLIT ? br_kernel_ptr<9, BR_SPLIT>()
-                               : (var == BR_SPLIT_LAT ? br_kernel_ptr<9, BR_SPLIT_LAT>()
-                                                      : br_kernel_ptr<9, BR_ONE>());
-    case 11:
-        return var == BR_SPLIT ? br_kernel_ptr<11, BR_SPLIT>()
-                               : (var == BR_SPLIT_LAT ? br_kernel_ptr<11, BR_SPLIT_LAT>()
-                                                      : br_kernel_ptr<11, BR_ONE>());
+    case 9: return br_kernel_of<9>(var);
+    case 11: return br_kernel_of<11>(var);
     }
     return nullptr;
 }
@@ -386,18 +417,19 @@ void launch_blind_rotate(int logn, int var, int grid, const u32 *ks, const u32 *
 #define PZ_BR_LAUNCH(L, V)                                                                       \
     hipLaunchKernelGGL((k_blind_rotate<L, V>), grid, tpb, lds, s, ks, gate, B, out, stats, c, k, \
                        keys, key_idx, out_idx)
-    switch (logn) {
-    case 9:
-        if (var == BR_SPLIT) PZ_BR_LAUNCH(9, BR_SPLIT);
-        else if (var == BR_SPLIT_LAT) PZ_BR_LAUNCH(9, BR_SPLIT_LAT);
-        else PZ_BR_LAUNCH(9, BR_ONE);
-        break;
-    case 11:
-        if (var == BR_SPLIT) PZ_BR_LAUNCH(11, BR_SPLIT);
-        else if (var == BR_SPLIT_LAT) PZ_BR_LAUNCH(11, BR_SPLIT_LAT);
-        else PZ_BR_LAUNCH(11, BR_ONE);
-        break;
+#define PZ_BR_LAUNCH_ALL(L)                                                                      \
+    switch (var) {                                                                               \
+    case BR_SPLIT: PZ_BR_LAUNCH(L, BR_SPLIT); break;                                             \
+    case BR_SPLIT_LAT: PZ_BR_LAUNCH(L, BR_SPLIT_LAT); break;                                     \
+    case BR_SPLIT4: PZ_BR_LAUNCH(L, BR_SPLIT4); break;                                           \
+    case BR_SPLIT4_LAT: PZ_BR_LAUNCH(L, BR_SPLIT4_LAT); break;                                   \
+    default: PZ_BR_LAUNCH(L, BR_ONE); break;                                                     \
     }
+    switch (logn) {
+    case 9: PZ_BR_LAUNCH_ALL(9); break;
+    case 11: PZ_BR_LAUNCH_ALL(11); break;
+    }
+#undef PZ_BR_LAUNCH_ALL
 #undef PZ_BR_LAUNCH
 }
 

@@ -599,7 +599,8 @@ class Context:
         """Pick the fastest blind-rotation launch shape for batches like B."""
         out = np.zeros(3, dtype=np.uint32)
         _check(self.L.pz_autotune(self.h, key.h, B, out.ctypes.data))
-        names = {0: "one_half", 1: "split_latency", 2: "split"}
+        names = {0: "one_half", 1: "split_latency", 2: "split", 3: "split4_latency",
+                 4: "split4"}
         return {"variant": names[int(out[0])], "groups_per_cu": int(out[1]),
                 "ms_per_batch": int(out[2]) / 1e3}
 
@@ -633,7 +634,8 @@ class Context:
         _check(self.L.pz_set_timing(self.h, 1 if on else 0))
 
     def set_blind_rotate_mode(self, mode):
-        """0 automatic, 1 one work group per bootstrap, 2 two half groups."""
+        """0 automatic, 1 one work group per bootstrap, 2 two half groups,
+        3 two half groups with 4 coefficients per thread."""
         _check(self.L.pz_set_blind_rotate_mode(self.h, mode))
 
     def valu_probe(self, kind, iters=4096):

@@ -78,7 +78,8 @@ def test_ext_product_batch_4096_uniform_rgsw(ctx):
 PLAIN = {0: lambda a, b: 1 - (a & b), 4: lambda a, b: a ^ b}
 
 
-@pytest.mark.parametrize("B,mode", [(1024, 0), (4096, 0), (1024, 1), (256, 0)])
+@pytest.mark.parametrize("B,mode", [(1024, 0), (4096, 0), (1024, 1), (256, 0), (4096, 3),
+                                    (256, 3)])
 def test_gates_full_batch(ctx, B, mode):
     o, s, z, key = keys_for("NI_2P")
     k = ctx.key_upload(key.rgsw, key.auto_key, key.lwe_ksk)
@@ -123,7 +124,8 @@ def test_autotune_keeps_outputs(ctx, B):
     ctx.gates(k, dops, d0, d1, dout, B)
     before = dout.download()
     choice = ctx.autotune(k, B)
-    assert choice["variant"] in ("one_half", "split_latency", "split")
+    assert choice["variant"] in ("one_half", "split_latency", "split", "split4_latency",
+                                 "split4")
     assert choice["groups_per_cu"] >= 1 and choice["ms_per_batch"] > 0
     ctx.gates(k, dops, d0, d1, dout, B)
     assert np.array_equal(dout.download(), before)

@@ -16,8 +16,9 @@ pytestmark = pytest.mark.gpu
 # NI_4P_HB_FR: br_q = 2048 (embedding factor 2); NI_8P: q_ks = 2^17 (32-bit KSK)
 PARAMS = ["NI_2P", "SP_TEST", "NI_4P_HB_FR", "NI_8P"]
 # blind-rotation kernels: 1 = one work group per bootstrap, 2 = two half
-# groups (d = 1 sets only); both must give the same words
-MODES = [(p, m) for p in PARAMS for m in ((1,) if p == "SP_TEST" else (1, 2))]
+# groups, 3 = two half groups with 4 coefficients per thread (2 and 3: d = 1
+# sets only); all must give the same words
+MODES = [(p, m) for p in PARAMS for m in ((1,) if p == "SP_TEST" else (1, 2, 3))]
 _ctx = {}
 _dkeys = {}
 
@@ -236,6 +237,7 @@ def test_gates_truth_and_parity(pname, mode):
 def test_split_mode_needs_single_digit_decompositions():
     import pzlib
     c = ctx_for("SP_TEST")
-    with pytest.raises(pzlib.PzError):
-        c.set_blind_rotate_mode(2)
+    for mode in (2, 3):
+        with pytest.raises(pzlib.PzError):
+            c.set_blind_rotate_mode(mode)
     c.set_blind_rotate_mode(0)
