@@ -127,24 +127,65 @@ def fheuint8_circuits(args):
             "ops_per_sec": 3 * args.u8_count / secs, "all_correct": bool(ok)}
 
 
+def plain_rate(ctx, key, s, steps):
+    """One NAND PBS per launch, synchronised after each: bootstraps/s, and
+    whether the output decrypts to NAND(1, 1) = 0."""
+    N = ctx.p.rlwe_n
+    c0 = ctx.encrypt_bools(s, np.array([1], dtype=np.uint8), 9)
+    c1 = ctx.encrypt_bools(s, np.array([1], dtype=np.uint8), 10)
+    d0p, d1p = ctx.to_device(c0), ctx.to_device(c1)
+    dopsp = ctx.to_device(np.zeros(1, dtype=np.uint32))
+    doutp = ctx.array((1, N + 1))
+    ctx.gates(key, dopsp, d0p, d1p, doutp, 1)
+    ctx.sync()
+    tp0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.gates(key, dopsp, d0p, d1p, doutp, 1)
+        ctx.sync()
+    rate = steps / (time.perf_counter() - tp0)
+    return rate, int(ctx.decrypt_bools(s, doutp.download())[0]) == 0
+
+
+SERVING_SETS = (("NI_2P", 2), ("NI_4P_HB_FR", 4))  # 2-party and 4-party aggregated keys
+
+
+def serving_leg(args):
+    """The serving measurement alone; the serial baseline is the one-PBS rate
+    measured here on an NI_2P key."""
+    import pzlib
+    ctx = pzlib.Context("NI_2P")
+    s, z, key = keys(ctx, 2)
+    rate, _ = plain_rate(ctx, key, s, args.plain_steps)
+    key.free()
+    ctx.close()
+    return serving(args, rate)
+
+
 def serving(args, plain_rate):
     """Poisson arrivals of FheUint8 programs (1..40 ops of add/sub/mul/div/
-    rem/lt/eq) from 64 NI_2P sessions, then single-multiply requests."""
+    rem/lt/eq) from 64 sessions, alternately with 2-party NI_2P and 4-party
+    NI_4P_HB_FR keys (one server, one batch per parameter set per step), then
+    single-multiply requests."""
     import pzlib
     import pzprog
-    ctx = pzlib.Context("NI_2P")
-    blobs = []
+    ctxs = {name: pzlib.Context(name) for name, _ in SERVING_SETS}
+    blobs = []  # (parameter set, secret, key rows)
     tk = time.perf_counter()
     for i in range(args.distinct_keys):
-        s, z, k = keys(ctx, 2, seed=300 + 2 * i)
-        blobs.append((s, ctx.key_download(k)))
+        name, parties = SERVING_SETS[i % len(SERVING_SETS)]
+        ctx = ctxs[name]
+        s, z, k = keys(ctx, parties, seed=300 + 2 * i)
+        blobs.append((name, s, ctx.key_download(k)))
         k.free()
     log("serving: %d keys generated in %.1f s" % (len(blobs), time.perf_counter() - tk))
     out = {}
     for mode in ("mixed", "mul"):
-        srv = pzlib.Server(ctx, hbm_budget_bytes=args.hbm_budget << 30, pool_rows=1 << 17,
-                           max_batch_pbs=args.max_batch)
-        sids = [srv.add_session(*blobs[i % len(blobs)][1]) for i in range(args.sessions)]
+        srv = pzlib.Server(ctxs[SERVING_SETS[0][0]], hbm_budget_bytes=args.hbm_budget << 30,
+                           pool_rows=1 << 17, max_batch_pbs=args.max_batch)
+        sids = []
+        for i in range(args.sessions):
+            name, _, rows = blobs[i % len(blobs)]
+            sids.append(srv.add_session(*rows, ctx=ctxs[name]))
         r = np.random.default_rng(400 if mode == "mixed" else 401)
         rate = args.rate if mode == "mixed" else args.mul_rate
         count = args.programs if mode == "mixed" else args.mul_requests
@@ -158,7 +199,8 @@ def serving(args, plain_rate):
                 prog = pzprog.Program(r, 1, ops=("mul",))
             circ = pzlib.Circuit(None)
             wires, nu, nb = prog.build(circ)
-            cts = ctx.encrypt_bools(blobs[si % len(blobs)][0], prog.input_bits(), 5000 + 97 * j)
+            name, sec, _ = blobs[si % len(blobs)]
+            cts = ctxs[name].encrypt_bools(sec, prog.input_bits(), 5000 + 97 * j)
             rids.append(srv.submit(sids[si], circ, cts, wires, len(prog.steps), t))
             circ.close()
             progs.append((prog, si))
@@ -171,9 +213,14 @@ def serving(args, plain_rate):
             tm = srv.times(rid)
             lat.append(tm["finished"] - tm["arrival"])
             per_op.append(lat[-1] / tm["ops"])
-            bits = ctx.decrypt_bools(blobs[si % len(blobs)][0], srv.result(rid))
+            name, sec, _ = blobs[si % len(blobs)]
+            bits = ctxs[name].decrypt_bools(sec, srv.result(rid))
             ok &= pzprog.decode(bits, *meta) == prog.plain()
+        sets = {}
+        for i in range(args.sessions):
+            sets[blobs[i % len(blobs)][0]] = sets.get(blobs[i % len(blobs)][0], 0) + 1
         res = {"requests": count, "arrival_rate_per_s": rate, "sessions": args.sessions,
+               "sessions_per_parameter_set": sets,
                "distinct_keys": args.distinct_keys, "total_pbs": st["total_pbs"],
                "launches": st["batches"], "wall_s": st["wall_s"],
                "bootstraps_per_sec": st["total_pbs"] / st["wall_s"],
@@ -186,7 +233,8 @@ def serving(args, plain_rate):
                "all_correct": bool(ok)}
         out[mode] = res
         srv.close()
-    ctx.close()
+    for c in ctxs.values():
+        c.close()
     return out
 
 
@@ -448,7 +496,7 @@ def main():
                     help="keep the built-in launch heuristic (profiling runs)")
     ap.add_argument("--no-extras", action="store_true", help="headline, plain and roofline only")
     ap.add_argument("--leg", choices=["kernel_batch", "input_path", "ni_aggregation",
-                                      "ni_aggregation_ni8p"],
+                                      "ni_aggregation_ni8p", "serving"],
                     help="run one secondary measurement alone (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB read + 256 MiB write elementwise kernel twice "
@@ -460,7 +508,8 @@ def main():
     if args.leg:
         legs = {"kernel_batch": kernel_batch, "input_path": input_path,
                 "ni_aggregation": ni_aggregation,
-                "ni_aggregation_ni8p": lambda: ni_aggregation("NI_8P", 8)}
+                "ni_aggregation_ni8p": lambda: ni_aggregation("NI_8P", 8),
+                "serving": lambda: serving_leg(args)}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
     if args.gpus != 1:
@@ -501,19 +550,8 @@ def main():
     roof_hbm = (key_bytes + ct_bytes) / br_s / 1e9
 
     # plain: one PBS per launch
-    c0 = ctx.encrypt_bools(s, np.array([1], dtype=np.uint8), 9)
-    c1 = ctx.encrypt_bools(s, np.array([1], dtype=np.uint8), 10)
-    d0p, d1p = ctx.to_device(c0), ctx.to_device(c1)
-    dopsp = ctx.to_device(np.zeros(1, dtype=np.uint32))
-    doutp = ctx.array((1, N + 1))
-    ctx.gates(key, dopsp, d0p, d1p, doutp, 1)
-    ctx.sync()
-    tp0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        ctx.gates(key, dopsp, d0p, d1p, doutp, 1)
-        ctx.sync()
-    plain = args.plain_steps / (time.perf_counter() - tp0)
-    ok &= int(ctx.decrypt_bools(s, doutp.download())[0]) == 0
+    plain, ok_plain = plain_rate(ctx, key, s, args.plain_steps)
+    ok &= ok_plain
     log("plain: %.1f bootstraps/s" % plain)
 
     # measured VALU rates: the instruction forms of the Shoup multiply, and
@@ -544,8 +582,8 @@ def main():
         log("ni aggregation NI_8P: %s" % extras["ni_aggregation_ni8p"])
         extras["fheuint8_ni8p"] = fheuint8_circuits(args)
         log("fheuint8: %s" % extras["fheuint8_ni8p"])
-        extras["serving_ni2p"] = serving(args, plain)
-        log("serving: %s" % extras["serving_ni2p"])
+        extras["serving"] = serving(args, plain)
+        log("serving: %s" % extras["serving"])
 
     cpu = None
     if not args.no_cpu:

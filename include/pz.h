@@ -219,8 +219,10 @@ int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t
  * ready gate level of every active request, across sessions and up to
  * max_batch_pbs bootstraps, as one batched launch; requests arriving while
  * others run join at the next step.  Arrival times are seconds from the
- * start of pz_server_run.  All sessions of a server share the context's
- * parameter set. */
+ * start of pz_server_run.  pz_server_add_session registers a session of the
+ * creating context's parameter set; pz_server_add_session_ctx one of another
+ * context's (same device, same ring size, else PZ_ERR_UNSUPPORTED), whose
+ * gates a step batches apart and launches on that context's stream. */
 typedef struct pz_server pz_server;
 typedef struct pz_server_config {
     size_t hbm_budget_bytes; /* resident key bytes allowed             */
@@ -232,6 +234,9 @@ int pz_server_create(pz_ctx *ctx, const pz_server_config *cfg, pz_server **out);
 int pz_server_destroy(pz_server *s);
 int pz_server_add_session(pz_server *s, const uint64_t *h_rgsw, const uint64_t *h_auto,
                           const uint64_t *h_lwe_ksk, uint32_t *session_id);
+int pz_server_add_session_ctx(pz_server *s, pz_ctx *ctx, const uint64_t *h_rgsw,
+                              const uint64_t *h_auto, const uint64_t *h_lwe_ksk,
+                              uint32_t *session_id);
 /* prog: a circuit (may be build-only); h_inputs: one [N+1] ciphertext per
  * input wire in order; out_wires: results to return; n_ops: FheUint8
  * operations in the program (for per-op latency). */

@@ -272,6 +272,7 @@ extern "C" int pz_ctx_destroy(pz_ctx *c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     drop_graphs(c);
+    pz_internal_exec_forget(c);
     hipFree(c->d_tables);
     hipFree(c->d_dlog);
     hipFree(c->d_tv);
@@ -428,6 +429,20 @@ int pz_internal_workspace(pz_ctx *c, size_t bytes, void **out)
     return PZ_OK;
 }
 
+// Device bytes of a server key of this context's parameter set (what
+// pz_key_bytes reports for a key uploaded to it).
+size_t pz_internal_key_footprint(const pz_ctx *c)
+{
+    const pz_params &p = c->p;
+    const size_t N = p.rlwe_n;
+    const size_t rgsw_words = (size_t)p.lwe_n * (2 * (size_t)p.rlrg_da + 2 * (size_t)p.rlrg_db) * N;
+    const size_t auto_words = (size_t)(p.w + 1) * 2 * (size_t)p.auto_d * N;
+    const size_t kb = p.lwe_q <= 65536 ? 2 : 4, ldk = ((size_t)p.lwe_n + 1 + 63) / 64 * 64;
+    return 16 * rgsw_words + 16 * auto_words + kb * N * p.lwe_d * ldk;
+}
+
+int pz_internal_ctx_device(const pz_ctx *c) { return c->device; }
+
 // Build a key from device rows already in the reference layout (evaluation
 // domain RGSW / auto rows, u64 KSK rows [N*d][n+1]): computes the Shoup twins
 // and the packed KSK.  Takes ownership of rgsw and auto_k on success.
@@ -475,6 +490,7 @@ int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows
     k->dk.ksk_bytes = kb;
     k->dk.ldk = ldk;
     k->bytes = 16 * rgsw_words + 16 * auto_words + kb * ksk_dev;
+    // pz_internal_key_footprint states the same sum
     *out = k;
     return PZ_OK;
 }

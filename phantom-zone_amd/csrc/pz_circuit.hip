@@ -9,6 +9,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -94,7 +96,38 @@ struct Staging {
     }
 };
 
-static thread_local Staging g_stage;
+// One staging area per context: batches of different contexts (a server
+// serving several parameter sets) run on their own streams at the same time,
+// and each waits only for its own stream before rewriting its pinned buffer.
+static std::mutex g_stage_mu;
+static std::map<const pz_ctx *, Staging *> g_stages;
+
+static Staging &stage_of(const pz_ctx *c)
+{
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    Staging *&st = g_stages[c];
+    if (!st) st = new Staging();
+    return *st;
+}
+
+}  // namespace pz
+
+void pz_internal_exec_forget(const pz_ctx *c)
+{
+    using pz::g_stage_mu;
+    using pz::g_stages;
+    pz::Staging *st = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        auto it = g_stages.find(c);
+        if (it == g_stages.end()) return;
+        st = it->second;
+        g_stages.erase(it);
+    }
+    delete st;  // the context's device is current (pz_ctx_destroy)
+}
+
+namespace pz {
 
 int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<GateRec> &recs,
                u64 *pool, hipStream_t st)
@@ -108,6 +141,7 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
     DevCtx dc;
     int rc = pz_internal_exec_view(c, B ? B : 1, &v, &dc);
     if (rc) return rc;
+    Staging &g_stage = stage_of(c);
     const u32 N1 = v.p.rlwe_n + 1, n1 = v.p.lwe_n + 1;
     // staging: ops | in0 | in1 | out | key | not_in | not_out | KS work list
     const size_t nn = nots.size();

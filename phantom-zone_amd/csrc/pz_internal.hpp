@@ -80,4 +80,8 @@ int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out)
 // same device and parameter set: keys made under one serve the other
 bool pz_internal_ctx_compatible(const pz_ctx *a, const pz_ctx *b);
 bool pz_internal_key_fits(const pz_ctx *c, const pz_key *k);
+size_t pz_internal_key_footprint(const pz_ctx *c);
+int pz_internal_ctx_device(const pz_ctx *c);
+// releases the batch executor's staging buffers of a context being destroyed
+void pz_internal_exec_forget(const pz_ctx *c);
 

@@ -9,8 +9,12 @@
 // next ready level of every active request -- across sessions, up to a
 // bootstrap budget per launch -- and runs them as ONE batch (per-ciphertext
 // keys), so requests that arrive while others are on the GPU join the next
-// step (continuous batching).  The reference has no server; its evaluation
-// is one gate at a time on one thread (src/bool/evaluator.rs:2136-2323).
+// step (continuous batching).  A server may hold sessions of several
+// parameter sets (one context each, same device and ring size, one wire
+// pool): a step forms one batch per context and launches them on the
+// contexts' own streams, where they run concurrently.  The reference has no
+// server; its evaluation is one gate at a time on one thread
+// (src/bool/evaluator.rs:2136-2323).
 #include <string.h>
 
 #include <algorithm>
@@ -33,6 +37,7 @@ namespace {
 
 struct Session {
     const uint64_t *h_rgsw, *h_auto, *h_ksk;
+    u32 ctx = 0;  // index into pz_server::ctxs
     pz_key *key = nullptr;
     size_t bytes = 0;
     int active = 0;
@@ -87,7 +92,7 @@ struct RowAlloc {
 }  // namespace
 
 struct pz_server {
-    pz_ctx *ctx = nullptr;
+    std::vector<pz_ctx *> ctxs;  // parameter sets served (ctxs[0]: the creating context)
     pz_server_config cfg;
     std::vector<Session> sessions;
     std::vector<Request> reqs;
@@ -106,7 +111,7 @@ extern "C" int pz_server_create(pz_ctx *ctx, const pz_server_config *cfg, pz_ser
     if (!ctx || !cfg || !out) return PZ_ERR_ARG;
     if (cfg->pool_rows == 0 || cfg->max_batch_pbs == 0) return PZ_ERR_ARG;
     pz_server *s = new pz_server();
-    s->ctx = ctx;
+    s->ctxs.push_back(ctx);
     s->cfg = *cfg;
     pz_exec_view v;
     DevCtx dc;
@@ -135,40 +140,58 @@ extern "C" int pz_server_destroy(pz_server *s)
     return PZ_OK;
 }
 
-extern "C" int pz_server_add_session(pz_server *s, const uint64_t *h_rgsw, const uint64_t *h_auto,
-                                     const uint64_t *h_ksk, uint32_t *sid)
+// Registers a session of context index ci; its key goes resident now while
+// the resident keys fit the budget without evicting anyone, else on admission.
+static int add_session_on(pz_server *s, u32 ci, const uint64_t *h_rgsw, const uint64_t *h_auto,
+                          const uint64_t *h_ksk, uint32_t *sid)
 {
-    if (!s || !h_rgsw || !h_auto || !h_ksk || !sid) return PZ_ERR_ARG;
     Session se;
     se.h_rgsw = h_rgsw;
     se.h_auto = h_auto;
     se.h_ksk = h_ksk;
+    se.ctx = ci;
     s->sessions.push_back(se);
     *sid = (uint32_t)(s->sessions.size() - 1);
-    // keys go resident at session creation while the budget holds them
-    // without evicting anyone; the rest are uploaded on admission
-    size_t need = 0;
-    for (auto &o : s->sessions)
-        if (o.key) {
-            need = o.bytes;
-            break;
-        }
-    if (s->resident_bytes + need <= s->cfg.hbm_budget_bytes) {
+    pz_ctx *ctx = s->ctxs[ci];
+    if (s->resident_bytes + pz_internal_key_footprint(ctx) <= s->cfg.hbm_budget_bytes) {
         Session &ns = s->sessions.back();
-        if (pz_key_upload(s->ctx, h_rgsw, h_auto, h_ksk, &ns.key) != PZ_OK) {
+        if (pz_key_upload(ctx, h_rgsw, h_auto, h_ksk, &ns.key) != PZ_OK) {
             ns.key = nullptr;
             return PZ_OK;  // retried on admission
         }
         pz_key_bytes(ns.key, &ns.bytes);
-        if (s->resident_bytes + ns.bytes > s->cfg.hbm_budget_bytes) {
-            pz_key_free(ns.key);
-            ns.key = nullptr;
-            return PZ_OK;
-        }
         s->resident_bytes += ns.bytes;
         s->key_uploads++;
     }
     return PZ_OK;
+}
+
+extern "C" int pz_server_add_session(pz_server *s, const uint64_t *h_rgsw, const uint64_t *h_auto,
+                                     const uint64_t *h_ksk, uint32_t *sid)
+{
+    if (!s || !h_rgsw || !h_auto || !h_ksk || !sid) return PZ_ERR_ARG;
+    return add_session_on(s, 0, h_rgsw, h_auto, h_ksk, sid);
+}
+
+extern "C" int pz_server_add_session_ctx(pz_server *s, pz_ctx *ctx, const uint64_t *h_rgsw,
+                                         const uint64_t *h_auto, const uint64_t *h_ksk,
+                                         uint32_t *sid)
+{
+    if (!s || !ctx || !h_rgsw || !h_auto || !h_ksk || !sid) return PZ_ERR_ARG;
+    u32 ci = 0;
+    while (ci < s->ctxs.size() && s->ctxs[ci] != ctx) ci++;
+    if (ci == s->ctxs.size()) {
+        // another parameter set: same device, same ring size (pool rows)
+        pz_exec_view v;
+        DevCtx dc;
+        int rc = pz_internal_exec_view(ctx, 1, &v, &dc);
+        if (rc) return rc;
+        if (pz_internal_ctx_device(ctx) != pz_internal_ctx_device(s->ctxs[0]) ||
+            v.p.rlwe_n + 1 != s->N1)
+            return PZ_ERR_UNSUPPORTED;
+        s->ctxs.push_back(ctx);
+    }
+    return add_session_on(s, ci, h_rgsw, h_auto, h_ksk, sid);
 }
 
 extern "C" int pz_server_submit(pz_server *s, uint32_t session, pz_circuit *prog,
@@ -223,14 +246,8 @@ static bool make_resident(pz_server *s, uint32_t sid)
 {
     Session &se = s->sessions[sid];
     if (se.key) return true;
-    // footprint of one key (same parameters for every session of a context)
-    size_t need = 0;
-    for (auto &o : s->sessions)
-        if (o.key) {
-            need = o.bytes;
-            break;
-        }
-    while (need && s->resident_bytes + need > s->cfg.hbm_budget_bytes) {
+    const size_t need = pz_internal_key_footprint(s->ctxs[se.ctx]);
+    while (s->resident_bytes > 0 && s->resident_bytes + need > s->cfg.hbm_budget_bytes) {
         // evict the least recently used idle session
         int victim = -1;
         for (size_t i = 0; i < s->sessions.size(); i++) {
@@ -245,7 +262,7 @@ static bool make_resident(pz_server *s, uint32_t sid)
         s->resident_bytes -= v.bytes;
         s->evictions++;
     }
-    if (pz_key_upload(s->ctx, se.h_rgsw, se.h_auto, se.h_ksk, &se.key) != PZ_OK) {
+    if (pz_key_upload(s->ctxs[se.ctx], se.h_rgsw, se.h_auto, se.h_ksk, &se.key) != PZ_OK) {
         se.key = nullptr;
         return false;
     }
@@ -263,9 +280,13 @@ static bool make_resident(pz_server *s, uint32_t sid)
 extern "C" int pz_server_run(pz_server *s)
 {
     if (!s) return PZ_ERR_ARG;
-    void *sv = nullptr;
-    pz_ctx_stream(s->ctx, &sv);
-    hipStream_t st = (hipStream_t)sv;
+    const size_t nc = s->ctxs.size();
+    std::vector<hipStream_t> streams(nc);
+    for (size_t c = 0; c < nc; c++) {
+        void *sv = nullptr;
+        pz_ctx_stream(s->ctxs[c], &sv);
+        streams[c] = (hipStream_t)sv;
+    }
     auto t0 = std::chrono::steady_clock::now();
     auto now = [&]() {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -279,8 +300,6 @@ extern "C" int pz_server_run(pz_server *s)
     std::vector<size_t> active;
     for (auto &r : s->reqs)
         if (r.state == 3) done++;
-    std::vector<u32> in_rows;
-    u32 *d_rows = nullptr;
     while (done < s->reqs.size()) {
         double t = now();
         while (next_arrival < order.size() && s->reqs[order[next_arrival]].arrival <= t) {
@@ -305,11 +324,11 @@ extern "C" int pz_server_run(pz_server *s)
             r.state = 2;
             r.admitted = now();
             s->sessions[r.session].active++;
-            // inputs into their pool rows
+            // inputs into their pool rows, on the stream of the session's context
             for (size_t i = 0; i < r.inputs.size(); i++)
                 (void)hipMemcpyAsync(s->pool + (r.base + r.inputs[i]) * s->N1,
                                      r.h_in.data() + i * s->N1, 8 * s->N1,
-                                     hipMemcpyHostToDevice, st);
+                                     hipMemcpyHostToDevice, streams[s->sessions[r.session].ctx]);
             active.push_back(*it);
             it = pending.erase(it);
         }
@@ -322,14 +341,15 @@ extern "C" int pz_server_run(pz_server *s)
             if (!pending.empty()) return PZ_ERR_STATE;  // cannot admit (budget too small)
             break;
         }
-        // one batch: the next level of every active request, oldest first
-        std::vector<const pz_key *> keys;
-        std::vector<int> key_of(s->sessions.size(), -1);
-        std::vector<GateRec> recs;
-        std::vector<size_t> stepped;
-        size_t pbs = 0;
+        // one batch per context: the next level of every active request of
+        // its sessions, oldest first
+        std::vector<std::vector<const pz_key *>> keys(nc);
+        std::vector<int> key_of(s->sessions.size(), -1);  // index in its context's list
+        std::vector<std::vector<GateRec>> recs(nc);
+        std::vector<size_t> stepped, pbs(nc, 0);
         for (size_t idx : active) {
             Request &r = s->reqs[idx];
+            const u32 c = s->sessions[r.session].ctx;
             while (r.next_level < r.levels.size() && r.levels[r.next_level].empty()) r.next_level++;
             if (r.next_level >= r.levels.size()) {
                 stepped.push_back(idx);
@@ -337,10 +357,10 @@ extern "C" int pz_server_run(pz_server *s)
             }
             size_t lp = 0;
             for (u32 n : r.levels[r.next_level]) lp += r.kind[n] == 1;
-            if (pbs > 0 && pbs + lp > s->cfg.max_batch_pbs) continue;
+            if (pbs[c] > 0 && pbs[c] + lp > s->cfg.max_batch_pbs) continue;
             if (key_of[r.session] < 0) {
-                key_of[r.session] = (int)keys.size();
-                keys.push_back(s->sessions[r.session].key);
+                key_of[r.session] = (int)keys[c].size();
+                keys[c].push_back(s->sessions[r.session].key);
             }
             for (u32 n : r.levels[r.next_level]) {
                 GateRec g;
@@ -349,16 +369,21 @@ extern "C" int pz_server_run(pz_server *s)
                 g.in1 = (u32)(r.base + r.b[n]);
                 g.out = (u32)(r.base + n);
                 g.key = (u32)key_of[r.session];
-                recs.push_back(g);
+                recs[c].push_back(g);
             }
-            pbs += lp;
+            pbs[c] += lp;
             stepped.push_back(idx);
         }
-        if (!recs.empty()) {
-            int rc = exec_batch(s->ctx, keys, recs, s->pool, st);
+        // launch every context's batch before waiting for any of them
+        for (size_t c = 0; c < nc; c++) {
+            if (recs[c].empty()) continue;
+            int rc = exec_batch(s->ctxs[c], keys[c], recs[c], s->pool, streams[c]);
             if (rc) return rc;
-            if (hipStreamSynchronize(st) != hipSuccess) return PZ_ERR_HIP;
-            s->total_pbs += pbs;
+        }
+        for (size_t c = 0; c < nc; c++) {
+            if (recs[c].empty()) continue;
+            if (hipStreamSynchronize(streams[c]) != hipSuccess) return PZ_ERR_HIP;
+            s->total_pbs += pbs[c];
             s->batches++;
         }
         s->tick++;
@@ -370,11 +395,12 @@ extern "C" int pz_server_run(pz_server *s)
             if (r.next_level >= r.levels.size()) {
                 // complete: fetch outputs, release rows
                 r.result.resize(r.outs.size() * (size_t)s->N1);
+                hipStream_t rs = streams[s->sessions[r.session].ctx];
                 for (size_t i = 0; i < r.outs.size(); i++)
                     (void)hipMemcpyAsync(r.result.data() + i * s->N1,
                                          s->pool + (r.base + r.outs[i]) * s->N1, 8 * s->N1,
-                                         hipMemcpyDeviceToHost, st);
-                (void)hipStreamSynchronize(st);
+                                         hipMemcpyDeviceToHost, rs);
+                (void)hipStreamSynchronize(rs);
                 r.finished = now();
                 r.state = 3;
                 s->rows.release(r.base, r.rows);
@@ -384,8 +410,6 @@ extern "C" int pz_server_run(pz_server *s)
             }
         }
     }
-    (void)d_rows;
-    (void)in_rows;
     s->wall = now();
     return PZ_OK;
 }

@@ -78,6 +78,7 @@ SIGNATURES = {
     "pz_server_create": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "pz_server_destroy": (C.c_int, [_vp]),
     "pz_server_add_session": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "pz_server_add_session_ctx": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "pz_server_submit": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32, C.c_uint32,
                                    C.c_double, _vp]),
     "pz_server_run": (C.c_int, [_vp]),
@@ -298,12 +299,20 @@ class Server:
         except Exception:
             pass
 
-    def add_session(self, rgsw, auto, ksk):
+    def add_session(self, rgsw, auto, ksk, ctx=None):
+        """A session with this server key; ctx: a context of another parameter
+        set (same device and ring size) whose gates run as their own batches."""
         arrs = [np.ascontiguousarray(x, dtype=np.uint64) for x in (rgsw, auto, ksk)]
         self._keep.append(arrs)
         sid = C.c_uint32()
-        _check(self.L.pz_server_add_session(self.h, arrs[0].ctypes.data, arrs[1].ctypes.data,
-                                            arrs[2].ctypes.data, C.byref(sid)))
+        if ctx is None or ctx is self.ctx:
+            _check(self.L.pz_server_add_session(self.h, arrs[0].ctypes.data, arrs[1].ctypes.data,
+                                                arrs[2].ctypes.data, C.byref(sid)))
+        else:
+            self._keep.append(ctx)
+            _check(self.L.pz_server_add_session_ctx(self.h, ctx.h, arrs[0].ctypes.data,
+                                                    arrs[1].ctypes.data, arrs[2].ctypes.data,
+                                                    C.byref(sid)))
         return sid.value
 
     def submit(self, session, circuit, inputs, out_wires, n_ops, arrival):

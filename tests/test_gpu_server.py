@@ -97,3 +97,63 @@ def test_serving_budget_forces_eviction():
         assert pzprog.decode(bits, *meta[j]) == prog.plain(), j
     srv.close()
     ctx.close()
+
+
+def test_serving_two_parameter_sets():
+    """Sessions of two parameter sets (2-party NI_2P and 4-party NI_4P_HB_FR
+    keys) in one server: each step launches one batch per parameter set on
+    its context's stream; every result decrypts correctly and equals the
+    request served alone by a server of its own parameter set."""
+    import pzlib
+    import pzprog
+    print("server two parameter sets", flush=True)
+    ctxs = {"NI_2P": pzlib.Context("NI_2P"), "NI_4P_HB_FR": pzlib.Context("NI_4P_HB_FR")}
+    parties = {"NI_2P": 2, "NI_4P_HB_FR": 4}
+    sessions = []  # (set name, secret, rgsw, auto, ksk)
+    for i in range(4):
+        name = "NI_2P" if i % 2 == 0 else "NI_4P_HB_FR"
+        c = ctxs[name]
+        s, z = c.keygen_secrets(500 + i, parties[name])
+        k = c.keygen_server(600 + i, s, z)
+        rgsw, auto, ksk = c.key_download(k)
+        k.free()
+        sessions.append((name, s, rgsw, auto, ksk))
+    r = np.random.default_rng(9)
+    progs = [pzprog.Program(r, 1, ops=("add", "lt", "eq")) for _ in range(8)]
+    sess_ids = [j % 4 for j in range(8)]
+    arrivals = [0.001 * j for j in range(8)]
+
+    def serve(server_ctx_name, only=None):
+        srv = pzlib.Server(ctxs[server_ctx_name], hbm_budget_bytes=1 << 34, pool_rows=1 << 13)
+        sid = {}
+        for si, se in enumerate(sessions):
+            if only is None or se[0] == only:
+                sid[si] = srv.add_session(se[2], se[3], se[4], ctx=ctxs[se[0]])
+        rids = {}
+        for j, (prog, si, t) in enumerate(zip(progs, sess_ids, arrivals)):
+            if si not in sid:
+                continue
+            circ = pzlib.Circuit(None)
+            wires, nu, nb = prog.build(circ)
+            cts = ctxs[sessions[si][0]].encrypt_bools(sessions[si][1], prog.input_bits(), 77 + j)
+            rids[j] = (srv.submit(sid[si], circ, cts, wires, len(prog.steps),
+                                  t if only is None else 0.0), nu, nb)
+            circ.close()
+        srv.run()
+        out = {j: (srv.result(rid), nu, nb) for j, (rid, nu, nb) in rids.items()}
+        st = srv.stats()
+        srv.close()
+        return out, st
+
+    mixed, st = serve("NI_2P")
+    assert len(mixed) == 8
+    for j, (ct, nu, nb) in mixed.items():
+        name, s = sessions[sess_ids[j]][0], sessions[sess_ids[j]][1]
+        bits = ctxs[name].decrypt_bools(s, ct)
+        assert pzprog.decode(bits, nu, nb) == progs[j].plain(), j
+    for name in ("NI_2P", "NI_4P_HB_FR"):
+        alone, _ = serve(name, only=name)
+        for j, (ct, _, _) in alone.items():
+            assert np.array_equal(ct, mixed[j][0]), (name, j)
+    for c in ctxs.values():
+        c.close()
