@@ -75,6 +75,11 @@ struct Tables {
 // stage k leaves values below (4k+1)q (< 45q < 2^60 for N = 2048).
 enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3 };
 
+// Unreduced tail of the IB = 8 fast inverse (see inv_stage).
+#ifndef PZ_INV_TAIL4
+#define PZ_INV_TAIL4 1
+#endif
+
 // One forward stage S inside a phase of stride ST. KIND: 0 = 0_to_4q
 // butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q),
 // 3 = unreduced (FWD_FAST), 4 = unreduced with the [0,4q) product (FWD_FAST4).
@@ -132,9 +137,14 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
 // reduces x' = x + y < 2^KR IB q = 2^10 q to [0,2q) with a Shoup multiply by
 // 1 (2^10 q < 2^64 for q < 2^54); later stages are the reference butterfly
 // again.  With IB = 8 the products of stages before KR come from shoup_mul4
-// ([0,4q), within the next stage's bound 2^k IB q); stage KR's are exact, so
-// the reference butterflies still see [0,2q).  Residues are the reference's
-// throughout, and the strict last stage returns the same words.
+// ([0,4q), within the next stage's bound 2^k IB q), and the tail is unreduced
+// as well (PZ_INV_TAIL4): stage KR leaves [0,4q) (shoup_mul4 by 1 for the
+// sum), stage k > KR receives values below 2^(k-KR+1) q and adds that bound
+// in its difference, and only the last stage reduces, its sum with an exact
+// Shoup multiply by 1 and both outputs from [0,2q) to [0,q).  Without the
+// tail stage KR's products are exact and the reference butterflies follow.
+// Residues are the reference's throughout, and the strict last stage returns
+// the same words.
 template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB,
           int R>
 PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
@@ -146,6 +156,7 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
     constexpr int K = LOGN - S;  // executed stage, 1-based
     constexpr int LIB = IB == 8 ? 3 : (IB == 4 ? 2 : 1);  // log2 IB
     constexpr int KR = 10 - LIB;
+    constexpr bool T4 = PZ_INV_TAIL4 && FAST && IB == 8;  // unreduced tail
     static_assert(!FAST || !SCALE, "the fast inverse leaves n^-1 to the key rows");
     static_assert(IB == 2 || ((IB == 4 || IB == 8) && FAST),
                   "inputs below 4q or 8q need the fast inverse");
@@ -162,13 +173,31 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
                 const u64 xv = v[p][r], yv = v[p][r + LD];
                 const u64 sum = xv + yv;
                 const u64 dif = xv + (tb.q << (K - 1 + LIB)) - yv;
-                v[p][r] = K == KR ? shoup_mul<SM>(sum, 1, tb.one_s, tb.q) : sum;
-                v[p][r + LD] = (IB == 8 && K < KR) ? shoup_mul4(dif, w, ws, tb.q)
-                                                   : shoup_mul<SM>(dif, w, ws, tb.q);
+                if constexpr (T4 && K == KR) {
+                    v[p][r] = shoup_mul4(sum, 1, tb.one_s, tb.q);
+                    v[p][r + LD] = shoup_mul4(dif, w, ws, tb.q);
+                } else {
+                    v[p][r] = K == KR ? shoup_mul<SM>(sum, 1, tb.one_s, tb.q) : sum;
+                    v[p][r + LD] = (IB == 8 && K < KR) ? shoup_mul4(dif, w, ws, tb.q)
+                                                       : shoup_mul<SM>(dif, w, ws, tb.q);
+                }
+            } else if constexpr (T4) {
+                const u64 xv = v[p][r], yv = v[p][r + LD];
+                const u64 sum = xv + yv;
+                const u64 dif = xv + (tb.q << (K - KR + 1)) - yv;  // inputs below 2^(K-KR+1) q
+                if constexpr (S == 0) {  // the last stage: exact, then [0,2q) -> [0,q)
+                    const u64 xo = shoup_mul<SM>(sum, 1, tb.one_s, tb.q);
+                    const u64 yo = shoup_mul<SM>(dif, w, ws, tb.q);
+                    v[p][r] = umin(xo, xo - tb.q);
+                    v[p][r + LD] = umin(yo, yo - tb.q);
+                } else {
+                    v[p][r] = sum;
+                    v[p][r + LD] = shoup_mul4(dif, w, ws, tb.q);
+                }
             } else {
                 ib_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
             }
-            if (S == 0 && !SCALE && STRICT) {
+            if (S == 0 && !SCALE && STRICT && !T4) {
                 v[p][r] = umin(v[p][r], v[p][r] - tb.q);
                 v[p][r + LD] = umin(v[p][r + LD], v[p][r + LD] - tb.q);
             } else if (S == 0 && SCALE) {
