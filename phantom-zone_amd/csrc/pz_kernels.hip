@@ -75,22 +75,21 @@ __global__ void k_decompose(const u64 *in, u64 *digits, size_t count, Decomp dc)
 // evaluator.rs:2118-2133), f64 moddown Q -> q_ks (src/pbs.rs:128-132) and
 // the LWE-KS decomposition (src/lwe.rs:31-35) into signed int8 digits.
 // mode: PREP_GATE (c0, c1, ops), PREP_PBS (in), PREP_KSONLY (in is mod q_ks).
+// One work group per tile of KS_CT ciphertexts x PREP_COLS mask columns: the
+// inputs are read along rows (8 consecutive words per ciphertext), the digits
+// are assembled in LDS in the key switch's layout [tile][column][digit][KS_CT]
+// and leave as one contiguous run of PREP_COLS * d * KS_CT bytes.
 // ---------------------------------------------------------------------------
-__global__ void k_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const u32 *idx1,
-                       const u32 *ops, u32 N1, size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out,
-                       DevCtx c)
+constexpr int PREP_COLS = 64;
+constexpr int PREP_MAXD = 17;  // lwe_logb * lwe_d <= log2(q_ks) <= 17
+
+PZ_DEV u64 prep_value(int mode, const u64 *c0, const u64 *c1, size_t r0, size_t r1, u32 op,
+                      u32 col, u32 N1, const DevCtx &c)
 {
-    const size_t bb = blockIdx.x;
-    const u32 col = blockIdx.y * blockDim.x + threadIdx.x;
-    if (bb >= B || col >= N1) return;
-    if (col < c.n_lwe + 1) ks_out[bb * (c.n_lwe + 1) + col] = 0;  // k_ks accumulates
     const u64 q = c.tb.q;
-    // inputs are rows bb of c0/c1, or rows idx0[bb]/idx1[bb] of a wire pool
-    const size_t r0 = idx0 ? idx0[bb] : bb, r1 = idx1 ? idx1[bb] : bb;
     u64 v = c0[r0 * N1 + col];
     if (mode == PREP_GATE) {
-        u64 v1 = c1[r1 * N1 + col];
-        u32 op = ops[bb];
+        const u64 v1 = c1[r1 * N1 + col];
         if (op <= 3) {
             v = add_mod(v, v1, q);
             if (col == 0) v = add_mod(v, c.qby4, q);
@@ -99,28 +98,75 @@ __global__ void k_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, 
             v = add_mod(v, v, q);
         }
     }
-    u64 md = v;
     if (mode != PREP_KSONLY) {
-        double x = ((double)v * c.lwe_qf) / c.rlwe_qf;
-        md = (u64)round(x);
+        const double x = ((double)v * c.lwe_qf) / c.rlwe_qf;
+        v = (u64)round(x);
     }
-    if (col == 0) {
-        ks_b[bb] = (u32)md;
-        return;
-    }
-    // digits layout [B/KS_CT][rows][KS_CT]: the key switch stages a row of
-    // KS_CT ciphertexts with contiguous loads
-    const u32 d = c.lwe_d;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_prep(int mode, const u64 *c0, const u64 *c1,
+                                              const u32 *idx0, const u32 *idx1, const u32 *ops,
+                                              u32 N1, size_t B, int8_t *digits, u32 *ks_b,
+                                              u32 *ks_out, DevCtx c)
+{
+    constexpr int CW = KS_CT / 4;  // u32 words of one digit row of a tile
+    __shared__ u32 sd[PREP_COLS * PREP_MAXD * CW];
+    const u32 tid = threadIdx.x;
+    const size_t b0 = (size_t)blockIdx.x * KS_CT;
+    const u32 cb = blockIdx.y * PREP_COLS;  // first mask column (0-based) of this group
+    const u32 d = c.lwe_d, n1 = c.n_lwe + 1;
     const u64 lq = c.lwe_mask + 1;
-    const size_t rows = (size_t)(N1 - 1) * d;
-    int8_t *out = digits + (bb / KS_CT) * rows * KS_CT + (size_t)(col - 1) * d * KS_CT +
-                  (bb % KS_CT);
-    u64 value = decomp_prepare(c.dec_lwe, md);
-    for (u32 r = 0; r < d; r++) {
-        u64 dg = decomp_next(c.dec_lwe, value);
-        long long sv = dg >= (lq >> 1) ? (long long)dg - (long long)lq : (long long)dg;
-        out[(size_t)r * KS_CT] = (int8_t)sv;
+    // this group's share of the key-switch output, cleared (k_ks accumulates)
+    for (u32 e = tid; e < KS_CT * PREP_COLS; e += blockDim.x) {
+        const size_t bb = b0 + e / PREP_COLS;
+        const u32 col = cb + e % PREP_COLS;
+        if (bb < B && col < n1) ks_out[bb * n1 + col] = 0;
     }
+    // body coefficient (column 0) of each ciphertext
+    if (blockIdx.y == 0 && tid < KS_CT && b0 + tid < B) {
+        const size_t bb = b0 + tid;
+        ks_b[bb] = (u32)prep_value(mode, c0, c1, idx0 ? idx0[bb] : bb, idx1 ? idx1[bb] : bb,
+                                   mode == PREP_GATE ? ops[bb] : 0, 0, N1, c);
+    }
+    // digits: thread (m, cl) takes ciphertexts 4m..4m+3 of the tile in column
+    // 1 + cb + cl and packs their digit bytes; ciphertexts past B give 0
+    const u32 m = tid % CW;
+#pragma unroll
+    for (int half = 0; half < PREP_COLS / 32; half++) {
+        const u32 cl = tid / CW + 32 * half;
+        const u32 col = 1 + cb + cl;
+        u32 w[PREP_MAXD];
+#pragma unroll
+        for (int r = 0; r < PREP_MAXD; r++) w[r] = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const size_t bb = b0 + 4 * m + i;
+            if (bb >= B) continue;
+            const u64 v = prep_value(mode, c0, c1, idx0 ? idx0[bb] : bb, idx1 ? idx1[bb] : bb,
+                                     mode == PREP_GATE ? ops[bb] : 0, col, N1, c);
+            u64 value = decomp_prepare(c.dec_lwe, v);
+#pragma unroll
+            for (int r = 0; r < PREP_MAXD; r++) {
+                if ((u32)r >= d) break;
+                const u64 dg = decomp_next(c.dec_lwe, value);
+                const int sv = dg >= (lq >> 1) ? (int)((long long)dg - (long long)lq) : (int)dg;
+                w[r] |= (u32)(uint8_t)(int8_t)sv << (8 * i);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < PREP_MAXD; r++) {
+            if ((u32)r >= d) break;
+            sd[(cl * d + r) * CW + m] = w[r];
+        }
+    }
+    __syncthreads();
+    // the group's columns are one contiguous run of the tile
+    const size_t rows = (size_t)(N1 - 1) * d;
+    uint4 *dst = reinterpret_cast<uint4 *>(digits + (b0 / KS_CT) * rows * KS_CT +
+                                           (size_t)cb * d * KS_CT);
+    const uint4 *src = reinterpret_cast<const uint4 *>(sd);
+    for (u32 e = tid; e < PREP_COLS * d * CW / 4; e += blockDim.x) dst[e] = src[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -413,7 +459,8 @@ void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const 
                  const DevCtx &c, hipStream_t s)
 {
     if (B == 0) return;
-    dim3 grid((unsigned)B, (N1 + 255) / 256);
+    // (N1 - 1) is a multiple of PREP_COLS for every supported ring size
+    dim3 grid((unsigned)((B + KS_CT - 1) / KS_CT), (N1 - 1) / PREP_COLS);
     hipLaunchKernelGGL(k_prep, grid, 256, 0, s, mode, c0, c1, idx0, idx1, ops, N1, B, digits,
                        ks_b, ks_out, c);
 }
