@@ -222,7 +222,8 @@ int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t
  * start of pz_server_run.  pz_server_add_session registers a session of the
  * creating context's parameter set; pz_server_add_session_ctx one of another
  * context's (same device, same ring size, else PZ_ERR_UNSUPPORTED), whose
- * gates a step batches apart and launches on that context's stream. */
+ * gates a step batches apart and launches on that context's stream.  Every
+ * context a server uses must outlive it. */
 typedef struct pz_server pz_server;
 typedef struct pz_server_config {
     size_t hbm_budget_bytes; /* resident key bytes allowed             */
