@@ -92,6 +92,39 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *
 }
 
 // ---------------------------------------------------------------------------
+// Batched RLWE x RGSW / automorphism on the split work group (d = 1 sets):
+// the blind rotation's step, one RLWE per 2 x N/8 threads; half h loads and
+// stores polynomial h.  LDS: the two halves' regions and the stash.
+// ---------------------------------------------------------------------------
+template <int LOGN>
+__global__ __launch_bounds__(2 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu(
+    PZ_BR_SPLIT_WAVES, PZ_BR_SPLIT_WAVES))) void k_step_split(bool is_auto, u64 *rlwe,
+                                                                const u32 *idx, const u8 *trivial,
+                                                                size_t B, DevCtx c, DevKey key)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int TPB = N / 8;
+    constexpr int W = Geom<LOGN>::LDS_WORDS;
+    extern __shared__ u64 lds[];
+    const u32 tid = threadIdx.x, j = tid % TPB;
+    const u32 h = (u32)__builtin_amdgcn_readfirstlane(tid / TPB);
+    const size_t rs = (size_t)(is_auto ? key.auto_rows : key.rgsw_rows) * N;
+    for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
+        u64 *poly = rlwe + (bb * 2 + h) * N;
+        u64 p[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) p[r] = poly[coef_x<LOGN>(j, r)];
+        const u32 k = (u32)__builtin_amdgcn_readfirstlane((int)idx[bb]);
+        const bool triv = __builtin_amdgcn_readfirstlane((int)trivial[bb]) != 0;
+        step_split<LOGN, 8>(p, (is_auto ? key.auto_k : key.rgsw) + k * rs,
+                            (is_auto ? key.auto_s : key.rgsw_s) + k * rs, is_auto,
+                            c.auto_el[is_auto ? k : 0], triv, c, j, h, lds, lds + 2 * W);
+#pragma unroll
+        for (int r = 0; r < 8; r++) poly[coef_x<LOGN>(j, r)] = p[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // RGSW x RGSW (rgsw_by_rgsw_inplace, src/rgsw/runtime.rs:883-985): each work
 // group takes one RLWE row pair of one RGSW0 (coefficient domain; rows (i,
 // da0 + i) of the -sm part, then (2 da0 + i, 2 da0 + db0 + i) of the m part)
@@ -354,6 +387,18 @@ static void launch_ext_t(bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv
 void launch_ext(int logn, bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv, size_t B,
                 const DevCtx &c, const DevKey &k, hipStream_t s)
 {
+    if (br_split(c)) {  // d = 1: the blind rotation's split step
+        if (B == 0) return;
+        const int grid = B < 65535 ? (int)B : 65535;
+        const size_t lds = 3 * 8 * (size_t)((1 << logn) + (1 << logn) / 8);
+        if (logn == 11)
+            hipLaunchKernelGGL(k_step_split<11>, grid, 2 << 8, lds, s, is_auto, rlwe, idx, triv,
+                               B, c, k);
+        else if (logn == 9)
+            hipLaunchKernelGGL(k_step_split<9>, grid, 2 << 6, lds, s, is_auto, rlwe, idx, triv,
+                               B, c, k);
+        return;
+    }
     switch (logn) {
     case 9: launch_ext_t<9>(is_auto, rlwe, idx, triv, B, c, k, s); break;
     case 11: launch_ext_t<11>(is_auto, rlwe, idx, triv, B, c, k, s); break;
