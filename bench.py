@@ -496,7 +496,7 @@ def main():
                     help="keep the built-in launch heuristic (profiling runs)")
     ap.add_argument("--no-extras", action="store_true", help="headline, plain and roofline only")
     ap.add_argument("--leg", choices=["kernel_batch", "input_path", "ni_aggregation",
-                                      "ni_aggregation_ni8p", "serving"],
+                                      "ni_aggregation_ni8p", "serving", "fheuint8"],
                     help="run one secondary measurement alone (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB read + 256 MiB write elementwise kernel twice "
@@ -509,7 +509,8 @@ def main():
         legs = {"kernel_batch": kernel_batch, "input_path": input_path,
                 "ni_aggregation": ni_aggregation,
                 "ni_aggregation_ni8p": lambda: ni_aggregation("NI_8P", 8),
-                "serving": lambda: serving_leg(args)}
+                "serving": lambda: serving_leg(args),
+                "fheuint8": lambda: fheuint8_circuits(args)}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
     if args.gpus != 1:

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -604,11 +605,85 @@ extern "C" int pz_circuit_reset_flags(pz_circuit *c)
     return PZ_OK;
 }
 
+// Level schedule.  The circuit's depth D (its longest chain of bootstrapped
+// gates) fixes the number of levels; gates off that chain have slack.  Each
+// gate gets its latest level that still keeps depth D (ALAP), and levels are
+// filled in order from the gates whose inputs are done, least slack first:
+// every gate whose latest level has come, then others up to a width of whole
+// rounds of the resident blind-rotation grid (at least the average of what
+// remains per level).  Wide early levels are thereby spread into the late
+// levels that a long chain (the running div-by-zero flag) leaves nearly
+// empty.  NOT gates run in the batch of the gate that produces their input
+// (level 0 for an input wire).  Order does not change any ciphertext: every
+// gate reads the same inputs.
 static void schedule(pz_circuit *c)
 {
     if (c->scheduled) return;
-    c->levels.assign(c->depth + 1, {});
-    for (u32 i = 0; i < c->nodes.size(); i++) {
+    const size_t nn = c->nodes.size();
+    const u32 D = c->depth;
+    u32 cap = 512;  // bootstraps of one resident round (default without a device)
+    if (c->ctx) {
+        uint32_t cu = 0, per = 0;
+        if (pz_ctx_info(c->ctx, nullptr, &cu, &per) == PZ_OK && cu * per > 0) cap = cu * per;
+    }
+    // producer of each wire's value: the bootstrapped gate behind any NOTs, or
+    // none (an input wire)
+    const u32 NONE = 0xffffffffu;
+    std::vector<u32> src(nn, NONE);
+    for (size_t i = 0; i < nn; i++) {
+        const Node &n = c->nodes[i];
+        src[i] = n.kind == 1 ? (u32)i : (n.kind == 2 ? src[n.a] : NONE);
+    }
+    // latest levels (ALAP) at depth D; nodes are in topological order
+    std::vector<u32> alap(nn, D);
+    for (size_t i = nn; i-- > 0;) {
+        const Node &n = c->nodes[i];
+        if (n.kind == 0) continue;
+        const u32 lim = n.kind == 1 ? alap[i] - 1 : alap[i];
+        alap[n.a] = std::min(alap[n.a], lim);
+        if (n.kind == 1) alap[n.b] = std::min(alap[n.b], lim);
+    }
+    // bootstrapped-gate dependency graph
+    std::vector<u32> indeg(nn, 0), level(nn, 0);
+    std::vector<std::vector<u32>> users(nn);
+    size_t remaining = 0;
+    for (size_t i = 0; i < nn; i++) {
+        const Node &n = c->nodes[i];
+        if (n.kind != 1) continue;
+        remaining++;
+        const u32 pa = src[n.a], pb = src[n.b];
+        if (pa != NONE) {
+            indeg[i]++;
+            users[pa].push_back((u32)i);
+        }
+        if (pb != NONE && pb != pa) {
+            indeg[i]++;
+            users[pb].push_back((u32)i);
+        }
+    }
+    typedef std::pair<u32, u32> Item;  // (latest level, gate)
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> ready;
+    for (size_t i = 0; i < nn; i++)
+        if (c->nodes[i].kind == 1 && indeg[i] == 0) ready.push({alap[i], (u32)i});
+    u32 nlev = 0;
+    for (u32 L = 1; remaining > 0 && !ready.empty(); L++) {
+        const size_t left = L <= D ? D - L + 1 : 1;
+        const size_t avg = (remaining + left - 1) / left;
+        const size_t width = (size_t)cap * ((std::max<size_t>(avg, 1) + cap - 1) / cap);
+        std::vector<u32> take;
+        while (!ready.empty() && (take.size() < width || ready.top().first <= L)) {
+            take.push_back(ready.top().second);
+            ready.pop();
+        }
+        for (u32 g : take) level[g] = L;
+        for (u32 g : take)
+            for (u32 u : users[g])
+                if (--indeg[u] == 0) ready.push({alap[u], u});
+        remaining -= take.size();
+        nlev = L;
+    }
+    c->levels.assign((size_t)std::max(nlev, D) + 1, {});
+    for (u32 i = 0; i < nn; i++) {
         const Node &n = c->nodes[i];
         if (n.kind == 0) continue;
         GateRec r;
@@ -617,7 +692,8 @@ static void schedule(pz_circuit *c)
         r.in1 = n.b;
         r.out = i;
         r.key = 0;
-        c->levels[n.level].push_back(r);
+        const u32 l = n.kind == 1 ? level[i] : (src[i] == NONE ? 0 : level[src[i]]);
+        c->levels[l].push_back(r);
     }
     c->scheduled = true;
 }
