@@ -206,6 +206,11 @@ int pz_circuit_reset_flags(pz_circuit *c);
 int pz_circuit_stats(pz_circuit *c, uint32_t *n_wires, uint32_t *n_pbs, uint32_t *depth);
 int pz_circuit_level_sizes(pz_circuit *c, uint32_t *h_sizes, uint32_t cap);
 int pz_circuit_nodes(pz_circuit *c, uint32_t *h_out, size_t cap);
+/* Level of every node in the batch schedule (input wires 0; a NOT shares the
+ * level of the gate producing its input).  Levels: each gate's latest level at
+ * the circuit's depth, filled least slack first up to whole rounds of the
+ * resident blind-rotation grid (512 bootstraps without a device). */
+int pz_circuit_node_levels(pz_circuit *c, uint32_t *h_levels, size_t cap);
 /* d_inputs[i] is the ciphertext of the i-th input wire ([N+1] u64 each) */
 int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inputs, void *stream);
 int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t *d_out,

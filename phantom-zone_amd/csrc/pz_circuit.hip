@@ -698,6 +698,19 @@ static void schedule(pz_circuit *c)
     c->scheduled = true;
 }
 
+// Level of every node in the schedule (input wires 0; a NOT runs in the
+// batch of its input's producer, after its bootstrapped gates).
+extern "C" int pz_circuit_node_levels(pz_circuit *c, uint32_t *h_levels, size_t cap)
+{
+    if (!c || !h_levels) return PZ_ERR_ARG;
+    schedule(c);
+    std::vector<uint32_t> lv(c->nodes.size(), 0);
+    for (size_t l = 0; l < c->levels.size(); l++)
+        for (auto &r : c->levels[l]) lv[r.out] = (uint32_t)l;
+    for (size_t i = 0; i < lv.size() && i < cap; i++) h_levels[i] = lv[i];
+    return PZ_OK;
+}
+
 extern "C" int pz_circuit_stats(pz_circuit *c, uint32_t *n_wires, uint32_t *n_pbs, uint32_t *depth)
 {
     if (!c) return PZ_ERR_ARG;

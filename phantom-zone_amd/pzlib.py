@@ -73,6 +73,7 @@ SIGNATURES = {
     "pz_circuit_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pz_circuit_level_sizes": (C.c_int, [_vp, _vp, C.c_uint32]),
     "pz_circuit_nodes": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "pz_circuit_node_levels": (C.c_int, [_vp, _vp, C.c_size_t]),
     "pz_circuit_run": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pz_circuit_read": (C.c_int, [_vp, _vp, C.c_size_t, _vp, _vp]),
     "pz_server_create": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
@@ -250,6 +251,12 @@ class Circuit:
         d = self.stats()["depth"]
         out = np.zeros(d + 1, dtype=np.uint32)
         _check(self.L.pz_circuit_level_sizes(self.h, out.ctypes.data, d + 1))
+        return out
+
+    def node_levels(self):
+        n = self.stats()["wires"]
+        out = np.zeros(n, dtype=np.uint32)
+        _check(self.L.pz_circuit_node_levels(self.h, out.ctypes.data, n))
         return out
 
     def nodes(self):
