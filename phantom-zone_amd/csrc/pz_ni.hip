@@ -53,7 +53,36 @@ PZ_DEV u32 ni_consumers(const NiArgs &A, u32 r)
 
 // One work group per (index, input polynomial, group of up to G consumers):
 // the 50 digit rows of the polynomial are transformed once for the whole
-// group and multiply-accumulated with each member's Ksk.
+// group and multiply-accumulated with each member's Ksk.  With PZ_NI_LAZY the
+// products are [0,4q) Shoup products left unreduced (the accumulator stays
+// below 4q times the row count, < 2^62 for d <= 64) and one Shoup multiply by
+// 1 brings each accumulator to [0,2q) at the end: the same residues, and the
+// strict outputs are the same words.
+#ifndef PZ_NI_LAZY
+#define PZ_NI_LAZY 1
+#endif
+PZ_DEV void ni_mac_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+                       const u64 (&D)[8], u32 j, u64 q, u64 q2)
+{
+#if PZ_NI_LAZY
+    u64 kv[8], ks[8];
+    load8(K + 8 * j, kv);
+    load8(Ks + 8 * j, ks);
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[r] = shoup_fma<true, false, true>(acc[r], kv[r], ks[r], D[r], q, q2);
+#else
+    fma_row<false>(acc, K, Ks, D, j, q, q2);
+#endif
+}
+// accumulator of ni_mac_row in [0,2q)
+PZ_DEV u64 ni_mac_done(u64 a, const Tables &tb)
+{
+#if PZ_NI_LAZY
+    return shoup_mul<true>(a, 1, tb.one_s, tb.q);
+#else
+    return a;
+#endif
+}
 template <int LOGN, int G>
 __global__ __launch_bounds__(1 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 k_ni_rows(NiArgs A, DevCtx c)
@@ -117,12 +146,18 @@ k_ni_rows(NiArgs A, DevCtx c)
             for (int m = 0; m < G; m++) {
                 if ((u32)m >= nmem) break;
                 const u64 *K = A.K[party[m]], *Ks = A.Ks[party[m]];
-                fma_row<false>(acc[m][0], K + (size_t)row * N, Ks + (size_t)row * N, D[0], j, q,
-                               q2);
-                fma_row<false>(acc[m][1], K + (size_t)(A.d + row) * N,
-                               Ks + (size_t)(A.d + row) * N, D[0], j, q, q2);
+                ni_mac_row(acc[m][0], K + (size_t)row * N, Ks + (size_t)row * N, D[0], j, q, q2);
+                ni_mac_row(acc[m][1], K + (size_t)(A.d + row) * N, Ks + (size_t)(A.d + row) * N,
+                           D[0], j, q, q2);
             }
         }
+#pragma unroll
+        for (int m = 0; m < G; m++)
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                acc[m][0][x] = ni_mac_done(acc[m][0][x], c.tb);
+                acc[m][1][x] = ni_mac_done(acc[m][1][x], c.tb);
+            }
         const int part = nsm ? 0 : 1;  // the NI row joins a for RLWE'(-sm), b for RLWE'(m)
 #pragma unroll
         for (int m = 0; m < G; m++) {

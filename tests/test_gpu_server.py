@@ -157,3 +157,22 @@ def test_serving_two_parameter_sets():
             assert np.array_equal(ct, mixed[j][0]), (name, j)
     for c in ctxs.values():
         c.close()
+
+
+def test_server_refuses_other_ring_size():
+    """A session of a context with another ring size cannot share the wire
+    pool: PZ_ERR_UNSUPPORTED, and the server keeps working."""
+    import pzlib
+    print("server refuses other ring size", flush=True)
+    big, small = pzlib.Context("NI_2P"), pzlib.Context("SP_TEST")
+    s, z = small.keygen_secrets(1, 1)
+    k = small.keygen_server(2, s, z)
+    rgsw, auto, ksk = small.key_download(k)
+    k.free()
+    srv = pzlib.Server(big, hbm_budget_bytes=1 << 34, pool_rows=1 << 10)
+    with pytest.raises(pzlib.PzError):
+        srv.add_session(rgsw, auto, ksk, ctx=small)
+    assert srv.stats()["key_uploads"] == 0
+    srv.close()
+    small.close()
+    big.close()
