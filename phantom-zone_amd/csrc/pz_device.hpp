@@ -134,6 +134,20 @@ PZ_DEV void fma_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restr
     for (int r = 0; r < 8; r++) acc[r] = shoup_fma<SPLIT>(acc[r], kv[r], ks[r], D[r], q, q2);
 }
 
+// acc[r] += K[8j+r] * D[r] as a [0,4q) Shoup product, unreduced: after n
+// rows from zero the accumulator is below 4nq (< 2^62 for n <= 64 rows and
+// q < 2^54); fma_done brings it to [0,2q) with a Shoup multiply by 1.
+PZ_DEV void fma_row_lazy4(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
+                          const u64 (&D)[8], u32 j, u64 q, u64 q2)
+{
+    u64 kv[8], ks[8];
+    load8(K + 8 * j, kv);
+    load8(Ks + 8 * j, ks);
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[r] = shoup_fma<true, false, true>(acc[r], kv[r], ks[r], D[r], q, q2);
+}
+PZ_DEV u64 fma_done(u64 a, const Tables &tb) { return shoup_mul<true>(a, 1, tb.one_s, tb.q); }
+
 // fma_row in two halves of four words, a scheduling fence between them:
 // sixteen fewer registers hold key words at any time.
 template <bool SPLIT, bool REDUCE = true, bool QUOT4 = false>

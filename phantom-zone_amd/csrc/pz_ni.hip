@@ -65,11 +65,7 @@ PZ_DEV void ni_mac_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__re
                        const u64 (&D)[8], u32 j, u64 q, u64 q2)
 {
 #if PZ_NI_LAZY
-    u64 kv[8], ks[8];
-    load8(K + 8 * j, kv);
-    load8(Ks + 8 * j, ks);
-#pragma unroll
-    for (int r = 0; r < 8; r++) acc[r] = shoup_fma<true, false, true>(acc[r], kv[r], ks[r], D[r], q, q2);
+    fma_row_lazy4(acc, K, Ks, D, j, q, q2);
 #else
     fma_row<false>(acc, K, Ks, D, j, q, q2);
 #endif
@@ -78,7 +74,7 @@ PZ_DEV void ni_mac_row(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__re
 PZ_DEV u64 ni_mac_done(u64 a, const Tables &tb)
 {
 #if PZ_NI_LAZY
-    return shoup_mul<true>(a, 1, tb.one_s, tb.q);
+    return fma_done(a, tb);
 #else
     return a;
 #endif
