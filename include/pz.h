@@ -146,13 +146,14 @@ int pz_decrypt_bools(pz_ctx *ctx, const int32_t *h_s, uint32_t dim, const uint64
 
 /* ---- rlwe_by_rgsw_shoup (src/rgsw/runtime.rs:754-855) on B RLWE
  * ciphertexts d_rlwe[B][2][N]; RLWE b is multiplied by RGSW(X^{z_i}) with
- * i = d_rgsw_idx[b]; d_trivial[b] != 0 skips part a (trivial RLWE). */
+ * i = d_rgsw_idx[b]; d_trivial[b] != 0 skips part a (trivial RLWE).  An index
+ * outside [0, lwe_n) leaves its RLWE unchanged (the reference panics). */
 int pz_rlwe_by_rgsw(pz_ctx *ctx, pz_key *key, uint64_t *d_rlwe, const uint32_t *d_rgsw_idx,
                     const uint8_t *d_trivial, size_t B, void *stream);
 
 /* ---- rlwe_auto_shoup (src/rgsw/runtime.rs:530-643): automorphism with key
  * index d_auto_idx[b] in [0, w] (0 = -g, k = g^k; src/bool/evaluator.rs:
- * 730-743). */
+ * 730-743); an index outside [0, w] leaves its RLWE unchanged. */
 int pz_rlwe_auto(pz_ctx *ctx, pz_key *key, uint64_t *d_rlwe, const uint32_t *d_auto_idx,
                  const uint8_t *d_trivial, size_t B, void *stream);
 

@@ -53,6 +53,7 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) PZ_BR_ATTR void k_ext(u64 *rlwe, c
             a[r] = p[coef_x<LOGN>(j, r)];
             b[r] = p[N + coef_x<LOGN>(j, r)];
         }
+        if (idx[bb] >= c.n_lwe) continue;  // no such key: unchanged
         size_t off = (size_t)idx[bb] * key.rgsw_rows * N;
         ext_product_blk<LOGN>(a, b, key.rgsw + off, key.rgsw_s + off, trivial[bb] != 0, c, j, lds);
 #pragma unroll
@@ -80,6 +81,7 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *
             b[r] = p[N + coef_x<LOGN>(j, r)];
         }
         u32 k = kidx[bb];
+        if (k > c.w) continue;  // no such key: unchanged
         size_t off = (size_t)k * key.auto_rows * N;
         auto_blk<LOGN>(a, b, key.auto_k + off, key.auto_s + off, c.auto_el[k], trivial[bb] != 0,
                        c, j, lds);
@@ -110,11 +112,12 @@ __global__ __launch_bounds__(2 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu
     const u32 h = (u32)__builtin_amdgcn_readfirstlane(tid / TPB);
     const size_t rs = (size_t)(is_auto ? key.auto_rows : key.rgsw_rows) * N;
     for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
+        const u32 k = (u32)__builtin_amdgcn_readfirstlane((int)idx[bb]);
+        if (k >= (is_auto ? c.w + 1 : c.n_lwe)) continue;  // no such key: unchanged
         u64 *poly = rlwe + (bb * 2 + h) * N;
         u64 p[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) p[r] = poly[coef_x<LOGN>(j, r)];
-        const u32 k = (u32)__builtin_amdgcn_readfirstlane((int)idx[bb]);
         const bool triv = __builtin_amdgcn_readfirstlane((int)trivial[bb]) != 0;
         step_split<LOGN, 8>(p, (is_auto ? key.auto_k : key.rgsw) + k * rs,
                             (is_auto ? key.auto_s : key.rgsw_s) + k * rs, is_auto,

@@ -59,6 +59,33 @@ def test_empty_batches(env):
     assert np.array_equal(d.download(), sentinel)
 
 
+def test_out_of_range_key_indices_leave_rlwes_unchanged(env):
+    """RLWE x RGSW / automorphism entries whose key index does not exist are
+    skipped (no key read past the end); the other entries of the batch are
+    the oracle's words."""
+    pzlib, ctx, k, s = env
+    log("edges: out-of-range key indices")
+    o, _, _, key = keys_for("NI_2P")
+    N, n, w = ctx.N, ctx.n, o.p.w
+    r = np.random.default_rng(17)
+    x = r.integers(0, ctx.Q, size=(4, 2, N), dtype=np.uint64)
+    triv = np.zeros(4, dtype=np.uint8)
+    for op, good, bad in (("ext", 5, n), ("auto", 3, w + 1)):
+        idx = np.array([good, bad, bad + 1000, 0xFFFFFFFF], dtype=np.uint32)
+        d = ctx.to_device(x)
+        if op == "ext":
+            ctx.rlwe_by_rgsw(k, d, ctx.to_device(idx), ctx.to_device(triv), 4)
+            ct, cts = key.rgsw_ct(good)
+            want0 = o.rlwe_by_rgsw_shoup(x[0], ct, cts, False)
+        else:
+            ctx.rlwe_auto(k, d, ctx.to_device(idx), ctx.to_device(triv), 4)
+            kt, kts = key.auto_ct(good)
+            want0 = o.rlwe_auto_shoup(x[0], kt, kts, good, False)
+        got = d.download()
+        assert np.array_equal(got[0], want0), op
+        assert np.array_equal(got[1:], x[1:]), op
+
+
 def test_null_and_range_errors(env):
     pzlib, ctx, k, s = env
     log("edges: null and range errors")
