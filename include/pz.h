@@ -289,7 +289,9 @@ int pz_encrypt_rlwe_bools(pz_ctx *ctx, const int32_t *h_u, const uint8_t *seed32
  * (d_out may equal d_in) */
 int pz_rlwe_key_switch(pz_ctx *ctx, pz_rlwe_ksk *ksk, const uint64_t *d_in, uint64_t *d_out,
                        size_t B, void *stream);
-/* d_lwe[t] = sample_extract(d_rlwe[d_src[2t]], index d_src[2t+1]) ([N+1]) */
+/* d_lwe[t] = sample_extract(d_rlwe[d_src[2t]], index d_src[2t+1]) ([N+1]);
+ * the RLWE indices must lie inside d_rlwe, an index d_src[2t+1] >= N gives an
+ * all-zero row */
 int pz_sample_extract(pz_ctx *ctx, const uint64_t *d_rlwe, const uint32_t *d_src, size_t count,
                       uint64_t *d_lwe, void *stream);
 

@@ -65,7 +65,9 @@ __global__ void k_sample_extract(const u64 *rlwe, const u32 *src, size_t count, 
         const u64 *r = rlwe + (size_t)src[2 * o] * 2 * N;
         const u32 k = src[2 * o + 1];
         u64 v;
-        if (col == 0) {
+        if (k >= N) {
+            v = 0;  // no such coefficient: an all-zero row, nothing read
+        } else if (col == 0) {
             v = r[N + k];                             // b at the index
         } else {
             const u32 i = col - 1;                    // a_i of the LWE
