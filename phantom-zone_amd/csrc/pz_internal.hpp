@@ -9,13 +9,17 @@
 typedef uint8_t u8;
 typedef uint16_t u16;
 
+#ifndef PZ_KS_CT
+#define PZ_KS_CT 32
+#endif
+
 namespace pz {
 
 enum { PREP_GATE = 0, PREP_PBS = 1, PREP_KSONLY = 2 };
 
 // LWE key-switch tiling: threads per group (one output column each),
 // ciphertexts per group, KSK rows staged per LDS round.
-constexpr int KS_CT = 32;     // ciphertexts per work group
+constexpr int KS_CT = PZ_KS_CT;  // ciphertexts per work group
 constexpr int KS_ROWS = 256;  // KSK rows per LDS round
 constexpr int KS_SPLIT = 2;   // minimum row splits (summed with atomics)
 constexpr int KS_GROUPS = 2048;  // work groups a key-switch launch aims for

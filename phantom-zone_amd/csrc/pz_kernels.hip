@@ -129,12 +129,12 @@ __global__ __launch_bounds__(256) void k_prep(int mode, const u64 *c0, const u64
         ks_b[bb] = (u32)prep_value(mode, c0, c1, idx0 ? idx0[bb] : bb, idx1 ? idx1[bb] : bb,
                                    mode == PREP_GATE ? ops[bb] : 0, 0, N1, c);
     }
-    // digits: thread (m, cl) takes ciphertexts 4m..4m+3 of the tile in column
+    // digits: item (m, cl) takes ciphertexts 4m..4m+3 of the tile in column
     // 1 + cb + cl and packs their digit bytes; ciphertexts past B give 0
-    const u32 m = tid % CW;
+    static_assert(PREP_COLS * CW % 256 == 0, "whole rounds of the 256 threads");
 #pragma unroll
-    for (int half = 0; half < PREP_COLS / 32; half++) {
-        const u32 cl = tid / CW + 32 * half;
+    for (u32 e0 = 0; e0 < PREP_COLS * CW; e0 += 256) {
+        const u32 m = (e0 + tid) % CW, cl = (e0 + tid) / CW;
         const u32 col = 1 + cb + cl;
         u32 w[PREP_MAXD];
 #pragma unroll
@@ -182,44 +182,55 @@ __global__ __launch_bounds__(256) void k_prep(int mode, const u64 *c0, const u64
 // ---------------------------------------------------------------------------
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
+// Ciphertexts accumulated per work group: the whole KS_CT tile with 16-bit
+// lanes, at most 32 with 32-bit accumulators (register budget); ks_subs work
+// groups share a tile.  (Tiles of 64 ran slower than 32: the accumulators
+// cost occupancy, and the key re-reads they save are not the bound.)
+template <bool PACK16>
+constexpr int ks_ct() { return PACK16 || KS_CT <= 32 ? KS_CT : KS_CT / 2; }
+template <bool PACK16>
+constexpr int ks_subs() { return KS_CT / ks_ct<PACK16>(); }
+
 template <bool PACK16>
 __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks_b,
                                              const void *ksk, u32 rows, u32 n1, u32 ldk,
                                              size_t seg_begin, size_t seg_end, u32 *out,
                                              const u32 *work, const DevKey *keys)
 {
-    __shared__ __attribute__((aligned(16))) u32 dig[KS_ROWS][KS_CT];
+    constexpr int CT = ks_ct<PACK16>(), SUBS = ks_subs<PACK16>();
+    __shared__ __attribute__((aligned(16))) u32 dig[KS_ROWS][CT];
     const u32 unit = threadIdx.x;
     const u32 units = PACK16 ? (n1 + 1) / 2 : n1;
-    // this group's tile of KS_CT ciphertexts; only those in [seg_begin, seg_end)
-    // (the ciphertexts of this key) are accumulated.  With a work list, entry
-    // blockIdx.x names the tile, the segment and the key (batches that mix
-    // server keys: one launch for every key).
-    size_t tile_id = seg_begin / KS_CT + blockIdx.x;
+    // this group's CT ciphertexts (part `sub` of a tile of KS_CT); only those
+    // in [seg_begin, seg_end) (the ciphertexts of this key) are accumulated.
+    // With a work list, entry blockIdx.x / SUBS names the tile, the segment
+    // and the key (batches that mix server keys: one launch for every key).
+    const u32 sub = blockIdx.x % SUBS, gx = blockIdx.x / SUBS;
+    size_t tile_id = seg_begin / KS_CT + gx;
     if (work) {
-        const u32 *e = work + 4 * (size_t)blockIdx.x;
+        const u32 *e = work + 4 * (size_t)gx;
         tile_id = e[0];
         seg_begin = e[1];
         seg_end = e[2];
         ksk = keys[e[3]].ksk;
     }
-    const size_t b0 = tile_id * KS_CT;
-    const int8_t *tile = digits + tile_id * rows * KS_CT;
+    const size_t b0 = tile_id * KS_CT + sub * CT;
+    const int8_t *tile = digits + tile_id * rows * KS_CT + sub * CT;
     const u32 per = (rows + gridDim.y - 1) / gridDim.y;
     const u32 rbeg = blockIdx.y * per;
     const u32 rend = rbeg + per < rows ? rbeg + per : rows;
     const uint32_t *k32 = reinterpret_cast<const uint32_t *>(ksk);
     const u32 ld32 = PACK16 ? ldk / 2 : ldk;
-    u32 acc[KS_CT];
+    u32 acc[CT];
 #pragma unroll
-    for (int t = 0; t < KS_CT; t++) acc[t] = 0;
+    for (int t = 0; t < CT; t++) acc[t] = 0;
     for (u32 r0 = rbeg; r0 < rend; r0 += KS_ROWS) {
         const u32 rmax = (rend - r0) < (u32)KS_ROWS ? (rend - r0) : (u32)KS_ROWS;
         __syncthreads();
         // stage: 4 digits (one u32 of int8) per step, broadcast to 16-bit lanes
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(tile + (size_t)r0 * KS_CT);
-        for (u32 e = threadIdx.x; e < rmax * KS_CT / 4; e += blockDim.x) {
-            u32 w4 = src[e];
+        for (u32 e = threadIdx.x; e < rmax * CT / 4; e += blockDim.x) {
+            const u32 rr = e / (CT / 4), c4 = e % (CT / 4);
+            u32 w4 = *reinterpret_cast<const uint32_t *>(tile + (size_t)(r0 + rr) * KS_CT + 4 * c4);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 int dv = (int)(int8_t)(w4 >> (8 * k));
@@ -238,7 +249,7 @@ __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks
                 for (int u = 0; u < 4; u++) {
                     const uint4 *dr = reinterpret_cast<const uint4 *>(&dig[rr + u][0]);
 #pragma unroll
-                    for (int t4 = 0; t4 < KS_CT / 4; t4++) {
+                    for (int t4 = 0; t4 < CT / 4; t4++) {
                         uint4 dd = dr[t4];
                         u32 dv[4] = {dd.x, dd.y, dd.z, dd.w};
 #pragma unroll
@@ -259,7 +270,7 @@ __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks
             for (; rr < rmax; rr++) {
                 u32 kv = kp[(size_t)rr * ld32];
 #pragma unroll
-                for (int t = 0; t < KS_CT; t++) {
+                for (int t = 0; t < CT; t++) {
                     if (PACK16) {
                         u16x2 a = __builtin_bit_cast(u16x2, acc[t]);
                         u16x2 dk = __builtin_bit_cast(u16x2, dig[rr][t]);
@@ -274,7 +285,7 @@ __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks
     }
     if (unit >= units) return;
 #pragma unroll
-    for (int t = 0; t < KS_CT; t++) {
+    for (int t = 0; t < CT; t++) {
         size_t bb = b0 + t;
         if (bb < seg_begin || bb >= seg_end) continue;
         u32 *o = out + bb * n1;
@@ -479,7 +490,7 @@ void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_b
     size_t max_split = rows / KS_ROWS > 0 ? rows / KS_ROWS : 1;
     if (split > max_split) split = max_split;
     if (split < KS_SPLIT) split = KS_SPLIT;
-    dim3 grid((unsigned)tiles, (unsigned)split);
+    dim3 grid((unsigned)(tiles * (pack ? ks_subs<true>() : ks_subs<false>())), (unsigned)split);
     if (pack)
         hipLaunchKernelGGL(k_ks<true>, grid, tpb, 0, s, digits, ks_b, ksk, rows, n1, ldk,
                            seg_begin, seg_end, out, nullptr, nullptr);
@@ -502,7 +513,7 @@ void launch_ks_multi(const int8_t *digits, const u32 *ks_b, const DevKey *keys, 
     size_t max_split = rows / KS_ROWS > 0 ? rows / KS_ROWS : 1;
     if (split > max_split) split = max_split;
     if (split < KS_SPLIT) split = KS_SPLIT;
-    dim3 grid((unsigned)n_work, (unsigned)split);
+    dim3 grid((unsigned)(n_work * (pack ? ks_subs<true>() : ks_subs<false>())), (unsigned)split);
     if (pack)
         hipLaunchKernelGGL(k_ks<true>, grid, tpb, 0, s, digits, ks_b, nullptr, rows, n1, ldk,
                            (size_t)0, (size_t)0, out, work, keys);
