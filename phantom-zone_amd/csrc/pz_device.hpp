@@ -424,8 +424,6 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     const bool work = !is_auto || h == 0;
     const Decomp dc = is_auto ? c.dec_auto : (h ? c.dec_b : c.dec_a);
     u64 D[1][R];
-#pragma unroll
-    for (int r = 0; r < R; r++) D[0][r] = decomp_digit0(dc, p[r]);
     // product rows: RLWE x RGSW -- own digits with the own-part row (half 0:
     // A(-sm), row 0; half 1: B(m), row 3), then the other half's digits with
     // the other row (A(m), row 2; B(-sm), row 1); automorphism -- half 0 row
@@ -437,6 +435,8 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     // 4q (8q with the [0,4q) products), the fast inverse's input bound
     constexpr bool Q4 = PZ_QUOT4 && PZ_LAZY_MAC && PZ_INV_FAST;
     if (work) {
+#pragma unroll
+        for (int r = 0; r < R; r++) D[0][r] = decomp_digit0(dc, p[r]);
         ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(D, j, mine,
                                                                                   c.tb);
 #pragma unroll
