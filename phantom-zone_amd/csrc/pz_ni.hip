@@ -137,7 +137,7 @@ k_ni_rows(NiArgs A, DevCtx c)
             u64 D[1][8];
 #pragma unroll
             for (int x = 0; x < 8; x++) D[0][x] = decomp_next(c.dec_uitos, val[x]);
-            ntt_fwd<LOGN, 1, PZ_FWD_MODE, false>(D, j, lds, c.tb);
+            ntt_fwd<LOGN, 1, PZ_NI_LAZY ? FWD_FAST4 : PZ_FWD_MODE, PZ_NI_LAZY != 0>(D, j, lds, c.tb);
 #pragma unroll
             for (int m = 0; m < G; m++) {
                 if ((u32)m >= nmem) break;
