@@ -257,6 +257,23 @@ struct pz_circuit {
     u32 depth = 0, n_pbs = 0;
     std::vector<std::vector<GateRec>> levels;  // schedule (built lazily)
     bool scheduled = false;
+    // launch plan of the schedule, one device copy: per level the bootstrapped
+    // gates' ops | in0 | in1 | out, then each NOT wave's in | out
+    struct Wave {
+        size_t off;
+        u32 count;
+    };
+    struct LevelPlan {
+        size_t off;
+        u32 B;
+        std::vector<Wave> waves;
+    };
+    std::vector<u32> plan;
+    std::vector<LevelPlan> lplan;
+    u32 max_B = 0;
+    u32 *d_plan = nullptr;
+    size_t d_plan_cap = 0;
+    bool plan_on_device = false;
 };
 
 typedef u32 W;
@@ -480,6 +497,7 @@ extern "C" int pz_circuit_destroy(pz_circuit *c)
 {
     if (!c) return PZ_OK;
     if (c->pool) (void)hipFree(c->pool);
+    if (c->d_plan) (void)hipFree(c->d_plan);
     delete c;
     return PZ_OK;
 }
@@ -619,6 +637,9 @@ extern "C" int pz_circuit_reset_flags(pz_circuit *c)
 static void schedule(pz_circuit *c)
 {
     if (c->scheduled) return;
+    c->plan.clear();
+    c->lplan.clear();
+    c->plan_on_device = false;
     const size_t nn = c->nodes.size();
     const u32 D = c->depth;
     u32 cap = 512;  // bootstraps of one resident round (default without a device)
@@ -744,6 +765,44 @@ static int ensure_pool(pz_circuit *c)
     return PZ_OK;
 }
 
+// The schedule's launch plan (host side): a NOT whose input an earlier NOT of
+// its level writes runs in a later wave.
+static void build_plan(pz_circuit *c)
+{
+    if (!c->lplan.empty() || c->levels.empty()) return;
+    c->max_B = 0;
+    for (auto &lvl : c->levels) {
+        pz_circuit::LevelPlan lp;
+        lp.off = c->plan.size();
+        std::vector<const GateRec *> pbs, nots;
+        for (auto &r : lvl) (r.op <= 5 ? pbs : nots).push_back(&r);
+        lp.B = (u32)pbs.size();
+        c->max_B = std::max(c->max_B, lp.B);
+        for (auto *r : pbs) c->plan.push_back(r->op);
+        for (auto *r : pbs) c->plan.push_back(r->in0);
+        for (auto *r : pbs) c->plan.push_back(r->in1);
+        for (auto *r : pbs) c->plan.push_back(r->out);
+        std::map<u32, int> wave_of;  // NOT output wire -> its wave
+        std::vector<std::vector<const GateRec *>> waves;
+        for (auto *r : nots) {
+            auto it = wave_of.find(r->in0);
+            const int w = it == wave_of.end() ? 0 : it->second + 1;
+            wave_of[r->out] = w;
+            if ((int)waves.size() <= w) waves.resize(w + 1);
+            waves[w].push_back(r);
+        }
+        for (auto &wv : waves) {
+            pz_circuit::Wave w;
+            w.off = c->plan.size();
+            w.count = (u32)wv.size();
+            for (auto *r : wv) c->plan.push_back(r->in0);
+            for (auto *r : wv) c->plan.push_back(r->out);
+            lp.waves.push_back(w);
+        }
+        c->lplan.push_back(lp);
+    }
+}
+
 extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inputs, void *stream)
 {
     if (!c || !key || (!d_inputs && !c->inputs.empty())) return PZ_ERR_ARG;
@@ -767,14 +826,47 @@ extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inpu
         (void)hipStreamSynchronize(st);
         (void)hipFree(d_rows);
     }
-    std::vector<const pz_key *> keys{key};
-    for (auto &lvl : c->levels) {
-        if (lvl.empty()) continue;
-        std::vector<GateRec> recs = lvl;
-        rc = exec_batch(c->ctx, keys, recs, c->pool, st);
-        if (rc) return rc;
+    // every level from one uploaded plan, launched back to back on the stream
+    // (no host work or synchronisation between levels)
+    build_plan(c);
+    if (!c->plan_on_device && !c->plan.empty()) {
+        if (c->d_plan_cap < c->plan.size()) {
+            if (c->d_plan) (void)hipFree(c->d_plan);
+            c->d_plan = nullptr;
+            c->d_plan_cap = 0;
+            if (hipMalloc(&c->d_plan, 4 * c->plan.size()) != hipSuccess) return PZ_ERR_HIP;
+            c->d_plan_cap = c->plan.size();
+        }
+        if (hipMemcpy(c->d_plan, c->plan.data(), 4 * c->plan.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return PZ_ERR_HIP;
+        c->plan_on_device = true;
     }
-    return PZ_OK;
+    pz_exec_view v;
+    DevCtx dc;
+    rc = pz_internal_exec_view(c->ctx, c->max_B ? c->max_B : 1, &v, &dc);
+    if (rc) return rc;
+    const DevKey *k0 = pz_internal_devkey(key);
+    const u32 N1 = v.p.rlwe_n + 1, n1 = v.p.lwe_n + 1;
+    for (auto &lp : c->lplan) {
+        const u32 *d = c->d_plan + lp.off;
+        const size_t B = lp.B;
+        if (B) {
+            launch_prep(PREP_GATE, c->pool, c->pool, d + B, d + 2 * B, d, N1, B, v.digits, v.ksb,
+                        v.ks, dc, st);
+            launch_ks(v.digits, v.ksb, k0->ksk, k0->ksk_bytes, v.p.rlwe_n * v.p.lwe_d, n1, k0->ldk,
+                      0, B, v.ks, st);
+            int var = BR_ONE, grid = 1;
+            pz_internal_br_plan(c->ctx, B, &var, &grid);
+            launch_blind_rotate(v.logn, var, grid, v.ks, d, B, c->pool, v.stats, dc, *k0, nullptr,
+                                nullptr, d + 3 * B, st);
+        }
+        for (auto &w : lp.waves)
+            hipLaunchKernelGGL(k_not_rows, grid_rows((size_t)w.count * N1), 256, 0, st, c->pool,
+                               c->d_plan + w.off, c->d_plan + w.off + w.count, (size_t)w.count, N1,
+                               v.p.rlwe_q);
+    }
+    return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
 }
 
 extern "C" int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t *d_out,
