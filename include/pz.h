@@ -212,7 +212,11 @@ int pz_circuit_nodes(pz_circuit *c, uint32_t *h_out, size_t cap);
  * the circuit's depth, filled least slack first up to whole rounds of the
  * resident blind-rotation grid (512 bootstraps without a device). */
 int pz_circuit_node_levels(pz_circuit *c, uint32_t *h_levels, size_t cap);
-/* d_inputs[i] is the ciphertext of the i-th input wire ([N+1] u64 each) */
+/* d_inputs[i] is the ciphertext of the i-th input wire ([N+1] u64 each).
+ * Every level is queued on the stream (NULL: the context's) without host
+ * synchronisation between levels; the schedule goes to the device once per
+ * circuit shape.  Runs of one circuit must not overlap (they share its wire
+ * pool), and a run is complete once the stream has drained. */
 int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inputs, void *stream);
 int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t *d_out,
                     void *stream);

@@ -830,6 +830,9 @@ extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inpu
     // (no host work or synchronisation between levels)
     build_plan(c);
     if (!c->plan_on_device && !c->plan.empty()) {
+        // a new schedule: the previous one's launches (this stream, which does
+        // not order with the null stream) may still read the old plan
+        if (hipStreamSynchronize(st) != hipSuccess) return PZ_ERR_HIP;
         if (c->d_plan_cap < c->plan.size()) {
             if (c->d_plan) (void)hipFree(c->d_plan);
             c->d_plan = nullptr;
