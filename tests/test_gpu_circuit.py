@@ -95,3 +95,43 @@ def test_u8_mul_lt_div_ni2p():
     ops = ["mul", "lt", "div_rem", "mul"]
     pairs = [(int(r.integers(0, 256)), int(r.integers(1, 256))) for _ in ops]
     run_case("NI_2P", ops, pairs, 777, check_serial=False)
+
+
+def test_circuit_grows_between_runs_sp_test():
+    """A circuit run, extended (a new operation on its outputs and a chain of
+    NOTs, which run in dependent waves of one level) and run again: the second
+    run uses the new schedule, and every output equals the oracle's serial
+    evaluation of the grown DAG."""
+    import pzlib
+    o, s, z, key = keys_for("SP_TEST")
+    ctx = pzlib.Context("SP_TEST")
+    k = ctx.key_upload(key.rgsw, key.auto_key, key.lwe_ksk)
+    circ = pzlib.Circuit(ctx)
+    a, b = 201, 77
+    wa, wb = circ.inputs(8), circ.inputs(8)
+    s1 = circ.u8("add", wa, wb)
+    cts = ctx.encrypt_bools(s, np.array(bits(a) + bits(b), dtype=np.uint8), 4242)
+    d_in = ctx.to_device(cts)
+    circ.run(k, d_in)
+    got = ctx.decrypt_bools(s, circ.read(s1))
+    assert unpack("add", got) == OPS["add"](a, b)
+    # grow: (a + b) * b, and NOT(NOT(NOT(bit 0 of the sum)))
+    p = circ.u8("mul", s1, wb)
+    n1 = circ.gate("not", int(s1[0]))
+    n2 = circ.gate("not", n1)
+    n3 = circ.gate("not", n2)
+    print("grown circuit pbs=%d" % circ.stats()["pbs"], flush=True)
+    circ.run(k, d_in)
+    wires = np.concatenate([s1, p, np.array([n1, n2, n3], dtype=np.uint32)])
+    got_ct = circ.read(wires)
+    got = ctx.decrypt_bools(s, got_ct)
+    assert unpack("add", got[:8]) == OPS["add"](a, b)
+    assert unpack("mul", got[8:16]) == OPS["mul"]((a + b) & 255, b)
+    bit0 = ((a + b) & 255) & 1
+    assert [int(x) for x in got[16:19]] == [1 - bit0, bit0, 1 - bit0]
+    vals = serial_oracle(o, key, circ.nodes(), cts)
+    for i, wire in enumerate(wires):
+        assert np.array_equal(got_ct[i], vals[int(wire)]), ("wire", int(wire))
+    circ.close()
+    k.free()
+    ctx.close()
