@@ -42,14 +42,6 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #define PZ_QUOT4 1
 #endif
 
-// Split blind rotation: key rows loaded whole (32 VGPRs), the first before the
-// barrier that precedes its product, the second before the read of the other
-// half's digits and the barrier after it: the load latencies overlap the
-// barrier waits instead of one exposed latency per half row.
-#ifndef PZ_KEY_EARLY
-#define PZ_KEY_EARLY 1
-#endif
-
 #ifndef PZ_MAC_FENCE
 #define PZ_MAC_FENCE 0
 #endif
@@ -147,27 +139,6 @@ PZ_DEV void fma_row_lazy4(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *_
     for (int r = 0; r < 8; r++) acc[r] = shoup_fma<true, false, true>(acc[r], kv[r], ks[r], D[r], q, q2);
 }
 PZ_DEV u64 fma_done(u64 a, const Tables &tb) { return shoup_mul<true>(a, 1, tb.one_s, tb.q); }
-
-// fma_row in two halves of four words, a scheduling fence between them:
-// sixteen fewer registers hold key words at any time.
-template <bool SPLIT, bool REDUCE = true, bool QUOT4 = false>
-PZ_DEV void fma_row_halves(u64 (&acc)[8], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
-                           const u64 (&D)[8], u32 j, u64 q, u64 q2)
-{
-#pragma unroll
-    for (int half = 0; half < 2; half++) {
-        const ulonglong2 *k2 = reinterpret_cast<const ulonglong2 *>(K + 8 * j + 4 * half);
-        const ulonglong2 *s2 = reinterpret_cast<const ulonglong2 *>(Ks + 8 * j + 4 * half);
-        const ulonglong2 k0 = k2[0], k1 = k2[1], s0 = s2[0], s1 = s2[1];
-        const u64 kv[4] = {k0.x, k0.y, k1.x, k1.y}, ks[4] = {s0.x, s0.y, s1.x, s1.y};
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-            acc[4 * half + r] =
-                shoup_fma<SPLIT, REDUCE, QUOT4>(acc[4 * half + r], kv[r], ks[r], D[4 * half + r],
-                                                q, q2);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-}
 
 // The thread's slice of one key row and its Shoup twins (R consecutive words
 // each), as 16-byte loads.
@@ -447,7 +418,10 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     u64 acc[1][R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[0][r] = 0;
-#if PZ_KEY_EARLY
+    // key rows loaded whole (32 VGPRs), the first before the barrier that
+    // precedes its product, the second before the read of the other half's
+    // digits and the barrier after it: the load latencies overlap the barrier
+    // waits instead of one exposed latency per half row
     KeySlice<R> k1, k2;
     if (work) load_slice(k1, K + r1, Ks + r1, j);
     __syncthreads();
@@ -460,19 +434,6 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     }
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (second) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k2, D[0], q, q2);
-#else
-    __syncthreads();
-    if (work)
-        fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], K + r1, Ks + r1, D[0], j, q, q2);
-    __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
-    if (second) {
-#pragma unroll
-        for (int r = 0; r < R; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN, R>(j, r))];
-    }
-    __syncthreads();  // both halves have read: the regions may be rewritten
-    if (second)
-        fma_row_halves<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], K + r2, Ks + r2, D[0], j, q, q2);
-#endif
     ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2), R,
             false>(acc, j, mine, c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
