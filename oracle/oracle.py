@@ -137,6 +137,8 @@ def lib():
             "pzo_not": (None, [vp, u64p]),
             "pzo_pbs_batch_threads": (C.c_double, [vp, C.POINTER(Key), C.c_uint32, u64p,
                                                    C.c_uint32, C.c_uint32]),
+            "pzo_gates_batch_threads": (C.c_double, [vp, C.POINTER(Key), u32p, u64p, u64p,
+                                                     C.c_uint32, C.c_uint32]),
             "pzo_encrypt_lwe": (None, [vp, u8p, i32p, C.c_uint32, C.c_uint64, u64p]),
             "pzo_keygen_user_secret": (None, [vp, u8p, i32p]),
             "pzo_keygen_rlwe_ksk": (None, [vp, u8p, i32p, i32p, u64p]),
@@ -339,6 +341,20 @@ class Oracle:
         secs = self.L.pzo_pbs_batch_threads(self.ev, C.byref(key.cstruct()),
                                             GATES.get(gate, gate), arr.reshape(-1), count, threads)
         return arr, secs
+
+    def gates_batch(self, key, gates, c0, c1, threads=None):
+        """gates[i] of (c0[i], c1[i]) for every i, on `threads` host threads
+        (default: every core); the same words as gate() on each item."""
+        out = np.ascontiguousarray(c0, dtype=np.uint64).copy()
+        b = np.ascontiguousarray(c1, dtype=np.uint64)
+        g = np.ascontiguousarray(gates, dtype=np.uint32)
+        count = out.shape[0]
+        if count == 0:
+            return out
+        threads = threads or os.cpu_count() or 1
+        self.L.pzo_gates_batch_threads(self.ev, C.byref(key.cstruct()), g, out.reshape(-1),
+                                       b.reshape(-1), count, min(threads, count))
+        return out
 
     # ---- input path (non-interactive sets) ----
     def keygen_user_secret(self, seed):

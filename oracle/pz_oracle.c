@@ -1162,19 +1162,43 @@ uint32_t pzo_br_schedule(const pzo_eval *e, const uint64_t *lwe_ks, uint32_t *op
     return c.nops;
 }
 
+/* Working memory of one PBS stream, allocated once per stream (the
+ * reference's evaluator likewise owns its scratch, src/bool/evaluator.rs:
+ * 186-206). */
+typedef struct {
+    uint64_t *scr;     /* scratch_rows x N */
+    uint64_t *ks;      /* n + 1            */
+    uint32_t *start;   /* br_q/2 + 1       */
+    uint32_t *idx;     /* n                */
+    uint64_t *rlwe;    /* 2 x N            */
+} pbs_scratch;
+
+static void pbs_scratch_init(const pzo_eval *e, pbs_scratch *w)
+{
+    w->scr = scratch_new(e);
+    w->ks = (uint64_t *)malloc(8 * (size_t)(e->p.lwe_n + 1));
+    w->start = (uint32_t *)malloc(4 * (size_t)((e->p.br_q >> 1) + 1));
+    w->idx = (uint32_t *)malloc(4 * (size_t)e->p.lwe_n);
+    w->rlwe = (uint64_t *)malloc(16 * (size_t)e->p.rlwe_n);
+}
+
+static void pbs_scratch_free(pbs_scratch *w)
+{
+    free(w->scr); free(w->ks); free(w->start); free(w->idx); free(w->rlwe);
+}
+
 /* pbs, src/pbs.rs:104-233.  lwe: [N+1] mod Q, overwritten with the output. */
 static void pbs_impl(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint64_t *lwe,
-                     pzo_pbs_stats *stats, uint64_t *scr)
+                     pzo_pbs_stats *stats, pbs_scratch *w)
 {
-    uint32_t N = e->p.rlwe_n, n1 = e->p.lwe_n + 1, brq = e->p.br_q, nb = brq >> 1;
+    uint32_t N = e->p.rlwe_n, brq = e->p.br_q, nb = brq >> 1;
+    uint64_t *scr = w->scr, *ks = w->ks;
+    uint32_t *start = w->start, *idx = w->idx;
     /* (1) moddown Q -> q_ks */
     for (uint32_t i = 0; i < N + 1; i++) lwe[i] = pzo_moddown(e, lwe[i]);
     /* (2) key switch */
-    uint64_t *ks = (uint64_t *)malloc(8 * (size_t)n1);
     pzo_lwe_key_switch(e, ks, lwe, key->lwe_ksk);
     /* (3) odd moddown + buckets */
-    uint32_t *start = (uint32_t *)malloc(4 * (size_t)(nb + 1));
-    uint32_t *idx = (uint32_t *)malloc(4 * (size_t)e->p.lwe_n);
     make_buckets(e, ks, start, idx);
     /* (4) b: rotate the test vector by X^{g*b} */
     double lq = (double)e->p.lwe_q, bq = (double)brq;
@@ -1182,7 +1206,8 @@ static void pbs_impl(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint6
     uint32_t exp = g_times_b;
     int sign = 1;
     if (exp > nb) { exp -= nb; sign = 0; }
-    uint64_t *rlwe = (uint64_t *)calloc(2 * (size_t)N, 8);
+    uint64_t *rlwe = w->rlwe;
+    memset(rlwe, 0, 16 * (size_t)N);
     const uint64_t *tvec = e->test_vec + (size_t)gate * nb;
     if (e->embedding == 1) {
         pzo_monomial_mul(e->p.rlwe_q, tvec, rlwe + N, exp, sign, nb);
@@ -1198,21 +1223,20 @@ static void pbs_impl(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint6
     if (stats) *stats = c.st;
     /* (6) sample extract at index 0 */
     pzo_sample_extract(e, lwe, rlwe, 0);
-    free(rlwe); free(start); free(idx); free(ks);
 }
 
 void pzo_pbs(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint64_t *lwe,
              pzo_pbs_stats *stats)
 {
-    uint64_t *s = scratch_new(e);
-    pbs_impl(e, key, gate, lwe, stats, s);
-    free(s);
+    pbs_scratch w;
+    pbs_scratch_init(e, &w);
+    pbs_impl(e, key, gate, lwe, stats, &w);
+    pbs_scratch_free(&w);
 }
 
 /* _add_and_shift_lwe_cts :2118-2123, _subtract_double_lwe_cts :2126-2133,
  * gates :2163-2250 */
-void pzo_gate(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint64_t *c0,
-              const uint64_t *c1)
+static void gate_linear(const pzo_eval *e, uint32_t gate, uint64_t *c0, const uint64_t *c1)
 {
     uint32_t N1 = e->p.rlwe_n + 1;
     const modq *m = &e->mq;
@@ -1223,6 +1247,12 @@ void pzo_gate(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint64_t *c0
         for (uint32_t i = 0; i < N1; i++) c0[i] = mm_sub(m, c0[i], c1[i]);
         for (uint32_t i = 0; i < N1; i++) c0[i] = mm_add(m, c0[i], c0[i]);
     }
+}
+
+void pzo_gate(const pzo_eval *e, const pzo_key *key, uint32_t gate, uint64_t *c0,
+              const uint64_t *c1)
+{
+    gate_linear(e, gate, c0, c1);
     pzo_pbs(e, key, gate, c0, NULL);
 }
 
@@ -1232,26 +1262,38 @@ void pzo_not(const pzo_eval *e, uint64_t *c0)
     for (uint32_t i = 0; i < e->p.rlwe_n + 1; i++) c0[i] = mm_neg(&e->mq, c0[i]);
 }
 
+/* Independent PBS streams, one per thread (the reference evaluator is
+ * single-threaded; this is the multi-core CPU baseline and a fast checker).
+ * gates: one gate id per item, or NULL for `gate` throughout; c1: the second
+ * operand of each gate's linear step, or NULL for a bare PBS of lwes[i]. */
 typedef struct {
     const pzo_eval *e;
     const pzo_key *key;
     uint32_t gate;
+    const uint32_t *gates;
     uint64_t *lwes;
+    const uint64_t *c1;
     uint32_t begin, end;
 } batch_job;
 
 static void *batch_worker(void *arg)
 {
     batch_job *j = (batch_job *)arg;
-    uint64_t *s = scratch_new(j->e);
-    for (uint32_t i = j->begin; i < j->end; i++)
-        pbs_impl(j->e, j->key, j->gate, j->lwes + (size_t)i * (j->e->p.rlwe_n + 1), NULL, s);
-    free(s);
+    const size_t N1 = (size_t)j->e->p.rlwe_n + 1;
+    pbs_scratch w;
+    pbs_scratch_init(j->e, &w);
+    for (uint32_t i = j->begin; i < j->end; i++) {
+        const uint32_t g = j->gates ? j->gates[i] : j->gate;
+        if (j->c1) gate_linear(j->e, g, j->lwes + i * N1, j->c1 + i * N1);
+        pbs_impl(j->e, j->key, g, j->lwes + i * N1, NULL, &w);
+    }
+    pbs_scratch_free(&w);
     return NULL;
 }
 
-double pzo_pbs_batch_threads(const pzo_eval *e, const pzo_key *key, uint32_t gate,
-                             uint64_t *lwes, uint32_t count, uint32_t threads)
+static double run_batch(const pzo_eval *e, const pzo_key *key, uint32_t gate,
+                        const uint32_t *gates, uint64_t *lwes, const uint64_t *c1,
+                        uint32_t count, uint32_t threads)
 {
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;
@@ -1260,7 +1302,8 @@ double pzo_pbs_batch_threads(const pzo_eval *e, const pzo_key *key, uint32_t gat
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     for (uint32_t t = 0; t < threads; t++) {
-        jobs[t].e = e; jobs[t].key = key; jobs[t].gate = gate; jobs[t].lwes = lwes;
+        jobs[t].e = e; jobs[t].key = key; jobs[t].gate = gate; jobs[t].gates = gates;
+        jobs[t].lwes = lwes; jobs[t].c1 = c1;
         jobs[t].begin = (uint32_t)(((uint64_t)count * t) / threads);
         jobs[t].end = (uint32_t)(((uint64_t)count * (t + 1)) / threads);
         pthread_create(&th[t], NULL, batch_worker, &jobs[t]);
@@ -1268,6 +1311,19 @@ double pzo_pbs_batch_threads(const pzo_eval *e, const pzo_key *key, uint32_t gat
     for (uint32_t t = 0; t < threads; t++) pthread_join(th[t], NULL);
     clock_gettime(CLOCK_MONOTONIC, &t1);
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+double pzo_pbs_batch_threads(const pzo_eval *e, const pzo_key *key, uint32_t gate,
+                             uint64_t *lwes, uint32_t count, uint32_t threads)
+{
+    return run_batch(e, key, gate, NULL, lwes, NULL, count, threads);
+}
+
+double pzo_gates_batch_threads(const pzo_eval *e, const pzo_key *key, const uint32_t *gates,
+                               uint64_t *c0, const uint64_t *c1, uint32_t count,
+                               uint32_t threads)
+{
+    return run_batch(e, key, 0, gates, c0, c1, count, threads);
 }
 
 /* ------------------------------------------------------------------------- */

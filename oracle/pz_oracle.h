@@ -169,6 +169,11 @@ void pzo_not(const pzo_eval *e, uint64_t *c0);
 /* batched PBS timing helper for the CPU baseline: returns seconds */
 double pzo_pbs_batch_threads(const pzo_eval *e, const pzo_key *key, uint32_t gate,
                              uint64_t *lwes, uint32_t count, uint32_t threads);
+/* gates[i] of (c0[i], c1[i]) into c0[i] for count items on `threads` threads
+ * (the same words as pzo_gate on each); returns seconds */
+double pzo_gates_batch_threads(const pzo_eval *e, const pzo_key *key, const uint32_t *gates,
+                               uint64_t *c0, const uint64_t *c1, uint32_t count,
+                               uint32_t threads);
 
 /* encryption helpers (tests) */
 /* ---- input path of the non-interactive protocol ---- */

@@ -62,3 +62,63 @@ def rng(seed):
 def uniform_mod(r, q, shape):
     """uint64 values uniform in [0, q)."""
     return r.integers(0, q, size=shape, dtype=np.uint64)
+
+
+def oracle_cone(o, key, nodes, inputs_ct, wires, threads=None):
+    """Oracle ciphertexts of `wires` in a gate DAG (nodes from
+    pzlib.Circuit.nodes(): kind 0 input, 1 gate, 2 NOT), evaluating only the
+    wires' ancestors.  Every gate is a deterministic function of its two
+    operands, so the words equal serial gate-by-gate evaluation in node order;
+    the independent gates of one depth run on host threads."""
+    nodes = [tuple(int(x) for x in nd) for nd in nodes]
+    need, stack = set(), [int(w) for w in wires]
+    while stack:
+        i = stack.pop()
+        if i in need:
+            continue
+        need.add(i)
+        kind, _, a, b = nodes[i]
+        if kind == 1:
+            stack += [a, b]
+        elif kind == 2:
+            stack.append(a)
+    vals, depth, k = {}, {}, 0
+    for i, (kind, _, a, b) in enumerate(nodes):
+        if kind == 0:
+            if i in need:
+                vals[i] = np.asarray(inputs_ct[k], dtype=np.uint64)
+            depth[i] = 0
+            k += 1
+        elif kind == 1:
+            depth[i] = max(depth[a], depth[b]) + 1
+        else:
+            depth[i] = depth[a]
+    by_depth = {}
+    for i in sorted(need):
+        if nodes[i][0] != 0:
+            by_depth.setdefault(depth[i], []).append(i)
+    for d in sorted(by_depth):
+        gates = [i for i in by_depth[d] if nodes[i][0] == 1]
+        if gates:
+            out = o.gates_batch(key, [nodes[i][1] for i in gates],
+                                np.stack([vals[nodes[i][2]] for i in gates]),
+                                np.stack([vals[nodes[i][3]] for i in gates]), threads)
+            for j, i in enumerate(gates):
+                vals[i] = out[j]
+        for i in by_depth[d]:  # NOTs of this depth, in node order (they may chain)
+            if nodes[i][0] == 2:
+                vals[i] = o.not_(vals[nodes[i][2]])
+    return {int(w): vals[int(w)] for w in wires}
+
+
+def oracle_key_from_rows(o, rgsw, auto, ksk):
+    """An oracle key holding server-key rows in the pz_key_upload layout (as
+    pzlib's key_download returns them), with its Shoup twins."""
+    import ctypes as C
+    from oracle.oracle import OracleKey
+    k = OracleKey(o.sizes, o.N)
+    k.rgsw[:] = np.ascontiguousarray(rgsw, dtype=np.uint64).reshape(-1)
+    k.auto_key[:] = np.ascontiguousarray(auto, dtype=np.uint64).reshape(-1)
+    k.lwe_ksk[:] = np.ascontiguousarray(ksk, dtype=np.uint64).reshape(-1)
+    o.L.pzo_key_shoup(o.ev, C.byref(k.cstruct()))
+    return k

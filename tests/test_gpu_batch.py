@@ -3,7 +3,8 @@
 - The kernel-level batch: 4096 forward + inverse NTTs of uniform rows, and
   4096 RLWE x RGSW products of uniform RLWEs with a uniform evaluation-domain
   RGSW key (logB = 17, d_a = d_b = 1), every output word against the oracle.
-- Batched gate bootstraps at B = 256, 1024 and 4096 (NAND / XOR mix, NI_2P),
+- Batched gate bootstraps at B = 256, 1024, 4096 and 16384 (NAND / XOR mix,
+  NI_2P),
   with the automatic kernel choice (split kernel, both variants) and with
   one work group per bootstrap forced:
   every output decrypts to the gate of the input bits, a sample of outputs
@@ -78,8 +79,8 @@ def test_ext_product_batch_4096_uniform_rgsw(ctx):
 PLAIN = {0: lambda a, b: 1 - (a & b), 4: lambda a, b: a ^ b}
 
 
-@pytest.mark.parametrize("B,mode", [(1024, 0), (4096, 0), (1024, 1), (256, 0), (4096, 3),
-                                    (256, 3)])
+@pytest.mark.parametrize("B,mode", [(1024, 0), (4096, 0), (16384, 0), (1024, 1), (256, 0),
+                                    (4096, 3), (256, 3)])
 def test_gates_full_batch(ctx, B, mode):
     o, s, z, key = keys_for("NI_2P")
     k = ctx.key_upload(key.rgsw, key.auto_key, key.lwe_ksk)

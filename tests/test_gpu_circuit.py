@@ -5,7 +5,7 @@ DAG, gate by gate in the reference's order."""
 import numpy as np
 import pytest
 
-from conftest import keys_for
+from conftest import keys_for, oracle_cone
 
 pytestmark = pytest.mark.gpu
 
@@ -132,6 +132,53 @@ def test_circuit_grows_between_runs_sp_test():
     vals = serial_oracle(o, key, circ.nodes(), cts)
     for i, wire in enumerate(wires):
         assert np.array_equal(got_ct[i], vals[int(wire)]), ("wire", int(wire))
+    circ.close()
+    k.free()
+    ctx.close()
+
+
+def test_u8_mul_divrem_lt_ni8p():
+    """FheUint8 at NI_8P (8-party keys, q_ks = 2^17: 32-bit key-switch
+    batches at circuit scale): 16 mul + 16 div_rem (one by zero) + 16 lt in
+    one run.  Every output decrypts to u8 arithmetic, the div-by-zero flag
+    decrypts to 1, and every wire of one mul and one lt equals the oracle's
+    evaluation of the same gates."""
+    import pzlib
+    name = "NI_8P"
+    o, s, z, key = keys_for(name)
+    ctx = pzlib.Context(name)
+    k = ctx.key_upload(key.rgsw, key.auto_key, key.lwe_ksk)
+    circ = pzlib.Circuit(ctx)
+    r = np.random.default_rng(92)
+    ops = ["mul"] * 16 + ["div_rem"] * 16 + ["lt"] * 16
+    pairs = [(int(r.integers(0, 256)), int(r.integers(0, 256))) for _ in ops]
+    pairs[16] = (173, 0)  # division by zero: quotient 255, remainder a
+    in_bits, outs = [], []
+    for op, (a, b) in zip(ops, pairs):
+        wa, wb = circ.inputs(8), circ.inputs(8)
+        in_bits += bits(a) + bits(b)
+        outs.append(circ.u8(op, wa, wb))
+    flag = circ.div_zero_flag()
+    st = circ.stats()
+    print("circuit NI_8P ops=%d pbs=%d depth=%d widest=%d" %
+          (len(ops), st["pbs"], st["depth"], int(circ.level_sizes().max())), flush=True)
+    cts = ctx.encrypt_bools(s, np.array(in_bits, dtype=np.uint8), 4321)
+    circ.run(k, ctx.to_device(cts))
+    all_out = np.concatenate(outs + [np.array([flag], dtype=np.uint32)])
+    got_ct = circ.read(all_out)
+    got_bits = ctx.decrypt_bools(s, got_ct)
+    pos = 0
+    for op, (a, b), w in zip(ops, pairs, outs):
+        assert unpack(op, got_bits[pos:pos + len(w)]) == OPS[op](a, b), (op, a, b)
+        pos += len(w)
+    assert int(got_bits[-1]) == 1  # the div-by-zero flag
+    # one mul (index 3) and one lt (index 40): every output wire vs the oracle
+    check = [int(w) for w in outs[3]] + [int(outs[40][0])]
+    print("oracle cone NI_8P wires=%d" % len(check), flush=True)
+    want = oracle_cone(o, key, circ.nodes(), cts, check)
+    pos_of = {int(w): i for i, w in enumerate(all_out)}
+    for w in check:
+        assert np.array_equal(got_ct[pos_of[w]], want[w]), ("wire", w)
     circ.close()
     k.free()
     ctx.close()

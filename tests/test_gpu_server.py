@@ -1,12 +1,14 @@
 """GPU: the serving loop.  Requests from several sessions (different keys)
 arrive over time and are batched across sessions level by level; every
 result decrypts to the plaintext program result and equals, bit for bit, the
-same request served alone.  A key budget smaller than the session count
-forces evictions and re-admission."""
+same request served alone and the oracle's evaluation of the same gates.  A
+key budget smaller than the session count forces evictions and
+re-admission.  64 sessions (32 two-party NI_2P and 32 four-party NI_4P_HB_FR
+keys, all resident) serve Poisson arrivals."""
 import numpy as np
 import pytest
 
-from conftest import oracle_for
+from conftest import oracle_cone, oracle_for, oracle_key_from_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +63,18 @@ def test_serving_mix_equals_alone():
     for j, prog in enumerate(progs):
         bits = ctx.decrypt_bools(sessions[sess_ids[j]][0], mixed[j])
         assert pzprog.decode(bits, *meta[j]) == prog.plain(), j
+    # the oracle's evaluation of the same gates gives the same ciphertexts
+    o = oracle_for(name)
+    for j in (1, 5):
+        se = sessions[sess_ids[j]]
+        okey = oracle_key_from_rows(o, se[2], se[3], se[4])
+        circ = pzlib.Circuit(None)
+        wires, _, _ = progs[j].build(circ)
+        cts = ctx.encrypt_bools(se[0], progs[j].input_bits(), 40 + 1000 * j)
+        want = oracle_cone(o, okey, circ.nodes(), cts, wires)
+        for i, w in enumerate(wires):
+            assert np.array_equal(mixed[j][i], want[int(w)]), (j, int(w))
+        circ.close()
     # each request alone gives the same ciphertexts
     for j in (0, 4, 8):
         solo = pzlib.Server(ctx, hbm_budget_bytes=1 << 34, pool_rows=1 << 14)
@@ -176,3 +190,82 @@ def test_server_refuses_other_ring_size():
     srv.close()
     small.close()
     big.close()
+
+
+def test_serving_64_sessions_poisson():
+    """64 sessions, each with its own key resident (32 two-party NI_2P and 32
+    four-party NI_4P_HB_FR), Poisson arrivals of programs of 1-6 FheUint8
+    operations.  Every result decrypts to the program's plaintext result;
+    four requests (two per parameter set) served alone give the same words;
+    one single-comparison request equals the oracle's gates word for word."""
+    import pzlib
+    import pzprog
+    sets = (("NI_2P", 2), ("NI_4P_HB_FR", 4))
+    ctxs = {name: pzlib.Context(name) for name, _ in sets}
+    sessions = []  # (set name, secret, rgsw, auto, ksk)
+    for i in range(64):
+        name, parties = sets[i % 2]
+        c = ctxs[name]
+        s, z = c.keygen_secrets(800 + i, parties)
+        k = c.keygen_server(900 + i, s, z)
+        rgsw, auto, ksk = c.key_download(k)
+        k.free()
+        sessions.append((name, s, rgsw, auto, ksk))
+    print("serving 64 sessions: keys ready", flush=True)
+    r = np.random.default_rng(64)
+    count = 64
+    progs, sess_ids, arrivals, t = [], [], [], 0.0
+    for j in range(count):
+        t += float(r.exponential(1.0 / 40.0))
+        arrivals.append(t)
+        sess_ids.append(int(r.integers(0, 64)))
+        progs.append(pzprog.Program(r, int(r.integers(1, 7))))
+    # request 0: one less-than at session 0 (NI_2P), checked against the oracle
+    progs[0] = pzprog.Program(np.random.default_rng(65), 1, ops=("lt",))
+    sess_ids[0] = 0
+
+    def serve(jobs):
+        srv = pzlib.Server(ctxs["NI_2P"], hbm_budget_bytes=1 << 36, pool_rows=1 << 16,
+                           max_batch_pbs=8192)
+        sid = [srv.add_session(se[2], se[3], se[4], ctx=ctxs[se[0]]) for se in sessions]
+        rids = {}
+        for j, arrival in jobs:
+            si = sess_ids[j]
+            circ = pzlib.Circuit(None)
+            wires, nu, nb = progs[j].build(circ)
+            cts = ctxs[sessions[si][0]].encrypt_bools(sessions[si][1], progs[j].input_bits(),
+                                                      6000 + 13 * j)
+            rids[j] = (srv.submit(sid[si], circ, cts, wires, len(progs[j].steps), arrival),
+                       nu, nb, wires, cts)
+            circ.close()
+        srv.run()
+        out = {j: (srv.result(rid), nu, nb, wires, cts)
+               for j, (rid, nu, nb, wires, cts) in rids.items()}
+        st = srv.stats()
+        srv.close()
+        return out, st
+
+    mixed, st = serve(list(enumerate(arrivals)))
+    print("serving 64 sessions: %d requests, %d bootstraps, %d launches, %.2f s" %
+          (count, st["total_pbs"], st["batches"], st["wall_s"]), flush=True)
+    assert st["key_uploads"] == 64 and st["evictions"] == 0
+    for j, (ct, nu, nb, _, _) in mixed.items():
+        name, s = sessions[sess_ids[j]][0], sessions[sess_ids[j]][1]
+        assert pzprog.decode(ctxs[name].decrypt_bools(s, ct), nu, nb) == progs[j].plain(), j
+    picks = [j for j in range(1, count) if sessions[sess_ids[j]][0] == "NI_2P"][:2] + \
+            [j for j in range(1, count) if sessions[sess_ids[j]][0] == "NI_4P_HB_FR"][:2]
+    alone, _ = serve([(j, 0.0) for j in picks])
+    for j in picks:
+        assert np.array_equal(alone[j][0], mixed[j][0]), j
+    o = oracle_for("NI_2P")
+    se = sessions[0]
+    okey = oracle_key_from_rows(o, se[2], se[3], se[4])
+    circ = pzlib.Circuit(None)
+    wires, _, _ = progs[0].build(circ)
+    print("serving 64 sessions: oracle cone of request 0", flush=True)
+    want = oracle_cone(o, okey, circ.nodes(), mixed[0][4], wires)
+    for i, w in enumerate(wires):
+        assert np.array_equal(mixed[0][0][i], want[int(w)]), int(w)
+    circ.close()
+    for c in ctxs.values():
+        c.close()
