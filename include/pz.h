@@ -218,6 +218,8 @@ int pz_circuit_node_levels(pz_circuit *c, uint32_t *h_levels, size_t cap);
  * circuit shape.  Runs of one circuit must not overlap (they share its wire
  * pool), and a run is complete once the stream has drained. */
 int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inputs, void *stream);
+/* d_out[i] = the ciphertext of wire wires[i] from the last pz_circuit_run;
+ * PZ_ERR_STATE before any run or for a wire added after the last run. */
 int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t *d_out,
                     void *stream);
 

@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -18,17 +19,33 @@ using namespace pz;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char *msg)
+// Records the message pz_last_error() returns: "<function>: <what>", plus the
+// HIP runtime's text for PZ_ERR_HIP when a HIP call left an error behind.
+int pz_internal_fail(int code, const char *fn, const char *what)
 {
-    g_err = msg;
+    g_err = std::string(fn) + ": " + what;
+    if (code == PZ_ERR_HIP) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) g_err += std::string(" (") + hipGetErrorString(e) + ")";
+    }
     return code;
 }
+
+int pz_internal_launch_status(const char *fn)
+{
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return PZ_OK;
+    g_err = std::string(fn) + ": kernel launch failed (" + hipGetErrorString(e) + ")";
+    return PZ_ERR_HIP;
+}
+
+#define fail(code, msg) pz_internal_fail((code), __func__, (msg))
 
 #define HIPCHK(x)                                                                     \
     do {                                                                              \
         hipError_t e_ = (x);                                                          \
         if (e_ != hipSuccess) {                                                       \
-            g_err = std::string(#x) + ": " + hipGetErrorString(e_);                   \
+            g_err = std::string(__func__) + ": " #x ": " + hipGetErrorString(e_);      \
             return PZ_ERR_HIP;                                                        \
         }                                                                             \
     } while (0)
@@ -72,17 +89,22 @@ struct pz_ctx {
     // synchronises the stream before it returns)
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    // captured gate batches
+    // captured gate batches, all made with the key whose id is graph_key_id
+    // (ids are never reused, so a key freed elsewhere -- its graphs then hold
+    // dangling pointers -- can never match a later key at the same address)
     std::map<GraphKey, hipGraphExec_t> graphs;
-    const pz_key *graph_key = nullptr;
+    uint64_t graph_key_id = 0;
     // timing
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     float t_prep = 0, t_ks = 0, t_br = 0;
 };
 
+static std::atomic<uint64_t> g_next_key_id{1};
+
 struct pz_key {
     pz_ctx *owner = nullptr;
+    uint64_t id = g_next_key_id.fetch_add(1);  // unique for the process's lifetime
     DevKey dk;
     u64 *rgsw = nullptr, *rgsw_s = nullptr, *auto_k = nullptr, *auto_s = nullptr;
     void *ksk = nullptr;
@@ -531,9 +553,9 @@ extern "C" int pz_key_free(pz_key *k)
     if (!k) return PZ_OK;
     if (k->owner) {
         hipStreamSynchronize(k->owner->stream);
-        if (k->owner->graph_key == k) {
+        if (k->owner->graph_key_id == k->id) {
             drop_graphs(k->owner);
-            k->owner->graph_key = nullptr;
+            k->owner->graph_key_id = 0;
         }
     }
     hipFree(k->rgsw);
@@ -906,9 +928,9 @@ extern "C" int pz_gates(pz_ctx *c, pz_key *k, const uint32_t *ops, const uint64_
         HIPCHK(hipGetLastError());
         return collect_timing(c, st);
     }
-    if (c->graph_key != k) {
+    if (c->graph_key_id != k->id) {
         drop_graphs(c);
-        c->graph_key = k;
+        c->graph_key_id = k->id;
     }
     GraphKey gk{ops, c0, c1, out, B};
     auto it = c->graphs.find(gk);

@@ -167,8 +167,10 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
     const size_t nw = work.size();
     // the previous batch's staging copies read the pinned buffer
     // asynchronously: wait for them before rewriting it
-    if (hipStreamSynchronize(st) != hipSuccess) return PZ_ERR_HIP;
-    if (!g_stage.reserve(5 * B + 2 * nn + nw + 8, keys.size())) return PZ_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipStreamSynchronize failed");
+    if (!g_stage.reserve(5 * B + 2 * nn + nw + 8, keys.size()))
+        return PZ_FAIL(PZ_ERR_HIP, "g_stage.reserve failed");
     u32 *h = g_stage.h;
     for (size_t i = 0; i < B; i++) {
         h[i] = pbs[i].op;
@@ -185,11 +187,11 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
     for (size_t k = 0; k < keys.size(); k++) g_stage.hk[k] = *pz_internal_devkey(keys[k]);
     u32 *d = g_stage.d;
     if (hipMemcpyAsync(d, h, 4 * (5 * B + 2 * nn + nw), hipMemcpyHostToDevice, st) != hipSuccess)
-        return PZ_ERR_HIP;
+        return PZ_FAIL(PZ_ERR_HIP, "hipMemcpyAsync failed");
     if (keys.size() > 1 &&
         hipMemcpyAsync(g_stage.dk, g_stage.hk, sizeof(DevKey) * keys.size(),
                        hipMemcpyHostToDevice, st) != hipSuccess)
-        return PZ_ERR_HIP;
+        return PZ_FAIL(PZ_ERR_HIP, "hipMemcpyAsync failed");
     if (B) {
         launch_prep(PREP_GATE, pool, pool, d + B, d + 2 * B, d, N1, B, v.digits, v.ksb, v.ks, dc,
                     st);
@@ -230,7 +232,7 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
             }
         }
     }
-    return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
+    return PZ_LAUNCH_STATUS();
 }
 
 // ---------------------------------------------------------------------------
@@ -253,6 +255,7 @@ struct pz_circuit {
     int div_flag = -1;        // running div-by-zero flag wire
     u64 *pool = nullptr;
     size_t pool_rows = 0;
+    size_t ran_nodes = 0;     // wires the last run computed (0: never run)
     u32 N1 = 0;
     u32 depth = 0, n_pbs = 0;
     std::vector<std::vector<GateRec>> levels;  // schedule (built lazily)
@@ -468,13 +471,11 @@ struct Builder {
     }
 };
 
-static thread_local std::string c_err;
-
 // ctx may be NULL: the circuit can then be built and inspected (stats,
 // nodes, level sizes) but not run.
 extern "C" int pz_circuit_create(pz_ctx *ctx, pz_circuit **out)
 {
-    if (!out) return PZ_ERR_ARG;
+    if (!out) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !out");
     pz_circuit *c = new pz_circuit();
     c->ctx = ctx;
     *out = c;
@@ -483,7 +484,7 @@ extern "C" int pz_circuit_create(pz_ctx *ctx, pz_circuit **out)
 
 static int bind_ctx(pz_circuit *c)
 {
-    if (!c->ctx) return PZ_ERR_STATE;
+    if (!c->ctx) return PZ_FAIL(PZ_ERR_STATE, "invalid state: !c->ctx");
     if (c->N1) return PZ_OK;
     pz_exec_view v;
     DevCtx dc;
@@ -504,7 +505,7 @@ extern "C" int pz_circuit_destroy(pz_circuit *c)
 
 extern "C" int pz_circuit_inputs(pz_circuit *c, uint32_t count, uint32_t *wires)
 {
-    if (!c || !wires) return PZ_ERR_ARG;
+    if (!c || !wires) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c || !wires");
     for (uint32_t i = 0; i < count; i++) {
         Node n;
         n.kind = 0;
@@ -524,7 +525,7 @@ static bool valid(pz_circuit *c, uint32_t w) { return w < c->nodes.size(); }
 extern "C" int pz_circuit_gate(pz_circuit *c, uint32_t op, uint32_t a, uint32_t b, uint32_t *out)
 {
     if (!c || !out || op > GATE_NOT || !valid(c, a) || (op != GATE_NOT && !valid(c, b)))
-        return PZ_ERR_ARG;
+        return PZ_FAIL(PZ_ERR_ARG, "null argument, unknown gate or operand wire out of range");
     Builder bl{c};
     *out = op == GATE_NOT ? bl.not_(a) : bl.add(1, op, a, b);
     return PZ_OK;
@@ -533,11 +534,13 @@ extern "C" int pz_circuit_gate(pz_circuit *c, uint32_t op, uint32_t a, uint32_t 
 extern "C" int pz_circuit_u8(pz_circuit *c, uint32_t op, const uint32_t *a8, const uint32_t *b8,
                              const uint32_t *sel, uint32_t *out, uint32_t *nout)
 {
-    if (!c || !a8 || !out || !nout) return PZ_ERR_ARG;
-    if (!b8) return PZ_ERR_ARG;
+    if (!c || !a8 || !out || !nout)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c || !a8 || !out || !nout");
+    if (!b8) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !b8");
     std::vector<W> a(a8, a8 + 8), b(b8, b8 + 8);
     for (size_t i = 0; i < 8; i++)
-        if (!valid(c, a[i]) || !valid(c, b[i])) return PZ_ERR_ARG;
+        if (!valid(c, a[i]) || !valid(c, b[i]))
+            return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !valid(c, a[i]) || !valid(c, b[i])");
     Builder bl{c};
     std::vector<W> o;
     switch (op) {
@@ -588,7 +591,8 @@ extern "C" int pz_circuit_u8(pz_circuit *c, uint32_t op, const uint32_t *a8, con
     case PZ_U8_LE: o = {bl.not_(bl.comparator(a, b))}; break;
     case PZ_U8_GE: o = {bl.not_(bl.comparator(b, a))}; break;
     case PZ_U8_MUX:
-        if (!sel || !valid(c, *sel)) return PZ_ERR_ARG;
+        if (!sel || !valid(c, *sel))
+            return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !sel || !valid(c, *sel)");
         o = bl.mux(*sel, a, b);
         break;
     case PZ_U8_MAX: {
@@ -601,7 +605,7 @@ extern "C" int pz_circuit_u8(pz_circuit *c, uint32_t op, const uint32_t *a8, con
         o = bl.mux(lt, a, b);
         break;
     }
-    default: return PZ_ERR_ARG;
+    default: return PZ_FAIL(PZ_ERR_ARG, "unknown FheUint8 operation");
     }
     for (size_t i = 0; i < o.size(); i++) out[i] = o[i];
     *nout = (uint32_t)o.size();
@@ -610,15 +614,15 @@ extern "C" int pz_circuit_u8(pz_circuit *c, uint32_t op, const uint32_t *a8, con
 
 extern "C" int pz_circuit_div_zero_flag(pz_circuit *c, uint32_t *wire)
 {
-    if (!c || !wire) return PZ_ERR_ARG;
-    if (c->div_flag < 0) return PZ_ERR_STATE;
+    if (!c || !wire) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c || !wire");
+    if (c->div_flag < 0) return PZ_FAIL(PZ_ERR_STATE, "invalid state: c->div_flag < 0");
     *wire = (uint32_t)c->div_flag;
     return PZ_OK;
 }
 
 extern "C" int pz_circuit_reset_flags(pz_circuit *c)
 {
-    if (!c) return PZ_ERR_ARG;
+    if (!c) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c");
     c->div_flag = -1;
     return PZ_OK;
 }
@@ -723,7 +727,7 @@ static void schedule(pz_circuit *c)
 // batch of its input's producer, after its bootstrapped gates).
 extern "C" int pz_circuit_node_levels(pz_circuit *c, uint32_t *h_levels, size_t cap)
 {
-    if (!c || !h_levels) return PZ_ERR_ARG;
+    if (!c || !h_levels) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c || !h_levels");
     schedule(c);
     std::vector<uint32_t> lv(c->nodes.size(), 0);
     for (size_t l = 0; l < c->levels.size(); l++)
@@ -734,7 +738,7 @@ extern "C" int pz_circuit_node_levels(pz_circuit *c, uint32_t *h_levels, size_t 
 
 extern "C" int pz_circuit_stats(pz_circuit *c, uint32_t *n_wires, uint32_t *n_pbs, uint32_t *depth)
 {
-    if (!c) return PZ_ERR_ARG;
+    if (!c) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c");
     if (n_wires) *n_wires = (uint32_t)c->nodes.size();
     if (n_pbs) *n_pbs = c->n_pbs;
     if (depth) *depth = c->depth;
@@ -743,7 +747,7 @@ extern "C" int pz_circuit_stats(pz_circuit *c, uint32_t *n_wires, uint32_t *n_pb
 
 extern "C" int pz_circuit_level_sizes(pz_circuit *c, uint32_t *h_sizes, uint32_t cap)
 {
-    if (!c || !h_sizes) return PZ_ERR_ARG;
+    if (!c || !h_sizes) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c || !h_sizes");
     schedule(c);
     for (uint32_t l = 0; l < cap && l < c->levels.size(); l++) {
         uint32_t k = 0;
@@ -760,7 +764,8 @@ static int ensure_pool(pz_circuit *c)
     if (c->pool) (void)hipFree(c->pool);
     c->pool = nullptr;
     size_t cap = std::max<size_t>(rows, 64);
-    if (hipMalloc(&c->pool, 8 * cap * c->N1) != hipSuccess) return PZ_ERR_HIP;
+    if (hipMalloc(&c->pool, 8 * cap * c->N1) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
     c->pool_rows = cap;
     return PZ_OK;
 }
@@ -805,8 +810,10 @@ static void build_plan(pz_circuit *c)
 
 extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inputs, void *stream)
 {
-    if (!c || !key || (!d_inputs && !c->inputs.empty())) return PZ_ERR_ARG;
-    if (!pz_internal_key_fits(c->ctx, key)) return PZ_ERR_ARG;
+    if (!c || !key || (!d_inputs && !c->inputs.empty()))
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c || !key || (!d_inputs && !c->inputs.empty())");
+    if (!pz_internal_key_fits(c->ctx, key))
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !pz_internal_key_fits(c->ctx, key)");
     int rc = bind_ctx(c);
     if (rc) return rc;
     schedule(c);
@@ -819,7 +826,8 @@ extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inpu
     size_t ni = c->inputs.size();
     if (ni) {
         u32 *d_rows = nullptr;
-        if (hipMalloc(&d_rows, 4 * ni) != hipSuccess) return PZ_ERR_HIP;
+        if (hipMalloc(&d_rows, 4 * ni) != hipSuccess)
+            return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
         (void)hipMemcpyAsync(d_rows, c->inputs.data(), 4 * ni, hipMemcpyHostToDevice, st);
         hipLaunchKernelGGL(k_copy_rows, grid_rows(ni * c->N1), 256, 0, st, c->pool, d_inputs,
                            d_rows, ni, c->N1);
@@ -832,17 +840,19 @@ extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inpu
     if (!c->plan_on_device && !c->plan.empty()) {
         // a new schedule: the previous one's launches (this stream, which does
         // not order with the null stream) may still read the old plan
-        if (hipStreamSynchronize(st) != hipSuccess) return PZ_ERR_HIP;
+        if (hipStreamSynchronize(st) != hipSuccess)
+            return PZ_FAIL(PZ_ERR_HIP, "hipStreamSynchronize failed");
         if (c->d_plan_cap < c->plan.size()) {
             if (c->d_plan) (void)hipFree(c->d_plan);
             c->d_plan = nullptr;
             c->d_plan_cap = 0;
-            if (hipMalloc(&c->d_plan, 4 * c->plan.size()) != hipSuccess) return PZ_ERR_HIP;
+            if (hipMalloc(&c->d_plan, 4 * c->plan.size()) != hipSuccess)
+                return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
             c->d_plan_cap = c->plan.size();
         }
         if (hipMemcpy(c->d_plan, c->plan.data(), 4 * c->plan.size(), hipMemcpyHostToDevice) !=
             hipSuccess)
-            return PZ_ERR_HIP;
+            return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
         c->plan_on_device = true;
     }
     pz_exec_view v;
@@ -869,21 +879,30 @@ extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inpu
                                c->d_plan + w.off, c->d_plan + w.off + w.count, (size_t)w.count, N1,
                                v.p.rlwe_q);
     }
-    return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
+    rc = PZ_LAUNCH_STATUS();
+    if (rc == PZ_OK) c->ran_nodes = c->nodes.size();
+    return rc;
 }
 
 extern "C" int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t *d_out,
                                void *stream)
 {
-    if (!c || !wires || !d_out) return PZ_ERR_ARG;
+    if (!c || !wires || !d_out) return PZ_FAIL(PZ_ERR_ARG, "null argument");
     for (size_t i = 0; i < count; i++)
-        if (!valid(c, wires[i])) return PZ_ERR_ARG;
-    if (!c->pool || bind_ctx(c)) return PZ_ERR_STATE;
+        if (!valid(c, wires[i])) return PZ_FAIL(PZ_ERR_ARG, "wire out of range");
+    if (!c->pool || c->ran_nodes == 0 || bind_ctx(c))
+        return PZ_FAIL(PZ_ERR_STATE, "the circuit has not been run on a device");
+    // the pool holds only the wires of the last run: a wire added since then
+    // has no value yet (and may lie past the pool's rows)
+    for (size_t i = 0; i < count; i++)
+        if (wires[i] >= c->ran_nodes)
+            return PZ_FAIL(PZ_ERR_STATE, "wire added after the last run: run the circuit again");
     void *sv = nullptr;
     pz_ctx_stream(c->ctx, &sv);
     hipStream_t st = stream ? (hipStream_t)stream : (hipStream_t)sv;
     u32 *d_rows = nullptr;
-    if (hipMalloc(&d_rows, 4 * std::max<size_t>(count, 1)) != hipSuccess) return PZ_ERR_HIP;
+    if (hipMalloc(&d_rows, 4 * std::max<size_t>(count, 1)) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
     (void)hipMemcpyAsync(d_rows, wires, 4 * count, hipMemcpyHostToDevice, st);
     hipLaunchKernelGGL(k_gather_rows, grid_rows(count * c->N1), 256, 0, st, c->pool, d_out, d_rows,
                        count, c->N1);
@@ -895,7 +914,7 @@ extern "C" int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t coun
 // Node list for inspection: kind, op, a, b per node (4 words each).
 extern "C" int pz_circuit_nodes(pz_circuit *c, uint32_t *h_out, size_t cap)
 {
-    if (!c || !h_out) return PZ_ERR_ARG;
+    if (!c || !h_out) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !c || !h_out");
     for (size_t i = 0; i < c->nodes.size() && i < cap; i++) {
         h_out[4 * i] = c->nodes[i].kind;
         h_out[4 * i + 1] = c->nodes[i].op;

@@ -98,13 +98,13 @@ int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out)
     k->d = d;
     if (hipMalloc(&k->rows_s, 8 * words) != hipSuccess) {
         delete k;
-        return PZ_ERR_HIP;
+        return PZ_FAIL(PZ_ERR_HIP, "hipMalloc of the Shoup twins failed");
     }
     launch_shoup(d_rows, k->rows_s, words, p.rlwe_q, st);
     if (hipStreamSynchronize(st) != hipSuccess) {
         hipFree(k->rows_s);
         delete k;
-        return PZ_ERR_HIP;
+        return PZ_FAIL(PZ_ERR_HIP, "Shoup twins of the Ksk rows failed");
     }
     k->rows = d_rows;
     *out = k;
@@ -117,16 +117,17 @@ extern "C" int pz_rlwe_ksk_upload(pz_ctx *c, const uint64_t *h_rows, pz_rlwe_ksk
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!h_rows || !out) return PZ_ERR_ARG;
+    if (!h_rows || !out) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !h_rows || !out");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (p.uitos_d == 0) return PZ_ERR_UNSUPPORTED;
+    if (p.uitos_d == 0) return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: p.uitos_d == 0");
     const size_t words = 2 * (size_t)p.uitos_d * p.rlwe_n;
     u64 *rows = nullptr;
-    if (hipMalloc(&rows, 8 * words) != hipSuccess) return PZ_ERR_HIP;
+    if (hipMalloc(&rows, 8 * words) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
     if (hipMemcpy(rows, h_rows, 8 * words, hipMemcpyHostToDevice) != hipSuccess) {
         hipFree(rows);
-        return PZ_ERR_HIP;
+        return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy of the Ksk rows failed");
     }
     rc = pz_internal_rlwe_ksk_adopt(c, rows, p.uitos_d, out);
     if (rc) hipFree(rows);
@@ -135,17 +136,18 @@ extern "C" int pz_rlwe_ksk_upload(pz_ctx *c, const uint64_t *h_rows, pz_rlwe_ksk
 
 extern "C" int pz_rlwe_ksk_download(pz_rlwe_ksk *k, uint64_t *h_rows)
 {
-    if (!k || !h_rows) return PZ_ERR_ARG;
+    if (!k || !h_rows) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !k || !h_rows");
     pz_params p;
     DevCtx dc;
     int logn;
     hipStream_t st;
     int rc = pz_internal_ctx_view(k->owner, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (hipStreamSynchronize(st) != hipSuccess) return PZ_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipStreamSynchronize failed");
     if (hipMemcpy(h_rows, k->rows, 16 * (size_t)k->d * p.rlwe_n, hipMemcpyDeviceToHost) !=
         hipSuccess)
-        return PZ_ERR_HIP;
+        return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
     return PZ_OK;
 }
 
@@ -170,11 +172,13 @@ extern "C" int pz_rlwe_key_switch(pz_ctx *c, pz_rlwe_ksk *k, const uint64_t *d_i
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!k || !d_in || !d_out) return PZ_ERR_ARG;
+    if (!k || !d_in || !d_out)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !k || !d_in || !d_out");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (p.uitos_d == 0) return PZ_ERR_UNSUPPORTED;
-    if (!pz_internal_ctx_compatible(c, k->owner)) return PZ_ERR_ARG;
+    if (p.uitos_d == 0) return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: p.uitos_d == 0");
+    if (!pz_internal_ctx_compatible(c, k->owner))
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !pz_internal_ctx_compatible(c, k->owner)");
     if (B == 0) return PZ_OK;
     if (stream) st = (hipStream_t)stream;
     const int grid = (int)(B < 65535 ? B : 65535);
@@ -189,9 +193,9 @@ extern "C" int pz_rlwe_key_switch(pz_ctx *c, pz_rlwe_ksk *k, const uint64_t *d_i
                            k->rows_s, k->d);
         break;
     default:
-        return PZ_ERR_UNSUPPORTED;
+        return PZ_FAIL(PZ_ERR_UNSUPPORTED, "ring size must be 512 or 2048");
     }
-    return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
+    return PZ_LAUNCH_STATUS();
 }
 
 extern "C" int pz_sample_extract(pz_ctx *c, const uint64_t *d_rlwe, const uint32_t *d_src,
@@ -201,7 +205,8 @@ extern "C" int pz_sample_extract(pz_ctx *c, const uint64_t *d_rlwe, const uint32
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!d_rlwe || !d_src || !d_lwe) return PZ_ERR_ARG;
+    if (!d_rlwe || !d_src || !d_lwe)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !d_rlwe || !d_src || !d_lwe");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     if (count == 0) return PZ_OK;
@@ -210,7 +215,7 @@ extern "C" int pz_sample_extract(pz_ctx *c, const uint64_t *d_rlwe, const uint32
     const int grid = (int)((total + 255) / 256 < 65535 * 4 ? (total + 255) / 256 : 65535 * 4);
     hipLaunchKernelGGL(k_sample_extract, grid, 256, 0, st, d_rlwe, d_src, count, p.rlwe_n,
                        p.rlwe_q, d_lwe);
-    return hipGetLastError() == hipSuccess ? PZ_OK : PZ_ERR_HIP;
+    return PZ_LAUNCH_STATUS();
 }
 
 // The context with the RGSW x RGSW decomposers in place of the RLWE x RGSW
@@ -240,10 +245,11 @@ extern "C" int pz_rgsw_by_rgsw(pz_ctx *c, uint64_t *d_rgsw0, const uint64_t *d_r
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!d_rgsw0 || !d_rgsw1) return PZ_ERR_ARG;
+    if (!d_rgsw0 || !d_rgsw1) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !d_rgsw0 || !d_rgsw1");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (p.rgrg_da == 0 || p.rgrg_db == 0) return PZ_ERR_UNSUPPORTED;
+    if (p.rgrg_da == 0 || p.rgrg_db == 0)
+        return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: p.rgrg_da == 0 || p.rgrg_db == 0");
     if (B == 0) return PZ_OK;
     if (stream) st = (hipStream_t)stream;
     const u32 N = p.rlwe_n, r0 = 2 * (p.rlrg_da + p.rlrg_db), r1 = 2 * (p.rgrg_da + p.rgrg_db);
@@ -260,6 +266,6 @@ extern "C" int pz_rgsw_by_rgsw(pz_ctx *c, uint64_t *d_rgsw0, const uint64_t *d_r
     launch_rgsw_rows(logn, d_rgsw0, r0, p.rlrg_da, p.rlrg_db, B, cc, k1, k1s, r1, st);
     hipError_t e = hipGetLastError();
     if (hipStreamSynchronize(st) != hipSuccess) e = hipErrorUnknown;  // workspace free again
-    return e == hipSuccess ? PZ_OK : PZ_ERR_HIP;
+    return e == hipSuccess ? PZ_OK : PZ_FAIL(PZ_ERR_HIP, hipGetErrorString(e));
 }
 

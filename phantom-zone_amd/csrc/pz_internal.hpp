@@ -72,6 +72,14 @@ void launch_pack_ksk(const u64 *in, void *out, size_t rows, u32 n1, u32 ldk, u32
 
 }  // namespace pz
 
+// Sets the message pz_last_error() returns ("<fn>: <what>"; for PZ_ERR_HIP
+// the pending HIP error's text is appended) and returns `code`.
+int pz_internal_fail(int code, const char *fn, const char *what);
+#define PZ_FAIL(code, what) pz_internal_fail((code), __func__, (what))
+// PZ_OK, or PZ_ERR_HIP with the error's text when the last launch failed
+int pz_internal_launch_status(const char *fn);
+#define PZ_LAUNCH_STATUS() pz_internal_launch_status(__func__)
+
 // Key-switching key for RLWE ciphertexts (Ksk(u -> s)): [2d][N] rows in the
 // evaluation domain, A_0..A_{d-1} then B_0..B_{d-1}, with Shoup twins.
 struct pz_rlwe_ksk {

@@ -124,8 +124,6 @@ extern int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, 
 extern int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows,
                                  pz_key **out);
 
-static thread_local std::string kg_err;
-
 extern "C" int pz_keygen_secrets(pz_ctx *c, const uint8_t *seed, uint32_t parties,
                                  int32_t *h_s, int32_t *h_z)
 {
@@ -133,7 +131,8 @@ extern "C" int pz_keygen_secrets(pz_ctx *c, const uint8_t *seed, uint32_t partie
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!seed || !h_s || !h_z || parties == 0) return PZ_ERR_ARG;
+    if (!seed || !h_s || !h_z || parties == 0)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !seed || !h_s || !h_z || parties == 0");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     host::keygen_secrets(p, seed, parties, h_s, h_z);
@@ -143,9 +142,16 @@ extern "C" int pz_keygen_secrets(pz_ctx *c, const uint8_t *seed, uint32_t partie
 #define KG_CHK(x)                                 \
     do {                                          \
         if ((x) != hipSuccess) {                  \
+            PZ_FAIL(PZ_ERR_HIP, #x);              \
             ok = false;                           \
             goto done;                            \
         }                                         \
+    } while (0)
+#define KG_FAIL(what)                             \
+    do {                                          \
+        PZ_FAIL(PZ_ERR_HIP, what);                \
+        ok = false;                               \
+        goto done;                                \
     } while (0)
 
 extern "C" int pz_keygen_server(pz_ctx *c, const uint8_t *seed, const int32_t *h_s,
@@ -155,7 +161,8 @@ extern "C" int pz_keygen_server(pz_ctx *c, const uint8_t *seed, const int32_t *h
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!seed || !h_s || !h_z || !out) return PZ_ERR_ARG;
+    if (!seed || !h_s || !h_z || !out)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !seed || !h_s || !h_z || !out");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     const u64 q = p.rlwe_q;
@@ -306,7 +313,7 @@ done:
     hipFree(d_zi);
     hipFree(d_err);
     hipFree(d_pairs);
-    if (!ok) return PZ_ERR_HIP;
+    if (!ok) return PZ_ERR_HIP;  // message set by KG_CHK / KG_FAIL
     return rc;
 }
 
@@ -331,7 +338,8 @@ extern "C" int pz_encrypt_bools(pz_ctx *c, const int32_t *h_s, uint32_t dim, con
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!h_s || !seed || !h_bits || !h_out) return PZ_ERR_ARG;
+    if (!h_s || !seed || !h_bits || !h_out)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !h_s || !seed || !h_bits || !h_out");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     const u64 q = p.rlwe_q, tru = q >> 3, fal = (q - 1) - tru + 1;
@@ -364,7 +372,8 @@ extern "C" int pz_decrypt_bools(pz_ctx *c, const int32_t *h_s, uint32_t dim, con
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!h_s || !h_cts || !h_bits) return PZ_ERR_ARG;
+    if (!h_s || !h_cts || !h_bits)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !h_s || !h_cts || !h_bits");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     const u64 q = p.rlwe_q, qby8 = q >> 3;
@@ -395,7 +404,7 @@ extern "C" int pz_keygen_user_secret(pz_ctx *c, const uint8_t *seed, int32_t *h_
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!seed || !h_u) return PZ_ERR_ARG;
+    if (!seed || !h_u) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !seed || !h_u");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     host::ChaCha8 r(seed);
@@ -413,10 +422,11 @@ extern "C" int pz_keygen_rlwe_ksk(pz_ctx *c, const uint8_t *seed, const int32_t 
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!seed || !h_u || !h_s || !out) return PZ_ERR_ARG;
+    if (!seed || !h_u || !h_s || !out)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !seed || !h_u || !h_s || !out");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (p.uitos_d == 0) return PZ_ERR_UNSUPPORTED;
+    if (p.uitos_d == 0) return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: p.uitos_d == 0");
     const u64 q = p.rlwe_q;
     const u32 N = p.rlwe_n, d = p.uitos_d;
     host::ChaCha8 r(seed);
@@ -480,7 +490,7 @@ done:
     hipFree(d_u);
     hipFree(d_rows);
     hipFree(d_pairs);
-    if (!ok) return PZ_ERR_HIP;
+    if (!ok) return PZ_ERR_HIP;  // message set by KG_CHK / KG_FAIL
     return rc;
 }
 
@@ -495,7 +505,8 @@ extern "C" int pz_encrypt_rlwe_bools(pz_ctx *c, const int32_t *h_u, const uint8_
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!h_u || !seed || !h_bits || !h_out) return PZ_ERR_ARG;
+    if (!h_u || !seed || !h_bits || !h_out)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !h_u || !seed || !h_bits || !h_out");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     if (count == 0) return PZ_OK;
@@ -597,8 +608,10 @@ static bool ni_combine(hipStream_t st, const DevCtx &dc, const std::vector<NiRow
 
 static int ni_check(const pz_params &p, uint32_t self, uint32_t total)
 {
-    if (p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0) return PZ_ERR_UNSUPPORTED;
-    if (total == 0 || total > PZ_NI_MAX_PARTIES || self >= total) return PZ_ERR_ARG;
+    if (p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0)
+        return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0");
+    if (total == 0 || total > PZ_NI_MAX_PARTIES || self >= total)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: total == 0 || total > PZ_NI_MAX_PARTIES || self >= total");
     return PZ_OK;
 }
 
@@ -611,7 +624,7 @@ extern "C" int pz_ni_share_uitos(pz_ctx *c, const uint8_t *crs, const uint8_t *n
     int logn;
     hipStream_t st;
     if (!crs || !noise_seed || !h_s || !h_u || !h_ksk_b || (total > 1 && !h_zero_others))
-        return PZ_ERR_ARG;
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !crs || !noise_seed || !h_s || !h_u || !h_ksk_b || (total > 1 && !h_zero_others");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     if ((rc = ni_check(p, self, total))) return rc;
@@ -647,7 +660,7 @@ extern "C" int pz_ni_share_uitos(pz_ctx *c, const uint8_t *crs, const uint8_t *n
     KG_CHK(hipMemcpyAsync(d_a, ha.data(), 8 * R * N, hipMemcpyHostToDevice, st));
     KG_CHK(hipMemcpyAsync(d_e, he.data(), 8 * R * N, hipMemcpyHostToDevice, st));
     KG_CHK(hipMemcpyAsync(d_u, hu.data(), 8 * (size_t)N, hipMemcpyHostToDevice, st));
-    if (!ni_products(logn, dc, st, d_a, R, h_s, N, q, d_prod)) KG_CHK(hipErrorUnknown);
+    if (!ni_products(logn, dc, st, d_a, R, h_s, N, q, d_prod)) KG_FAIL("products with the secret failed");
     for (size_t r = 0; r < R; r++) {
         NiRow &x = rows[r];
         x.prod = d_prod + r * N;
@@ -658,7 +671,7 @@ extern "C" int pz_ni_share_uitos(pz_ctx *c, const uint8_t *crs, const uint8_t *n
         x.mono_pos = 0;
         x.mono_val = 0;
     }
-    if (!ni_combine(st, dc, rows, N)) KG_CHK(hipErrorUnknown);
+    if (!ni_combine(st, dc, rows, N)) KG_FAIL("row combination failed");
     KG_CHK(hipMemcpy(h_ksk_b, d_out, 8 * (size_t)d * N, hipMemcpyDeviceToHost));
     if (total > 1)
         KG_CHK(hipMemcpy(h_zero_others, d_out + (size_t)d * N, 8 * (R - d) * N,
@@ -681,11 +694,13 @@ extern "C" int pz_ni_share_rgsw(pz_ctx *c, const uint8_t *crs, const uint8_t *no
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!crs || !noise_seed || !h_s || !h_u || !h_z || !h_zero || !h_cts) return PZ_ERR_ARG;
+    if (!crs || !noise_seed || !h_s || !h_u || !h_z || !h_zero || !h_cts)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !crs || !noise_seed || !h_s || !h_u || !h_z || !h_zero || !h_cts");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
     if ((rc = ni_check(p, self, total))) return rc;
-    if ((uint64_t)l0 + count > p.lwe_n) return PZ_ERR_ARG;
+    if ((uint64_t)l0 + count > p.lwe_n)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: (uint64_t)l0 + count > p.lwe_n");
     if (count == 0) return PZ_OK;
     const u64 q = p.rlwe_q;
     const u32 N = p.rlwe_n, da = p.rgrg_da;
@@ -728,9 +743,9 @@ extern "C" int pz_ni_share_rgsw(pz_ctx *c, const uint8_t *crs, const uint8_t *no
     KG_CHK(hipMemcpyAsync(d_ec, hec.data(), 8 * A, hipMemcpyHostToDevice, st));
     KG_CHK(hipMemsetAsync(d_cts, 0, 8 * A, st));
     if (!ni_products(logn, dc, st, d_a, (size_t)count * dmax, h_s, N, q, d_ps))
-        KG_CHK(hipErrorUnknown);
+        KG_FAIL("products with the secret failed");
     if (!ni_products(logn, dc, st, d_a, (size_t)count * dmax, h_u, N, q, d_pu))
-        KG_CHK(hipErrorUnknown);
+        KG_FAIL("products with the secret failed");
     for (u32 t = 0; t < count; t++) {
         const u32 l = l0 + t;
         const bool lead = l / per == self;
@@ -752,7 +767,7 @@ extern "C" int pz_ni_share_rgsw(pz_ctx *c, const uint8_t *crs, const uint8_t *no
                             1ull << (ib + logb * (i - first)), pos, val});
         }
     }
-    if (!ni_combine(st, dc, rows, N)) KG_CHK(hipErrorUnknown);
+    if (!ni_combine(st, dc, rows, N)) KG_FAIL("row combination failed");
     KG_CHK(hipMemcpy(h_zero, d_zero, 8 * Z, hipMemcpyDeviceToHost));
     KG_CHK(hipMemcpy(h_cts, d_cts, 8 * A, hipMemcpyDeviceToHost));
 done:

@@ -258,11 +258,13 @@ extern "C" int pz_ni_uitos_ksk(pz_ctx *c, const uint8_t *crs, uint32_t total, ui
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!crs || !d_ksk_b || !out || (total > 1 && !d_zero_for_user)) return PZ_ERR_ARG;
+    if (!crs || !d_ksk_b || !out || (total > 1 && !d_zero_for_user))
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !crs || !d_ksk_b || !out || (total > 1 && !d_zero_for_user)");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (p.uitos_d == 0) return PZ_ERR_UNSUPPORTED;
-    if (total == 0 || total > PZ_NI_MAX_PARTIES || user >= total) return PZ_ERR_ARG;
+    if (p.uitos_d == 0) return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: p.uitos_d == 0");
+    if (total == 0 || total > PZ_NI_MAX_PARTIES || user >= total)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: total == 0 || total > PZ_NI_MAX_PARTIES || user >= total");
     const u32 N = p.rlwe_n, d = p.uitos_d;
     const size_t words = (size_t)d * N;
     uint8_t sd[32];
@@ -270,7 +272,8 @@ extern "C" int pz_ni_uitos_ksk(pz_ctx *c, const uint8_t *crs, uint32_t total, ui
     void *ws = nullptr;
     if ((rc = pz_internal_workspace(c, 64, &ws))) return rc;
     u64 *rows = nullptr;
-    if (hipMalloc(&rows, 16 * words) != hipSuccess) return PZ_ERR_HIP;
+    if (hipMalloc(&rows, 16 * words) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
     bool ok = expand_rows(st, sd, 1, words, 0, p.rlwe_q, rows, (u32 *)ws) &&
               hipMemcpyAsync(rows + words, d_ksk_b, 8 * words, hipMemcpyDeviceToDevice, st) ==
                   hipSuccess;
@@ -284,7 +287,7 @@ extern "C" int pz_ni_uitos_ksk(pz_ctx *c, const uint8_t *crs, uint32_t total, ui
     if (ok) rc = pz_internal_rlwe_ksk_adopt(c, rows, d, out);
     if (!ok || rc) {
         hipFree(rows);
-        return ok ? rc : PZ_ERR_HIP;
+        return ok ? rc : PZ_FAIL(PZ_ERR_HIP, "seed expansion or transform of the Ksk rows failed");
     }
     return PZ_OK;
 }
@@ -298,17 +301,23 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
     DevCtx dc;
     int logn;
     hipStream_t st;
-    if (!crs || !ksks || !d_zero || !d_cts || !d_out) return PZ_ERR_ARG;
+    if (!crs || !ksks || !d_zero || !d_cts || !d_out)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !crs || !ksks || !d_zero || !d_cts || !d_out");
     int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0) return PZ_ERR_UNSUPPORTED;
+    if (p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0)
+        return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: p.uitos_d == 0 || p.rgrg_da == 0 || p.rgrg_db == 0");
     // the leader's rows are the last of each part (the reference requires
     // RLWE x RGSW counts below the RGSW x RGSW ones, src/bool/evaluator.rs:1707-1716)
-    if (p.rlrg_da > p.rgrg_da || p.rlrg_db > p.rgrg_db) return PZ_ERR_UNSUPPORTED;
-    if (total == 0 || total > PZ_NI_MAX_PARTIES) return PZ_ERR_ARG;
-    if ((uint64_t)l0 + count > p.lwe_n) return PZ_ERR_ARG;
+    if (p.rlrg_da > p.rgrg_da || p.rlrg_db > p.rgrg_db)
+        return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: p.rlrg_da > p.rgrg_da || p.rlrg_db > p.rgrg_db");
+    if (total == 0 || total > PZ_NI_MAX_PARTIES)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: total == 0 || total > PZ_NI_MAX_PARTIES");
+    if ((uint64_t)l0 + count > p.lwe_n)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: (uint64_t)l0 + count > p.lwe_n");
     for (u32 o = 0; o < total; o++)
-        if (!ksks[o] || !pz_internal_ctx_compatible(c, ksks[o]->owner)) return PZ_ERR_ARG;
+        if (!ksks[o] || !pz_internal_ctx_compatible(c, ksks[o]->owner))
+            return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !ksks[o] || !pz_internal_ctx_compatible(c, ksks[o]->owner)");
     if (count == 0) return PZ_OK;
     if (stream) st = (hipStream_t)stream;
     const u32 N = p.rlwe_n, da = p.rgrg_da, db = p.rgrg_db, dmax = da > db ? da : db;
@@ -368,5 +377,5 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
         }
         ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
     }
-    return ok ? PZ_OK : PZ_ERR_HIP;
+    return ok ? PZ_OK : PZ_FAIL(PZ_ERR_HIP, "aggregation launch failed");
 }

@@ -108,8 +108,9 @@ struct pz_server {
 
 extern "C" int pz_server_create(pz_ctx *ctx, const pz_server_config *cfg, pz_server **out)
 {
-    if (!ctx || !cfg || !out) return PZ_ERR_ARG;
-    if (cfg->pool_rows == 0 || cfg->max_batch_pbs == 0) return PZ_ERR_ARG;
+    if (!ctx || !cfg || !out) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !ctx || !cfg || !out");
+    if (cfg->pool_rows == 0 || cfg->max_batch_pbs == 0)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: cfg->pool_rows == 0 || cfg->max_batch_pbs == 0");
     pz_server *s = new pz_server();
     s->ctxs.push_back(ctx);
     s->cfg = *cfg;
@@ -123,7 +124,7 @@ extern "C" int pz_server_create(pz_ctx *ctx, const pz_server_config *cfg, pz_ser
     s->N1 = v.p.rlwe_n + 1;
     if (hipMalloc(&s->pool, 8 * (size_t)cfg->pool_rows * s->N1) != hipSuccess) {
         delete s;
-        return PZ_ERR_HIP;
+        return PZ_FAIL(PZ_ERR_HIP, "hipMalloc of the wire pool failed");
     }
     s->rows.init(cfg->pool_rows);
     *out = s;
@@ -169,7 +170,8 @@ static int add_session_on(pz_server *s, u32 ci, const uint64_t *h_rgsw, const ui
 extern "C" int pz_server_add_session(pz_server *s, const uint64_t *h_rgsw, const uint64_t *h_auto,
                                      const uint64_t *h_ksk, uint32_t *sid)
 {
-    if (!s || !h_rgsw || !h_auto || !h_ksk || !sid) return PZ_ERR_ARG;
+    if (!s || !h_rgsw || !h_auto || !h_ksk || !sid)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s || !h_rgsw || !h_auto || !h_ksk || !sid");
     return add_session_on(s, 0, h_rgsw, h_auto, h_ksk, sid);
 }
 
@@ -177,7 +179,8 @@ extern "C" int pz_server_add_session_ctx(pz_server *s, pz_ctx *ctx, const uint64
                                          const uint64_t *h_auto, const uint64_t *h_ksk,
                                          uint32_t *sid)
 {
-    if (!s || !ctx || !h_rgsw || !h_auto || !h_ksk || !sid) return PZ_ERR_ARG;
+    if (!s || !ctx || !h_rgsw || !h_auto || !h_ksk || !sid)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s || !ctx || !h_rgsw || !h_auto || !h_ksk || !sid");
     u32 ci = 0;
     while (ci < s->ctxs.size() && s->ctxs[ci] != ctx) ci++;
     if (ci == s->ctxs.size()) {
@@ -188,7 +191,7 @@ extern "C" int pz_server_add_session_ctx(pz_server *s, pz_ctx *ctx, const uint64
         if (rc) return rc;
         if (pz_internal_ctx_device(ctx) != pz_internal_ctx_device(s->ctxs[0]) ||
             v.p.rlwe_n + 1 != s->N1)
-            return PZ_ERR_UNSUPPORTED;
+            return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: pz_internal_ctx_device(ctx) != pz_internal_ctx_device(s->ctxs[0]) || v.p.rlwe_n + 1 != s->N1");
         s->ctxs.push_back(ctx);
     }
     return add_session_on(s, ci, h_rgsw, h_auto, h_ksk, sid);
@@ -198,7 +201,8 @@ extern "C" int pz_server_submit(pz_server *s, uint32_t session, pz_circuit *prog
                                 const uint64_t *h_inputs, const uint32_t *out_wires,
                                 uint32_t n_out, uint32_t n_ops, double arrival, uint32_t *rid)
 {
-    if (!s || !prog || !out_wires || !rid || session >= s->sessions.size()) return PZ_ERR_ARG;
+    if (!s || !prog || !out_wires || !rid || session >= s->sessions.size())
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s || !prog || !out_wires || !rid || session >= s->sessions.size()");
     uint32_t nw = 0, npbs = 0, depth = 0;
     pz_circuit_stats(prog, &nw, &npbs, &depth);
     std::vector<uint32_t> nodes(4 * (size_t)nw);
@@ -226,17 +230,19 @@ extern "C" int pz_server_submit(pz_server *s, uint32_t session, pz_circuit *prog
     r.levels.assign(maxl + 1, {});
     for (uint32_t i = 0; i < nw; i++)
         if (r.kind[i] != 0) r.levels[r.level[i]].push_back(i);
-    if (!r.inputs.empty() && !h_inputs) return PZ_ERR_ARG;
+    if (!r.inputs.empty() && !h_inputs)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !r.inputs.empty() && !h_inputs");
     r.h_in.assign(h_inputs, h_inputs + r.inputs.size() * (size_t)s->N1);
     for (uint32_t i = 0; i < n_out; i++) {
-        if (out_wires[i] >= nw) return PZ_ERR_ARG;
+        if (out_wires[i] >= nw) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: out_wires[i] >= nw");
         r.outs.push_back(out_wires[i]);
     }
     r.n_ops = n_ops ? n_ops : 1;
     r.n_pbs = npbs;
     r.arrival = arrival;
     r.rows = nw;
-    if (nw > s->cfg.pool_rows) return PZ_ERR_UNSUPPORTED;
+    if (nw > s->cfg.pool_rows)
+        return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: nw > s->cfg.pool_rows");
     s->reqs.push_back(std::move(r));
     *rid = (uint32_t)(s->reqs.size() - 1);
     return PZ_OK;
@@ -279,7 +285,7 @@ static bool make_resident(pz_server *s, uint32_t sid)
 
 extern "C" int pz_server_run(pz_server *s)
 {
-    if (!s) return PZ_ERR_ARG;
+    if (!s) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s");
     const size_t nc = s->ctxs.size();
     std::vector<hipStream_t> streams(nc);
     for (size_t c = 0; c < nc; c++) {
@@ -338,7 +344,7 @@ extern "C" int pz_server_run(pz_server *s)
                 if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
                 continue;
             }
-            if (!pending.empty()) return PZ_ERR_STATE;  // cannot admit (budget too small)
+            if (!pending.empty()) return PZ_FAIL(PZ_ERR_STATE, "invalid state: !pending.empty()");  // cannot admit (budget too small)
             break;
         }
         // one batch per context: the next level of every active request of
@@ -382,7 +388,8 @@ extern "C" int pz_server_run(pz_server *s)
         }
         for (size_t c = 0; c < nc; c++) {
             if (recs[c].empty()) continue;
-            if (hipStreamSynchronize(streams[c]) != hipSuccess) return PZ_ERR_HIP;
+            if (hipStreamSynchronize(streams[c]) != hipSuccess)
+                return PZ_FAIL(PZ_ERR_HIP, "hipStreamSynchronize failed");
             s->total_pbs += pbs[c];
             s->batches++;
         }
@@ -416,9 +423,10 @@ extern "C" int pz_server_run(pz_server *s)
 
 extern "C" int pz_server_result(pz_server *s, uint32_t rid, uint64_t *h_out)
 {
-    if (!s || rid >= s->reqs.size() || !h_out) return PZ_ERR_ARG;
+    if (!s || rid >= s->reqs.size() || !h_out)
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s || rid >= s->reqs.size() || !h_out");
     Request &r = s->reqs[rid];
-    if (r.state != 3) return PZ_ERR_STATE;
+    if (r.state != 3) return PZ_FAIL(PZ_ERR_STATE, "invalid state: r.state != 3");
     memcpy(h_out, r.result.data(), 8 * r.result.size());
     return PZ_OK;
 }
@@ -427,7 +435,8 @@ extern "C" int pz_server_request_times(pz_server *s, uint32_t rid, double *arriv
                                        double *admitted, double *finished, uint32_t *n_pbs,
                                        uint32_t *n_ops)
 {
-    if (!s || rid >= s->reqs.size()) return PZ_ERR_ARG;
+    if (!s || rid >= s->reqs.size())
+        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s || rid >= s->reqs.size()");
     Request &r = s->reqs[rid];
     if (arrival) *arrival = r.arrival;
     if (admitted) *admitted = r.admitted;
@@ -440,7 +449,7 @@ extern "C" int pz_server_request_times(pz_server *s, uint32_t rid, double *arriv
 extern "C" int pz_server_stats(pz_server *s, uint64_t *total_pbs, uint64_t *batches,
                                uint64_t *key_uploads, uint64_t *evictions, double *wall_s)
 {
-    if (!s) return PZ_ERR_ARG;
+    if (!s) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s");
     if (total_pbs) *total_pbs = s->total_pbs;
     if (batches) *batches = s->batches;
     if (key_uploads) *key_uploads = s->key_uploads;

@@ -185,3 +185,64 @@ def test_ksk_across_contexts():
         a.close()
         b.close()
         c.close()
+
+
+def test_graph_cache_after_key_freed_elsewhere(env):
+    """A key used (graph-captured) on a second context, then freed by its
+    owner, then replaced by another key: the second context's next gates on
+    the same buffers use the new key (the cache is keyed by key id, never by
+    an address that a new key can reuse) and equal the oracle's words."""
+    import pzlib
+    pzlib_, ctx, k, s = env
+    log("edges: graph cache after a key is freed on another context")
+    o, s0, z0, key = keys_for("NI_2P")
+    _, _, _, key2 = keys_for("NI_2P", seed_key=7)  # same secret, other key material
+    other = pzlib.Context("NI_2P")
+    try:
+        r = np.random.default_rng(41)
+        B = 4
+        b0 = r.integers(0, 2, B).astype(np.uint8)
+        b1 = r.integers(0, 2, B).astype(np.uint8)
+        c0, c1 = other.encrypt_bools(s0, b0, 610), other.encrypt_bools(s0, b1, 620)
+        d0, d1 = other.to_device(c0), other.to_device(c1)
+        ops = np.array([0, 4, 0, 4], dtype=np.uint32)
+        dops, dout = other.to_device(ops), other.array((B, other.N + 1))
+        ka = ctx.key_upload(key.rgsw, key.auto_key, key.lwe_ksk)
+        other.gates(ka, dops, d0, d1, dout, B)  # captures a graph with ka on `other`
+        first = dout.download()
+        ka.free()
+        kb = ctx.key_upload(key2.rgsw, key2.auto_key, key2.lwe_ksk)
+        other.gates(kb, dops, d0, d1, dout, B)
+        got = dout.download()
+        for i in range(B):
+            assert np.array_equal(got[i], o.gate(key2, int(ops[i]), c0[i], c1[i])), i
+            assert np.array_equal(first[i], o.gate(key, int(ops[i]), c0[i], c1[i])), i
+        kb.free()
+    finally:
+        other.close()
+
+
+def test_circuit_read_refuses_wires_added_after_run(env):
+    """Reading a wire added after the last run is refused (PZ_ERR_STATE): the
+    pool holds no value for it and may not hold its row; the wires of the run
+    stay readable, and after another run the new wire is readable too."""
+    import pzlib
+    pzlib_, ctx, k, s = env
+    log("edges: circuit read of a wire added after the run")
+    circ = pzlib.Circuit(ctx)
+    w = circ.inputs(2)
+    g = circ.gate("nand", int(w[0]), int(w[1]))
+    cts = ctx.encrypt_bools(s, np.array([1, 1], dtype=np.uint8), 630)
+    d_in = ctx.to_device(cts)
+    with pytest.raises(pzlib.PzError):
+        circ.read([g])  # never run
+    circ.run(k, d_in)
+    assert ctx.decrypt_bools(s, circ.read([g]))[0] == 0
+    extra = [circ.gate("xor", g, int(w[0])) for _ in range(80)]  # past the pool's 64 rows
+    with pytest.raises(pzlib.PzError) as e:
+        circ.read([extra[-1]])
+    assert "after the last run" in str(e.value)
+    assert ctx.decrypt_bools(s, circ.read([g]))[0] == 0
+    circ.run(k, d_in)
+    assert ctx.decrypt_bools(s, circ.read([extra[-1]]))[0] == 1  # NAND(1,1) xor 1
+    circ.close()
