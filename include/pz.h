@@ -354,6 +354,95 @@ int pz_ni_aggregate_rgsw(pz_ctx *ctx, const uint8_t *crs, uint32_t total,
                          const uint64_t *d_zero, const uint64_t *d_cts, uint64_t *d_out,
                          void *stream);
 
+/* Party side: the automorphism-key and LWE-KSK shares
+ * (_common_rountine_multi_party_auto_keys_share_gen and
+ * _common_rountine_multi_party_lwe_ksk_share_gen, src/bool/evaluator.rs:
+ * 1952-1965, 1981-2046, over seeded_auto_key_gen src/rgsw/keygen.rs:380-430
+ * and seeded_lwe_ksk_keygen src/lwe.rs:46-101) of a party with secrets h_s
+ * (RLWE) and h_z (LWE): b parts only, h_auto_b[w+1][auto_d][N] (coefficient
+ * domain, keys for -g, g, .., g^w) and h_ksk_b[N * lwe_d] (mod q_ks); the a
+ * parts are the CRS streams that pz_ni_expand_server_key regenerates. */
+int pz_ni_share_auto_ksk(pz_ctx *ctx, const uint8_t *crs, const uint8_t *noise_seed,
+                         const int32_t *h_s, const int32_t *h_z, uint64_t *h_auto_b,
+                         uint64_t *h_ksk_b);
+/* Server: the automorphism-key and LWE-KSK parts of the aggregation
+ * (src/bool/evaluator.rs:1574-1612): d_auto_b = sum of
+ * d_auto_shares[total][w+1][auto_d][N] mod Q, d_ksk_b = sum of
+ * d_ksk_shares[total][N * lwe_d] mod q_ks. */
+int pz_ni_aggregate_auto_ksk(pz_ctx *ctx, uint32_t total, const uint64_t *d_auto_shares,
+                             const uint64_t *d_ksk_shares, uint64_t *d_auto_b,
+                             uint64_t *d_ksk_b, void *stream);
+/* Server: the evaluation-domain server key of a seeded non-interactive
+ * server key (NonInteractiveServerKeyEvaluationDomain::from, src/bool/keys.rs:
+ * 851-945, with the Shoup twins of :1088-1155): d_rgsw[n][2 rlrg_da +
+ * 2 rlrg_db][N] (coefficient domain, from pz_ni_aggregate_rgsw), d_auto_b and
+ * d_ksk_b (the summed b parts); the a parts are expanded on the device from
+ * the CRS seeds.  The inputs are left unchanged. */
+int pz_ni_expand_server_key(pz_ctx *ctx, const uint8_t *crs, const uint64_t *d_rgsw,
+                            const uint64_t *d_auto_b, const uint64_t *d_ksk_b, pz_key **out);
+
+/* ---- seeded wire formats of the non-interactive server key.  A blob is a
+ * pz_ni_blob_header followed by its sections: arrays of little-endian u64
+ * words at 8-byte aligned offsets (offset[i] bytes from the blob start,
+ * words[i] long; unused entries 0).  Only b parts travel; every a part is
+ * regenerated from the common reference seed.  Blob buffers must be 8-byte
+ * aligned.
+ *   A party's share ("PZNISHR", CommonReferenceSeededNonInteractive
+ *   MultiPartyServerKeyShare, src/bool/keys.rs:1240-1394), sections:
+ *     0 self_leader_ni_rgsw_cts  [lead][max(rlrg_da, rlrg_db)][N], the LWE
+ *       indices [lb, le) the party leads (src/bool/evaluator.rs:613-626)
+ *     1 not_self_leader_ni_rgsw_cts [lwe_n - lead][max(rgrg_da, rgrg_db)][N],
+ *       the other indices in ascending order
+ *     2 ni_rgsw_zero_encs        [lwe_n][rgrg_da][N]
+ *     3 ui_to_s_ksk (b parts)    [uitos_d][N]
+ *     4 ksk_zero_encs_for_others [total - 1][uitos_d][N], ascending party
+ *       order without the party itself
+ *     5 auto_keys_share          [w + 1][auto_d][N]
+ *     6 lwe_ksk_share            [N * lwe_d]
+ *   The seeded server key ("PZNIKEY", SeededNonInteractiveMultiPartyServerKey,
+ *   src/bool/keys.rs:1029-1063), sections:
+ *     0 ui_to_s_ksks (b parts)   [total][uitos_d][N], coefficient domain
+ *     1 rgsw_cts                 [lwe_n][2 rlrg_da + 2 rlrg_db][N],
+ *       coefficient domain
+ *     2 auto_keys (b parts)      [w + 1][auto_d][N]
+ *     3 lwe_ksk (b parts)        [N * lwe_d]
+ */
+#define PZ_NI_SECTIONS 8
+#define PZ_NI_BLOB_VERSION 1
+typedef struct pz_ni_blob_header {
+    char magic[8];       /* "PZNISHR" or "PZNIKEY", NUL-padded             */
+    uint32_t version;    /* PZ_NI_BLOB_VERSION                              */
+    uint32_t user;       /* the party's id (share); 0 (seeded server key)    */
+    uint32_t total;      /* parties                                         */
+    uint32_t nsections;
+    pz_params params;    /* the parameter set (88 bytes, no padding)        */
+    uint8_t crs[32];     /* common reference seed                           */
+    uint64_t offset[PZ_NI_SECTIONS];
+    uint64_t words[PZ_NI_SECTIONS];
+} pz_ni_blob_header; /* 272 bytes */
+/* Blob sizes in bytes (no device needed). */
+int pz_ni_share_bytes(const pz_params *p, uint32_t self, uint32_t total, size_t *bytes);
+int pz_ni_seeded_key_bytes(const pz_params *p, uint32_t total, size_t *bytes);
+/* Party `self` of `total`: its whole share into h_share
+ * (gen_non_interactive_multi_party_key_share, src/bool/evaluator.rs:
+ * 1624-1979; the parts as pz_ni_share_uitos, pz_ni_share_rgsw over every
+ * index and pz_ni_share_auto_ksk make them, from the same noise_seed). */
+int pz_ni_key_share(pz_ctx *ctx, const uint8_t *crs, const uint8_t *noise_seed, uint32_t self,
+                    uint32_t total, const int32_t *h_s, const int32_t *h_u, const int32_t *h_z,
+                    uint8_t *h_share);
+/* Server: the seeded server key of `total` shares, in any order
+ * (aggregate_non_interactive_multi_party_server_key_shares,
+ * src/bool/evaluator.rs:1222-1622), computed on the device.  PZ_ERR_ARG if
+ * the shares disagree on the parameter set, party count or seed, or two come
+ * from one party. */
+int pz_ni_aggregate_shares(pz_ctx *ctx, const uint8_t *const *h_shares, uint32_t total,
+                           uint8_t *h_seeded_key);
+/* Server: the evaluation-domain server key (and, if ksks is not NULL, the
+ * parties' Ksk(u_j -> s), ksks[total]) of a seeded server key
+ * (src/bool/keys.rs:851-1005, :1088-1155), expanded on the device. */
+int pz_ni_server_key_from_seeded(pz_ctx *ctx, const uint8_t *h_seeded_key, pz_key **key,
+                                 pz_rlwe_ksk **ksks);
+
 /* Autotuner: times every blind-rotation launch shape that fits the device
  * (kernel variant x resident work groups per CU) on B synthetic gates with
  * this key, and uses the fastest for batches in B's power-of-two bucket

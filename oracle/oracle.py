@@ -156,6 +156,9 @@ def lib():
                                       C.c_int, u64p]),
             "pzo_ni_aggregate_rgsw": (None, [vp, u8p, C.c_uint32, u64p, C.c_uint32, C.c_uint32,
                                              u64p, u64p, u64p]),
+            "pzo_ni_share_auto_ksk": (None, [vp, u8p, u8p, i32p, i32p, u64p, u64p]),
+            "pzo_ni_aggregate_auto_ksk": (None, [vp, C.c_uint32, u64p, u64p, u64p, u64p]),
+            "pzo_ni_expand_server_key": (None, [vp, u8p, u64p, u64p, u64p, C.POINTER(Key)]),
             "pzo_decrypt_lwe": (C.c_uint64, [vp, u64p, i32p, C.c_uint32]),
             "pzo_decode_bool": (C.c_int, [vp, C.c_uint64]),
             "pzo_true_el": (C.c_uint64, [vp]),
@@ -467,6 +470,34 @@ class Oracle:
         self.L.pzo_ni_aggregate_rgsw(self.ev, seed_bytes(crs), total, _u64(ksks), l0, count,
                                      _u64(zero), _u64(cts), out.reshape(-1))
         return out
+
+    def ni_share_auto_ksk(self, crs, noise_seed, s, z):
+        """One party's automorphism-key b parts [(w+1)][d_auto][N]
+        (coefficient domain) and LWE-KSK b parts [N * d_ks] (mod q_ks)."""
+        p, N = self.p, self.N
+        auto_b = np.zeros((p.w + 1, p.auto_d, N), dtype=np.uint64)
+        ksk_b = np.zeros(N * p.lwe_d, dtype=np.uint64)
+        self.L.pzo_ni_share_auto_ksk(self.ev, seed_bytes(crs), seed_bytes(noise_seed), _i32(s),
+                                     _i32(z), auto_b.reshape(-1), ksk_b)
+        return auto_b, ksk_b
+
+    def ni_aggregate_auto_ksk(self, auto_shares, ksk_shares):
+        p, N = self.p, self.N
+        a = np.ascontiguousarray(auto_shares, dtype=np.uint64)
+        k = np.ascontiguousarray(ksk_shares, dtype=np.uint64)
+        auto_b = np.zeros((p.w + 1, p.auto_d, N), dtype=np.uint64)
+        ksk_b = np.zeros(N * p.lwe_d, dtype=np.uint64)
+        self.L.pzo_ni_aggregate_auto_ksk(self.ev, a.shape[0], a.reshape(-1), k.reshape(-1),
+                                         auto_b.reshape(-1), ksk_b)
+        return auto_b, ksk_b
+
+    def ni_expand_server_key(self, crs, rgsw, auto_b, ksk_b):
+        """Evaluation-domain server key (with Shoup twins) from the seeded
+        server key's parts: coefficient-domain RGSWs and summed b parts."""
+        key = OracleKey(self.sizes, self.N)
+        self.L.pzo_ni_expand_server_key(self.ev, seed_bytes(crs), _u64(rgsw), _u64(auto_b),
+                                        _u64(ksk_b), C.byref(key.cstruct()))
+        return key
 
     # ---- encryption ----
     def encrypt_bool(self, s, bit, seed):

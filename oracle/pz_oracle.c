@@ -1924,3 +1924,133 @@ int pzo_decode_bool(const pzo_eval *e, uint64_t m)
     if (v == 1) return 1;
     return -1;
 }
+
+/* ---- non-interactive automorphism-key and LWE-KSK shares, their sums and
+ * the seeded server key's expansion ----
+ * Seeds (NonInteractiveMultiPartyCrs, src/bool/evaluator.rs:148-156):
+ *   auto keys a parts:  puncture(puncture(crs, 1), 2)
+ *   LWE KSK a parts:    puncture(puncture(crs, 1), 3)
+ * Noise streams are the oracle's own: ChaCha8(puncture(noise_seed, 3)) for
+ * the automorphism keys, ChaCha8(puncture(noise_seed, 4)) for the LWE KSK. */
+static void ni_seed_key_part(const uint8_t crs[32], uint32_t part, uint8_t out[32])
+{
+    uint8_t k[32];
+    pzo_puncture(crs, 1, k);
+    pzo_puncture(k, part, out);
+}
+
+/* _common_rountine_multi_party_auto_keys_share_gen and
+ * _common_rountine_multi_party_lwe_ksk_share_gen (src/bool/evaluator.rs:
+ * 1952-1965, 1981-2046) over seeded_auto_key_gen / seeded_rlwe_ksk_gen
+ * (src/rgsw/keygen.rs:306-430) and seeded_lwe_ksk_keygen (src/lwe.rs:46-101):
+ *   auto_b[k][j] = a_{k,j}*s + e + beta_j*(-s(X^{el_k})), k over the dlogs
+ *                  [0, 1, .., w] (el_0 = -g, el_k = g^k), j < d_auto, the
+ *                  a rows drawn in that order from the auto-key seed;
+ *   ksk_b[i*d + j] = <a_{i,j}, z> + beta_j*(-s_i) + e mod q_ks, the n
+ *                  entries of each a row drawn in row order from the KSK
+ *                  seed. */
+void pzo_ni_share_auto_ksk(const pzo_eval *e, const uint8_t crs[32], const uint8_t noise_seed[32],
+                           const int32_t *s, const int32_t *z, uint64_t *auto_b, uint64_t *ksk_b)
+{
+    const pzo_params *p = &e->p;
+    uint32_t N = p->rlwe_n, n = p->lwe_n, dau = p->auto_d;
+    uint64_t q = p->rlwe_q;
+    const modq *m = &e->mq;
+    uint8_t sd[32], ns[32];
+    uint64_t *s_eval = secret_eval(e, s);
+    uint64_t *a = (uint64_t *)malloc(8 * (size_t)N), *msg = (uint64_t *)malloc(8 * (size_t)N);
+    /* automorphism keys */
+    ni_seed_key_part(crs, 2, sd);
+    pzo_rng *pr = pzo_rng_new(sd);
+    pzo_puncture(noise_seed, 3, ns);
+    pzo_rng *noise = pzo_rng_new(ns);
+    for (uint32_t k = 0; k <= p->w; k++) {
+        const uint32_t *to = e->auto_idx + (size_t)k * N;
+        const uint8_t *sg = e->auto_sign + (size_t)k * N;
+        for (uint32_t i = 0; i < N; i++) {  /* -s(X^k): the sign opposite to the map's */
+            uint64_t el = map_i64(q, s[i]);
+            msg[to[i]] = sg[i] ? mm_neg(m, el) : el;
+        }
+        for (uint32_t j = 0; j < dau; j++) {
+            for (uint32_t i = 0; i < N; i++) a[i] = pzo_rng_uniform(pr, q);
+            uint64_t beta = 1ULL << (p->auto_logb * j + e->dec_auto.ignore_bits);
+            ni_row(e, a, s_eval, noise, msg, beta, auto_b + ((size_t)k * dau + j) * N);
+        }
+    }
+    pzo_rng_free(pr);
+    pzo_rng_free(noise);
+    /* LWE key-switching key from s (dimension N) to z (dimension n) */
+    ni_seed_key_part(crs, 3, sd);
+    pr = pzo_rng_new(sd);
+    pzo_puncture(noise_seed, 4, ns);
+    noise = pzo_rng_new(ns);
+    uint64_t lq = p->lwe_q, mask = e->lwe_mask;
+    for (uint32_t i = 0; i < N; i++) {
+        uint64_t neg_si = (0 - map_i64(lq, s[i])) & mask;
+        for (uint32_t j = 0; j < p->lwe_d; j++) {
+            uint64_t beta = 1ULL << (p->lwe_logb * j + e->dec_lwe.ignore_bits);
+            uint64_t az = 0;
+            for (uint32_t k = 0; k < n; k++)
+                az = (az + pzo_rng_uniform(pr, lq) * map_i64(lq, z[k])) & mask;
+            uint64_t b = (az + beta * neg_si) & mask;
+            ksk_b[(size_t)i * p->lwe_d + j] = (b + map_i64(lq, pzo_rng_gauss_round(noise))) & mask;
+        }
+    }
+    pzo_rng_free(pr);
+    pzo_rng_free(noise);
+    free(s_eval); free(a); free(msg);
+}
+
+/* The automorphism-key and LWE-KSK parts of
+ * aggregate_non_interactive_multi_party_server_key_shares
+ * (src/bool/evaluator.rs:1574-1612): sums of the parties' b parts, mod Q and
+ * mod q_ks.  shares: [total][(w+1) d_auto N] and [total][N d_ks]. */
+void pzo_ni_aggregate_auto_ksk(const pzo_eval *e, uint32_t total, const uint64_t *auto_shares,
+                               const uint64_t *ksk_shares, uint64_t *auto_b, uint64_t *ksk_b)
+{
+    const pzo_params *p = &e->p;
+    size_t aw = (size_t)(p->w + 1) * p->auto_d * p->rlwe_n, kw = (size_t)p->rlwe_n * p->lwe_d;
+    memset(auto_b, 0, 8 * aw);
+    memset(ksk_b, 0, 8 * kw);
+    for (uint32_t t = 0; t < total; t++) {
+        for (size_t i = 0; i < aw; i++) auto_b[i] = mm_add(&e->mq, auto_b[i], auto_shares[t * aw + i]);
+        for (size_t i = 0; i < kw; i++) ksk_b[i] = (ksk_b[i] + ksk_shares[t * kw + i]) & e->lwe_mask;
+    }
+}
+
+/* NonInteractiveServerKeyEvaluationDomain::from(&SeededNonInteractive
+ * MultiPartyServerKey) (src/bool/keys.rs:851-945) and its Shoup twins
+ * (:1088-1155): the RGSW rows (coefficient domain) sent to the evaluation
+ * domain; each automorphism key's a rows drawn from the auto-key seed, its b
+ * rows copied, every row sent to the evaluation domain; each LWE-KSK row's n
+ * a entries drawn from the KSK seed after its b. */
+void pzo_ni_expand_server_key(const pzo_eval *e, const uint8_t crs[32], const uint64_t *rgsw,
+                              const uint64_t *auto_b, const uint64_t *ksk_b, pzo_key *key)
+{
+    const pzo_params *p = &e->p;
+    uint32_t N = p->rlwe_n, n = p->lwe_n, dau = p->auto_d;
+    size_t rw = (size_t)n * e->rgsw_rows * N;
+    memcpy(key->rgsw, rgsw, 8 * rw);
+    for (size_t r = 0; r < (size_t)n * e->rgsw_rows; r++) ntt_fwd(e->ntt, key->rgsw + r * N, 1);
+    uint8_t sd[32];
+    ni_seed_key_part(crs, 2, sd);
+    pzo_rng *pr = pzo_rng_new(sd);
+    size_t as = (size_t)e->auto_rows * N;
+    for (uint32_t k = 0; k <= p->w; k++) {
+        uint64_t *ct = key->auto_key + k * as;
+        for (size_t i = 0; i < (size_t)dau * N; i++) ct[i] = pzo_rng_uniform(pr, p->rlwe_q);
+        memcpy(ct + (size_t)dau * N, auto_b + (size_t)k * dau * N, 8 * (size_t)dau * N);
+        for (uint32_t r = 0; r < e->auto_rows; r++) ntt_fwd(e->ntt, ct + (size_t)r * N, 1);
+    }
+    pzo_rng_free(pr);
+    ni_seed_key_part(crs, 3, sd);
+    pr = pzo_rng_new(sd);
+    size_t n1 = (size_t)n + 1;
+    for (size_t r = 0; r < (size_t)N * p->lwe_d; r++) {
+        uint64_t *row = key->lwe_ksk + r * n1;
+        for (uint32_t k = 0; k < n; k++) row[1 + k] = pzo_rng_uniform(pr, p->lwe_q);
+        row[0] = ksk_b[r];
+    }
+    pzo_rng_free(pr);
+    pzo_key_shoup(e, key);
+}

@@ -246,6 +246,16 @@ int pzo_decode_bool(const pzo_eval *e, uint64_t m);
 uint64_t pzo_true_el(const pzo_eval *e);
 uint64_t pzo_false_el(const pzo_eval *e);
 
+/* non-interactive automorphism-key and LWE-KSK shares (b parts, coefficient
+ * domain / mod q_ks), their sums, and the expansion of a seeded server key
+ * (coefficient-domain RGSWs + summed b parts) into an evaluation-domain key */
+void pzo_ni_share_auto_ksk(const pzo_eval *e, const uint8_t crs[32], const uint8_t noise_seed[32],
+                           const int32_t *s, const int32_t *z, uint64_t *auto_b, uint64_t *ksk_b);
+void pzo_ni_aggregate_auto_ksk(const pzo_eval *e, uint32_t total, const uint64_t *auto_shares,
+                               const uint64_t *ksk_shares, uint64_t *auto_b, uint64_t *ksk_b);
+void pzo_ni_expand_server_key(const pzo_eval *e, const uint8_t crs[32], const uint64_t *rgsw,
+                              const uint64_t *auto_b, const uint64_t *ksk_b, pzo_key *key);
+
 #ifdef __cplusplus
 }
 #endif

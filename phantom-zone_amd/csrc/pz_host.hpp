@@ -304,6 +304,15 @@ inline void ni_seed_user(const uint8_t crs[32], uint32_t j, uint8_t out[32])
     puncture(k, j + 1, out);
 }
 
+// seeds of the server key's seeded parts (src/bool/evaluator.rs:148-156):
+// part 2 the automorphism keys' a rows, part 3 the LWE KSK's a entries
+inline void ni_seed_key_part(const uint8_t crs[32], uint32_t part, uint8_t out[32])
+{
+    uint8_t k[32];
+    puncture(crs, 1, k);
+    puncture(k, part, out);
+}
+
 // multi_party_user_id_lwe_segment (src/bool/evaluator.rs:613-626): indices
 // per party; the leader of index l is l / ni_per_party
 inline uint32_t ni_per_party(uint32_t lwe_n, uint32_t total) { return (lwe_n + total - 1) / total; }

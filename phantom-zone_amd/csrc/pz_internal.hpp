@@ -80,6 +80,14 @@ int pz_internal_fail(int code, const char *fn, const char *what);
 int pz_internal_launch_status(const char *fn);
 #define PZ_LAUNCH_STATUS() pz_internal_launch_status(__func__)
 
+// Uniform rows of S seeded ChaCha8 streams (UniformInt<u64> mod q, rand
+// 0.8.5) into d_out[S][draws - keep_from], expanded on the device; d_kf is
+// device workspace of 12*S words.  A stream with a rejected draw is
+// regenerated sequentially on the host, so the rows always equal the
+// sequential streams.  Synchronises the stream.
+bool pz_internal_expand_uniform(hipStream_t st, const uint8_t *seeds, u32 S, u64 draws,
+                                u64 keep_from, u64 q, u64 *d_out, u32 *d_kf);
+
 // Key-switching key for RLWE ciphertexts (Ksk(u -> s)): [2d][N] rows in the
 // evaluation domain, A_0..A_{d-1} then B_0..B_{d-1}, with Shoup twins.
 struct pz_rlwe_ksk {

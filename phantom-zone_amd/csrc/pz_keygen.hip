@@ -15,6 +15,7 @@
 // row: n uniform a (mod q_ks), one noise e.
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/pz.h"
@@ -110,6 +111,31 @@ __global__ void k_ni_combine(const NiRow *rows, size_t nrows, u32 N, Barrett br)
         if (r.msg) m = r.msg[i];
         else if (r.mono_val && i == r.mono_pos) m = r.mono_val;
         r.out[i] = add_mod(add_mod(r.prod[i], r.e[i], q), barrett_mul(br, m, r.beta), q);
+    }
+}
+
+// NI LWE-KSK share row r = i*d + j (seeded_lwe_ksk_keygen, src/lwe.rs:46-101):
+// b[r] = <a_r, z> + beta_j * (-s_i) + e_r mod q_ks, a_r the n entries of row r
+// (rows of the CRS stream).  One wave per row: lanes sum strided entries,
+// then a butterfly reduction (all arithmetic wraps mod 2^64, masked at the
+// end, so the order of the sum does not matter).
+__global__ __launch_bounds__(64) void k_ksk_share_b(const u64 *a, const int32_t *s,
+                                                    const int32_t *z, const int64_t *err, u32 N,
+                                                    u32 d, u32 n, u32 logb, u32 ib, u64 mask,
+                                                    u64 *b)
+{
+    const size_t rows = (size_t)N * d;
+    for (size_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const u64 *ar = a + r * n;
+        u64 acc = 0;
+        for (u32 k = threadIdx.x; k < n; k += 64) acc += ar[k] * (u64)(int64_t)z[k];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (threadIdx.x == 0) {
+            const u32 i = (u32)(r / d), j = (u32)(r % d);
+            const u64 beta = 1ull << (logb * j + ib);
+            const u64 neg_si = 0 - (u64)(int64_t)s[i];
+            b[r] = (acc + beta * neg_si + (u64)err[r]) & mask;
+        }
     }
 }
 
@@ -781,3 +807,111 @@ done:
     return ok ? PZ_OK : PZ_ERR_HIP;
 }
 
+
+// ---- non-interactive automorphism-key and LWE-KSK shares (client side;
+// _common_rountine_multi_party_auto_keys_share_gen and
+// _common_rountine_multi_party_lwe_ksk_share_gen, src/bool/evaluator.rs:
+// 1952-1965, 1981-2046).  The a parts are the CRS streams
+// puncture(puncture(crs, 1), 2) (automorphism keys, rows in dlog order
+// [-g, g, .., g^w], d_auto rows each) and puncture(puncture(crs, 1), 3) (LWE
+// KSK, n entries per row, rows i*d + j); noise comes from
+// ChaCha8(puncture(noise_seed, 3)) and ChaCha8(puncture(noise_seed, 4)).
+extern "C" int pz_ni_share_auto_ksk(pz_ctx *c, const uint8_t *crs, const uint8_t *noise_seed,
+                                    const int32_t *h_s, const int32_t *h_z, uint64_t *h_auto_b,
+                                    uint64_t *h_ksk_b)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!crs || !noise_seed || !h_s || !h_z || !h_auto_b || !h_ksk_b)
+        return PZ_FAIL(PZ_ERR_ARG, "null argument");
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    const u64 q = p.rlwe_q;
+    const u32 N = p.rlwe_n, n = p.lwe_n, dau = p.auto_d;
+    const size_t R = (size_t)(p.w + 1) * dau;  // automorphism-key rows
+    const size_t krows = (size_t)N * p.lwe_d;   // LWE-KSK rows
+    uint8_t sd_auto[32], sd_ksk[32], n_auto[32], n_ksk[32];
+    host::ni_seed_key_part(crs, 2, sd_auto);
+    host::ni_seed_key_part(crs, 3, sd_ksk);
+    host::puncture(noise_seed, 3, n_auto);
+    host::puncture(noise_seed, 4, n_ksk);
+    // automorphism keys: a rows, noise and the messages -s(X^k) on the host
+    std::vector<u64> ha(R * N), he(R * N), hmsg((size_t)(p.w + 1) * N);
+    {
+        host::ChaCha8 pr(sd_auto), noise(n_auto);
+        for (size_t i = 0; i < R * N; i++) ha[i] = pr.uniform(q);
+        for (size_t i = 0; i < R * N; i++) he[i] = host::map_i64(q, host::gauss_round(noise));
+    }
+    for (u32 k = 0; k <= p.w; k++) {
+        const uint64_t kk = dc.auto_el[k];
+        for (u32 i = 0; i < N; i++) {
+            uint64_t to = ((uint64_t)i * kk) % (2ull * N);
+            const bool pos = to < N;
+            if (!pos) to -= N;
+            const u64 el = host::map_i64(q, h_s[i]);
+            hmsg[(size_t)k * N + to] = pos ? (el ? q - el : 0) : el;
+        }
+    }
+    std::vector<int64_t> herr(krows);
+    {
+        host::ChaCha8 noise(n_ksk);
+        for (size_t r = 0; r < krows; r++)
+            herr[r] = (int64_t)host::map_i64(p.lwe_q, host::gauss_round(noise));
+    }
+    bool ok = true;
+    u64 *d_a = nullptr, *d_e = nullptr, *d_prod = nullptr, *d_msg = nullptr, *d_out = nullptr;
+    u64 *d_ka = nullptr, *d_kb = nullptr;
+    u32 *d_kf = nullptr;
+    int32_t *d_si = nullptr, *d_zi = nullptr;
+    int64_t *d_err = nullptr;
+    std::vector<NiRow> rows(R);
+    const u32 ib_lwe = host::ref_log2(p.lwe_q) - p.lwe_logb * p.lwe_d;
+    KG_CHK(hipMalloc(&d_a, 8 * R * N));
+    KG_CHK(hipMalloc(&d_e, 8 * R * N));
+    KG_CHK(hipMalloc(&d_prod, 8 * R * N));
+    KG_CHK(hipMalloc(&d_out, 8 * R * N));
+    KG_CHK(hipMalloc(&d_msg, 8 * hmsg.size()));
+    KG_CHK(hipMalloc(&d_ka, 8 * krows * n));
+    KG_CHK(hipMalloc(&d_kb, 8 * krows));
+    KG_CHK(hipMalloc(&d_kf, 4 * 12));
+    KG_CHK(hipMalloc(&d_si, 4 * (size_t)N));
+    KG_CHK(hipMalloc(&d_zi, 4 * (size_t)n));
+    KG_CHK(hipMalloc(&d_err, 8 * krows));
+    KG_CHK(hipMemcpyAsync(d_a, ha.data(), 8 * R * N, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_e, he.data(), 8 * R * N, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_msg, hmsg.data(), 8 * hmsg.size(), hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_si, h_s, 4 * (size_t)N, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_zi, h_z, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    KG_CHK(hipMemcpyAsync(d_err, herr.data(), 8 * krows, hipMemcpyHostToDevice, st));
+    // b = a*s + e + beta_j * (-s(X^k))  (seeded_rlwe_ksk_gen, src/rgsw/keygen.rs:306-358)
+    if (!ni_products(logn, dc, st, d_a, R, h_s, N, q, d_prod)) KG_FAIL("products with the secret failed");
+    for (size_t r = 0; r < R; r++) {
+        const u32 k = (u32)(r / dau), j = (u32)(r % dau);
+        rows[r] = {d_prod + r * N, d_e + r * N, d_msg + (size_t)k * N, d_out + r * N,
+                   1ull << (p.auto_logb * j + dc.dec_auto.ignore_bits), 0, 0};
+    }
+    if (!ni_combine(st, dc, rows, N)) KG_FAIL("row combination failed");
+    // LWE KSK: the a rows expanded from the CRS stream on the device
+    if (!pz_internal_expand_uniform(st, sd_ksk, 1, krows * n, 0, p.lwe_q, d_ka, d_kf))
+        KG_FAIL("expansion of the KSK a parts failed");
+    hipLaunchKernelGGL(k_ksk_share_b, (int)std::min<size_t>(krows, 65535), 64, 0, st, d_ka, d_si,
+                       d_zi, d_err, N, p.lwe_d, n, p.lwe_logb, ib_lwe, (u64)(p.lwe_q - 1), d_kb);
+    KG_CHK(hipGetLastError());
+    KG_CHK(hipMemcpy(h_auto_b, d_out, 8 * R * N, hipMemcpyDeviceToHost));
+    KG_CHK(hipMemcpy(h_ksk_b, d_kb, 8 * krows, hipMemcpyDeviceToHost));
+done:
+    hipFree(d_a);
+    hipFree(d_e);
+    hipFree(d_prod);
+    hipFree(d_out);
+    hipFree(d_msg);
+    hipFree(d_ka);
+    hipFree(d_kb);
+    hipFree(d_kf);
+    hipFree(d_si);
+    hipFree(d_zi);
+    hipFree(d_err);
+    return ok ? PZ_OK : PZ_ERR_HIP;  // message set by KG_CHK / KG_FAIL
+}

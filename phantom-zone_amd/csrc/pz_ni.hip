@@ -218,14 +218,16 @@ using namespace pz;
 extern int pz_internal_ctx_view(pz_ctx *c, pz_params *p, DevCtx *dc, int *logn, hipStream_t *s);
 extern int pz_internal_rlwe_ksk_adopt(pz_ctx *c, u64 *d_rows, u32 d, pz_rlwe_ksk **out);
 extern int pz_internal_workspace(pz_ctx *c, size_t bytes, void **out);
+extern int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows,
+                                 pz_key **out);
 
 // Uniform rows of S seeded streams into d_out ([S][draws - keep_from]) on
 // the device; d_kf is workspace for the keys and the reject flags (12*S
 // words).  A stream with a draw outside the acceptance zone (probability
 // about 2^-38 per draw for the 54-bit modulus) is regenerated sequentially on
 // the host, so the rows always equal the sequential streams.
-static bool expand_rows(hipStream_t st, const uint8_t *seeds, u32 S, u64 draws, u64 keep_from,
-                        u64 q, u64 *d_out, u32 *d_kf)
+bool pz_internal_expand_uniform(hipStream_t st, const uint8_t *seeds, u32 S, u64 draws,
+                                u64 keep_from, u64 q, u64 *d_out, u32 *d_kf)
 {
     u32 *d_keys = d_kf, *d_flags = d_kf + 8 * (size_t)S;
     std::vector<u32> flags(S);
@@ -274,7 +276,7 @@ extern "C" int pz_ni_uitos_ksk(pz_ctx *c, const uint8_t *crs, uint32_t total, ui
     u64 *rows = nullptr;
     if (hipMalloc(&rows, 16 * words) != hipSuccess)
         return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
-    bool ok = expand_rows(st, sd, 1, words, 0, p.rlwe_q, rows, (u32 *)ws) &&
+    bool ok = pz_internal_expand_uniform(st, sd, 1, words, 0, p.rlwe_q, rows, (u32 *)ws) &&
               hipMemcpyAsync(rows + words, d_ksk_b, 8 * words, hipMemcpyDeviceToDevice, st) ==
                   hipSuccess;
     if (ok) {
@@ -330,8 +332,9 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
     u64 *o1 = (u64 *)ws, *o1s = o1 + w1, *d_a = o1s + w1;
     std::vector<uint8_t> seeds(32 * (size_t)count);
     host::ni_index_seeds(crs, l0, count, seeds.data());
-    bool ok = expand_rows(st, seeds.data(), count, (u64)dmax * N, (u64)(dmax - db) * N, p.rlwe_q,
-                          d_a, (u32 *)(d_a + wa));
+    bool ok = pz_internal_expand_uniform(st, seeds.data(), count, (u64)dmax * N,
+                                         (u64)(dmax - db) * N, p.rlwe_q, d_a,
+                                         (u32 *)(d_a + wa));
     if (ok) {
         NiArgs A;
         A.zero = d_zero;
@@ -378,4 +381,142 @@ extern "C" int pz_ni_aggregate_rgsw(pz_ctx *c, const uint8_t *crs, uint32_t tota
         ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
     }
     return ok ? PZ_OK : PZ_FAIL(PZ_ERR_HIP, "aggregation launch failed");
+}
+
+// ---- automorphism keys and LWE KSK of the non-interactive server key ----
+
+namespace pz {
+
+// out = sum of `total` share arrays of `words` entries mod 2^k (mask)
+__global__ void k_ni_sum_pow2(u64 *out, const u64 *shares, u32 total, size_t words, u64 mask)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words;
+         i += (size_t)gridDim.x * blockDim.x) {
+        u64 v = 0;
+        for (u32 t = 0; t < total; t++) v += shares[(size_t)t * words + i];
+        out[i] = v & mask;
+    }
+}
+
+// automorphism key k: rows [a_0..a_{d-1} | b_0..b_{d-1}] of N words, the a
+// rows taken in order from the expanded auto-key stream
+__global__ void k_ni_auto_rows(u64 *key, const u64 *a, const u64 *b, u32 K, u32 d, u32 N)
+{
+    const size_t total = (size_t)K * 2 * d * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const size_t row = t / N, i = t % N;
+        const size_t k = row / (2 * d), r = row % (2 * d);
+        key[t] = r < d ? a[(k * d + r) * N + i] : b[(k * d + r - d) * N + i];
+    }
+}
+
+// LWE-KSK rows [b_r, a_r[0..n)]
+__global__ void k_ni_ksk_rows(u64 *rows, const u64 *a, const u64 *b, size_t nrows, u32 n)
+{
+    const size_t n1 = (size_t)n + 1, total = nrows * n1;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = t / n1, c = t % n1;
+        rows[t] = c == 0 ? b[r] : a[r * n + c - 1];
+    }
+}
+
+}  // namespace pz
+
+// The automorphism-key and LWE-KSK parts of
+// aggregate_non_interactive_multi_party_server_key_shares
+// (src/bool/evaluator.rs:1574-1612): the parties' b parts summed, mod Q and
+// mod q_ks.
+extern "C" int pz_ni_aggregate_auto_ksk(pz_ctx *c, uint32_t total,
+                                        const uint64_t *d_auto_shares,
+                                        const uint64_t *d_ksk_shares, uint64_t *d_auto_b,
+                                        uint64_t *d_ksk_b, void *stream)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!d_auto_shares || !d_ksk_shares || !d_auto_b || !d_ksk_b)
+        return PZ_FAIL(PZ_ERR_ARG, "null argument");
+    if (total == 0 || total > PZ_NI_MAX_PARTIES)
+        return PZ_FAIL(PZ_ERR_ARG, "total must be 1..PZ_NI_MAX_PARTIES");
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    if (stream) st = (hipStream_t)stream;
+    const size_t aw = (size_t)(p.w + 1) * p.auto_d * p.rlwe_n, kw = (size_t)p.rlwe_n * p.lwe_d;
+    if (hipMemcpyAsync(d_auto_b, d_auto_shares, 8 * aw, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipMemcpyAsync failed");
+    if (total > 1)
+        hipLaunchKernelGGL(k_ni_sum_rows, ni_grid(aw), 256, 0, st, d_auto_b, d_auto_shares + aw,
+                           total - 1, aw, p.rlwe_q);
+    hipLaunchKernelGGL(k_ni_sum_pow2, ni_grid(kw), 256, 0, st, d_ksk_b, d_ksk_shares, total, kw,
+                       (u64)(p.lwe_q - 1));
+    return PZ_LAUNCH_STATUS();
+}
+
+// NonInteractiveServerKeyEvaluationDomain::from(&SeededNonInteractive
+// MultiPartyServerKey) (src/bool/keys.rs:851-945) with the Shoup twins of
+// ShoupNonInteractiveServerKeyEvaluationDomain (:1088-1155): the RGSWs
+// (coefficient domain) sent to the evaluation domain; each automorphism key's
+// a rows expanded from ChaCha8(puncture(puncture(crs, 1), 2)) in dlog order,
+// its b rows copied, every row sent to the evaluation domain; each LWE-KSK
+// row's n a entries expanded from ChaCha8(puncture(puncture(crs, 1), 3)) in
+// row order after its b.  The caller's buffers are left unchanged.
+extern "C" int pz_ni_expand_server_key(pz_ctx *c, const uint8_t *crs, const uint64_t *d_rgsw,
+                                       const uint64_t *d_auto_b, const uint64_t *d_ksk_b,
+                                       pz_key **out)
+{
+    pz_params p;
+    DevCtx dc;
+    int logn;
+    hipStream_t st;
+    if (!crs || !d_rgsw || !d_auto_b || !d_ksk_b || !out)
+        return PZ_FAIL(PZ_ERR_ARG, "null argument");
+    int rc = pz_internal_ctx_view(c, &p, &dc, &logn, &st);
+    if (rc) return rc;
+    const u32 N = p.rlwe_n, n = p.lwe_n, d = p.auto_d, K = p.w + 1;
+    const size_t rg_rows = (size_t)n * (2 * p.rlrg_da + 2 * p.rlrg_db);
+    const size_t au_rows = (size_t)K * 2 * d, krows = (size_t)N * p.lwe_d;
+    uint8_t sd_auto[32], sd_ksk[32];
+    host::ni_seed_key_part(crs, 2, sd_auto);
+    host::ni_seed_key_part(crs, 3, sd_ksk);
+    u64 *rg = nullptr, *au = nullptr, *au_a = nullptr, *kr = nullptr, *ka = nullptr;
+    u32 *kf = nullptr;
+    bool ok = hipMalloc(&rg, 8 * rg_rows * N) == hipSuccess &&
+              hipMalloc(&au, 8 * au_rows * N) == hipSuccess &&
+              hipMalloc(&au_a, 8 * (size_t)K * d * N) == hipSuccess &&
+              hipMalloc(&kr, 8 * krows * (n + 1)) == hipSuccess &&
+              hipMalloc(&ka, 8 * krows * n) == hipSuccess && hipMalloc(&kf, 4 * 12) == hipSuccess;
+    const char *what = "hipMalloc failed";
+    if (ok) {
+        what = "expansion of the seeded parts failed";
+        ok = hipMemcpyAsync(rg, d_rgsw, 8 * rg_rows * N, hipMemcpyDeviceToDevice, st) ==
+                 hipSuccess &&
+             pz_internal_expand_uniform(st, sd_auto, 1, (u64)K * d * N, 0, p.rlwe_q, au_a, kf) &&
+             pz_internal_expand_uniform(st, sd_ksk, 1, (u64)krows * n, 0, p.lwe_q, ka, kf);
+    }
+    if (ok) {
+        launch_ntt(logn, 1, rg, rg_rows, dc.tb, st);
+        hipLaunchKernelGGL(k_ni_auto_rows, ni_grid(au_rows * N), 256, 0, st, au, au_a, d_auto_b,
+                           K, d, N);
+        launch_ntt(logn, 1, au, au_rows, dc.tb, st);
+        hipLaunchKernelGGL(k_ni_ksk_rows, ni_grid(krows * (n + 1)), 256, 0, st, kr, ka, d_ksk_b,
+                           krows, n);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+        what = "transform or assembly of the key rows failed";
+    }
+    if (ok) {
+        rc = pz_internal_key_adopt(c, rg, au, kr, out);  // owns rg and au on success
+        if (rc == PZ_OK) rg = au = nullptr;
+    } else {
+        rc = PZ_FAIL(PZ_ERR_HIP, what);
+    }
+    hipFree(rg);
+    hipFree(au);
+    hipFree(au_a);
+    hipFree(kr);
+    hipFree(ka);
+    hipFree(kf);
+    return rc;
 }

@@ -33,6 +33,21 @@ class Params(C.Structure):
     ]
 
 
+class NiBlobHeader(C.Structure):
+    """pz_ni_blob_header: the header of a seeded share / server key blob."""
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("user", C.c_uint32),
+                ("total", C.c_uint32), ("nsections", C.c_uint32), ("params", Params),
+                ("crs", C.c_uint8 * 32), ("offset", C.c_uint64 * 8), ("words", C.c_uint64 * 8)]
+
+
+def ni_blob_sections(blob):
+    """(header, [u64 arrays of the sections]) of a wire-format blob."""
+    h = NiBlobHeader.from_buffer_copy(bytes(blob[:C.sizeof(NiBlobHeader)]))
+    secs = [np.frombuffer(bytes(blob[h.offset[i]:h.offset[i] + 8 * h.words[i]]), dtype=np.uint64)
+            for i in range(h.nsections)]
+    return h, secs
+
+
 # Every symbol declared in include/pz.h, with its ctypes signature.
 _vp = C.c_void_p
 SIGNATURES = {
@@ -115,6 +130,15 @@ SIGNATURES = {
                                   C.POINTER(_vp)]),
     "pz_ni_aggregate_rgsw": (C.c_int, [_vp, _vp, C.c_uint32, _vp, C.c_uint32, C.c_uint32, _vp,
                                        _vp, _vp, _vp]),
+    "pz_ni_share_auto_ksk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pz_ni_share_bytes": (C.c_int, [C.POINTER(Params), C.c_uint32, C.c_uint32,
+                                    C.POINTER(C.c_size_t)]),
+    "pz_ni_seeded_key_bytes": (C.c_int, [C.POINTER(Params), C.c_uint32, C.POINTER(C.c_size_t)]),
+    "pz_ni_key_share": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp, _vp, _vp, _vp]),
+    "pz_ni_aggregate_shares": (C.c_int, [_vp, _vp, C.c_uint32, _vp]),
+    "pz_ni_server_key_from_seeded": (C.c_int, [_vp, _vp, C.POINTER(_vp), _vp]),
+    "pz_ni_aggregate_auto_ksk": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
+    "pz_ni_expand_server_key": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
 }
 
 _lib = None
@@ -143,6 +167,25 @@ def load():
 def _check(rc):
     if rc != 0:
         raise PzError("pz error %d: %s" % (rc, load().pz_last_error().decode()))
+
+
+def aligned_bytes(nbytes):
+    """A zeroed uint8 buffer of nbytes whose data is 8-byte aligned."""
+    return np.zeros((nbytes + 7) // 8, dtype=np.uint64).view(np.uint8)[:nbytes]
+
+
+def ni_share_bytes(params, me, total):
+    p = params_named(params) if isinstance(params, str) else params
+    n = C.c_size_t()
+    _check(load().pz_ni_share_bytes(C.byref(p), me, total, C.byref(n)))
+    return n.value
+
+
+def ni_seeded_key_bytes(params, total):
+    p = params_named(params) if isinstance(params, str) else params
+    n = C.c_size_t()
+    _check(load().pz_ni_seeded_key_bytes(C.byref(p), total, C.byref(n)))
+    return n.value
 
 
 def params_named(name):
@@ -610,6 +653,66 @@ class Context:
         crs_b = self.seed32(crs)
         _check(self.L.pz_ni_aggregate_rgsw(self.h, crs_b.ctypes.data, total, arr, l0,
                                            count, d_zero.ptr, d_cts.ptr, d_out.ptr, None))
+
+    def ni_share_auto_ksk(self, crs, noise_seed, s, z):
+        """A party's automorphism-key b parts [(w+1)][d_auto][N] and LWE-KSK
+        b parts [N * d_ks]."""
+        p, N = self.p, self.N
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        z = np.ascontiguousarray(z, dtype=np.int32)
+        auto_b = np.zeros((p.w + 1, p.auto_d, N), dtype=np.uint64)
+        ksk_b = np.zeros(N * p.lwe_d, dtype=np.uint64)
+        crs_b, ns_b = self.seed32(crs), self.seed32(noise_seed)
+        _check(self.L.pz_ni_share_auto_ksk(self.h, crs_b.ctypes.data, ns_b.ctypes.data,
+                                           s.ctypes.data, z.ctypes.data, auto_b.ctypes.data,
+                                           ksk_b.ctypes.data))
+        return auto_b, ksk_b
+
+    def ni_aggregate_auto_ksk(self, total, d_auto_shares, d_ksk_shares, d_auto_b, d_ksk_b):
+        _check(self.L.pz_ni_aggregate_auto_ksk(self.h, total, d_auto_shares.ptr,
+                                               d_ksk_shares.ptr, d_auto_b.ptr, d_ksk_b.ptr, None))
+
+    def ni_expand_server_key(self, crs, d_rgsw, d_auto_b, d_ksk_b):
+        h = C.c_void_p()
+        crs_b = self.seed32(crs)
+        _check(self.L.pz_ni_expand_server_key(self.h, crs_b.ctypes.data, d_rgsw.ptr,
+                                              d_auto_b.ptr, d_ksk_b.ptr, C.byref(h)))
+        return Key(self, h.value)
+
+    def ni_key_share(self, crs, noise_seed, me, total, s, u, z):
+        """Party `me`'s whole share as a wire-format blob (uint8 array)."""
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        z = np.ascontiguousarray(z, dtype=np.int32)
+        out = aligned_bytes(ni_share_bytes(self.p, me, total))
+        crs_b, ns_b = self.seed32(crs), self.seed32(noise_seed)
+        _check(self.L.pz_ni_key_share(self.h, crs_b.ctypes.data, ns_b.ctypes.data, me, total,
+                                      s.ctypes.data, u.ctypes.data, z.ctypes.data,
+                                      out.ctypes.data))
+        return out
+
+    def ni_aggregate_shares(self, shares):
+        """The seeded server key blob of a list of share blobs."""
+        total = len(shares)
+        keep = []
+        for b in shares:
+            if b.ctypes.data % 8:
+                a = aligned_bytes(len(b))
+                a[:] = b
+                b = a
+            keep.append(b)
+        arr = (C.c_void_p * total)(*[b.ctypes.data for b in keep])
+        out = aligned_bytes(ni_seeded_key_bytes(self.p, total))
+        _check(self.L.pz_ni_aggregate_shares(self.h, arr, total, out.ctypes.data))
+        return out
+
+    def ni_server_key_from_seeded(self, blob, total=None):
+        """(Key, [RlweKsk] or None) from a seeded server key blob; with
+        total given, the parties' Ksk(u_j -> s) are made too."""
+        h = C.c_void_p()
+        ks = (C.c_void_p * total)() if total else None
+        _check(self.L.pz_ni_server_key_from_seeded(self.h, blob.ctypes.data, C.byref(h), ks))
+        return Key(self, h.value), ([RlweKsk(self, k) for k in ks] if total else None)
 
     def autotune(self, key, B):
         """Pick the fastest blind-rotation launch shape for batches like B."""

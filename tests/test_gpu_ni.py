@@ -7,11 +7,13 @@ word for word, and end to end.
 - Ksk(u_j -> s) from the shares equals the oracle's.
 - The aggregated RGSWs of indices straddling a leader boundary equal the
   oracle's (every produced RLWE and the product chain).
+- Automorphism-key and LWE-KSK shares, their sums and the expansion of the
+  seeded server key equal the oracle's (NI_2P with 2 parties, NI_8P with 3).
 - Full NI_2P aggregation, 2 parties, all 520 LWE indices on the device: a
-  sample of indices equals the oracle, and a server key whose RGSW part is the
-  aggregate (automorphism keys and LWE KSK generated under the same ideal
-  secret, as the reference's share sums give) bootstraps NAND / XOR gates that
-  decrypt correctly under the ideal secret.
+  sample of indices equals the oracle; the server key built from the parties'
+  shares alone (RGSWs aggregated, automorphism keys and LWE KSK summed, a parts
+  expanded from the CRS) equals the oracle's expansion word for word and
+  bootstraps NAND / XOR gates that decrypt correctly under the ideal secret.
 """
 import sys
 
@@ -127,13 +129,22 @@ def test_ni_full_aggregation_ni2p_bootstraps():
             want = o.ni_aggregate_rgsw(CRS, total, np.stack(oks), l, 1, zero[:, l:l + 1],
                                        cts[:, l:l + 1])
             assert np.array_equal(got[l], want[0]), l
-        # server key: aggregated RGSWs in the evaluation domain
-        ctx.ntt("forward", d_out, n * rows)
-        rgsw_eval = d_out.download()
-        base = ctx.keygen_server(5, s, z)
-        _, auto, ksk = ctx.key_download(base)
-        base.free()
-        key = ctx.key_upload(rgsw_eval, auto, ksk)
+        # automorphism keys and LWE KSK from the parties' shares, summed, and
+        # the whole server key expanded from the seeded parts on the device
+        log("ni full auto/ksk shares NI_2P")
+        ak = [ctx.ni_share_auto_ksk(CRS, 6000 + j, ps[j][0], ps[j][1]) for j in range(total)]
+        d_as = ctx.to_device(np.stack([x[0] for x in ak]))
+        d_ks = ctx.to_device(np.stack([x[1] for x in ak]))
+        d_ab = ctx.array((p.w + 1, p.auto_d, N))
+        d_kb = ctx.array((N * p.lwe_d,))
+        ctx.ni_aggregate_auto_ksk(total, d_as, d_ks, d_ab, d_kb)
+        key = ctx.ni_expand_server_key(CRS, d_out, d_ab, d_kb)
+        assert np.array_equal(d_out.download(), got)  # inputs unchanged
+        okey = o.ni_expand_server_key(CRS, got, d_ab.download(), d_kb.download())
+        g_rgsw, g_auto, g_ksk = ctx.key_download(key)
+        assert np.array_equal(g_rgsw.reshape(-1), okey.rgsw)
+        assert np.array_equal(g_auto.reshape(-1), okey.auto_key)
+        assert np.array_equal(g_ksk.reshape(-1), okey.lwe_ksk)
         B = 256
         r = rng(77)
         b0 = r.integers(0, 2, size=B).astype(np.uint8)
@@ -150,5 +161,40 @@ def test_ni_full_aggregation_ni2p_bootstraps():
         key.free()
         for k in ksks:
             k.free()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("name,total", [("NI_2P", 2), ("NI_8P", 3)])
+def test_ni_auto_ksk_shares_sums_and_expansion_match_oracle(name, total):
+    import pzlib
+    o = oracle_for(name)
+    p, N = o.p, o.N
+    ctx = pzlib.Context(name)
+    try:
+        ps = party_secrets(ctx, total)
+        log("ni auto/ksk shares %s parties=%d" % (name, total))
+        sh = []
+        for j in range(total):
+            g = ctx.ni_share_auto_ksk(CRS, 7000 + j, ps[j][0], ps[j][1])
+            w = o.ni_share_auto_ksk(CRS, 7000 + j, ps[j][0], ps[j][1])
+            assert np.array_equal(g[0], w[0]) and np.array_equal(g[1], w[1]), j
+            sh.append(g)
+        d_as = ctx.to_device(np.stack([x[0] for x in sh]))
+        d_ks = ctx.to_device(np.stack([x[1] for x in sh]))
+        d_ab, d_kb = ctx.array((p.w + 1, p.auto_d, N)), ctx.array((N * p.lwe_d,))
+        ctx.ni_aggregate_auto_ksk(total, d_as, d_ks, d_ab, d_kb)
+        wa, wk = o.ni_aggregate_auto_ksk(np.stack([x[0] for x in sh]), np.stack([x[1] for x in sh]))
+        assert np.array_equal(d_ab.download(), wa) and np.array_equal(d_kb.download(), wk)
+        rows = 2 * (p.rlrg_da + p.rlrg_db)
+        rgsw = rng(31).integers(0, o.Q, size=(p.lwe_n, rows, N), dtype=np.uint64)
+        log("ni expand server key %s" % name)
+        key = ctx.ni_expand_server_key(CRS, ctx.to_device(rgsw), d_ab, d_kb)
+        okey = o.ni_expand_server_key(CRS, rgsw, wa, wk)
+        g_rgsw, g_auto, g_ksk = ctx.key_download(key)
+        assert np.array_equal(g_rgsw.reshape(-1), okey.rgsw)
+        assert np.array_equal(g_auto.reshape(-1), okey.auto_key)
+        assert np.array_equal(g_ksk.reshape(-1), okey.lwe_ksk)
+        key.free()
     finally:
         ctx.close()
