@@ -174,6 +174,17 @@ int pz_pbs(pz_ctx *ctx, pz_key *key, const uint32_t *d_gate, uint64_t *d_lwe, si
  * same buffers replay it. */
 int pz_gates(pz_ctx *ctx, pz_key *key, const uint32_t *d_ops, const uint64_t *d_c0,
              const uint64_t *d_c1, uint64_t *d_out, size_t B, void *stream);
+/* pz_gates that allocates nothing.  No counterpart in the reference (its
+ * evaluator allocates per gate): for callers that cannot call hipMalloc --
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation.  The pipeline scratch is a fixed 16 MiB device array per GPU,
+ * held until the process ends, so B is at most what it holds (576 at NI_2P,
+ * 512 at NI_8P; larger B: PZ_ERR_ARG with the limit in the message).  The
+ * launches are eager (no captured graph) and give the same words as
+ * pz_gates.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (from any context or thread) race. */
+int pz_gates_noalloc(pz_ctx *ctx, pz_key *key, const uint32_t *d_ops, const uint64_t *d_c0,
+                     const uint64_t *d_c1, uint64_t *d_out, size_t B, void *stream);
 /* not_inplace (src/bool/evaluator.rs:2236-2239) on B ciphertexts. */
 int pz_not(pz_ctx *ctx, uint64_t *d_c, size_t B, void *stream);
 

@@ -120,6 +120,62 @@ bool pz_internal_key_fits(const pz_ctx *c, const pz_key *k);
             return fail(PZ_ERR_ARG, "key made for another parameter set or device"); \
     } while (0)
 
+// Pipeline scratch of one gate / PBS batch: key-switch digits (tile-major),
+// bodies, key-switch outputs and automorphism counts.
+struct PbsScratch {
+    int8_t *digits;
+    u32 *ksb, *ks, *stats;
+};
+
+// ---- fixed pipeline scratch of pz_gates_noalloc: one array per GPU, held
+// until the process ends.  Host code never passes the symbol itself (in host
+// code it names a host-side stand-in): each device's address comes from
+// hipGetSymbolAddress at the first context creation on that device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Records the device's scratch address; `device` must be current.
+static int noalloc_setup(int device)
+{
+    if (device >= NOALLOC_MAX_DEVICES) return PZ_OK;  // pz_gates_noalloc refuses it
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return PZ_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(PZ_ERR_HIP, "address of the fixed scratch not available");
+    g_scratch_ptr[device] = ptr;
+    return PZ_OK;
+}
+
+// Carves the pipeline scratch of `cap` ciphertexts (a multiple of the key
+// switch's tile) out of `base`, each part 256-byte aligned; returns the bytes
+// used (base = nullptr only measures).
+static size_t noalloc_layout(const pz_params &p, size_t cap, uint8_t *base, PbsScratch *s)
+{
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t digits = up((size_t)p.rlwe_n * p.lwe_d * cap), ksb = up(4 * cap);
+    const size_t ks = up(4 * cap * ((size_t)p.lwe_n + 1)), stats = up(4 * cap);
+    if (s && base) {
+        s->digits = (int8_t *)base;
+        s->ksb = (u32 *)(base + digits);
+        s->ks = (u32 *)(base + digits + ksb);
+        s->stats = (u32 *)(base + digits + ksb + ks);
+    }
+    return digits + ksb + ks + stats;
+}
+
+// Largest batch the fixed scratch holds for a parameter set.
+static size_t noalloc_max_batch(const pz_params &p)
+{
+    size_t cap = 0;
+    while (noalloc_layout(p, cap + KS_CT, nullptr, nullptr) <= NOALLOC_SCRATCH_BYTES) cap += KS_CT;
+    return cap;
+}
+
 extern "C" int pz_params_named(const char *name, pz_params *out)
 {
     if (!name || !out) return fail(PZ_ERR_ARG, "null argument");
@@ -185,6 +241,7 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(PZ_ERR_ARG, "no such device");
     HIPCHK(hipSetDevice(device));
+    if ((rc = noalloc_setup(device))) return rc;
     pz_ctx *c = new pz_ctx();
     c->p = *p;
     c->device = device;
@@ -784,16 +841,18 @@ extern "C" int pz_set_blind_rotate_mode(pz_ctx *c, int mode)
 // blind-rotation variant and grid (the autotuner's candidates)
 static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, const uint64_t *c1,
                          const uint32_t *ops, uint64_t *out, size_t B, hipStream_t st,
-                         int force_var = -1, int force_grid = 0)
+                         int force_var = -1, int force_grid = 0, const PbsScratch *sc = nullptr)
 {
     const pz_params &p = c->p;
     u32 n1 = p.lwe_n + 1;
+    const PbsScratch own{c->d_digits, c->d_ksb, c->d_ks, c->d_stats};
+    if (!sc) sc = &own;  // the context's grow-on-demand scratch
     if (c->timing) hipEventRecord(c->ev[0], st);
-    launch_prep(mode, c0, c1, nullptr, nullptr, ops, p.rlwe_n + 1, B, c->d_digits, c->d_ksb,
-                c->d_ks, c->dc, st);
+    launch_prep(mode, c0, c1, nullptr, nullptr, ops, p.rlwe_n + 1, B, sc->digits, sc->ksb,
+                sc->ks, c->dc, st);
     if (c->timing) hipEventRecord(c->ev[1], st);
-    launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
-              k->dk.ldk, 0, B, c->d_ks, st);
+    launch_ks(sc->digits, sc->ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
+              k->dk.ldk, 0, B, sc->ks, st);
     if (c->timing) hipEventRecord(c->ev[2], st);
     int var = BR_ONE, grid = 1;
     if (force_var >= 0) {
@@ -802,7 +861,7 @@ static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, con
     } else {
         br_plan(c, B, &var, &grid);
     }
-    launch_blind_rotate(c->logn, var, grid, c->d_ks, ops, B, out, c->d_stats, c->dc, k->dk,
+    launch_blind_rotate(c->logn, var, grid, sc->ks, ops, B, out, sc->stats, c->dc, k->dk,
                         nullptr, nullptr, nullptr, st);
     if (c->timing) hipEventRecord(c->ev[3], st);
 }
@@ -949,6 +1008,38 @@ extern "C" int pz_gates(pz_ctx *c, pz_key *k, const uint32_t *ops, const uint64_
     }
     HIPCHK(hipGraphLaunch(it->second, st));
     return PZ_OK;
+}
+
+// pz_gates without any allocation: the pipeline scratch is this device's
+// fixed array (no grow-on-demand buffers, no captured graph), so a call holds
+// at most noalloc_max_batch bootstraps.  Every call on a device uses the same
+// array; each kernel writes the parts it reads before reading them (the prep
+// kernel clears the key-switch outputs), so no call depends on what an
+// earlier one left, but calls that overlap on one device race.
+extern "C" int pz_gates_noalloc(pz_ctx *c, pz_key *k, const uint32_t *ops, const uint64_t *c0,
+                                const uint64_t *c1, uint64_t *out, size_t B, void *s)
+{
+    if (!c || !k || !ops || !c0 || !c1 || !out) return fail(PZ_ERR_ARG, "null argument");
+    KEY_OK(c, k);
+    if (B == 0) return PZ_OK;
+    const size_t bmax = noalloc_max_batch(c->p);
+    if (B > bmax) {
+        char msg[160];
+        snprintf(msg, sizeof(msg),
+                 "batch of %zu exceeds the fixed scratch: at most %zu bootstraps per call for "
+                 "this parameter set (%zu MiB per device)",
+                 B, bmax, NOALLOC_SCRATCH_BYTES >> 20);
+        return fail(PZ_ERR_ARG, msg);
+    }
+    if (c->device >= NOALLOC_MAX_DEVICES || !g_scratch_ptr[c->device])
+        return fail(PZ_ERR_UNSUPPORTED, "no fixed scratch on this device");
+    PbsScratch sc;
+    const size_t cap = (B + KS_CT - 1) / KS_CT * KS_CT;
+    noalloc_layout(c->p, cap, (uint8_t *)g_scratch_ptr[c->device], &sc);
+    hipStream_t st = S(c, s);
+    pbs_sequence(c, k, PREP_GATE, c0, c1, ops, out, B, st, -1, 0, &sc);
+    HIPCHK(hipGetLastError());
+    return collect_timing(c, st);
 }
 
 extern "C" int pz_not(pz_ctx *c, uint64_t *d, size_t B, void *s)

@@ -106,6 +106,7 @@ SIGNATURES = {
     "pz_lwe_key_switch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_pbs": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_gates": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "pz_gates_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "pz_not": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     "pz_last_auto_counts": (C.c_int, [_vp, _vp, C.c_size_t]),
     "pz_set_timing": (C.c_int, [_vp, C.c_int]),
