@@ -4,9 +4,10 @@ plus plain PBS, FheUint8 circuits and serving latency.
 Headline (the JSON line's `value`): a step is one batched launch of B = 4096
 independent NAND gate bootstraps (gate linear step + f64 moddown + LWE key
 switch + persistent blind rotation + sample extract), keys and ciphertexts
-resident in HBM, replayed from a captured hipGraph.  Keys are the seeded
-2-party ideal-secret NI_2P server key; inputs are encryptions of seeded
-random bits; outputs are decrypted and checked after timing.
+resident in HBM, replayed from a captured hipGraph.  The key is the 2-party
+NI_2P server key aggregated from the parties' seeded shares; inputs are
+encryptions of seeded random bits under the ideal secret; outputs are
+decrypted and checked after timing.
 
 Extra fields: batch sweep (1024 / 16384), plain one-PBS-per-launch rate,
 FheUint8 at NI_8P (256 mul + 256 div_rem + 256 lt, level by level), serving
@@ -53,9 +54,24 @@ def modmul_per_pbs(N, n, autos):
     return n * (2 * F + 2 * I + 4 * N) + autos * (F + 2 * I + 2 * N)
 
 
+NI_CRS = bytes(range(7, 39))
+
+
 def keys(ctx, parties, seed=0):
-    s, z = ctx.keygen_secrets(seed, parties)
-    return s, z, ctx.keygen_server(seed + 1, s, z)
+    """The non-interactive server key of `parties` parties, from their seeded
+    shares alone: each party draws its secrets (s_j, z_j, u_j) from seeds
+    derived from `seed` and makes its share blob; the shares are aggregated
+    into the seeded server key, which is expanded on the device.  Returns the
+    ideal secrets s = sum s_j, z = sum z_j and the key."""
+    blobs, s, z = [], None, None
+    for j in range(parties):
+        s_j, z_j = ctx.keygen_secrets(1000 * seed + 10 * j + 1, 1)
+        u_j = ctx.keygen_user_secret(1000 * seed + 10 * j + 2)
+        blobs.append(ctx.ni_key_share(NI_CRS, 1000 * seed + 10 * j + 3, j, parties, s_j, u_j, z_j))
+        s = s_j.astype(np.int64) if s is None else s + s_j
+        z = z_j.astype(np.int64) if z is None else z + z_j
+    key, _ = ctx.ni_server_key_from_seeded(ctx.ni_aggregate_shares(blobs))
+    return s.astype(np.int32), z.astype(np.int32), key
 
 
 def batched(ctx, key, s, B, steps, warmup, seed):
@@ -334,9 +350,6 @@ def input_path():
             "bits_per_sec": count / (ks_s + ex_s), "sample_correct": ok}
 
 
-NI_CRS = bytes(range(7, 39))
-
-
 def ni_shares(ctx, total, l0, count):
     """Party secrets and NI server-key shares (client side) for indices
     [l0, l0 + count)."""
@@ -408,6 +421,19 @@ def ni_aggregation(params="NI_2P", total=2):
             "share_gen_ms_per_party": share_s * 1e3, "gates_correct": ok}
 
 
+def cpu_share():
+    """Host cores this process may run on: its affinity mask, capped by the
+    cgroup CPU quota when one is set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(quota) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
 def cpu_baseline(params, threads, per_thread):
     """Oracle PBS throughput on the host cores (one PBS stream per thread)."""
     from oracle import Oracle
@@ -438,6 +464,8 @@ def cpu_baseline(params, threads, per_thread):
         agg = {"aggregate_ms_est": per_index * o.p.lwe_n * 1e3, "cores": 1,
                "sample": "%d of %d LWE indices, 2 parties, scaled" % (cnt, o.p.lwe_n)}
     return {"value": count / secs, "unit": "bootstraps/sec", "cores": threads,
+            "host_cores": os.cpu_count(), "cpu_share": cpu_share(),
+            "build": "gcc -O3 (oracle/Makefile)",
             "kind": "port", "single_thread_pbs_ms": secs1 / 4 * 1e3,
             "ni_aggregation_ni2p": agg,
             "sample": "%d NAND PBS at %s (C restatement of the reference's Rust PBS), "
@@ -535,11 +563,16 @@ def main():
     value, secs, ok, bufs = batched(ctx, key, s, B, args.steps, args.warmup, seed=3)
     log("batched B=%d: %.0f bootstraps/s" % (B, value))
 
-    # per-kernel durations (HIP events on the launch stream) of one more step
+    # per-kernel durations: HIP events around each kernel on the launch
+    # stream, averaged over as many eager steps as were timed (the graph
+    # replays above launch the same three kernels with the same shapes)
     dops, d0, d1, dout = bufs
     ctx.set_timing(True)
-    ctx.gates(key, dops, d0, d1, dout, B)
-    tim = ctx.last_timing()
+    tim = {"prep_ms": 0.0, "ks_ms": 0.0, "blind_rotate_ms": 0.0}
+    for _ in range(args.steps):
+        ctx.gates(key, dops, d0, d1, dout, B)
+        for name, v in ctx.last_timing().items():
+            tim[name] += v / args.steps
     ctx.set_timing(False)
     autos = ctx.last_auto_counts(B)
     mm = float(sum(modmul_per_pbs(N, n, int(a)) for a in autos))
@@ -588,7 +621,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or cpu_share()
         cpu = cpu_baseline(args.params, threads, args.cpu_per_thread)
         log("cpu: %s" % cpu)
 
@@ -604,7 +637,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
-        "data": "synthetic: seeded 2-party NI_2P keys, encryptions of seeded random bits; "
+        "data": "synthetic: 2-party NI_2P server key aggregated from the parties' seeded "
+                "shares (seed 0), encryptions of seeded random bits under the ideal secret; "
                 "outputs decrypted and checked",
         "config": {"workload": "batched NAND PBS, %s (N=%d, n=%d), B=%d per launch" %
                                (args.params, N, n, B), "batch": B, "params": args.params},
@@ -613,10 +647,19 @@ def main():
         "plain_bootstraps_per_sec": plain,
         "batched_over_plain": value / plain,
         "kernel_ms": tim,
+        "timing": {"ms_per_step": "wall clock between synchronisations over the timed "
+                                  "hipGraph replays",
+                   "kernel_ms": "HIP events around each kernel on the launch stream, "
+                                "averaged over %d eager launches of the same batch" % args.steps,
+                   "roofline_achieved": "kernel_ms.blind_rotate_ms"},
         "autos_mean": float(np.mean(autos)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak,
                      "unit": "Gmodmul/s", "frac": achieved / peak,
                      "traffic": measured_traffic(args.params, B),
+                     "traffic_source": "not measured in this run: rocprofv3 --pmc FETCH_SIZE / "
+                                       "WRITE_SIZE of this workload, scaled by the copy "
+                                       "calibration (profiles/k_blind_rotate_%s_b%d_pmc.md)" %
+                                       (args.params.lower().replace("_", ""), B),
                      "kernel": "k_blind_rotate",
                      "note": "54-bit Shoup modmuls/s of the blind rotation; peak = "
                              "VALU lane-op peak / %d ops per Shoup multiply" %
