@@ -59,6 +59,32 @@ PZ_DEV u64 shoup_mul4(u64 a, u64 w, u64 ws, u64 q)
     return shoup_rem(0, a, w, shoup_quot4(a, ws), q);
 }
 
+// Shoup multiply by 1 (ws = floor(2^64/q)), the fused inverse's reduction of
+// an unreduced sum, for q > 2^32 so that ws < 2^32: the quotient needs two
+// 32-bit products instead of the general four, the remainder v - q*h two.
+// shoup_one: the exact quotient floor(v*ws/2^64) (v1*ws + hi(v0*ws), high
+// word), the same word as shoup_mul(v, 1, ws, q), in [0, 2q).
+// shoup_one4: the quotient shoup_quot4 forms when ws < 2^32 (hi(v1*ws)), the
+// same word as shoup_mul4(v, 1, ws, q), in [0, 4q) (in fact [0, 3q)).
+PZ_DEV u64 shoup_one_rem(u64 v, u32 h, u64 q)
+{
+    const u64 nq = 0 - q;  // v + (2^64 - q)*h mod 2^64
+    const u64 lo = (u64)(u32)nq * h + v;
+    u32 hi = (u32)(lo >> 32);
+    const u32 m = (u32)(nq >> 32) * h;
+    asm("v_add_u32 %0, %1, %2" : "=v"(hi) : "v"(hi), "v"(m));
+    return ((u64)hi << 32) | (u32)lo;
+}
+PZ_DEV u64 shoup_one(u64 v, u32 ws, u64 q)
+{
+    const u64 t = (u64)(u32)(v >> 32) * ws + __umulhi((u32)v, ws);
+    return shoup_one_rem(v, (u32)(t >> 32), q);
+}
+PZ_DEV u64 shoup_one4(u64 v, u32 ws, u64 q)
+{
+    return shoup_one_rem(v, __umulhi((u32)(v >> 32), ws), q);
+}
+
 // ShoupMul::mul, src/utils.rs:43-46: b*a - q*floor(b_shoup*a/2^64) (wrapping),
 // result in [0, 2q) for any a < 2^64.
 // SPLIT selects the two-step form above (two fewer VALU instructions per

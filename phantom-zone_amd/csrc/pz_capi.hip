@@ -194,6 +194,9 @@ static int check_params(const pz_params *p, int *logn)
     // (forward), so q < 2^54
     if (p->rlwe_q >= (1ull << 54) || p->rlwe_q % (2ull * N) != 1)
         return fail(PZ_ERR_ARG, "rlwe_q must be < 2^54 and 1 mod 2N");
+    // the N = 2048 transforms reduce with floor(2^64/q) as a 32-bit factor
+    if (l == 11 && p->rlwe_q <= (1ull << 32))
+        return fail(PZ_ERR_UNSUPPORTED, "rlwe_q must exceed 2^32 at N = 2048");
     if (p->lwe_q & (p->lwe_q - 1) || p->lwe_q > (1ull << 17) || p->lwe_q < 2)
         return fail(PZ_ERR_UNSUPPORTED, "lwe_q must be a power of two <= 2^17");
     if (p->br_q & (p->br_q - 1) || p->br_q > 2 * N || p->br_q < 16)

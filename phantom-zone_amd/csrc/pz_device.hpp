@@ -437,11 +437,15 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2), R,
             false>(acc, j, mine, c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
+        // acc + (+-v) mod q, branch-free: the negated value q - v is selected,
+        // then one conditional subtraction (acc < q, q - v <= q); the same word
+        // as sub_mod / add_mod
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const u64 bp = stash[lds_ix(coef_x<LOGN, R>(j, r))];
             const u64 v = bp & ~SIGN;
-            p[r] = (bp & SIGN) ? sub_mod(acc[0][r], v, q) : add_mod(acc[0][r], v, q);
+            const u64 o = acc[0][r] + ((bp & SIGN) ? q - v : v);
+            p[r] = umin(o, o - q);
         }
     } else {
 #pragma unroll

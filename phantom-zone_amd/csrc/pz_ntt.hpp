@@ -157,6 +157,9 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
     constexpr int LIB = IB == 8 ? 3 : (IB == 4 ? 2 : 1);  // log2 IB
     constexpr int KR = 10 - LIB;
     constexpr bool T4 = PZ_INV_TAIL4 && FAST && IB == 8;  // unreduced tail
+    // N = 2048 contexts have q > 2^32 (pz_ctx_create), so floor(2^64/q) fits
+    // 32 bits and the reductions by 1 take the short forms
+    constexpr bool ONE32 = LOGN >= 11;
     static_assert(!FAST || !SCALE, "the fast inverse leaves n^-1 to the key rows");
     static_assert(IB == 2 || ((IB == 4 || IB == 8) && FAST),
                   "inputs below 4q or 8q need the fast inverse");
@@ -174,10 +177,13 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
                 const u64 sum = xv + yv;
                 const u64 dif = xv + (tb.q << (K - 1 + LIB)) - yv;
                 if constexpr (T4 && K == KR) {
-                    v[p][r] = shoup_mul4(sum, 1, tb.one_s, tb.q);
+                    v[p][r] = ONE32 ? shoup_one4(sum, (u32)tb.one_s, tb.q)
+                                    : shoup_mul4(sum, 1, tb.one_s, tb.q);
                     v[p][r + LD] = shoup_mul4(dif, w, ws, tb.q);
                 } else {
-                    v[p][r] = K == KR ? shoup_mul<SM>(sum, 1, tb.one_s, tb.q) : sum;
+                    v[p][r] = K != KR ? sum
+                                      : (ONE32 ? shoup_one(sum, (u32)tb.one_s, tb.q)
+                                               : shoup_mul<SM>(sum, 1, tb.one_s, tb.q));
                     v[p][r + LD] = (IB == 8 && K < KR) ? shoup_mul4(dif, w, ws, tb.q)
                                                        : shoup_mul<SM>(dif, w, ws, tb.q);
                 }
@@ -186,7 +192,8 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
                 const u64 sum = xv + yv;
                 const u64 dif = xv + (tb.q << (K - KR + 1)) - yv;  // inputs below 2^(K-KR+1) q
                 if constexpr (S == 0) {  // the last stage: exact, then [0,2q) -> [0,q)
-                    const u64 xo = shoup_mul<SM>(sum, 1, tb.one_s, tb.q);
+                    const u64 xo = ONE32 ? shoup_one(sum, (u32)tb.one_s, tb.q)
+                                         : shoup_mul<SM>(sum, 1, tb.one_s, tb.q);
                     const u64 yo = shoup_mul<SM>(dif, w, ws, tb.q);
                     v[p][r] = umin(xo, xo - tb.q);
                     v[p][r + LD] = umin(yo, yo - tb.q);
