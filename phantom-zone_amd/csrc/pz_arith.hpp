@@ -108,6 +108,15 @@ PZ_DEV void fb_4q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)
 
 PZ_DEV u64 umin(u64 a, u64 b) { return a < b ? a : b; }
 
+// 2x + c (c uniform) as one v_lshl_add_u64: written as plain C the compiler
+// turns (x << 1) + c into ((x + c/2) << 1), an add and a 64-bit shift
+PZ_DEV u64 twice_plus(u64 x, u64 c)
+{
+    u64 r;
+    asm("v_lshl_add_u64 %0, %1, 1, %2" : "=v"(r) : "v"(x), "s"(c));
+    return r;
+}
+
 // forward_butterly_0_to_2q, src/ntt.rs:51-75
 template <bool SPLIT = true>
 PZ_DEV void fb_2q(u64 &x, u64 &y, u64 w, u64 ws, u64 q, u64 q2)

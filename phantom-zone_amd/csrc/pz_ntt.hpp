@@ -110,10 +110,12 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
                 v[p][r] = x + t;
                 v[p][r + LD] = x + tb.q2 - t;
             } else if (KIND == 4) {
-                const u64 x = v[p][r];
-                const u64 t = shoup_mul4(v[p][r + LD], w, ws, tb.q);
-                v[p][r] = x + t;
-                v[p][r + LD] = x + (tb.q2 << 1) - t;
+                // x + t with x as the remainder's addend (no separate add), and
+                // x + 4q - t as 2x + 4q - (x + t)
+                const u64 x = v[p][r], a = v[p][r + LD];
+                const u64 xt = shoup_rem(x, a, w, shoup_quot4(a, ws), tb.q);
+                v[p][r] = xt;
+                v[p][r + LD] = twice_plus(x, tb.q2 << 1) - xt;
             } else {
                 fb_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
                 if (KIND == 2) {
