@@ -19,6 +19,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -184,7 +185,8 @@ def serving(args, plain_rate):
     """Poisson arrivals of FheUint8 programs (1..40 ops of add/sub/mul/div/
     rem/lt/eq) from 64 sessions, alternately with 2-party NI_2P and 4-party
     NI_4P_HB_FR keys (one server, one batch per parameter set per step), then
-    single-multiply requests."""
+    single-multiply requests.  A client thread submits each request at its
+    arrival time while the server loop runs (the online path)."""
     import pzlib
     import pzprog
     ctxs = {name: pzlib.Context(name) for name, _ in SERVING_SETS}
@@ -208,9 +210,9 @@ def serving(args, plain_rate):
         r = np.random.default_rng(400 if mode == "mixed" else 401)
         rate = args.rate if mode == "mixed" else args.mul_rate
         count = args.programs if mode == "mixed" else args.mul_requests
-        t, rids, progs, metas = 0.0, [], [], []
+        gaps, jobs, progs, metas = [], [], [], []
         for j in range(count):
-            t += float(r.exponential(1.0 / rate))
+            gaps.append(float(r.exponential(1.0 / rate)))
             si = int(r.integers(0, args.sessions))
             if mode == "mixed":
                 prog = pzprog.Program(r, int(r.integers(1, 41)))
@@ -220,12 +222,33 @@ def serving(args, plain_rate):
             wires, nu, nb = prog.build(circ)
             name, sec, _ = blobs[si % len(blobs)]
             cts = ctxs[name].encrypt_bools(sec, prog.input_bits(), 5000 + 97 * j)
-            rids.append(srv.submit(sids[si], circ, cts, wires, len(prog.steps), t))
-            circ.close()
+            jobs.append((sids[si], circ, cts, wires, len(prog.steps)))
             progs.append((prog, si))
             metas.append((nu, nb))
-        log("serving %s: %d requests submitted" % (mode, count))
+        # online: a client thread submits each request at its Poisson arrival
+        # time while the loop serves the others (arrival stamped at submission)
+        rids, errs = [None] * count, []
+
+        def client():
+            try:
+                for j, job in enumerate(jobs):
+                    time.sleep(gaps[j])
+                    rids[j] = srv.submit(*job, -1.0)
+            except Exception as e:  # noqa: BLE001 -- reported below
+                errs.append(e)
+            finally:
+                srv.intake(False)
+
+        srv.intake(True)
+        th = threading.Thread(target=client)
+        th.start()
         srv.run()
+        th.join()
+        if errs:
+            raise errs[0]
+        for job in jobs:
+            job[1].close()
+        log("serving %s: %d requests served online" % (mode, count))
         st = srv.stats()
         lat, per_op, ok = [], [], True
         for rid, (prog, si), meta in zip(rids, progs, metas):
@@ -239,6 +262,8 @@ def serving(args, plain_rate):
         for i in range(args.sessions):
             sets[blobs[i % len(blobs)][0]] = sets.get(blobs[i % len(blobs)][0], 0) + 1
         res = {"requests": count, "arrival_rate_per_s": rate, "sessions": args.sessions,
+               "submission": "online: a client thread submits at each Poisson arrival while "
+                             "the loop serves; latency = completion - submission",
                "sessions_per_parameter_set": sets,
                "distinct_keys": args.distinct_keys, "total_pbs": st["total_pbs"],
                "launches": st["batches"], "wall_s": st["wall_s"],

@@ -241,8 +241,9 @@ int pz_circuit_read(pz_circuit *c, const uint32_t *wires, size_t count, uint64_t
  * sessions are then evicted LRU to make room).  Each step runs the next
  * ready gate level of every active request, across sessions and up to
  * max_batch_pbs bootstraps, as one batched launch; requests arriving while
- * others run join at the next step.  Arrival times are seconds from the
- * start of pz_server_run.  pz_server_add_session registers a session of the
+ * others run join at the next step.  Steps are queued without waiting for
+ * the previous one (up to two in flight per context).  Arrival times are
+ * seconds from the start of pz_server_run.  pz_server_add_session registers a session of the
  * creating context's parameter set; pz_server_add_session_ctx one of another
  * context's (same device, same ring size, else PZ_ERR_UNSUPPORTED), whose
  * gates a step batches apart and launches on that context's stream.  Every
@@ -263,11 +264,21 @@ int pz_server_add_session_ctx(pz_server *s, pz_ctx *ctx, const uint64_t *h_rgsw,
                               uint32_t *session_id);
 /* prog: a circuit (may be build-only); h_inputs: one [N+1] ciphertext per
  * input wire in order; out_wires: results to return; n_ops: FheUint8
- * operations in the program (for per-op latency). */
+ * operations in the program (for per-op latency).  Thread-safe with respect
+ * to a running pz_server_run (the only call that may overlap it; sessions are
+ * added before).  arrival < 0: the request arrives when submitted. */
 int pz_server_submit(pz_server *s, uint32_t session, pz_circuit *prog, const uint64_t *h_inputs,
                      const uint32_t *out_wires, uint32_t n_out, uint32_t n_ops, double arrival,
                      uint32_t *request_id);
+/* Intake open (open = 1): pz_server_run keeps serving, waiting for
+ * submissions when idle, until the intake is closed (open = 0, from any
+ * thread) and every request is complete.  Closed (the default): it returns
+ * once every request submitted so far is complete.  No counterpart in the
+ * reference (it has no server). */
+int pz_server_intake(pz_server *s, int open);
 int pz_server_run(pz_server *s);
+/* Result ciphertexts of a completed request ([n_out][N+1]); PZ_ERR_STATE
+ * while it is not complete (may be called while the loop runs). */
 int pz_server_result(pz_server *s, uint32_t request_id, uint64_t *h_out);
 int pz_server_request_times(pz_server *s, uint32_t request_id, double *arrival, double *admitted,
                             double *finished, uint32_t *n_pbs, uint32_t *n_ops);

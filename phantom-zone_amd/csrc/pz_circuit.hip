@@ -97,17 +97,33 @@ struct Staging {
     }
 };
 
-// One staging area per context: batches of different contexts (a server
-// serving several parameter sets) run on their own streams at the same time,
-// and each waits only for its own stream before rewriting its pinned buffer.
+// Two staging areas per context, used in turn: a batch's index arrays stay
+// untouched until the batch has finished (an event recorded after its last
+// launch), so the host prepares and queues the next batch while the previous
+// one runs.  Batches of different contexts (a server serving several
+// parameter sets) run on their own streams with their own staging.
+struct StagingPair {
+    Staging slot[2];
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    int next = 0;
+    ~StagingPair()
+    {
+        for (int i = 0; i < 2; i++)
+            if (done[i]) {
+                (void)hipEventSynchronize(done[i]);
+                (void)hipEventDestroy(done[i]);
+            }
+    }
+};
 static std::mutex g_stage_mu;
-static std::map<const pz_ctx *, Staging *> g_stages;
+static std::map<const pz_ctx *, StagingPair *> g_stages;
 
-static Staging &stage_of(const pz_ctx *c)
+static StagingPair &stage_of(const pz_ctx *c)
 {
     std::lock_guard<std::mutex> lk(g_stage_mu);
-    Staging *&st = g_stages[c];
-    if (!st) st = new Staging();
+    StagingPair *&st = g_stages[c];
+    if (!st) st = new StagingPair();
     return *st;
 }
 
@@ -117,7 +133,7 @@ void pz_internal_exec_forget(const pz_ctx *c)
 {
     using pz::g_stage_mu;
     using pz::g_stages;
-    pz::Staging *st = nullptr;
+    pz::StagingPair *st = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_stage_mu);
         auto it = g_stages.find(c);
@@ -142,7 +158,9 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
     DevCtx dc;
     int rc = pz_internal_exec_view(c, B ? B : 1, &v, &dc);
     if (rc) return rc;
-    Staging &g_stage = stage_of(c);
+    StagingPair &pair = stage_of(c);
+    const int slot = pair.next;
+    Staging &g_stage = pair.slot[slot];
     const u32 N1 = v.p.rlwe_n + 1, n1 = v.p.lwe_n + 1;
     // staging: ops | in0 | in1 | out | key | not_in | not_out | KS work list
     const size_t nn = nots.size();
@@ -165,10 +183,13 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
         }
     }
     const size_t nw = work.size();
-    // the previous batch's staging copies read the pinned buffer
-    // asynchronously: wait for them before rewriting it
-    if (hipStreamSynchronize(st) != hipSuccess)
-        return PZ_FAIL(PZ_ERR_HIP, "hipStreamSynchronize failed");
+    // the batch before last used this slot (its copies read the pinned
+    // buffer, its kernels the device copy): wait for it to finish
+    if (!pair.done[slot] && hipEventCreateWithFlags(&pair.done[slot], hipEventDisableTiming) !=
+                                hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipEventCreate failed");
+    if (pair.used[slot] && hipEventSynchronize(pair.done[slot]) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipEventSynchronize failed");
     if (!g_stage.reserve(5 * B + 2 * nn + nw + 8, keys.size()))
         return PZ_FAIL(PZ_ERR_HIP, "g_stage.reserve failed");
     u32 *h = g_stage.h;
@@ -232,7 +253,13 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
             }
         }
     }
-    return PZ_LAUNCH_STATUS();
+    rc = PZ_LAUNCH_STATUS();
+    if (rc) return rc;
+    if (hipEventRecord(pair.done[slot], st) != hipSuccess)
+        return PZ_FAIL(PZ_ERR_HIP, "hipEventRecord failed");
+    pair.used[slot] = true;
+    pair.next = slot ^ 1;
+    return PZ_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -3,13 +3,17 @@
 // Sessions own server keys; a session's key is uploaded to HBM when the
 // session is created if the keys' total footprint fits the budget, otherwise
 // when one of its requests is admitted (idle resident sessions are then
-// evicted, least recently used first).  Requests
-// are gate DAGs (pz_circuit) with input ciphertexts and arrival times.  Each
-// loop step admits arrived requests as HBM and pool rows allow, then takes the
-// next ready level of every active request -- across sessions, up to a
+// evicted, least recently used first).  Requests are gate DAGs (pz_circuit)
+// with input ciphertexts and arrival times; they may be submitted from
+// another thread while the loop runs, and then arrive when submitted.  Each
+// loop step admits arrived requests as HBM and pool rows allow, then takes
+// the next ready level of every active request -- across sessions, up to a
 // bootstrap budget per launch -- and runs them as ONE batch (per-ciphertext
 // keys), so requests that arrive while others are on the GPU join the next
-// step (continuous batching).  A server may hold sessions of several
+// step (continuous batching).  The loop does not wait for a step to finish
+// before queueing the next one (each context keeps up to two steps in
+// flight); a request is complete when the event recorded after its last
+// level has passed, and only then are its rows reused.  A server may hold sessions of several
 // parameter sets (one context each, same device and ring size, one wire
 // pool): a step forms one batch per context and launches them on the
 // contexts' own streams, where they run concurrently.  The reference has no
@@ -19,7 +23,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -95,7 +101,17 @@ struct pz_server {
     std::vector<pz_ctx *> ctxs;  // parameter sets served (ctxs[0]: the creating context)
     pz_server_config cfg;
     std::vector<Session> sessions;
-    std::vector<Request> reqs;
+    // requests: a deque keeps every element in place while submissions from
+    // another thread append during a run; mu guards the appends, the request
+    // states the other entry points read, the intake flag and the clock
+    std::deque<Request> reqs;
+    std::mutex mu;
+    std::condition_variable cv;  // a submission wakes an idle loop
+    bool intake_open = false;
+    bool running = false;
+    bool failed = false;  // a run ended in an error: request states are not kept
+    std::chrono::steady_clock::time_point t0;
+    std::vector<hipEvent_t> free_events;
     u64 *pool = nullptr;
     RowAlloc rows;
     u32 N1 = 0;
@@ -104,6 +120,10 @@ struct pz_server {
     // stats
     uint64_t total_pbs = 0, batches = 0, key_uploads = 0, evictions = 0;
     double wall = 0;
+    double now() const
+    {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
 };
 
 extern "C" int pz_server_create(pz_ctx *ctx, const pz_server_config *cfg, pz_server **out)
@@ -136,6 +156,7 @@ extern "C" int pz_server_destroy(pz_server *s)
     if (!s) return PZ_OK;
     for (auto &se : s->sessions)
         if (se.key) pz_key_free(se.key);
+    for (auto e : s->free_events) (void)hipEventDestroy(e);
     if (s->pool) (void)hipFree(s->pool);
     delete s;
     return PZ_OK;
@@ -201,10 +222,12 @@ extern "C" int pz_server_submit(pz_server *s, uint32_t session, pz_circuit *prog
                                 const uint64_t *h_inputs, const uint32_t *out_wires,
                                 uint32_t n_out, uint32_t n_ops, double arrival, uint32_t *rid)
 {
-    if (!s || !prog || !out_wires || !rid || session >= s->sessions.size())
-        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s || !prog || !out_wires || !rid || session >= s->sessions.size()");
+    if (!s || !prog || !out_wires || !rid) return PZ_FAIL(PZ_ERR_ARG, "null argument");
+    if (session >= s->sessions.size()) return PZ_FAIL(PZ_ERR_ARG, "no such session");
     uint32_t nw = 0, npbs = 0, depth = 0;
     pz_circuit_stats(prog, &nw, &npbs, &depth);
+    if (nw > s->cfg.pool_rows)
+        return PZ_FAIL(PZ_ERR_UNSUPPORTED, "program has more wires than the pool has rows");
     std::vector<uint32_t> nodes(4 * (size_t)nw);
     pz_circuit_nodes(prog, nodes.data(), nw);
     Request r;
@@ -230,21 +253,34 @@ extern "C" int pz_server_submit(pz_server *s, uint32_t session, pz_circuit *prog
     r.levels.assign(maxl + 1, {});
     for (uint32_t i = 0; i < nw; i++)
         if (r.kind[i] != 0) r.levels[r.level[i]].push_back(i);
-    if (!r.inputs.empty() && !h_inputs)
-        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !r.inputs.empty() && !h_inputs");
+    if (!r.inputs.empty() && !h_inputs) return PZ_FAIL(PZ_ERR_ARG, "null input ciphertexts");
     r.h_in.assign(h_inputs, h_inputs + r.inputs.size() * (size_t)s->N1);
     for (uint32_t i = 0; i < n_out; i++) {
-        if (out_wires[i] >= nw) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: out_wires[i] >= nw");
+        if (out_wires[i] >= nw) return PZ_FAIL(PZ_ERR_ARG, "output wire out of range");
         r.outs.push_back(out_wires[i]);
     }
     r.n_ops = n_ops ? n_ops : 1;
     r.n_pbs = npbs;
-    r.arrival = arrival;
     r.rows = nw;
-    if (nw > s->cfg.pool_rows)
-        return PZ_FAIL(PZ_ERR_UNSUPPORTED, "unsupported: nw > s->cfg.pool_rows");
-    s->reqs.push_back(std::move(r));
-    *rid = (uint32_t)(s->reqs.size() - 1);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        // a negative arrival is "now": the loop's clock while it runs, 0 before
+        r.arrival = arrival >= 0 ? arrival : (s->running ? s->now() : 0.0);
+        s->reqs.push_back(std::move(r));
+        *rid = (uint32_t)(s->reqs.size() - 1);
+    }
+    s->cv.notify_one();
+    return PZ_OK;
+}
+
+extern "C" int pz_server_intake(pz_server *s, int open)
+{
+    if (!s) return PZ_FAIL(PZ_ERR_ARG, "null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->intake_open = open != 0;
+    }
+    s->cv.notify_one();
     return PZ_OK;
 }
 
@@ -283,9 +319,21 @@ static bool make_resident(pz_server *s, uint32_t sid)
     return true;
 }
 
+static hipEvent_t take_event(pz_server *s)
+{
+    if (!s->free_events.empty()) {
+        hipEvent_t e = s->free_events.back();
+        s->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+}
+
 extern "C" int pz_server_run(pz_server *s)
 {
-    if (!s) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s");
+    if (!s) return PZ_FAIL(PZ_ERR_ARG, "null server");
     const size_t nc = s->ctxs.size();
     std::vector<hipStream_t> streams(nc);
     for (size_t c = 0; c < nc; c++) {
@@ -293,60 +341,107 @@ extern "C" int pz_server_run(pz_server *s)
         pz_ctx_stream(s->ctxs[c], &sv);
         streams[c] = (hipStream_t)sv;
     }
-    auto t0 = std::chrono::steady_clock::now();
-    auto now = [&]() {
-        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    size_t scanned = 0, total = 0, done = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->failed) return PZ_FAIL(PZ_ERR_STATE, "an earlier run of this server failed");
+        s->t0 = std::chrono::steady_clock::now();
+        s->running = true;
+        for (auto &r : s->reqs)
+            if (r.state == 3) done++;
+    }
+    struct Finishing {
+        size_t req;
+        hipEvent_t ev;  // recorded after the request's last level
     };
-    std::vector<size_t> order(s->reqs.size());
-    for (size_t i = 0; i < order.size(); i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t x, size_t y) { return s->reqs[x].arrival < s->reqs[y].arrival; });
-    size_t next_arrival = 0, done = 0;
-    std::deque<size_t> pending;
-    std::vector<size_t> active;
-    for (auto &r : s->reqs)
-        if (r.state == 3) done++;
-    while (done < s->reqs.size()) {
-        double t = now();
-        while (next_arrival < order.size() && s->reqs[order[next_arrival]].arrival <= t) {
-            Request &r = s->reqs[order[next_arrival]];
-            if (r.state == 0) {
-                r.state = 1;
-                pending.push_back(order[next_arrival]);
-            }
-            next_arrival++;
+    std::vector<size_t> waiting;  // taken in, arrival still ahead (sorted by arrival)
+    std::deque<size_t> pending;   // arrived, not admitted
+    std::vector<size_t> active;   // admitted, levels left to queue
+    std::vector<Finishing> finishing;
+    int rc = PZ_OK;
+    for (;;) {
+        bool open;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (; scanned < s->reqs.size(); scanned++)
+                if (s->reqs[scanned].state == 0) waiting.push_back(scanned);
+            total = s->reqs.size();
+            open = s->intake_open;
         }
+        std::stable_sort(waiting.begin(), waiting.end(), [&](size_t x, size_t y) {
+            return s->reqs[x].arrival < s->reqs[y].arrival;
+        });
+        double t = s->now();
+        size_t k = 0;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            while (k < waiting.size() && s->reqs[waiting[k]].arrival <= t) {
+                s->reqs[waiting[k]].state = 1;
+                pending.push_back(waiting[k++]);
+            }
+        }
+        waiting.erase(waiting.begin(), waiting.begin() + k);
+        // completions: outputs copied once the last level's event has passed,
+        // then the rows go back to the pool
+        for (size_t i = 0; i < finishing.size();) {
+            const hipError_t q = hipEventQuery(finishing[i].ev);
+            if (q == hipErrorNotReady) {
+                i++;
+                continue;
+            }
+            if (q != hipSuccess) {
+                rc = PZ_FAIL(PZ_ERR_HIP, "a served batch failed");
+                break;
+            }
+            Request &r = s->reqs[finishing[i].req];
+            std::vector<uint64_t> out(r.outs.size() * (size_t)s->N1);
+            for (size_t o = 0; o < r.outs.size() && rc == PZ_OK; o++)
+                if (hipMemcpy(out.data() + o * s->N1, s->pool + (r.base + r.outs[o]) * s->N1,
+                              8 * s->N1, hipMemcpyDeviceToHost) != hipSuccess)
+                    rc = PZ_FAIL(PZ_ERR_HIP, "hipMemcpy of a result failed");
+            if (rc) break;
+            s->rows.release(r.base, r.rows);
+            s->sessions[r.session].active--;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                r.result.swap(out);
+                r.finished = s->now();
+                r.state = 3;
+            }
+            done++;
+            s->free_events.push_back(finishing[i].ev);
+            finishing.erase(finishing.begin() + i);
+        }
+        if (rc) break;
         // admission in arrival order (a request whose session cannot be made
         // resident does not block later requests of resident sessions)
         for (auto it = pending.begin(); it != pending.end();) {
             Request &r = s->reqs[*it];
-            if (active.size() >= s->cfg.max_active) break;
+            if (active.size() + finishing.size() >= s->cfg.max_active) break;
             size_t base;
             if (!make_resident(s, r.session) || !s->rows.alloc(r.rows, &base)) {
                 ++it;
                 continue;
             }
             r.base = base;
-            r.state = 2;
-            r.admitted = now();
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                r.state = 2;
+                r.admitted = s->now();
+            }
             s->sessions[r.session].active++;
-            // inputs into their pool rows, on the stream of the session's context
-            for (size_t i = 0; i < r.inputs.size(); i++)
-                (void)hipMemcpyAsync(s->pool + (r.base + r.inputs[i]) * s->N1,
-                                     r.h_in.data() + i * s->N1, 8 * s->N1,
-                                     hipMemcpyHostToDevice, streams[s->sessions[r.session].ctx]);
+            // inputs into their rows: the rows have no work in flight, and
+            // the synchronous copy completes before the request's first level
+            // is queued
+            for (size_t i = 0; i < r.inputs.size() && rc == PZ_OK; i++)
+                if (hipMemcpy(s->pool + (r.base + r.inputs[i]) * s->N1,
+                              r.h_in.data() + i * s->N1, 8 * s->N1,
+                              hipMemcpyHostToDevice) != hipSuccess)
+                    rc = PZ_FAIL(PZ_ERR_HIP, "hipMemcpy of an input failed");
             active.push_back(*it);
             it = pending.erase(it);
         }
-        if (active.empty()) {
-            if (next_arrival < order.size()) {
-                double wait = s->reqs[order[next_arrival]].arrival - now();
-                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-                continue;
-            }
-            if (!pending.empty()) return PZ_FAIL(PZ_ERR_STATE, "invalid state: !pending.empty()");  // cannot admit (budget too small)
-            break;
-        }
+        if (rc) break;
         // one batch per context: the next level of every active request of
         // its sessions, oldest first
         std::vector<std::vector<const pz_key *>> keys(nc);
@@ -380,53 +475,77 @@ extern "C" int pz_server_run(pz_server *s)
             pbs[c] += lp;
             stepped.push_back(idx);
         }
-        // launch every context's batch before waiting for any of them
-        for (size_t c = 0; c < nc; c++) {
+        // queue every context's batch
+        for (size_t c = 0; c < nc && rc == PZ_OK; c++) {
             if (recs[c].empty()) continue;
-            int rc = exec_batch(s->ctxs[c], keys[c], recs[c], s->pool, streams[c]);
-            if (rc) return rc;
-        }
-        for (size_t c = 0; c < nc; c++) {
-            if (recs[c].empty()) continue;
-            if (hipStreamSynchronize(streams[c]) != hipSuccess)
-                return PZ_FAIL(PZ_ERR_HIP, "hipStreamSynchronize failed");
+            rc = exec_batch(s->ctxs[c], keys[c], recs[c], s->pool, streams[c]);
+            if (rc) break;
             s->total_pbs += pbs[c];
             s->batches++;
         }
+        if (rc) break;
         s->tick++;
         for (size_t idx : stepped) {
             Request &r = s->reqs[idx];
             s->sessions[r.session].last_used = s->tick;
             if (r.next_level < r.levels.size()) r.next_level++;
             while (r.next_level < r.levels.size() && r.levels[r.next_level].empty()) r.next_level++;
-            if (r.next_level >= r.levels.size()) {
-                // complete: fetch outputs, release rows
-                r.result.resize(r.outs.size() * (size_t)s->N1);
-                hipStream_t rs = streams[s->sessions[r.session].ctx];
-                for (size_t i = 0; i < r.outs.size(); i++)
-                    (void)hipMemcpyAsync(r.result.data() + i * s->N1,
-                                         s->pool + (r.base + r.outs[i]) * s->N1, 8 * s->N1,
-                                         hipMemcpyDeviceToHost, rs);
-                (void)hipStreamSynchronize(rs);
-                r.finished = now();
-                r.state = 3;
-                s->rows.release(r.base, r.rows);
-                s->sessions[r.session].active--;
-                done++;
-                active.erase(std::find(active.begin(), active.end(), idx));
+            if (r.next_level < r.levels.size()) continue;
+            // every level queued: complete once its context's stream passes
+            // an event recorded here
+            const u32 c = s->sessions[r.session].ctx;
+            hipEvent_t e = take_event(s);
+            if (!e || hipEventRecord(e, streams[c]) != hipSuccess) {
+                rc = PZ_FAIL(PZ_ERR_HIP, "hipEventRecord failed");
+                break;
             }
+            active.erase(std::find(active.begin(), active.end(), idx));
+            finishing.push_back({idx, e});
         }
+        if (rc) break;
+        if (done == total && !open) break;
+        if (!stepped.empty()) continue;
+        // nothing queued this round: wait for a completion, an arrival or a
+        // submission
+        if (!finishing.empty()) {
+            if (hipEventSynchronize(finishing.front().ev) != hipSuccess) {
+                rc = PZ_FAIL(PZ_ERR_HIP, "a served batch failed");
+                break;
+            }
+            continue;
+        }
+        if (!pending.empty() && waiting.empty() && !open) {
+            rc = PZ_FAIL(PZ_ERR_STATE, "a request cannot be admitted (key budget or pool rows)");
+            break;
+        }
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->reqs.size() > scanned || s->intake_open != open) continue;
+        double wait = 0.01;  // an open intake, or requests that cannot be admitted yet
+        if (!waiting.empty())
+            wait = std::min(wait, std::max(0.0, s->reqs[waiting.front()].arrival - s->now()));
+        if (wait > 0)
+            s->cv.wait_for(lk, std::chrono::duration<double>(wait),
+                           [&]() { return s->reqs.size() > scanned || s->intake_open != open; });
     }
-    s->wall = now();
-    return PZ_OK;
+    // on an error, let queued work finish before the events go back
+    for (auto &f : finishing) {
+        (void)hipEventSynchronize(f.ev);
+        s->free_events.push_back(f.ev);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->running = false;
+    s->failed = rc != PZ_OK;
+    s->wall = s->now();
+    return rc;
 }
 
 extern "C" int pz_server_result(pz_server *s, uint32_t rid, uint64_t *h_out)
 {
-    if (!s || rid >= s->reqs.size() || !h_out)
-        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s || rid >= s->reqs.size() || !h_out");
+    if (!s || !h_out) return PZ_FAIL(PZ_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rid >= s->reqs.size()) return PZ_FAIL(PZ_ERR_ARG, "no such request");
     Request &r = s->reqs[rid];
-    if (r.state != 3) return PZ_FAIL(PZ_ERR_STATE, "invalid state: r.state != 3");
+    if (r.state != 3) return PZ_FAIL(PZ_ERR_STATE, "request not complete");
     memcpy(h_out, r.result.data(), 8 * r.result.size());
     return PZ_OK;
 }
@@ -435,8 +554,9 @@ extern "C" int pz_server_request_times(pz_server *s, uint32_t rid, double *arriv
                                        double *admitted, double *finished, uint32_t *n_pbs,
                                        uint32_t *n_ops)
 {
-    if (!s || rid >= s->reqs.size())
-        return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s || rid >= s->reqs.size()");
+    if (!s) return PZ_FAIL(PZ_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rid >= s->reqs.size()) return PZ_FAIL(PZ_ERR_ARG, "no such request");
     Request &r = s->reqs[rid];
     if (arrival) *arrival = r.arrival;
     if (admitted) *admitted = r.admitted;
@@ -449,7 +569,7 @@ extern "C" int pz_server_request_times(pz_server *s, uint32_t rid, double *arriv
 extern "C" int pz_server_stats(pz_server *s, uint64_t *total_pbs, uint64_t *batches,
                                uint64_t *key_uploads, uint64_t *evictions, double *wall_s)
 {
-    if (!s) return PZ_FAIL(PZ_ERR_ARG, "invalid argument: !s");
+    if (!s) return PZ_FAIL(PZ_ERR_ARG, "null server");
     if (total_pbs) *total_pbs = s->total_pbs;
     if (batches) *batches = s->batches;
     if (key_uploads) *key_uploads = s->key_uploads;

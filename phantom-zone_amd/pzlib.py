@@ -98,6 +98,7 @@ SIGNATURES = {
     "pz_server_submit": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32, C.c_uint32,
                                    C.c_double, _vp]),
     "pz_server_run": (C.c_int, [_vp]),
+    "pz_server_intake": (C.c_int, [_vp, C.c_int]),
     "pz_server_result": (C.c_int, [_vp, C.c_uint32, _vp]),
     "pz_server_request_times": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
     "pz_server_stats": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -377,6 +378,9 @@ class Server:
 
     def run(self):
         _check(self.L.pz_server_run(self.h))
+
+    def intake(self, open_):
+        _check(self.L.pz_server_intake(self.h, 1 if open_ else 0))
 
     def result(self, rid):
         out = np.zeros((self.outs[rid], self.ctx.N + 1), dtype=np.uint64)

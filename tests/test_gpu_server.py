@@ -269,3 +269,73 @@ def test_serving_64_sessions_poisson():
     circ.close()
     for c in ctxs.values():
         c.close()
+
+
+def test_online_submissions_from_another_thread():
+    """Requests submitted by a second thread while the loop serves others
+    (intake open, arrival stamped at submission): later requests arrive while
+    earlier ones are on the GPU, every result decrypts correctly and equals
+    the same request served alone."""
+    import threading
+    import time
+
+    import pzlib
+    import pzprog
+    name = "NI_2P"
+    print("server online submissions %s" % name, flush=True)
+    ctx = pzlib.Context(name)
+    sessions = make_sessions(ctx, name, 3)
+    r = np.random.default_rng(12)
+    count = 12
+    progs = [pzprog.Program(r, int(r.integers(1, 3)), ops=("add", "lt", "mul")) for _ in range(count)]
+    sess_ids = [j % 3 for j in range(count)]
+    gaps = [float(g) for g in r.exponential(1.0 / 40.0, size=count)]
+    built = []
+    for j, prog in enumerate(progs):
+        circ = pzlib.Circuit(None)
+        wires, nu, nb = prog.build(circ)
+        cts = ctx.encrypt_bools(sessions[sess_ids[j]][0], prog.input_bits(), 900 + 7 * j)
+        built.append((circ, wires, nu, nb, cts))
+    srv = pzlib.Server(ctx, hbm_budget_bytes=1 << 34, pool_rows=1 << 15)
+    srv.sid_map = [srv.add_session(se[2], se[3], se[4]) for se in sessions]
+    rids = [None] * count
+    errors = []
+
+    def producer():
+        try:
+            for j in range(count):
+                time.sleep(gaps[j])
+                circ, wires, _, _, cts = built[j]
+                rids[j] = srv.submit(srv.sid_map[sess_ids[j]], circ, cts, wires,
+                                     len(progs[j].steps), -1.0)
+        except Exception as e:  # noqa: BLE001 -- reported by the main thread
+            errors.append(e)
+        finally:
+            srv.intake(False)
+
+    srv.intake(True)
+    th = threading.Thread(target=producer)
+    th.start()
+    srv.run()  # returns once the intake is closed and every request is done
+    th.join()
+    assert not errors, errors
+    times = [srv.times(i) for i in rids]
+    # online: a request arrived while an earlier one was still being served
+    assert any(times[j]["arrival"] < times[i]["finished"] for i in range(count)
+               for j in range(i + 1, count))
+    mixed = [srv.result(i) for i in rids]
+    for j, prog in enumerate(progs):
+        bits = ctx.decrypt_bools(sessions[sess_ids[j]][0], mixed[j])
+        assert pzprog.decode(bits, built[j][2], built[j][3]) == prog.plain(), j
+    for j in (0, count // 2, count - 1):
+        solo = pzlib.Server(ctx, hbm_budget_bytes=1 << 34, pool_rows=1 << 14)
+        sid = [solo.add_session(se[2], se[3], se[4]) for se in sessions]
+        circ, wires, _, _, cts = built[j]
+        rid = solo.submit(sid[sess_ids[j]], circ, cts, wires, len(progs[j].steps), 0.0)
+        solo.run()
+        assert np.array_equal(solo.result(rid), mixed[j]), j
+        solo.close()
+    for b in built:
+        b[0].close()
+    srv.close()
+    ctx.close()
