@@ -85,6 +85,10 @@ struct pz_ctx {
     struct Tuned {
         int var = -1, per_cu = 0;
     } tuned[40];
+    // key-index order of a batched RLWE x RGSW / automorphism launch (B
+    // words), grown on demand
+    u32 *d_order = nullptr;
+    size_t order_cap = 0;
     // workspace of the key-preparation operators (grow-only; every user
     // synchronises the stream before it returns)
     void *ws = nullptr;
@@ -362,6 +366,7 @@ extern "C" int pz_ctx_destroy(pz_ctx *c)
     hipFree(c->d_ksb);
     hipFree(c->d_ks);
     hipFree(c->d_stats);
+    hipFree(c->d_order);
     hipFree(c->ws);
     for (int i = 0; i < 4; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
@@ -673,26 +678,50 @@ extern "C" int pz_key_bytes(pz_key *k, size_t *bytes)
     return PZ_OK;
 }
 
-extern "C" int pz_rlwe_by_rgsw(pz_ctx *c, pz_key *k, uint64_t *rlwe, const uint32_t *idx,
-                               const uint8_t *triv, size_t B, void *s)
+// Batched RLWE x RGSW (is_auto = false) or automorphism.  Where the split step
+// runs (d = 1), the batch is ordered by key index on the device and the
+// launch is persistent (one resident grid, each work group a contiguous run
+// of the order), so consecutive RLWEs of a work group share key slices.
+static int batched_rlwe_op(pz_ctx *c, pz_key *k, bool is_auto, uint64_t *rlwe,
+                           const uint32_t *idx, const uint8_t *triv, size_t B, void *s)
 {
     if (!c || !k || !rlwe || !idx || !triv) return fail(PZ_ERR_ARG, "null argument");
     KEY_OK(c, k);
     if (B == 0) return PZ_OK;
-    launch_ext(c->logn, false, rlwe, idx, triv, B, c->dc, k->dk, S(c, s));
+    hipStream_t st = S(c, s);
+    if (!br_split(c->dc)) {
+        launch_ext(c->logn, is_auto, rlwe, idx, triv, B, c->dc, k->dk, st);
+        HIPCHK(hipGetLastError());
+        return PZ_OK;
+    }
+    const u32 nkeys = is_auto ? c->p.w + 1 : c->p.lwe_n;
+    const size_t need = B;
+    if (need > c->order_cap) {
+        // a launch on any stream may still read the old order
+        HIPCHK(hipDeviceSynchronize());
+        hipFree(c->d_order);
+        c->d_order = nullptr;
+        c->order_cap = 0;
+        HIPCHK(hipMalloc(&c->d_order, 4 * need));
+        c->order_cap = need;
+    }
+    u32 *order = launch_key_order(idx, B, nkeys, c->d_order, st) ? c->d_order : nullptr;
+    const int grid = c->num_cu * step_split_per_cu(c->logn);
+    launch_ext(c->logn, is_auto, rlwe, idx, triv, B, c->dc, k->dk, st, order, grid);
     HIPCHK(hipGetLastError());
     return PZ_OK;
+}
+
+extern "C" int pz_rlwe_by_rgsw(pz_ctx *c, pz_key *k, uint64_t *rlwe, const uint32_t *idx,
+                               const uint8_t *triv, size_t B, void *s)
+{
+    return batched_rlwe_op(c, k, false, rlwe, idx, triv, B, s);
 }
 
 extern "C" int pz_rlwe_auto(pz_ctx *c, pz_key *k, uint64_t *rlwe, const uint32_t *idx,
                             const uint8_t *triv, size_t B, void *s)
 {
-    if (!c || !k || !rlwe || !idx || !triv) return fail(PZ_ERR_ARG, "null argument");
-    KEY_OK(c, k);
-    if (B == 0) return PZ_OK;
-    launch_ext(c->logn, true, rlwe, idx, triv, B, c->dc, k->dk, S(c, s));
-    HIPCHK(hipGetLastError());
-    return PZ_OK;
+    return batched_rlwe_op(c, k, true, rlwe, idx, triv, B, s);
 }
 
 // grow the PBS pipeline scratch to hold B ciphertexts

@@ -30,7 +30,13 @@ void launch_fma(u64 *out, const u64 *a, const u64 *as, const u64 *b, u32 d, size
 void launch_decompose(const u64 *in, u64 *digits, size_t count, const Decomp &dc,
                       hipStream_t s);
 void launch_ext(int logn, bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv, size_t B,
-                const DevCtx &c, const DevKey &k, hipStream_t s);
+                const DevCtx &c, const DevKey &k, hipStream_t s, const u32 *order = nullptr,
+                int grid = 0);
+// resident work groups per CU of the split step (d = 1 batched operators)
+int step_split_per_cu(int logn);
+// order[] = the batch's positions sorted by key index (idx >= nkeys last),
+// one work group; false (nothing launched) for more than 8191 keys
+bool launch_key_order(const u32 *idx, size_t B, u32 nkeys, u32 *order, hipStream_t s);
 void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const u32 *idx1,
                  const u32 *ops, u32 N1, size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out,
                  const DevCtx &c, hipStream_t s);

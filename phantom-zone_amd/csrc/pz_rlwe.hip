@@ -5,6 +5,8 @@
 //   k_blind_rotate    persistent: dlog buckets, test-vector rotation, LMKCY+
 //                     blind rotation, sample extract (one PBS per work group
 //                     at a time, RLWE resident in VGPRs)
+#include <algorithm>
+
 #include "pz_internal.hpp"
 
 #ifndef PZ_BR_WAVES
@@ -94,15 +96,69 @@ __global__ __launch_bounds__(1 << (LOGN - 3)) void k_auto(u64 *rlwe, const u32 *
 }
 
 // ---------------------------------------------------------------------------
+// Order of a batch by key index: a counting sort in one work group (counts,
+// exclusive prefix sum and placement in LDS; indices past the last key share
+// the final counter).  The order within one index is arbitrary, and each
+// RLWE's result does not depend on it.  nkeys + 1 <= KEY_ORDER_MAX.
+// ---------------------------------------------------------------------------
+constexpr u32 KEY_ORDER_MAX = 8192;
+__global__ __launch_bounds__(1024) void k_key_order(const u32 *idx, size_t B, u32 nkeys,
+                                                    u32 *order)
+{
+    __shared__ u32 cnt[KEY_ORDER_MAX];
+    __shared__ u32 part[1024];
+    const u32 t = threadIdx.x, n = nkeys + 1;
+    for (u32 i = t; i < n; i += 1024) cnt[i] = 0;
+    __syncthreads();
+    for (size_t b = t; b < B; b += 1024) atomicAdd(&cnt[idx[b] < nkeys ? idx[b] : nkeys], 1u);
+    __syncthreads();
+    // exclusive prefix sum: thread t takes ceil(n / 1024) consecutive counters
+    const u32 per = (n + 1023) / 1024, lo = t * per;
+    u32 sum = 0;
+    for (u32 i = 0; i < per && lo + i < n; i++) sum += cnt[lo + i];
+    part[t] = sum;
+    __syncthreads();
+    for (u32 off = 1; off < 1024; off <<= 1) {
+        const u32 v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    u32 run = part[t] - sum;
+    for (u32 i = 0; i < per && lo + i < n; i++) {
+        const u32 c = cnt[lo + i];
+        cnt[lo + i] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (size_t b = t; b < B; b += 1024)
+        order[atomicAdd(&cnt[idx[b] < nkeys ? idx[b] : nkeys], 1u)] = (u32)b;
+}
+
+bool launch_key_order(const u32 *idx, size_t B, u32 nkeys, u32 *order, hipStream_t s)
+{
+    if (nkeys + 1 > KEY_ORDER_MAX) return false;
+    hipLaunchKernelGGL(k_key_order, 1, 1024, 0, s, idx, B, nkeys, order);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // Batched RLWE x RGSW / automorphism on the split work group (d = 1 sets):
 // the blind rotation's step, one RLWE per 2 x N/8 threads; half h loads and
 // stores polynomial h.  LDS: the two halves' regions and the stash.
 // ---------------------------------------------------------------------------
+// Persistent.  With `order` (the batch sorted by key index) the positions are
+// dealt XCD by XCD: work groups are dispatched to the 8 XCDs in turn (work
+// group g on XCD g % 8), so round t gives the G/8 work groups of XCD x the
+// consecutive positions [(8t + x) G/8, (8t + x + 1) G/8) -- a few key
+// indices per XCD at a time, whose key slices the XCD's L2 then serves to
+// every RLWE of that index.  Without an order, contiguous runs.
 template <int LOGN>
 __global__ __launch_bounds__(2 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu(
     PZ_BR_SPLIT_WAVES, PZ_BR_SPLIT_WAVES))) void k_step_split(bool is_auto, u64 *rlwe,
                                                                 const u32 *idx, const u8 *trivial,
-                                                                size_t B, DevCtx c, DevKey key)
+                                                                const u32 *order, size_t B,
+                                                                DevCtx c, DevKey key)
 {
     constexpr int N = 1 << LOGN;
     constexpr int TPB = N / 8;
@@ -111,7 +167,18 @@ __global__ __launch_bounds__(2 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu
     const u32 tid = threadIdx.x, j = tid % TPB;
     const u32 h = (u32)__builtin_amdgcn_readfirstlane(tid / TPB);
     const size_t rs = (size_t)(is_auto ? key.auto_rows : key.rgsw_rows) * N;
-    for (size_t bb = blockIdx.x; bb < B; bb += gridDim.x) {
+    constexpr u32 XCDS = 8;
+    const bool dealt = order && gridDim.x % XCDS == 0;
+    const size_t per_x = gridDim.x / XCDS, x = blockIdx.x % XCDS, m = blockIdx.x / XCDS;
+    const size_t p0 = dealt ? 0 : blockIdx.x * B / gridDim.x;
+    const size_t p1 = dealt ? B : (blockIdx.x + 1) * B / gridDim.x;
+    for (size_t it = p0; it < p1; it++) {
+        size_t pos = it;
+        if (dealt) {
+            pos = ((it * XCDS) + x) * per_x + m;  // round it of this work group
+            if (pos >= B) break;
+        }
+        const size_t bb = order ? (size_t)__builtin_amdgcn_readfirstlane((int)order[pos]) : pos;
         const u32 k = (u32)__builtin_amdgcn_readfirstlane((int)idx[bb]);
         if (k >= (is_auto ? c.w + 1 : c.n_lwe)) continue;  // no such key: unchanged
         u64 *poly = rlwe + (bb * 2 + h) * N;
@@ -387,19 +454,35 @@ static void launch_ext_t(bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv
         hipLaunchKernelGGL((k_ext<LOGN>), grid, TPB, lds, s, rlwe, idx, triv, B, c, k);
 }
 
-void launch_ext(int logn, bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv, size_t B,
-                const DevCtx &c, const DevKey &k, hipStream_t s)
+// Resident work groups of the split step per CU (occupancy query, cached).
+int step_split_per_cu(int logn)
 {
-    if (br_split(c)) {  // d = 1: the blind rotation's split step
+    static int per[2] = {0, 0};
+    int &v = per[logn == 11 ? 1 : 0];
+    if (v == 0) {
+        const size_t lds = 3 * 8 * (size_t)((1 << logn) + (1 << logn) / 8);
+        int n = 0;
+        const hipError_t e =
+            logn == 11 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step_split<11>, 2 << 8, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step_split<9>, 2 << 6, lds);
+        v = (e == hipSuccess && n > 0) ? n : 1;
+    }
+    return v;
+}
+
+void launch_ext(int logn, bool is_auto, u64 *rlwe, const u32 *idx, const u8 *triv, size_t B,
+                const DevCtx &c, const DevKey &k, hipStream_t s, const u32 *order, int grid)
+{
+    if (br_split(c)) {  // d = 1: the blind rotation's split step, persistent
         if (B == 0) return;
-        const int grid = B < 65535 ? (int)B : 65535;
+        if (grid <= 0 || (size_t)grid > B) grid = B < 65535 ? (int)B : 65535;
         const size_t lds = 3 * 8 * (size_t)((1 << logn) + (1 << logn) / 8);
         if (logn == 11)
             hipLaunchKernelGGL(k_step_split<11>, grid, 2 << 8, lds, s, is_auto, rlwe, idx, triv,
-                               B, c, k);
+                               order, B, c, k);
         else if (logn == 9)
             hipLaunchKernelGGL(k_step_split<9>, grid, 2 << 6, lds, s, is_auto, rlwe, idx, triv,
-                               B, c, k);
+                               order, B, c, k);
         return;
     }
     switch (logn) {
