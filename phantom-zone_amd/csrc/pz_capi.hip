@@ -649,17 +649,18 @@ extern "C" int pz_key_download(pz_key *k, uint64_t *h_rgsw, uint64_t *h_auto, ui
             h[i] = (uint64_t)(((unsigned __int128)h[i] * N) % q);
     };
     if (h_rgsw) {
-        HIPCHK(hipMemcpy(h_rgsw, k->rgsw, 8 * rgsw_words, hipMemcpyDeviceToHost));
+        HIPCHK(pz_copy_sync(h_rgsw, k->rgsw, 8 * rgsw_words, hipMemcpyDeviceToHost, c->stream));
         unscale(h_rgsw, rgsw_words);
     }
     if (h_auto) {
-        HIPCHK(hipMemcpy(h_auto, k->auto_k, 8 * auto_words, hipMemcpyDeviceToHost));
+        HIPCHK(pz_copy_sync(h_auto, k->auto_k, 8 * auto_words, hipMemcpyDeviceToHost, c->stream));
         unscale(h_auto, auto_words);
     }
     if (h_ksk) {
         size_t rows = (size_t)N * p.lwe_d, n1 = p.lwe_n + 1, ld = k->dk.ldk;
         std::vector<uint8_t> packed(rows * ld * k->dk.ksk_bytes);
-        HIPCHK(hipMemcpy(packed.data(), k->ksk, packed.size(), hipMemcpyDeviceToHost));
+        HIPCHK(pz_copy_sync(packed.data(), k->ksk, packed.size(), hipMemcpyDeviceToHost,
+                            c->stream));
         for (size_t r = 0; r < rows; r++)
             for (size_t col = 0; col < n1; col++) {
                 size_t i = r * ld + col;

@@ -125,7 +125,7 @@ extern "C" int pz_rlwe_ksk_upload(pz_ctx *c, const uint64_t *h_rows, pz_rlwe_ksk
     u64 *rows = nullptr;
     if (hipMalloc(&rows, 8 * words) != hipSuccess)
         return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
-    if (hipMemcpy(rows, h_rows, 8 * words, hipMemcpyHostToDevice) != hipSuccess) {
+    if (pz_copy_sync(rows, h_rows, 8 * words, hipMemcpyHostToDevice, st) != hipSuccess) {
         hipFree(rows);
         return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy of the Ksk rows failed");
     }
@@ -143,11 +143,9 @@ extern "C" int pz_rlwe_ksk_download(pz_rlwe_ksk *k, uint64_t *h_rows)
     hipStream_t st;
     int rc = pz_internal_ctx_view(k->owner, &p, &dc, &logn, &st);
     if (rc) return rc;
-    if (hipStreamSynchronize(st) != hipSuccess)
-        return PZ_FAIL(PZ_ERR_HIP, "hipStreamSynchronize failed");
-    if (hipMemcpy(h_rows, k->rows, 16 * (size_t)k->d * p.rlwe_n, hipMemcpyDeviceToHost) !=
+    if (pz_copy_sync(h_rows, k->rows, 16 * (size_t)k->d * p.rlwe_n, hipMemcpyDeviceToHost, st) !=
         hipSuccess)
-        return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
+        return PZ_FAIL(PZ_ERR_HIP, "copy failed");
     return PZ_OK;
 }
 

@@ -86,6 +86,18 @@ int pz_internal_fail(int code, const char *fn, const char *what);
 int pz_internal_launch_status(const char *fn);
 #define PZ_LAUNCH_STATUS() pz_internal_launch_status(__func__)
 
+// Host-blocking copy ordered on stream st.  The contexts' streams are
+// non-blocking: a plain hipMemcpy (null stream) neither waits for the work
+// queued on them nor is waited for by it, so every copy next to such work --
+// reading what a kernel wrote, or refilling a buffer a kernel may still read
+// -- is queued on the same stream and waited for.
+inline hipError_t pz_copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind,
+                               hipStream_t st)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    return e != hipSuccess ? e : hipStreamSynchronize(st);
+}
+
 // Uniform rows of S seeded ChaCha8 streams (UniformInt<u64> mod q, rand
 // 0.8.5) into d_out[S][draws - keep_from], expanded on the device; d_kf is
 // device workspace of 12*S words.  A stream with a rejected draw is

@@ -415,7 +415,7 @@ extern "C" int pz_decrypt_bools(pz_ctx *c, const int32_t *h_s, uint32_t dim, con
         if (v > 1) bad = 1;
         h_bits[t] = (uint8_t)(v == 1);
     }
-    return bad ? PZ_ERR_STATE : PZ_OK;
+    return bad ? PZ_FAIL(PZ_ERR_STATE, "a ciphertext decodes to neither 0 nor 1") : PZ_OK;
 }
 
 // ---- input path of the non-interactive protocol (the library's own seeded
@@ -698,10 +698,10 @@ extern "C" int pz_ni_share_uitos(pz_ctx *c, const uint8_t *crs, const uint8_t *n
         x.mono_val = 0;
     }
     if (!ni_combine(st, dc, rows, N)) KG_FAIL("row combination failed");
-    KG_CHK(hipMemcpy(h_ksk_b, d_out, 8 * (size_t)d * N, hipMemcpyDeviceToHost));
+    KG_CHK(pz_copy_sync(h_ksk_b, d_out, 8 * (size_t)d * N, hipMemcpyDeviceToHost, st));
     if (total > 1)
-        KG_CHK(hipMemcpy(h_zero_others, d_out + (size_t)d * N, 8 * (R - d) * N,
-                         hipMemcpyDeviceToHost));
+        KG_CHK(pz_copy_sync(h_zero_others, d_out + (size_t)d * N, 8 * (R - d) * N,
+                            hipMemcpyDeviceToHost, st));
 done:
     hipFree(d_a);
     hipFree(d_e);
@@ -794,8 +794,8 @@ extern "C" int pz_ni_share_rgsw(pz_ctx *c, const uint8_t *crs, const uint8_t *no
         }
     }
     if (!ni_combine(st, dc, rows, N)) KG_FAIL("row combination failed");
-    KG_CHK(hipMemcpy(h_zero, d_zero, 8 * Z, hipMemcpyDeviceToHost));
-    KG_CHK(hipMemcpy(h_cts, d_cts, 8 * A, hipMemcpyDeviceToHost));
+    KG_CHK(pz_copy_sync(h_zero, d_zero, 8 * Z, hipMemcpyDeviceToHost, st));
+    KG_CHK(pz_copy_sync(h_cts, d_cts, 8 * A, hipMemcpyDeviceToHost, st));
 done:
     hipFree(d_a);
     hipFree(d_ez);
@@ -899,8 +899,8 @@ extern "C" int pz_ni_share_auto_ksk(pz_ctx *c, const uint8_t *crs, const uint8_t
     hipLaunchKernelGGL(k_ksk_share_b, (int)std::min<size_t>(krows, 65535), 64, 0, st, d_ka, d_si,
                        d_zi, d_err, N, p.lwe_d, n, p.lwe_logb, ib_lwe, (u64)(p.lwe_q - 1), d_kb);
     KG_CHK(hipGetLastError());
-    KG_CHK(hipMemcpy(h_auto_b, d_out, 8 * R * N, hipMemcpyDeviceToHost));
-    KG_CHK(hipMemcpy(h_ksk_b, d_kb, 8 * krows, hipMemcpyDeviceToHost));
+    KG_CHK(pz_copy_sync(h_auto_b, d_out, 8 * R * N, hipMemcpyDeviceToHost, st));
+    KG_CHK(pz_copy_sync(h_ksk_b, d_kb, 8 * krows, hipMemcpyDeviceToHost, st));
 done:
     hipFree(d_a);
     hipFree(d_e);

@@ -246,7 +246,8 @@ bool pz_internal_expand_uniform(hipStream_t st, const uint8_t *seeds, u32 S, u64
         host::ChaCha8 r(seeds + 32 * (size_t)s);
         for (u64 k = 0; k < keep_from; k++) r.uniform(q);
         for (u64 k = 0; k < kept; k++) h[k] = r.uniform(q);
-        if (hipMemcpy(d_out + s * kept, h.data(), 8 * kept, hipMemcpyHostToDevice) != hipSuccess)
+        if (pz_copy_sync(d_out + s * kept, h.data(), 8 * kept, hipMemcpyHostToDevice, st) !=
+            hipSuccess)
             return false;
     }
     return true;

@@ -295,11 +295,11 @@ extern "C" int pz_ni_aggregate_shares(pz_ctx *c, const uint8_t *const *h_shares,
                     bj[i] = v >= p.rlwe_q ? v - p.rlwe_q : v;
                 }
             }
-            if (hipMemcpy(d_b.p, own, 8 * uw, hipMemcpyHostToDevice) != hipSuccess ||
+            if (pz_copy_sync(d_b.p, own, 8 * uw, hipMemcpyHostToDevice, st) != hipSuccess ||
                 (total > 1 &&
-                 hipMemcpy(d_z.p, zf.data(), 8 * zf.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+                 pz_copy_sync(d_z.p, zf.data(), 8 * zf.size(), hipMemcpyHostToDevice, st) != hipSuccess)) {
                 free_ksks();
-                return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
+                return PZ_FAIL(PZ_ERR_HIP, "copy failed");
             }
             if ((rc = pz_ni_uitos_ksk(c, crs, total, j, d_b.u(), total > 1 ? d_z.u() : nullptr,
                                       &ksks[j]))) {
@@ -336,22 +336,22 @@ extern "C" int pz_ni_aggregate_shares(pz_ctx *c, const uint8_t *const *h_shares,
                     rest += (size_t)dm * N;
                 }
             }
-            ok = hipMemcpy(d_zero.u() + j * zw, section(by_user[j], sh[j], 2), 8 * zw,
-                           hipMemcpyHostToDevice) == hipSuccess &&
-                 hipMemcpy(d_cts.u() + j * cw, cts.data(), 8 * cw, hipMemcpyHostToDevice) ==
+            ok = pz_copy_sync(d_zero.u() + j * zw, section(by_user[j], sh[j], 2), 8 * zw,
+                           hipMemcpyHostToDevice, st) == hipSuccess &&
+                 pz_copy_sync(d_cts.u() + j * cw, cts.data(), 8 * cw, hipMemcpyHostToDevice, st) ==
                      hipSuccess;
         }
         if (!ok) {
             free_ksks();
-            return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
+            return PZ_FAIL(PZ_ERR_HIP, "copy failed");
         }
         rc = pz_ni_aggregate_rgsw(c, crs, total, ksks.data(), 0, n, d_zero.u(), d_cts.u(),
                                   d_out.u(), nullptr);
         free_ksks();
         if (rc) return rc;
-        if (hipMemcpy(section(h_key, kh, 1), d_out.p, 8 * kw[1], hipMemcpyDeviceToHost) !=
+        if (pz_copy_sync(section(h_key, kh, 1), d_out.p, 8 * kw[1], hipMemcpyDeviceToHost, st) !=
             hipSuccess)
-            return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
+            return PZ_FAIL(PZ_ERR_HIP, "copy failed");
     }
     // automorphism keys and LWE KSK: sums of the b parts
     {
@@ -360,19 +360,19 @@ extern "C" int pz_ni_aggregate_shares(pz_ctx *c, const uint8_t *const *h_shares,
             !d_ab.alloc(8 * kw[2]) || !d_kb.alloc(8 * kw[3]))
             return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
         for (uint32_t j = 0; j < total; j++)
-            if (hipMemcpy(d_as.u() + j * kw[2], section(by_user[j], sh[j], 5), 8 * kw[2],
-                          hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(d_ks.u() + j * kw[3], section(by_user[j], sh[j], 6), 8 * kw[3],
-                          hipMemcpyHostToDevice) != hipSuccess)
-                return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
+            if (pz_copy_sync(d_as.u() + j * kw[2], section(by_user[j], sh[j], 5), 8 * kw[2],
+                          hipMemcpyHostToDevice, st) != hipSuccess ||
+                pz_copy_sync(d_ks.u() + j * kw[3], section(by_user[j], sh[j], 6), 8 * kw[3],
+                          hipMemcpyHostToDevice, st) != hipSuccess)
+                return PZ_FAIL(PZ_ERR_HIP, "copy failed");
         if ((rc = pz_ni_aggregate_auto_ksk(c, total, d_as.u(), d_ks.u(), d_ab.u(), d_kb.u(),
                                            nullptr)))
             return rc;
-        if (hipMemcpy(section(h_key, kh, 2), d_ab.p, 8 * kw[2], hipMemcpyDeviceToHost) !=
+        if (pz_copy_sync(section(h_key, kh, 2), d_ab.p, 8 * kw[2], hipMemcpyDeviceToHost, st) !=
                 hipSuccess ||
-            hipMemcpy(section(h_key, kh, 3), d_kb.p, 8 * kw[3], hipMemcpyDeviceToHost) !=
+            pz_copy_sync(section(h_key, kh, 3), d_kb.p, 8 * kw[3], hipMemcpyDeviceToHost, st) !=
                 hipSuccess)
-            return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
+            return PZ_FAIL(PZ_ERR_HIP, "copy failed");
     }
     return PZ_OK;
 }
@@ -397,13 +397,13 @@ extern "C" int pz_ni_server_key_from_seeded(pz_ctx *c, const uint8_t *h_key, pz_
         if (!d_rg.alloc(8 * kh.words[1]) || !d_ab.alloc(8 * kh.words[2]) ||
             !d_kb.alloc(8 * kh.words[3]))
             return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
-        if (hipMemcpy(d_rg.p, section(h_key, kh, 1), 8 * kh.words[1], hipMemcpyHostToDevice) !=
+        if (pz_copy_sync(d_rg.p, section(h_key, kh, 1), 8 * kh.words[1], hipMemcpyHostToDevice, st) !=
                 hipSuccess ||
-            hipMemcpy(d_ab.p, section(h_key, kh, 2), 8 * kh.words[2], hipMemcpyHostToDevice) !=
+            pz_copy_sync(d_ab.p, section(h_key, kh, 2), 8 * kh.words[2], hipMemcpyHostToDevice, st) !=
                 hipSuccess ||
-            hipMemcpy(d_kb.p, section(h_key, kh, 3), 8 * kh.words[3], hipMemcpyHostToDevice) !=
+            pz_copy_sync(d_kb.p, section(h_key, kh, 3), 8 * kh.words[3], hipMemcpyHostToDevice, st) !=
                 hipSuccess)
-            return PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
+            return PZ_FAIL(PZ_ERR_HIP, "copy failed");
         if ((rc = pz_ni_expand_server_key(c, kh.crs, d_rg.u(), d_ab.u(), d_kb.u(), key)))
             return rc;
     }
@@ -414,18 +414,19 @@ extern "C" int pz_ni_server_key_from_seeded(pz_ctx *c, const uint8_t *h_key, pz_
     const size_t uw = (size_t)p.uitos_d * p.rlwe_n;
     DevBuf d_b, d_z;
     if (!d_b.alloc(8 * uw) || !d_z.alloc(8 * uw * std::max<uint32_t>(kh.total - 1, 1)) ||
-        hipMemset(d_z.p, 0, 8 * uw * std::max<uint32_t>(kh.total - 1, 1)) != hipSuccess) {
+        hipMemsetAsync(d_z.p, 0, 8 * uw * std::max<uint32_t>(kh.total - 1, 1), st) !=
+            hipSuccess) {
         pz_key_free(*key);
         *key = nullptr;
         return PZ_FAIL(PZ_ERR_HIP, "hipMalloc failed");
     }
     for (uint32_t j = 0; j < kh.total; j++) ksks[j] = nullptr;
     for (uint32_t j = 0; j < kh.total; j++) {
-        rc = hipMemcpy(d_b.p, section(h_key, kh, 0) + (size_t)j * uw, 8 * uw,
-                       hipMemcpyHostToDevice) == hipSuccess
+        rc = pz_copy_sync(d_b.p, section(h_key, kh, 0) + (size_t)j * uw, 8 * uw,
+                       hipMemcpyHostToDevice, st) == hipSuccess
                  ? pz_ni_uitos_ksk(c, kh.crs, kh.total, j, d_b.u(),
                                    kh.total > 1 ? d_z.u() : nullptr, &ksks[j])
-                 : PZ_FAIL(PZ_ERR_HIP, "hipMemcpy failed");
+                 : PZ_FAIL(PZ_ERR_HIP, "copy failed");
         if (rc) {
             for (uint32_t o = 0; o < j; o++) pz_rlwe_ksk_free(ksks[o]);
             pz_key_free(*key);
