@@ -383,27 +383,30 @@ def input_path():
 def ni_shares(ctx, total, l0, count):
     """Party secrets and NI server-key shares (client side) for indices
     [l0, l0 + count)."""
-    ps, uit, rg = [], [], []
+    ps, uit, rg, ak = [], [], [], []
     for j in range(total):
         s_j, z_j = ctx.keygen_secrets(1000 + j, 1)
         u_j = ctx.keygen_user_secret(2000 + j)
         ps.append((s_j, z_j, u_j))
         uit.append(ctx.ni_share_uitos(NI_CRS, 3000 + j, j, total, s_j, u_j))
         rg.append(ctx.ni_share_rgsw(NI_CRS, 4000 + j, j, total, s_j, u_j, z_j, l0, count))
-    return ps, uit, rg
+        ak.append(ctx.ni_share_auto_ksk(NI_CRS, 5000 + j, s_j, z_j))
+    return ps, uit, rg, ak
 
 
 def ni_aggregation(params="NI_2P", total=2):
     """Server-key aggregation for `total` parties over all LWE indices: the
-    u->s key switching keys from the shares and every RGSW (produced RLWEs
-    and the leader x other RGSW products), shares resident in HBM; then a
-    batch of gates on the aggregated key, decrypted with the ideal secret."""
+    u->s key switching keys from the shares, every RGSW (produced RLWEs and
+    the leader x other RGSW products) and the automorphism-key / LWE-KSK share
+    sums, shares resident in HBM; then the evaluation-domain key expanded from
+    the aggregate and a batch of gates on it, decrypted with the ideal
+    secret."""
     import pzlib
     ctx = pzlib.Context(params)
     p, N = ctx.p, ctx.N
     n = p.lwe_n
     t0 = time.perf_counter()
-    ps, uit, rg = ni_shares(ctx, total, 0, n)
+    ps, uit, rg, ak = ni_shares(ctx, total, 0, n)
     share_s = (time.perf_counter() - t0) / total
     d_kb = [ctx.to_device(uit[j][0]) for j in range(total)]
     d_zf = [ctx.to_device(np.stack([uit[k][1][j if j < k else j - 1]
@@ -412,10 +415,15 @@ def ni_aggregation(params="NI_2P", total=2):
     d_cts = ctx.to_device(np.stack([x[1] for x in rg]))
     rows = 2 * (p.rlrg_da + p.rlrg_db)
     d_out = ctx.array((n, rows, N))
+    d_as = ctx.to_device(np.stack([x[0] for x in ak]))
+    d_ks = ctx.to_device(np.stack([x[1] for x in ak]))
+    d_ab = ctx.array(ak[0][0].shape)
+    d_kk = ctx.array(ak[0][1].shape)
 
     def run():
         ksks = [ctx.ni_uitos_ksk(NI_CRS, total, j, d_kb[j], d_zf[j]) for j in range(total)]
         ctx.ni_aggregate_rgsw(NI_CRS, total, ksks, 0, n, d_zero, d_cts, d_out)
+        ctx.ni_aggregate_auto_ksk(total, d_as, d_ks, d_ab, d_kk)
         ctx.sync()
         return ksks
 
@@ -427,13 +435,11 @@ def ni_aggregation(params="NI_2P", total=2):
         for k in run():
             k.free()
     agg_s = (time.perf_counter() - t0) / reps
-    ctx.ntt("forward", d_out, n * rows)
+    ctx.sync()
+    t0 = time.perf_counter()
+    key = ctx.ni_expand_server_key(NI_CRS, d_out, d_ab, d_kk)
+    expand_s = time.perf_counter() - t0
     s = sum(x[0].astype(np.int64) for x in ps).astype(np.int32)
-    z = sum(x[1].astype(np.int64) for x in ps).astype(np.int32)
-    base = ctx.keygen_server(5, s, z)
-    _, auto, ksk = ctx.key_download(base)
-    base.free()
-    key = ctx.key_upload(d_out.download(), auto, ksk)
     B = 256
     r = np.random.default_rng(600)
     b0 = r.integers(0, 2, size=B).astype(np.uint8)
@@ -448,7 +454,8 @@ def ni_aggregation(params="NI_2P", total=2):
     produced = n * (p.rlrg_da + p.rlrg_db + (total - 1) * (p.rgrg_da + p.rgrg_db))
     return {"params": params, "parties": total, "lwe_indices": n,
             "aggregate_ms": agg_s * 1e3, "produced_rlwes": produced,
-            "share_gen_ms_per_party": share_s * 1e3, "gates_correct": ok}
+            "expand_ms": expand_s * 1e3, "share_gen_ms_per_party": share_s * 1e3,
+            "gates_correct": ok}
 
 
 def cpu_share():

@@ -28,6 +28,7 @@ def named(name="NI_2P"):
 CASES = [
     ("rlwe_q", lambda p: setattr(p, "rlwe_q", (1 << 54) + 1), PZ_ERR_ARG),       # above 2^54
     ("rlwe_q", lambda p: setattr(p, "rlwe_q", p.rlwe_q + 2), PZ_ERR_ARG),         # not 1 mod 2N
+    ("rlwe_q", lambda p: setattr(p, "rlwe_q", 4096 * 1000 + 1), PZ_ERR_UNSUPPORTED),  # < 2^32 at N = 2048
     ("rlwe_n", lambda p: setattr(p, "rlwe_n", 1024), PZ_ERR_UNSUPPORTED),
     ("rlwe_n", lambda p: setattr(p, "rlwe_n", 3000), PZ_ERR_UNSUPPORTED),
     ("lwe_q", lambda p: setattr(p, "lwe_q", 3 << 14), PZ_ERR_UNSUPPORTED),
