@@ -106,25 +106,37 @@ __global__ __launch_bounds__(1024) void k_key_order(const u32 *idx, size_t B, u3
                                                     u32 *order)
 {
     __shared__ u32 cnt[KEY_ORDER_MAX];
-    __shared__ u32 part[1024];
-    const u32 t = threadIdx.x, n = nkeys + 1;
+    __shared__ u32 wsum[16];
+    const u32 t = threadIdx.x, n = nkeys + 1, lane = t & 63, wv = t >> 6;
     for (u32 i = t; i < n; i += 1024) cnt[i] = 0;
     __syncthreads();
     for (size_t b = t; b < B; b += 1024) atomicAdd(&cnt[idx[b] < nkeys ? idx[b] : nkeys], 1u);
     __syncthreads();
-    // exclusive prefix sum: thread t takes ceil(n / 1024) consecutive counters
+    // exclusive prefix sum: thread t takes ceil(n / 1024) consecutive
+    // counters; their sums are scanned within each wave (shuffles), then the
+    // 16 wave totals by wave 0
     const u32 per = (n + 1023) / 1024, lo = t * per;
     u32 sum = 0;
     for (u32 i = 0; i < per && lo + i < n; i++) sum += cnt[lo + i];
-    part[t] = sum;
-    __syncthreads();
-    for (u32 off = 1; off < 1024; off <<= 1) {
-        const u32 v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    u32 v = sum;
+#pragma unroll
+    for (u32 off = 1; off < 64; off <<= 1) {
+        const u32 y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
     }
-    u32 run = part[t] - sum;
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    if (wv == 0) {
+        u32 w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (u32 off = 1; off < 16; off <<= 1) {
+            const u32 y = __shfl_up(w, off, 64);
+            if (lane >= off) w += y;
+        }
+        if (lane < 16) wsum[lane] = w;
+    }
+    __syncthreads();
+    u32 run = v - sum + (wv ? wsum[wv - 1] : 0);
     for (u32 i = 0; i < per && lo + i < n; i++) {
         const u32 c = cnt[lo + i];
         cnt[lo + i] = run;
@@ -172,6 +184,13 @@ __global__ __launch_bounds__(2 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu
     const size_t per_x = gridDim.x / XCDS, x = blockIdx.x % XCDS, m = blockIdx.x / XCDS;
     const size_t p0 = dealt ? 0 : blockIdx.x * B / gridDim.x;
     const size_t p1 = dealt ? B : (blockIdx.x + 1) * B / gridDim.x;
+    // A result waits in this thread's own words of its half's LDS region
+    // (the stride-layout words its inverse transform loaded last, which the
+    // next forward transform stores to first) and is written out after the
+    // next RLWE's loads are issued: loads and stores share one in-order
+    // counter, so a load issued after the stores would wait for them.
+    u64 *mine = lds + h * W;
+    u64 *parked = nullptr;
     for (size_t it = p0; it < p1; it++) {
         size_t pos = it;
         if (dealt) {
@@ -185,12 +204,22 @@ __global__ __launch_bounds__(2 << (LOGN - 3)) __attribute__((amdgpu_waves_per_eu
         u64 p[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) p[r] = poly[coef_x<LOGN>(j, r)];
+        if (parked) {
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+                parked[coef_x<LOGN>(j, r)] = mine[lds_ix(coef_x<LOGN>(j, r))];
+        }
         const bool triv = __builtin_amdgcn_readfirstlane((int)trivial[bb]) != 0;
         step_split<LOGN, 8>(p, (is_auto ? key.auto_k : key.rgsw) + k * rs,
                             (is_auto ? key.auto_s : key.rgsw_s) + k * rs, is_auto,
                             c.auto_el[is_auto ? k : 0], triv, c, j, h, lds, lds + 2 * W);
 #pragma unroll
-        for (int r = 0; r < 8; r++) poly[coef_x<LOGN>(j, r)] = p[r];
+        for (int r = 0; r < 8; r++) mine[lds_ix(coef_x<LOGN>(j, r))] = p[r];
+        parked = poly;
+    }
+    if (parked) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) parked[coef_x<LOGN>(j, r)] = mine[lds_ix(coef_x<LOGN>(j, r))];
     }
 }
 
