@@ -397,14 +397,19 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     u64 D[1][R];
     // product rows: RLWE x RGSW -- own digits with the own-part row (half 0:
     // A(-sm), row 0; half 1: B(m), row 3), then the other half's digits with
-    // the other row (A(m), row 2; B(-sm), row 1); automorphism -- half 0 row
-    // A (0) with its digits, half 1 row B (1) with half 0's digits
-    const size_t r1 = (size_t)(is_auto ? 0 : (h ? 3 : 0)) * N;
-    const size_t r2 = (size_t)(is_auto ? 1 : (h ? 1 : 2)) * N;
-    const bool second = !is_auto || h == 1;
+    // the other row (A(m), row 2; B(-sm), row 1); automorphism -- one product
+    // each, half 0 row A (0) with its digits, half 1 row B (1) with half 0's
+    const size_t r1 = (size_t)(is_auto ? (h ? 1 : 0) : (h ? 3 : 0)) * N;
+    const size_t r2 = (size_t)(h ? 1 : 2) * N;
     // unreduced products: at most two of them, so the accumulator stays below
     // 4q (8q with the [0,4q) products), the fast inverse's input bound
     constexpr bool Q4 = PZ_QUOT4 && PZ_LAZY_MAC && PZ_INV_FAST;
+    // key rows loaded whole (32 VGPRs), the first before the barrier that
+    // precedes its product (half 1 of an automorphism, with no transform to
+    // run, before it waits out half 0's), the second before the read of the
+    // other half's digits and the barrier after it: the load latencies
+    // overlap the barrier waits instead of one exposed latency per half row
+    KeySlice<R> k1, k2;
     if (work) {
 #pragma unroll
         for (int r = 0; r < R; r++) D[0][r] = decomp_digit0(dc, p[r]);
@@ -412,28 +417,31 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
                                                                                   c.tb);
 #pragma unroll
         for (int r = 0; r < R; r++) mine[lds_ix(eval_x<LOGN, R>(j, r))] = D[0][r];
+        // not hoisted into the transform: its twiddle loads would wait behind
+        // the key row on the in-order load counter
+        __builtin_amdgcn_sched_barrier(0);
+        load_slice(k1, K + r1, Ks + r1, j);
     } else {
+        load_slice(k1, K + r1, Ks + r1, j);
         ntt_fwd_idle<LOGN, R>();
     }
     u64 acc[1][R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[0][r] = 0;
-    // key rows loaded whole (32 VGPRs), the first before the barrier that
-    // precedes its product, the second before the read of the other half's
-    // digits and the barrier after it: the load latencies overlap the barrier
-    // waits instead of one exposed latency per half row
-    KeySlice<R> k1, k2;
-    if (work) load_slice(k1, K + r1, Ks + r1, j);
     __syncthreads();
-    if (work) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k1, D[0], q, q2);
+    if (!work) {
+#pragma unroll
+        for (int r = 0; r < R; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN, R>(j, r))];
+    }
+    fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k1, D[0], q, q2);
     __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
-    if (second) load_slice(k2, K + r2, Ks + r2, j);
-    if (second) {
+    if (!is_auto) {
+        load_slice(k2, K + r2, Ks + r2, j);
 #pragma unroll
         for (int r = 0; r < R; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN, R>(j, r))];
     }
     __syncthreads();  // both halves have read: the regions may be rewritten
-    if (second) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k2, D[0], q, q2);
+    if (!is_auto) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k2, D[0], q, q2);
     ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2), R,
             false>(acc, j, mine, c.tb);  // n^-1: key rows
     if (is_auto && h == 1) {
