@@ -45,7 +45,7 @@ void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_b
 size_t br_lds_bytes(int logn, bool split, u32 br_q, u32 n_lwe);
 // blind-rotation kernel variants: one work group per bootstrap, or two half
 // groups (d = 1) with the register budget of 2 (latency) or 4 waves per SIMD
-// blind-rotation kernel variants (BR_SPLIT4*: 4 coefficients per thread)
+// (BR_SPLIT4*: 4 coefficients per thread)
 enum { BR_ONE = 0, BR_SPLIT_LAT = 1, BR_SPLIT = 2, BR_SPLIT4_LAT = 3, BR_SPLIT4 = 4, BR_NVAR = 5 };
 const void *blind_rotate_kernel(int logn, int var);
 bool br_split(const DevCtx &c);
