@@ -241,3 +241,35 @@ def test_split_mode_needs_single_digit_decompositions():
         with pytest.raises(pzlib.PzError):
             c.set_blind_rotate_mode(mode)
     c.set_blind_rotate_mode(0)
+
+
+# The reference's other parameter sets (src/bool/parameters.rs:526-596,
+# :652-677): q_ks = 2^15 with 12 key-switch digits (I_2P_LB_SR), n = 580 /
+# 620 / 660, br_q = 2048 and 4096, the interactive sets' ternary LWE secret.
+OTHER_SETS = ["I_2P_LB_SR", "I_4P", "I_8P", "NI_4P_LB_SR"]
+
+
+@pytest.mark.parametrize("pname", OTHER_SETS)
+def test_pbs_and_gates_other_sets(pname):
+    o, s, z, key = keys_for(pname)
+    c = ctx_for(pname)
+    k = dkey_for(pname)
+    B = 6
+    log("pbs + gates %s B=%d" % (pname, B))
+    r = rng(71)
+    x = uniform_mod(r, o.Q, (B, o.N + 1))
+    x[0, :8] = o.Q - 1
+    gate = (np.arange(B) % 6).astype(np.uint32)
+    want = np.stack([o.pbs(key, int(gate[b]), x[b])[0] for b in range(B)])
+    d = c.to_device(x)
+    c.pbs(k, c.to_device(gate), d, B)
+    assert np.array_equal(d.download(), want)
+    bits0, bits1, c0, c1 = _gate_inputs(o, s, B, 72)
+    ops = (np.arange(B) % 6).astype(np.uint32)
+    want = np.stack([o.gate(key, int(ops[b]), c0[b], c1[b]) for b in range(B)])
+    dout = c.array((B, o.N + 1))
+    c.gates(k, c.to_device(ops), c.to_device(c0), c.to_device(c1), dout, B)
+    got = dout.download()
+    assert np.array_equal(got, want)
+    for b in range(B):
+        assert o.decrypt_bool(s, got[b]) == PLAIN[int(ops[b])](int(bits0[b]), int(bits1[b]))

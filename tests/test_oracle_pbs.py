@@ -38,7 +38,24 @@ def test_gate_truth_tables(name):
         assert o.decrypt_bool(s, o.not_(c)) == 1 - a
 
 
-@pytest.mark.parametrize("name", ["NI_2P", "NI_8P", "NI_4P_HB_FR", "SP_TEST"])
+# the reference's other parameter sets (src/bool/parameters.rs:526-596,
+# :652-677): NAND (sum + Q/4) and XOR (2(c0 - c1)), the two linear steps
+@pytest.mark.parametrize("name", ["I_2P_LB_SR", "I_4P", "I_8P", "NI_4P_LB_SR"])
+def test_gate_truth_tables_other_sets(name):
+    o, s, z, key = keys_for(name)
+    seed = 3000
+    for g in ("nand", "xor"):
+        for a in (0, 1):
+            for b in (0, 1):
+                c0 = o.encrypt_bool(s, a, seed)
+                c1 = o.encrypt_bool(s, b, seed + 1)
+                seed += 2
+                out = o.gate(key, g, c0, c1)
+                assert o.decrypt_bool(s, out) == PLAIN[g](a, b), (name, g, a, b)
+
+
+@pytest.mark.parametrize("name", ["NI_2P", "NI_8P", "NI_4P_HB_FR", "SP_TEST", "I_2P_LB_SR",
+                                  "I_8P"])
 def test_dlog_map(name):
     from conftest import oracle_for
     o = oracle_for(name)
@@ -144,7 +161,8 @@ def py_br_schedule(ks, p):
     return ops
 
 
-@pytest.mark.parametrize("name", ["NI_2P", "NI_4P_HB_FR", "NI_8P", "SP_TEST"])
+@pytest.mark.parametrize("name", ["NI_2P", "NI_4P_HB_FR", "NI_8P", "SP_TEST", "I_2P_LB_SR",
+                                  "I_4P", "I_8P", "NI_4P_LB_SR"])
 def test_schedule_matches_python_restatement(name):
     """The oracle's recorded op order equals an independent restatement of
     the reference's loop, including the trivial-RLWE flags, for uniform key
