@@ -280,8 +280,11 @@ int pz_server_run(pz_server *s);
 /* Result ciphertexts of a completed request ([n_out][N+1]); PZ_ERR_STATE
  * while it is not complete (may be called while the loop runs). */
 int pz_server_result(pz_server *s, uint32_t request_id, uint64_t *h_out);
+/* Times of a request (seconds from the start of the run; -1 until admitted /
+ * finished); may be called while the loop runs. */
 int pz_server_request_times(pz_server *s, uint32_t request_id, double *arrival, double *admitted,
                             double *finished, uint32_t *n_pbs, uint32_t *n_ops);
+/* Totals of the last run (read after pz_server_run has returned). */
 int pz_server_stats(pz_server *s, uint64_t *total_pbs, uint64_t *batches, uint64_t *key_uploads,
                     uint64_t *evictions, double *wall_s);
 

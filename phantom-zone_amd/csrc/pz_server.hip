@@ -358,25 +358,31 @@ extern "C" int pz_server_run(pz_server *s)
     std::deque<size_t> pending;   // arrived, not admitted
     std::vector<size_t> active;   // admitted, levels left to queue
     std::vector<Finishing> finishing;
+    // the loop's view of the requests: element pointers taken under the lock
+    // (a deque keeps its elements in place on push_back, but indexing it
+    // while another thread appends reads a block map that may be replaced)
+    std::vector<Request *> R;
     int rc = PZ_OK;
     for (;;) {
         bool open;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (; scanned < s->reqs.size(); scanned++)
-                if (s->reqs[scanned].state == 0) waiting.push_back(scanned);
+            for (; scanned < s->reqs.size(); scanned++) {
+                R.push_back(&s->reqs[scanned]);
+                if (R.back()->state == 0) waiting.push_back(scanned);
+            }
             total = s->reqs.size();
             open = s->intake_open;
         }
         std::stable_sort(waiting.begin(), waiting.end(), [&](size_t x, size_t y) {
-            return s->reqs[x].arrival < s->reqs[y].arrival;
+            return R[x]->arrival < R[y]->arrival;
         });
         double t = s->now();
         size_t k = 0;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            while (k < waiting.size() && s->reqs[waiting[k]].arrival <= t) {
-                s->reqs[waiting[k]].state = 1;
+            while (k < waiting.size() && R[waiting[k]]->arrival <= t) {
+                R[waiting[k]]->state = 1;
                 pending.push_back(waiting[k++]);
             }
         }
@@ -393,7 +399,7 @@ extern "C" int pz_server_run(pz_server *s)
                 rc = PZ_FAIL(PZ_ERR_HIP, "a served batch failed");
                 break;
             }
-            Request &r = s->reqs[finishing[i].req];
+            Request &r = *R[finishing[i].req];
             std::vector<uint64_t> out(r.outs.size() * (size_t)s->N1);
             for (size_t o = 0; o < r.outs.size() && rc == PZ_OK; o++)
                 if (hipMemcpy(out.data() + o * s->N1, s->pool + (r.base + r.outs[o]) * s->N1,
@@ -416,7 +422,7 @@ extern "C" int pz_server_run(pz_server *s)
         // admission in arrival order (a request whose session cannot be made
         // resident does not block later requests of resident sessions)
         for (auto it = pending.begin(); it != pending.end();) {
-            Request &r = s->reqs[*it];
+            Request &r = *R[*it];
             if (active.size() + finishing.size() >= s->cfg.max_active) break;
             size_t base;
             if (!make_resident(s, r.session) || !s->rows.alloc(r.rows, &base)) {
@@ -449,7 +455,7 @@ extern "C" int pz_server_run(pz_server *s)
         std::vector<std::vector<GateRec>> recs(nc);
         std::vector<size_t> stepped, pbs(nc, 0);
         for (size_t idx : active) {
-            Request &r = s->reqs[idx];
+            Request &r = *R[idx];
             const u32 c = s->sessions[r.session].ctx;
             while (r.next_level < r.levels.size() && r.levels[r.next_level].empty()) r.next_level++;
             if (r.next_level >= r.levels.size()) {
@@ -486,7 +492,7 @@ extern "C" int pz_server_run(pz_server *s)
         if (rc) break;
         s->tick++;
         for (size_t idx : stepped) {
-            Request &r = s->reqs[idx];
+            Request &r = *R[idx];
             s->sessions[r.session].last_used = s->tick;
             if (r.next_level < r.levels.size()) r.next_level++;
             while (r.next_level < r.levels.size() && r.levels[r.next_level].empty()) r.next_level++;
@@ -522,7 +528,7 @@ extern "C" int pz_server_run(pz_server *s)
         if (s->reqs.size() > scanned || s->intake_open != open) continue;
         double wait = 0.01;  // an open intake, or requests that cannot be admitted yet
         if (!waiting.empty())
-            wait = std::min(wait, std::max(0.0, s->reqs[waiting.front()].arrival - s->now()));
+            wait = std::min(wait, std::max(0.0, R[waiting.front()]->arrival - s->now()));
         if (wait > 0)
             s->cv.wait_for(lk, std::chrono::duration<double>(wait),
                            [&]() { return s->reqs.size() > scanned || s->intake_open != open; });
