@@ -629,6 +629,7 @@ def main():
     # the Shoup multiply itself over independent chains (its achievable peak)
     probe = {name: ctx.valu_probe(kind) for kind, name in
              enumerate(("add_u32", "mul_lo_u32", "mul_hi_u32", "mad_u64_u32"))}
+    probe["lshl_add_u64"] = ctx.valu_probe(5)
     probe["shoup_mul_per_s"] = ctx.valu_probe(4, 1024)
     log("valu probe: %s" % probe)
 

@@ -502,8 +502,9 @@ int pz_last_timing(pz_ctx *ctx, float *ms_prep, float *ms_ks, float *ms_blind_ro
 /* VALU issue-rate probe.  No counterpart in the reference: it measures the
  * integer VALU rates the PBS kernels' roofline rests on.  kind: 0 v_add_u32,
  * 1 v_mul_lo_u32, 2 v_mul_hi_u32, 3 v_mad_u64_u32 (lane operations per
- * second), 4 one 54-bit Shoup multiply (multiplies per second), each over
- * 8 independent chains per thread on every CU. */
+ * second), 4 one 54-bit Shoup multiply (multiplies per second), 5
+ * v_lshl_add_u64 (lane operations per second), each over 8 independent
+ * chains per thread on every CU. */
 int pz_valu_probe(pz_ctx *ctx, int kind, uint32_t iters, double *ops_per_s);
 
 #ifdef __cplusplus

@@ -1113,7 +1113,7 @@ extern "C" int pz_last_timing(pz_ctx *c, float *a, float *b, float *d)
 // integer VALU rates for the roofline of the PBS kernels).
 extern "C" int pz_valu_probe(pz_ctx *c, int kind, uint32_t iters, double *ops_per_s)
 {
-    if (!c || !ops_per_s || kind < 0 || kind > 4 || iters == 0)
+    if (!c || !ops_per_s || kind < 0 || kind > 5 || iters == 0)
         return fail(PZ_ERR_ARG, "bad probe arguments");
     HIPCHK(hipSetDevice(c->device));
     const int grid = c->num_cu * 16;  // 16 work groups of 4 waves per CU: 2 rounds at 8 waves/SIMD

@@ -343,7 +343,8 @@ __global__ void k_not(u64 *c0, size_t n, u64 q)
 // PBS kernels are built from.  Each thread runs PROBE_CH independent chains
 // for `iters` steps (an empty asm keeps every step live) and stores the xor
 // of its chains.  kind: 0 v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32,
-// 3 v_mad_u64_u32, 4 one 54-bit Shoup multiply (shoup_mul).
+// 3 v_mad_u64_u32, 4 one 54-bit Shoup multiply (shoup_mul), 5 v_lshl_add_u64
+// (the butterflies' 64-bit 2x + c).
 // ---------------------------------------------------------------------------
 constexpr int PROBE_CH = 8;
 
@@ -361,6 +362,7 @@ __global__ __launch_bounds__(256) void k_valu_probe(u64 *out, u32 iters, u32 c, 
 #pragma unroll
             for (int k = 0; k < PROBE_CH; k++) {
                 if (KIND == 3) x[k] = (u64)(u32)x[k] * c + x[k];
+                else if (KIND == 5) x[k] = twice_plus(x[k], w);
                 else x[k] = shoup_mul(x[k], w, ws, q);
                 asm volatile("" : "+v"(x[k]));
             }
@@ -394,6 +396,7 @@ void launch_valu_probe(int kind, int grid, u64 *out, u32 iters, u32 c, u64 w, u6
     case 1: hipLaunchKernelGGL(k_valu_probe<1>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
     case 2: hipLaunchKernelGGL(k_valu_probe<2>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
     case 3: hipLaunchKernelGGL(k_valu_probe<3>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
+    case 5: hipLaunchKernelGGL(k_valu_probe<5>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
     default: hipLaunchKernelGGL(k_valu_probe<4>, grid, 256, 0, s, out, iters, c, w, ws, q); break;
     }
 }
