@@ -29,11 +29,30 @@
  *   LWE key switch       brute force over Python ints; decryption under z.
  *   moddown / odd switch Python floats (IEEE doubles) at rounding boundaries.
  *   PBS / gates          gate truth tables after decryption for all 4 inputs
- *                        of all 6 gates.
+ *                        of all 6 gates (NAND / XOR on the interactive sets);
+ *                        the recorded op order equals an independent Python
+ *                        restatement of the bucket loop for every named set;
+ *                        dlog maps against powers of g; multi-threaded gate
+ *                        batches equal one-at-a-time gates.
+ *   u -> s key switch,   exact big-int closed form of every output word;
+ *   sample extract       the extracted LWE decrypts to the RLWE phase.
+ *   RGSW x RGSW          exact big-int closed form of every row pair.
+ *   NI shares, Ksk from  CRS seed-tree bytes against the Python ChaCha8
+ *   shares, produced     stream; leader segments by brute force; Ksk rows,
+ *   RGSWs, aggregation   produced RGSW rows and the aggregate decrypt under the
+ *                        ideal secret with noise below the smallest gadget
+ *                        power.
+ *   NI auto-key / LWE-   the a parts equal the CRS streams drawn by the Python
+ *   KSK shares, sums,    ChaCha8 / UniformInt restatement; share rows decrypt
+ *   key expansion        to RLWE'(s(X^k)) and beta * s_i; sums exact; a key
+ *                        from shares alone bootstraps correctly.
  *   RNG streams (uniform / Gaussian / ternary sampling) and therefore the key
  *   material:            PARITY UNPINNED -- this is the project's own seeded
  *                        procedure (rand / rand_distr sources are not
- *                        available), not the Rust crate's stream.
+ *                        available), not the Rust crate's stream.  The CRS
+ *                        seed tree and UniformInt draws follow rand 0.8.5's
+ *                        published algorithm (pinned as above); the Gaussian
+ *                        noise words of keys and shares stay unpinned.
  */
 #define _GNU_SOURCE
 #include "pz_oracle.h"
