@@ -101,6 +101,38 @@ def batched(ctx, key, s, B, steps, warmup, seed):
     return B * steps / secs, secs, ok, (dops, d0, d1, dout)
 
 
+def host_path(ctx, key, s, B, steps, seed):
+    """Batches whose ciphertexts start and end in host memory: per step both
+    input arrays are copied to the device, the gates run and the outputs are
+    copied back (pageable numpy buffers, through the library's stream-ordered
+    copies), so the rate includes the PCIe transfers."""
+    p = ctx.p
+    r = np.random.default_rng(seed)
+    b0 = r.integers(0, 2, size=B).astype(np.uint8)
+    b1 = r.integers(0, 2, size=B).astype(np.uint8)
+    c0 = ctx.encrypt_bools(s, b0, seed * 1000 + 1)
+    c1 = ctx.encrypt_bools(s, b1, seed * 1000 + 1 + B)
+    d0, d1 = ctx.array((B, p.rlwe_n + 1)), ctx.array((B, p.rlwe_n + 1))
+    dops = ctx.to_device(np.zeros(B, dtype=np.uint32))
+    dout = ctx.array((B, p.rlwe_n + 1))
+
+    def step():
+        d0.upload(c0)
+        d1.upload(c1)
+        ctx.gates(key, dops, d0, d1, dout, B)
+        return dout.download()
+
+    step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = step()
+    secs = time.perf_counter() - t0
+    ok = bool(np.array_equal(ctx.decrypt_bools(s, got), (1 - (b0 & b1)).astype(np.uint8)))
+    return {"bootstraps_per_sec": B * steps / secs, "ms_per_step": secs / steps * 1e3,
+            "copied_bytes_per_step": 3 * B * (p.rlwe_n + 1) * 8, "host_memory": "pageable",
+            "outputs_correct": ok}
+
+
 def fheuint8_circuits(args):
     """256 mul + 256 div_rem + 256 lt on uniform u8 at NI_8P, one circuit,
     every ready gate of a level in one launch."""
@@ -642,6 +674,8 @@ def main():
             log("batched B=%d: %.0f bootstraps/s" % (bs, v))
         sweep[str(B)] = {"bootstraps_per_sec": value, "outputs_correct": ok}
         extras["batch_sweep"] = sweep
+        extras["host_path"] = host_path(ctx, key, s, B, 2, seed=5)
+        log("host path (PCIe included): %s" % extras["host_path"])
         key.free()
         ctx.close()
         extras["kernel_batch"] = kernel_batch()
