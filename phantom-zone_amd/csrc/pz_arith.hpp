@@ -28,13 +28,24 @@ PZ_DEV u64 shoup_quot(u64 a, u64 ws)
     return (u64)s1 * a1 + (t >> 32) + (u >> 32);
 }
 // c + w*a - q*h (mod 2^64), written as c + w*a + (2^64 - q)*h: the same word,
-// with the high cross products folded into one 32-bit sum.
+// with the high cross products folded into one 32-bit sum.  The four cross
+// products form one chain of v_mad_u64_u32, each taking the previous result
+// as its 64-bit addend (only the low word is used, so the addend's high word
+// is free and no add merges them: 4 instructions instead of v_mul_lo_u32,
+// v_mad, two v_mul_lo_u32 and v_add3), the products of w first, which do not
+// wait for the quotient.  The empty asm keeps the compiler from narrowing the
+// chain back to 32-bit multiplies and adds.
 PZ_DEV u64 shoup_rem(u64 c, u64 a, u64 w, u64 h, u64 q)
 {
     const u32 a0 = (u32)a, a1 = (u32)(a >> 32);
     const u64 nq = 0 - q;
     const u32 h0 = (u32)h, h1 = (u32)(h >> 32);
-    const u32 m = (u32)w * a1 + (u32)(w >> 32) * a0 + (u32)nq * h1 + (u32)(nq >> 32) * h0;
+    u64 mc = (u64)(u32)(w >> 32) * a0;
+    mc = (u64)(u32)w * a1 + mc;
+    mc = (u64)h1 * (u32)nq + mc;
+    mc = (u64)h0 * (u32)(nq >> 32) + mc;
+    asm("" : "+v"(mc));
+    const u32 m = (u32)mc;
     const u64 lo = (u64)(u32)w * a0 + (u64)(u32)nq * h0 + c;
     // m << 32 only changes the high word: one 32-bit add (as a 64-bit add the
     // compiler spends a move on a zero low word as well)
@@ -257,6 +268,21 @@ PZ_DEV u64 decomp_digit0(const Decomp &dc, u64 v)
     // of k + (that bit) against B/2, no divergent branch
     const bool carry = k + ((value >> dc.logb) & 1u) > half;
     return (u64)k + (carry ? dc.q - b : 0);  // q + k - B for a negative digit
+}
+
+// Digit of a d = 1 decomposition as q + digit, a representative in
+// [q - B/2, q + B/2] of the reference's digit (the consumer is a transform
+// that accepts inputs below 2q).  With d = 1 the decomposer ignores
+// bitlen(q) - logb bits, so the rounded signed value lies in [-B/2, B/2] and
+// the carry rule never changes it: the digit is the value itself
+// (negative values as q + value; the value B/2 with an even next digit stays
+// B/2, the value -B/2 carries into q - B/2).  Needs logb + ignore_bits <= 63
+// and ignore_bits > 0 (round_c = 2^(ignore_bits - 1)).
+PZ_DEV u64 decomp_digit_d1(const Decomp &dc, u64 v, u64 round_c)
+{
+    // v_s + 2^(ib-1) in two's complement, v_s = v - q when v >= floor(q/2)
+    const u64 t = v + (v >= (dc.q >> 1) ? round_c - dc.q : round_c);
+    return dc.q + (u64)((int64_t)t >> dc.ignore_bits);  // floor, then + q
 }
 
 PZ_DEV u64 decomp_digit(const Decomp &dc, u64 v, u32 r)

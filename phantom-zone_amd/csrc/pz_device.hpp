@@ -46,6 +46,30 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #define PZ_MAC_FENCE 0
 #endif
 
+// Split step transforms with each phase's twiddles loaded ahead of the LDS
+// exchange before the phase (the first phases' before the digits and before
+// the second key product).
+// Bits: 1 forward; 2 inverse, its first phase's twiddles issued before the
+// second key product; 4 inverse, its first phase's issued at its start.
+// PZ_TW_PREFETCH for 8 coefficients per thread, PZ_TW_PREFETCH4 for 4.
+// Measured at B = 4096 (8 coefficients): forward 1.9 % faster, with the
+// inverse (4) 4.1 %; with 2 the inverse twiddles and the second key row are
+// live together and the step spills (21 % slower).  One bootstrap
+// (4 coefficients): 8 % faster with 5.
+#ifndef PZ_TW_PREFETCH
+#define PZ_TW_PREFETCH 5
+#endif
+#ifndef PZ_TW_PREFETCH4
+#define PZ_TW_PREFETCH4 5
+#endif
+
+// Split step digits as q + digit (decomp_digit_d1, 8 instructions) instead of
+// the reference's representative in [0, q) (decomp_digit0, 15): the same
+// residue, for the unreduced transform.
+#ifndef PZ_DIGIT_D1
+#define PZ_DIGIT_D1 1
+#endif
+
 // Everything a kernel needs about the parameter set (passed by value).
 struct DevCtx {
     Tables tb;
@@ -410,11 +434,25 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     // other half's digits and the barrier after it: the load latencies
     // overlap the barrier waits instead of one exposed latency per half row
     KeySlice<R> k1, k2;
+    constexpr int PF = R == 8 ? PZ_TW_PREFETCH : PZ_TW_PREFETCH4;
+    PhaseTw<R> tw;  // the first phase's twiddles of the forward, then of the inverse
     if (work) {
+        if constexpr (PF & 1) fwd_tw_first<LOGN, R>(tw, j, c.tb);  // overlap the digits
+#if PZ_DIGIT_D1
+        // q + digit (below 2q): the unreduced forward transform takes it as is
+        const u64 rc = dc.ignore_bits ? 1ull << (dc.ignore_bits - 1) : 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) D[0][r] = decomp_digit_d1(dc, p[r], rc);
+#else
 #pragma unroll
         for (int r = 0; r < R; r++) D[0][r] = decomp_digit0(dc, p[r]);
-        ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(D, j, mine,
-                                                                                  c.tb);
+#endif
+        if constexpr (PF & 1)
+            ntt_fwd_pf<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(
+                D, j, mine, c.tb, tw);
+        else
+            ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(D, j,
+                                                                                      mine, c.tb);
 #pragma unroll
         for (int r = 0; r < R; r++) mine[lds_ix(eval_x<LOGN, R>(j, r))] = D[0][r];
         // not hoisted into the transform: its twiddle loads would wait behind
@@ -440,10 +478,17 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
 #pragma unroll
         for (int r = 0; r < R; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN, R>(j, r))];
     }
+    // after the key row, which is needed first
+    if constexpr ((PF & 6) == 2) inv_tw_first<LOGN, R>(tw, j, c.tb);
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (!is_auto) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k2, D[0], q, q2);
-    ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2), R,
-            false>(acc, j, mine, c.tb);  // n^-1: key rows
+    constexpr int IBND = Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2);  // accumulators below IBND q
+    if constexpr ((PF & 6) == 4) inv_tw_first<LOGN, R>(tw, j, c.tb);
+    if constexpr ((PF & 6) != 0)
+        ntt_inv_pf<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, IBND, R, false>(acc, j, mine,
+                                                                                 c.tb, tw);
+    else  // n^-1: key rows
+        ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, IBND, R, false>(acc, j, mine, c.tb);
     if (is_auto && h == 1) {
         // acc + (+-v) mod q, branch-free: the negated value q - v is selected,
         // then one conditional subtraction (acc < q, q - v <= q); the same word

@@ -72,7 +72,8 @@ struct Tables {
 // only used where a Shoup product consumes the result (Shoup accepts any
 // 64-bit input), so every ciphertext stays identical to the reference.
 // FWD_FAST4 is FWD_FAST with the [0,4q) product (shoup_mul4) and y' = x + 4q - t:
-// stage k leaves values below (4k+1)q (< 45q < 2^60 for N = 2048).
+// from inputs below 2q (the split step's digits, q + digit) stage k leaves
+// values below (4k+2)q (< 46q < 2^60 for N = 2048).
 enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3 };
 
 // Unreduced tail of the IB = 8 fast inverse (see inv_stage).
@@ -86,8 +87,8 @@ enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3 };
 // SM selects the two-step Shoup form
 // (shoup_mul<true>, fewer instructions); the one-step form has fewer live
 // registers, which keeps the one-work-group blind rotation within 256 VGPRs.
-template <int LOGN, int S, int ST, int NP, int KIND, bool SM, int R>
-PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
+template <int LOGN, int S, int ST, int NP, int KIND, bool SM, int R, bool PRE = false>
+PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *pre = nullptr)
 {
     constexpr int N = 1 << LOGN;
     constexpr int T = N >> (S + 1);
@@ -98,7 +99,7 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
         if (r & LD) continue;
         const u32 x = base + ST * r;
         const u32 i = x / (2 * T);
-        const u64x2 tw = gld2(tb.fw + M + i);
+        const u64x2 tw = PRE ? pre[r] : gld2(tb.fw + M + i);
         const u64 w = tw.x, ws = tw.y;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -148,8 +149,8 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
 // Residues are the reference's throughout, and the strict last stage returns
 // the same words.
 template <int LOGN, int S, int ST, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB,
-          int R>
-PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
+          int R, bool PRE = false>
+PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *pre = nullptr)
 {
     constexpr int N = 1 << LOGN;
     constexpr int T = N >> (S + 1);
@@ -170,7 +171,7 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
         if (r & LD) continue;
         const u32 x = base + ST * r;
         const u32 i = x / (2 * T);
-        const u64x2 tw = gld2(tb.iw + M + i);
+        const u64x2 tw = PRE ? pre[r] : gld2(tb.iw + M + i);
         const u64 w = tw.x, ws = tw.y;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -224,18 +225,40 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb)
 }
 
 PZ_DEV struct Tables fresh(const Tables &tb);
-#ifndef PZ_TWIDDLE_PER_STAGE
-#define PZ_TWIDDLE_PER_STAGE 0
-#endif
-#if PZ_TWIDDLE_PER_STAGE
-#define STAGE_TB(t) fresh(t)
-#else
-#define STAGE_TB(t) (t)
-#endif
 
-// Forward phase starting at stage S0 (up to log2 R stages).
-template <int LOGN, int S0, int NP, int MODE, bool SM, int R>
-PZ_DEV void fwd_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
+// The twiddles of one phase, [stage in the phase][r] (only entries with
+// !(r & LD) are used; the loads of equal indices are one load).  Loaded ahead
+// of the LDS exchange that precedes the phase (PF transforms): the exchange's
+// fence keeps the compiler from moving a load above it, and loaded after it
+// each stage's twiddle latency is exposed once the register budget stops the
+// scheduler from issuing several together.
+template <int R>
+struct PhaseTw {
+    u64x2 t[3][R];
+};
+template <int LOGN, int S0, int R, bool INV>
+PZ_DEV void phase_tw(PhaseTw<R> &pt, u32 j, const Tables &tb)
+{
+    using P = Phase<LOGN, S0, R>;
+    const u32 base = P::base(j);
+    const u64x2 *tab = INV ? tb.iw : tb.fw;
+#pragma unroll
+    for (int e = 0; e < P::NS; e++) {
+        const int S = S0 + e;
+        const int T = (1 << LOGN) >> (S + 1);
+        const int LD = T / P::ST;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (r & LD) continue;
+            pt.t[e][r] = gld2(tab + (1 << S) + (base + P::ST * r) / (2 * T));
+        }
+    }
+}
+
+// Forward phase starting at stage S0 (up to log2 R stages); PRE: twiddles
+// from pt.
+template <int LOGN, int S0, int NP, int MODE, bool SM, int R, bool PRE = false>
+PZ_DEV void fwd_phase(u64 (&v)[NP][R], u32 j, const Tables &tb, const PhaseTw<R> *pt = nullptr)
 {
     using P = Phase<LOGN, S0, R>;
     const u32 base = P::base(j);
@@ -243,28 +266,31 @@ PZ_DEV void fwd_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
     constexpr int K_LAST =
         MODE == FWD_STRICT ? 2 : (MODE == FWD_LAZY ? 1 : (MODE == FWD_FAST ? 3 : 4));
     constexpr int K_MID = MODE == FWD_FAST ? 3 : (MODE == FWD_FAST4 ? 4 : 0);
-    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM, R>(v, base, STAGE_TB(tb));
+    fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM, R, PRE>(
+        v, base, tb, PRE ? pt->t[0] : nullptr);
     if constexpr (P::NS > 1)
-        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID, SM, R>(
-            v, base, STAGE_TB(tb));
+        fwd_stage<LOGN, S0 + 1, P::ST, NP, (S0 + 1 == LAST) ? K_LAST : K_MID, SM, R, PRE>(
+            v, base, tb, PRE ? pt->t[1] : nullptr);
     if constexpr (P::NS > 2)
-        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID, SM, R>(
-            v, base, STAGE_TB(tb));
+        fwd_stage<LOGN, S0 + 2, P::ST, NP, (S0 + 2 == LAST) ? K_LAST : K_MID, SM, R, PRE>(
+            v, base, tb, PRE ? pt->t[2] : nullptr);
 }
 
 // Inverse phase covering stages S0+NS-1 .. S0 (descending).
-template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB, int R>
-PZ_DEV void inv_phase(u64 (&v)[NP][R], u32 j, const Tables &tb)
+template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB, int R,
+          bool PRE = false>
+PZ_DEV void inv_phase(u64 (&v)[NP][R], u32 j, const Tables &tb, const PhaseTw<R> *pt = nullptr)
 {
     using P = Phase<LOGN, S0, R>;
     const u32 base = P::base(j);
     if constexpr (P::NS > 2)
-        inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R>(v, base,
-                                                                           STAGE_TB(tb));
+        inv_stage<LOGN, S0 + 2, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R, PRE>(
+            v, base, tb, PRE ? pt->t[2] : nullptr);
     if constexpr (P::NS > 1)
-        inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R>(v, base,
-                                                                           STAGE_TB(tb));
-    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R>(v, base, STAGE_TB(tb));
+        inv_stage<LOGN, S0 + 1, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R, PRE>(
+            v, base, tb, PRE ? pt->t[1] : nullptr);
+    inv_stage<LOGN, S0, P::ST, NP, STRICT, SCALE, SM, FAST, IB, R, PRE>(
+        v, base, tb, PRE ? pt->t[0] : nullptr);
 }
 
 // Store the thread's coefficients (layout of phase SA) into LDS, barrier,
@@ -343,14 +369,26 @@ PZ_DEV Tables fresh(const Tables &tb)
 // ([0,2q)), FWD_FAST = unreduced (see above).
 // Phases S0, S0 + log2 R, ... with an LDS exchange between consecutive ones.
 // TAIL = false: exchanges without their second barrier (see exchange).
-template <int LOGN, int S0, int NP, int MODE, bool SM, int R, bool TAIL>
-PZ_DEV void fwd_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
+// PF: each phase's twiddles come from a PhaseTw loaded before the exchange
+// that precedes the phase (the first phase's from the caller's pt).
+template <int LOGN, int S0, int NP, int MODE, bool SM, int R, bool TAIL, bool PF = false>
+PZ_DEV void fwd_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb,
+                     const PhaseTw<R> *pt = nullptr)
 {
     constexpr int LR = RadixLog<R>::value;
-    fwd_phase<LOGN, S0, NP, MODE, SM, R>(v, j, fresh(tb));
+    if constexpr (PF) fwd_phase<LOGN, S0, NP, MODE, SM, R, true>(v, j, tb, pt);
+    else fwd_phase<LOGN, S0, NP, MODE, SM, R>(v, j, fresh(tb));
     if constexpr (S0 + LR < LOGN) {
-        exchange<LOGN, S0, S0 + LR, NP, R, TAIL>(v, j, lds);
-        fwd_from<LOGN, S0 + LR, NP, MODE, SM, R, TAIL>(v, j, lds, tb);
+        if constexpr (PF) {
+            PhaseTw<R> nt;
+            phase_tw<LOGN, S0 + LR, R, false>(nt, j, fresh(tb));
+            __builtin_amdgcn_sched_barrier(0);  // issued ahead of the exchange
+            exchange<LOGN, S0, S0 + LR, NP, R, TAIL>(v, j, lds);
+            fwd_from<LOGN, S0 + LR, NP, MODE, SM, R, TAIL, PF>(v, j, lds, tb, &nt);
+        } else {
+            exchange<LOGN, S0, S0 + LR, NP, R, TAIL>(v, j, lds);
+            fwd_from<LOGN, S0 + LR, NP, MODE, SM, R, TAIL>(v, j, lds, tb);
+        }
     }
 }
 
@@ -376,19 +414,47 @@ PZ_DEV void ntt_fwd(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
     fwd_from<LOGN, 0, NP, MODE, SM, R, TAIL>(v, j, lds, tb);
 }
 
+// The same with every phase's twiddles loaded ahead of its exchange; pt holds
+// the first phase's (fwd_tw_first), which the caller issues early.
+template <int LOGN, int R>
+PZ_DEV void fwd_tw_first(PhaseTw<R> &pt, u32 j, const Tables &tb)
+{
+    phase_tw<LOGN, 0, R, false>(pt, j, fresh(tb));
+}
+template <int LOGN, int NP, int MODE, bool SM = true, int R = 8, bool TAIL = true>
+PZ_DEV void ntt_fwd_pf(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb, const PhaseTw<R> &pt)
+{
+    static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
+    static_assert((MODE != FWD_FAST && MODE != FWD_FAST4) || LOGN <= 11,
+                  "unreduced values must stay below 2^64");
+    static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
+    fwd_from<LOGN, 0, NP, MODE, SM, R, TAIL, true>(v, j, lds, tb, &pt);
+}
+
 // Inverse NTT of NP polynomials: contiguous layout in, stride layout out.
 // STRICT = Ntt::backward ([0,q)), else backward_lazy ([0,2q)); SCALE = false
 // leaves out the n^-1 factor; FAST = unreduced early stages (see inv_stage).
 // Phases S0, S0 - log2 R, ..., 0 with an LDS exchange between consecutive ones.
 template <int LOGN, int S0, int NP, bool STRICT, bool SCALE, bool SM, bool FAST, int IB, int R,
-          bool TAIL>
-PZ_DEV void inv_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
+          bool TAIL, bool PF = false>
+PZ_DEV void inv_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb,
+                     const PhaseTw<R> *pt = nullptr)
 {
     constexpr int LR = RadixLog<R>::value;
-    inv_phase<LOGN, S0, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, fresh(tb));
+    if constexpr (PF) inv_phase<LOGN, S0, NP, STRICT, SCALE, SM, FAST, IB, R, true>(v, j, tb, pt);
+    else inv_phase<LOGN, S0, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, fresh(tb));
     if constexpr (S0 > 0) {
-        exchange<LOGN, S0, S0 - LR, NP, R, TAIL>(v, j, lds);
-        inv_from<LOGN, S0 - LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL>(v, j, lds, tb);
+        if constexpr (PF) {
+            PhaseTw<R> nt;
+            phase_tw<LOGN, S0 - LR, R, true>(nt, j, fresh(tb));
+            __builtin_amdgcn_sched_barrier(0);  // issued ahead of the exchange
+            exchange<LOGN, S0, S0 - LR, NP, R, TAIL>(v, j, lds);
+            inv_from<LOGN, S0 - LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL, PF>(v, j, lds, tb,
+                                                                                 &nt);
+        } else {
+            exchange<LOGN, S0, S0 - LR, NP, R, TAIL>(v, j, lds);
+            inv_from<LOGN, S0 - LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL>(v, j, lds, tb);
+        }
     }
 }
 
@@ -401,6 +467,25 @@ PZ_DEV void ntt_inv(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
     constexpr int LR = RadixLog<R>::value;
     inv_from<LOGN, ((LOGN - 1) / LR) * LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL>(v, j, lds,
                                                                                     tb);
+}
+
+// The inverse with every phase's twiddles loaded ahead of its exchange; pt
+// holds the first phase's (inv_tw_first).
+template <int LOGN, int R>
+PZ_DEV void inv_tw_first(PhaseTw<R> &pt, u32 j, const Tables &tb)
+{
+    constexpr int LR = RadixLog<R>::value;
+    phase_tw<LOGN, ((LOGN - 1) / LR) * LR, R, true>(pt, j, fresh(tb));
+}
+template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool FAST = false,
+          int IB = 2, int R = 8, bool TAIL = true>
+PZ_DEV void ntt_inv_pf(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb, const PhaseTw<R> &pt)
+{
+    static_assert(!FAST || (STRICT && LOGN <= 12), "fast inverse: strict output");
+    static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
+    constexpr int LR = RadixLog<R>::value;
+    inv_from<LOGN, ((LOGN - 1) / LR) * LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL, true>(
+        v, j, lds, tb, &pt);
 }
 
 // Coefficient index of register r in the stride (coefficient-domain) layout
