@@ -278,10 +278,17 @@ PZ_DEV u64 decomp_digit0(const Decomp &dc, u64 v)
 // (negative values as q + value; the value B/2 with an even next digit stays
 // B/2, the value -B/2 carries into q - B/2).  Needs logb + ignore_bits <= 63
 // and ignore_bits > 0 (round_c = 2^(ignore_bits - 1)).
+// LOOSE takes v in [0, 2q] (a representative the reference would hold as
+// v mod q): v_s = v - 2q from q + floor(q/2) on, the reference's rule on
+// v - q there.
+template <bool LOOSE = false>
 PZ_DEV u64 decomp_digit_d1(const Decomp &dc, u64 v, u64 round_c)
 {
     // v_s + 2^(ib-1) in two's complement, v_s = v - q when v >= floor(q/2)
-    const u64 t = v + (v >= (dc.q >> 1) ? round_c - dc.q : round_c);
+    const u64 h = dc.q >> 1;
+    u64 c = v >= h ? round_c - dc.q : round_c;
+    if constexpr (LOOSE) c = v >= dc.q + h ? round_c - 2 * dc.q : c;
+    const u64 t = v + c;
     return dc.q + (u64)((int64_t)t >> dc.ignore_bits);  // floor, then + q
 }
 

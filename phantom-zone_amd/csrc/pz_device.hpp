@@ -63,11 +63,12 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #define PZ_TW_PREFETCH4 5
 #endif
 
-// Split step digits as q + digit (decomp_digit_d1, 8 instructions) instead of
-// the reference's representative in [0, q) (decomp_digit0, 15): the same
+// Split-step digits are q + digit (decomp_digit_d1, 8 instructions) instead
+// of the reference's representative in [0, q) (decomp_digit0, 15): the same
 // residue, for the unreduced transform.
-#ifndef PZ_DIGIT_D1
-#define PZ_DIGIT_D1 1
+// The blind rotation's RLWE stays in [0, 2q) between steps.
+#ifndef PZ_LOOSE_STEPS
+#define PZ_LOOSE_STEPS 1
 #endif
 
 // Everything a kernel needs about the parameter set (passed by value).
@@ -360,14 +361,28 @@ PZ_DEV void ntt_fwd_idle()
 // stores of an automorphism's permutation need a barrier before them.  The
 // other half reads this half's digit transform, so a barrier separates that
 // read from the inverse's first store.
-template <int LOGN, int R>
+//
+// TRIV = false (the blind rotation) runs a trivial RLWE's automorphism as a
+// general one: its a part is zero, so the key products are zero and the result
+// has the same residues, while zero coefficients that the map negates come out
+// as 0 instead of the reference's q (q - 0).  The next RLWE x RGSW step reduces
+// both alike, and a blind rotation always ends in non-trivial steps, so every
+// output word is the same; the loop then has one path.  The batched operator
+// returns the RLWE itself and keeps the exact form (TRIV = true).
+template <int LOGN, int R, bool TRIV = true>
 PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__restrict__ Ks,
-                       bool is_auto, u64 kexp, bool trivial, const DevCtx &c, u32 j, u32 h,
+                       bool is_auto, u64 kexp, bool trivial_in, const DevCtx &c, u32 j, u32 h,
                        u64 *lds, u64 *stash)
 {
+    const bool trivial = TRIV && trivial_in;
     constexpr int N = 1 << LOGN;
     constexpr int W = Geom<LOGN>::LDS_WORDS;
     const u64 q = c.tb.q, q2 = c.tb.q2;
+    // the blind rotation keeps its RLWE in [0, 2q) between steps (the
+    // inverse's last reduction to [0, q) left out); the batched operator
+    // returns the reference's words
+    constexpr bool LOOSE = !TRIV && PZ_LOOSE_STEPS;
+    const u64 qn = LOOSE ? q2 : q;  // negation: qn - v
     const u64 SIGN = 1ull << 63;
     u64 *mine = lds + h * W;
     const u64 *theirs = lds + (h ^ 1) * W;
@@ -387,7 +402,7 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
                 const bool pos = to < (u32)N;
                 to = pos ? to : to - N;
                 const u64 v = p[r];
-                perm[lds_ix(to)] = pos ? v : neg_mod(v, q);
+                perm[lds_ix(to)] = pos ? v : qn - v;
             }
         } else if (move) {
 #pragma unroll
@@ -437,16 +452,14 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     constexpr int PF = R == 8 ? PZ_TW_PREFETCH : PZ_TW_PREFETCH4;
     PhaseTw<R> tw;  // the first phase's twiddles of the forward, then of the inverse
     if (work) {
-        if constexpr (PF & 1) fwd_tw_first<LOGN, R>(tw, j, c.tb);  // overlap the digits
-#if PZ_DIGIT_D1
+        if constexpr (PF & 1) {
+            fwd_tw_first<LOGN, R>(tw, j, c.tb);
+            __builtin_amdgcn_sched_barrier(0);  // issued ahead of the digits
+        }
         // q + digit (below 2q): the unreduced forward transform takes it as is
         const u64 rc = dc.ignore_bits ? 1ull << (dc.ignore_bits - 1) : 0;
 #pragma unroll
-        for (int r = 0; r < R; r++) D[0][r] = decomp_digit_d1(dc, p[r], rc);
-#else
-#pragma unroll
-        for (int r = 0; r < R; r++) D[0][r] = decomp_digit0(dc, p[r]);
-#endif
+        for (int r = 0; r < R; r++) D[0][r] = decomp_digit_d1<LOOSE>(dc, p[r], rc);
         if constexpr (PF & 1)
             ntt_fwd_pf<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(
                 D, j, mine, c.tb, tw);
@@ -456,21 +469,22 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
 #pragma unroll
         for (int r = 0; r < R; r++) mine[lds_ix(eval_x<LOGN, R>(j, r))] = D[0][r];
         // not hoisted into the transform: its twiddle loads would wait behind
-        // the key row on the in-order load counter
+        // the key row on the in-order load counter (issued after the last
+        // phase's twiddles instead, measured no faster)
         __builtin_amdgcn_sched_barrier(0);
         load_slice(k1, K + r1, Ks + r1, j);
     } else {
         load_slice(k1, K + r1, Ks + r1, j);
         ntt_fwd_idle<LOGN, R>();
     }
-    u64 acc[1][R];
-#pragma unroll
-    for (int r = 0; r < R; r++) acc[0][r] = 0;
     __syncthreads();
     if (!work) {
 #pragma unroll
         for (int r = 0; r < R; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN, R>(j, r))];
     }
+    u64 acc[1][R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[0][r] = 0;
     fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k1, D[0], q, q2);
     __builtin_amdgcn_sched_barrier(0);  // one key row in registers at a time
     if (!is_auto) {
@@ -485,20 +499,22 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     constexpr int IBND = Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2);  // accumulators below IBND q
     if constexpr ((PF & 6) == 4) inv_tw_first<LOGN, R>(tw, j, c.tb);
     if constexpr ((PF & 6) != 0)
-        ntt_inv_pf<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, IBND, R, false>(acc, j, mine,
-                                                                                 c.tb, tw);
+        ntt_inv_pf<LOGN, 1, !LOOSE, false, PZ_SPLIT_SM, PZ_INV_FAST, IBND, R, false>(acc, j, mine,
+                                                                                   c.tb, tw);
     else  // n^-1: key rows
-        ntt_inv<LOGN, 1, true, false, PZ_SPLIT_SM, PZ_INV_FAST, IBND, R, false>(acc, j, mine, c.tb);
+        ntt_inv<LOGN, 1, !LOOSE, false, PZ_SPLIT_SM, PZ_INV_FAST, IBND, R, false>(acc, j, mine,
+                                                                                c.tb);
     if (is_auto && h == 1) {
         // acc + (+-v) mod q, branch-free: the negated value q - v is selected,
         // then one conditional subtraction (acc < q, q - v <= q); the same word
-        // as sub_mod / add_mod
+        // as sub_mod / add_mod.  LOOSE: acc and v in [0, 2q), the negation
+        // 2q - v, the result in [0, 2q)
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const u64 bp = stash[lds_ix(coef_x<LOGN, R>(j, r))];
             const u64 v = bp & ~SIGN;
-            const u64 o = acc[0][r] + ((bp & SIGN) ? q - v : v);
-            p[r] = umin(o, o - q);
+            const u64 o = acc[0][r] + ((bp & SIGN) ? qn - v : v);
+            p[r] = umin(o, o - qn);
         }
     } else {
 #pragma unroll

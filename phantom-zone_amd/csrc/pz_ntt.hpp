@@ -194,12 +194,12 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *
                 const u64 xv = v[p][r], yv = v[p][r + LD];
                 const u64 sum = xv + yv;
                 const u64 dif = xv + (tb.q << (K - KR + 1)) - yv;  // inputs below 2^(K-KR+1) q
-                if constexpr (S == 0) {  // the last stage: exact, then [0,2q) -> [0,q)
+                if constexpr (S == 0) {  // the last stage: exact, [0,2q) (-> [0,q) if STRICT)
                     const u64 xo = ONE32 ? shoup_one(sum, (u32)tb.one_s, tb.q)
                                          : shoup_mul<SM>(sum, 1, tb.one_s, tb.q);
                     const u64 yo = shoup_mul<SM>(dif, w, ws, tb.q);
-                    v[p][r] = umin(xo, xo - tb.q);
-                    v[p][r + LD] = umin(yo, yo - tb.q);
+                    v[p][r] = STRICT ? umin(xo, xo - tb.q) : xo;
+                    v[p][r + LD] = STRICT ? umin(yo, yo - tb.q) : yo;
                 } else {
                     v[p][r] = sum;
                     v[p][r + LD] = shoup_mul4(dif, w, ws, tb.q);
@@ -462,7 +462,8 @@ template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool
           int IB = 2, int R = 8, bool TAIL = true>
 PZ_DEV void ntt_inv(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
-    static_assert(!FAST || (STRICT && LOGN <= 12), "fast inverse: strict output");
+    static_assert(!FAST || ((STRICT || (IB == 8 && PZ_INV_TAIL4)) && LOGN <= 12),
+                  "fast inverse: strict output, or [0,2q) from the unreduced tail");
     static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
     constexpr int LR = RadixLog<R>::value;
     inv_from<LOGN, ((LOGN - 1) / LR) * LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL>(v, j, lds,
@@ -481,7 +482,8 @@ template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool
           int IB = 2, int R = 8, bool TAIL = true>
 PZ_DEV void ntt_inv_pf(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb, const PhaseTw<R> &pt)
 {
-    static_assert(!FAST || (STRICT && LOGN <= 12), "fast inverse: strict output");
+    static_assert(!FAST || ((STRICT || (IB == 8 && PZ_INV_TAIL4)) && LOGN <= 12),
+                  "fast inverse: strict output, or [0,2q) from the unreduced tail");
     static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
     constexpr int LR = RadixLog<R>::value;
     inv_from<LOGN, ((LOGN - 1) / LR) * LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL, true>(

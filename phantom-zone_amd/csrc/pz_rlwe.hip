@@ -432,7 +432,7 @@ k_blind_rotate(
             const u32 arg = op & (SCHED_AUTO - 1);
             if constexpr (SPLIT) {
                 const size_t off = is_auto ? arg * as : arg * rs;
-                step_split<LOGN, R>(a, (is_auto ? key.auto_k : key.rgsw) + off,
+                step_split<LOGN, R, false>(a, (is_auto ? key.auto_k : key.rgsw) + off,
                                  (is_auto ? key.auto_s : key.rgsw_s) + off, is_auto,
                                  c.auto_el[is_auto ? arg : 0], trivial, c, j, h, lds, stash);
             } else if constexpr (R == 8) {
@@ -447,7 +447,12 @@ k_blind_rotate(
             trivial = trivial && is_auto;
         }
 
-        // (6) sample_extract at index 0 (src/pbs.rs:423-449)
+        // (6) sample_extract at index 0 (src/pbs.rs:423-449); the split
+        // steps leave the RLWE in [0, 2q) (PZ_LOOSE_STEPS): reduced here
+        if constexpr (SPLIT && PZ_LOOSE_STEPS) {
+#pragma unroll
+            for (int r = 0; r < R; r++) a[r] = umin(a[r], a[r] - q);
+        }
         if (stats && tid == 0) stats[bb] = n_auto;
         u64 *o = out + (out_idx ? (size_t)out_idx[bb] : bb) * (size_t)(N + 1);
         const bool has_a = !SPLIT || h == 0, has_b = !SPLIT || h == 1;
