@@ -32,10 +32,13 @@ for _p in (ROOT, PKG):
 
 # MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.32e12 32-bit VALU lane-ops/s.
 VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9
-# VALU instructions of one 54-bit Shoup multiply in the gfx950 ISA of
-# pz_arith.hpp's two-step form (shoup_quot + shoup_rem), the form the batched
-# blind rotation uses: 6 v_mad_u64_u32, 3 v_mul_lo_u32, 1 v_mul_hi_u32,
-# 2 v_lshl_add_u64, 1 v_add3_u32, 4 v_mov_b32.
+# VALU instructions of one exact 54-bit Shoup multiply in the gfx950 ISA of
+# pz_arith.hpp's two-step form (shoup_quot + shoup_rem) as first built:
+# 6 v_mad_u64_u32, 3 v_mul_lo_u32, 1 v_mul_hi_u32, 2 v_lshl_add_u64,
+# 1 v_add3_u32, 4 v_mov_b32.  The roofline keeps this divisor so fractions
+# compare across builds; the exact form now takes 15 instructions (the cross
+# products chained through v_mad_u64_u32 addends) and the [0,4q) form most
+# of the blind rotation's products use 12.
 VALU_OPS_PER_MODMUL = 17
 HBM_PEAK_GBS = 8000.0
 PLAIN = {0: lambda a, b: 1 - (a & b)}
