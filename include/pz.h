@@ -495,7 +495,10 @@ int pz_set_blind_rotate_mode(pz_ctx *ctx, int mode);
 int pz_last_auto_counts(pz_ctx *ctx, uint32_t *h_counts, size_t B);
 
 /* Per-kernel timing of the last pz_gates / pz_pbs call made with timing
- * enabled (milliseconds from HIP events on the launch stream). */
+ * enabled (milliseconds from HIP events on the launch stream: pz_gates records
+ * them as nodes of its captured graph, around each of its kernels; pz_pbs
+ * around eager launches).  With timing enabled the call waits for its
+ * kernels. */
 int pz_set_timing(pz_ctx *ctx, int enabled);
 int pz_last_timing(pz_ctx *ctx, float *ms_prep, float *ms_ks, float *ms_blind_rotate);
 

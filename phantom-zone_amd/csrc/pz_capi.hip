@@ -53,9 +53,11 @@ int pz_internal_launch_status(const char *fn)
 struct GraphKey {
     const void *ops, *c0, *c1, *out;
     size_t B;
+    bool timed;  // captured with the per-kernel timing events
     bool operator<(const GraphKey &o) const
     {
-        return std::tie(ops, c0, c1, out, B) < std::tie(o.ops, o.c0, o.c1, o.out, o.B);
+        return std::tie(ops, c0, c1, out, B, timed) <
+               std::tie(o.ops, o.c0, o.c1, o.out, o.B, o.timed);
     }
 };
 
@@ -874,19 +876,21 @@ extern "C" int pz_set_blind_rotate_mode(pz_ctx *c, int mode)
 // blind-rotation variant and grid (the autotuner's candidates)
 static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, const uint64_t *c1,
                          const uint32_t *ops, uint64_t *out, size_t B, hipStream_t st,
-                         int force_var = -1, int force_grid = 0, const PbsScratch *sc = nullptr)
+                         int force_var = -1, int force_grid = 0, const PbsScratch *sc = nullptr,
+                         bool events = true)
 {
     const pz_params &p = c->p;
     u32 n1 = p.lwe_n + 1;
     const PbsScratch own{c->d_digits, c->d_ksb, c->d_ks, c->d_stats};
     if (!sc) sc = &own;  // the context's grow-on-demand scratch
-    if (c->timing) hipEventRecord(c->ev[0], st);
+    events = events && c->timing;
+    if (events) hipEventRecord(c->ev[0], st);
     launch_prep(mode, c0, c1, nullptr, nullptr, ops, p.rlwe_n + 1, B, sc->digits, sc->ksb,
                 sc->ks, c->dc, st);
-    if (c->timing) hipEventRecord(c->ev[1], st);
+    if (events) hipEventRecord(c->ev[1], st);
     launch_ks(sc->digits, sc->ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
               k->dk.ldk, 0, B, sc->ks, st);
-    if (c->timing) hipEventRecord(c->ev[2], st);
+    if (events) hipEventRecord(c->ev[2], st);
     int var = BR_ONE, grid = 1;
     if (force_var >= 0) {
         var = force_var;
@@ -896,16 +900,61 @@ static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, con
     }
     launch_blind_rotate(c->logn, var, grid, sc->ks, ops, B, out, sc->stats, c->dc, k->dk,
                         nullptr, nullptr, nullptr, st);
-    if (c->timing) hipEventRecord(c->ev[3], st);
+    if (events) hipEventRecord(c->ev[3], st);
+}
+
+// Timing events as nodes of a captured gate graph: the capture holds the
+// three kernels (prep, key switch, blind rotation) in one chain; an event
+// record node goes in front of the first and after each, and each kernel
+// also waits for the record in front of it.  (Events recorded on a
+// capturing stream cannot be timed.)
+static hipError_t add_timing_nodes(pz_ctx *c, hipGraph_t g)
+{
+    size_t nn = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &nn);
+    if (e != hipSuccess) return e;
+    if (nn != 3) return hipErrorNotSupported;
+    hipGraphNode_t nodes[3], chain[3];
+    e = hipGraphGetNodes(g, nodes, &nn);
+    if (e != hipSuccess) return e;
+    // order the chain: the node that depends on the previous one comes next
+    hipGraphNode_t prev = nullptr;
+    for (int pos = 0; pos < 3; pos++) {
+        chain[pos] = nullptr;
+        for (int i = 0; i < 3; i++) {
+            size_t nd = 0;
+            hipGraphNode_t dep = nullptr;
+            if ((e = hipGraphNodeGetDependencies(nodes[i], nullptr, &nd)) != hipSuccess) return e;
+            if (nd > 1) return hipErrorNotSupported;
+            if (nd == 1 && (e = hipGraphNodeGetDependencies(nodes[i], &dep, &nd)) != hipSuccess)
+                return e;
+            if ((prev == nullptr && nd == 0) || (prev != nullptr && nd == 1 && dep == prev))
+                chain[pos] = nodes[i];
+        }
+        if (!chain[pos]) return hipErrorNotSupported;
+        hipGraphNodeType t;
+        if ((e = hipGraphNodeGetType(chain[pos], &t)) != hipSuccess) return e;
+        if (t != hipGraphNodeTypeKernel) return hipErrorNotSupported;
+        prev = chain[pos];
+    }
+    hipGraphNode_t rec[4];
+    for (int i = 0; i < 4; i++) {
+        const hipGraphNode_t *dep = i ? &chain[i - 1] : nullptr;
+        if ((e = hipGraphAddEventRecordNode(&rec[i], g, dep, i ? 1 : 0, c->ev[i])) != hipSuccess)
+            return e;
+        if (i < 3 && (e = hipGraphAddDependencies(g, &rec[i], &chain[i], 1)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 static int collect_timing(pz_ctx *c, hipStream_t st)
 {
     if (!c->timing) return PZ_OK;
     HIPCHK(hipEventSynchronize(c->ev[3]));
-    hipEventElapsedTime(&c->t_prep, c->ev[0], c->ev[1]);
-    hipEventElapsedTime(&c->t_ks, c->ev[1], c->ev[2]);
-    hipEventElapsedTime(&c->t_br, c->ev[2], c->ev[3]);
+    HIPCHK(hipEventElapsedTime(&c->t_prep, c->ev[0], c->ev[1]));
+    HIPCHK(hipEventElapsedTime(&c->t_ks, c->ev[1], c->ev[2]));
+    HIPCHK(hipEventElapsedTime(&c->t_br, c->ev[2], c->ev[3]));
     (void)st;
     return PZ_OK;
 }
@@ -1014,24 +1063,24 @@ extern "C" int pz_gates(pz_ctx *c, pz_key *k, const uint32_t *ops, const uint64_
     int rc = ensure_scratch(c, B);
     if (rc) return rc;
     hipStream_t st = S(c, s);
-    if (c->timing) {
-        // eager launches so the per-kernel events bracket each kernel
-        pbs_sequence(c, k, PREP_GATE, c0, c1, ops, out, B, st);
-        HIPCHK(hipGetLastError());
-        return collect_timing(c, st);
-    }
     if (c->graph_key_id != k->id) {
         drop_graphs(c);
         c->graph_key_id = k->id;
     }
-    GraphKey gk{ops, c0, c1, out, B};
+    // with timing on, the graph carries event-record nodes around each kernel,
+    // so the per-kernel times come from the same replays as the batch's
+    GraphKey gk{ops, c0, c1, out, B, c->timing};
     auto it = c->graphs.find(gk);
     if (it == c->graphs.end()) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        pbs_sequence(c, k, PREP_GATE, c0, c1, ops, out, B, c->stream);
+        pbs_sequence(c, k, PREP_GATE, c0, c1, ops, out, B, c->stream, -1, 0, nullptr, false);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (e != hipSuccess) return fail(PZ_ERR_HIP, hipGetErrorString(e));
+        if (c->timing && (e = add_timing_nodes(c, g)) != hipSuccess) {
+            hipGraphDestroy(g);
+            return fail(PZ_ERR_HIP, (std::string("timing nodes: ") + hipGetErrorString(e)).c_str());
+        }
         hipGraphExec_t ge = nullptr;
         e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
@@ -1040,7 +1089,7 @@ extern "C" int pz_gates(pz_ctx *c, pz_key *k, const uint32_t *ops, const uint64_
         it = c->graphs.emplace(gk, ge).first;
     }
     HIPCHK(hipGraphLaunch(it->second, st));
-    return PZ_OK;
+    return collect_timing(c, st);
 }
 
 // pz_gates without any allocation: the pipeline scratch is this device's

@@ -246,3 +246,39 @@ def test_circuit_read_refuses_wires_added_after_run(env):
     circ.run(k, d_in)
     assert ctx.decrypt_bools(s, circ.read([extra[-1]]))[0] == 1  # NAND(1,1) xor 1
     circ.close()
+
+
+def test_timed_gates_equal_untimed(env):
+    """pz_gates with timing on replays a graph that carries HIP events around
+    each kernel: the outputs equal the untimed replay's (and the oracle's),
+    every kernel time is positive, and the three add up to less than the
+    call's own wall time."""
+    import time
+    pzlib_, ctx, k, s = env
+    log("edges: timed graph replay")
+    o, _, _, key = keys_for("NI_2P")
+    r = np.random.default_rng(43)
+    B = 64
+    b0 = r.integers(0, 2, B).astype(np.uint8)
+    b1 = r.integers(0, 2, B).astype(np.uint8)
+    c0, c1 = ctx.encrypt_bools(s, b0, 630), ctx.encrypt_bools(s, b1, 640)
+    d0, d1 = ctx.to_device(c0), ctx.to_device(c1)
+    ops = (np.arange(B) % 6).astype(np.uint32)
+    dops, dout = ctx.to_device(ops), ctx.array((B, ctx.N + 1))
+    ctx.gates(k, dops, d0, d1, dout, B)
+    untimed = dout.download()
+    ctx.set_timing(True)
+    try:
+        for rep in range(3):  # the first call captures the timed graph
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.gates(k, dops, d0, d1, dout, B)
+            wall_ms = (time.perf_counter() - t0) * 1e3
+            tim = ctx.last_timing()
+            assert all(v > 0 for v in tim.values()), tim
+            assert sum(tim.values()) <= wall_ms, (tim, wall_ms)
+            assert np.array_equal(dout.download(), untimed), rep
+    finally:
+        ctx.set_timing(False)
+    for i in range(0, B, 16):
+        assert np.array_equal(untimed[i], o.gate(key, int(ops[i]), c0[i], c1[i])), i
