@@ -111,9 +111,10 @@ static std::atomic<uint64_t> g_next_key_id{1};
 struct pz_key {
     pz_ctx *owner = nullptr;
     uint64_t id = g_next_key_id.fetch_add(1);  // unique for the process's lifetime
-    DevKey dk;
+    DevKey dk{};
     u64 *rgsw = nullptr, *rgsw_s = nullptr, *auto_k = nullptr, *auto_s = nullptr;
     void *ksk = nullptr;
+    int8_t *ksk_planes = nullptr;
     size_t bytes = 0;
 };
 
@@ -527,7 +528,7 @@ size_t pz_internal_key_footprint(const pz_ctx *c)
     const size_t rgsw_words = (size_t)p.lwe_n * (2 * (size_t)p.rlrg_da + 2 * (size_t)p.rlrg_db) * N;
     const size_t auto_words = (size_t)(p.w + 1) * 2 * (size_t)p.auto_d * N;
     const size_t kb = p.lwe_q <= 65536 ? 2 : 4, ldk = ((size_t)p.lwe_n + 1 + 63) / 64 * 64;
-    return 16 * rgsw_words + 16 * auto_words + kb * N * p.lwe_d * ldk;
+    return 16 * rgsw_words + 16 * auto_words + (kb + (PZ_KS_MFMA ? KS_PLANES : 0)) * N * p.lwe_d * ldk;
 }
 
 int pz_internal_ctx_device(const pz_ctx *c) { return c->device; }
@@ -550,7 +551,8 @@ int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows
     k->owner = c;
     bool ok = hipMalloc(&k->rgsw_s, 8 * rgsw_words) == hipSuccess &&
               hipMalloc(&k->auto_s, 8 * auto_words) == hipSuccess &&
-              hipMalloc(&k->ksk, kb * ksk_dev) == hipSuccess;
+              hipMalloc(&k->ksk, kb * ksk_dev) == hipSuccess &&
+              (!PZ_KS_MFMA || hipMalloc(&k->ksk_planes, KS_PLANES * ksk_dev) == hipSuccess);
     if (!ok) {
         pz_key_free(k);
         return fail(PZ_ERR_HIP, "key allocation failed");
@@ -562,6 +564,8 @@ int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows
     launch_shoup(rgsw, k->rgsw_s, rgsw_words, p.rlwe_q, s);
     launch_shoup(auto_k, k->auto_s, auto_words, p.rlwe_q, s);
     launch_pack_ksk(ksk_rows, k->ksk, (size_t)N * p.lwe_d, n1, ldk, kb, s);
+    if (k->ksk_planes)
+        launch_pack_ksk_planes(ksk_rows, k->ksk_planes, (size_t)N * p.lwe_d, n1, ldk, s);
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         pz_key_free(k);
@@ -574,11 +578,13 @@ int pz_internal_key_adopt(pz_ctx *c, u64 *rgsw, u64 *auto_k, const u64 *ksk_rows
     k->dk.auto_k = k->auto_k;
     k->dk.auto_s = k->auto_s;
     k->dk.ksk = k->ksk;
+    k->dk.ksk_planes = k->ksk_planes;
     k->dk.rgsw_rows = (u32)rgsw_rows;
     k->dk.auto_rows = (u32)auto_rows;
     k->dk.ksk_bytes = kb;
     k->dk.ldk = ldk;
-    k->bytes = 16 * rgsw_words + 16 * auto_words + kb * ksk_dev;
+    k->bytes = 16 * rgsw_words + 16 * auto_words + kb * ksk_dev +
+               (k->ksk_planes ? KS_PLANES * ksk_dev : 0);
     // pz_internal_key_footprint states the same sum
     *out = k;
     return PZ_OK;
@@ -630,6 +636,7 @@ extern "C" int pz_key_free(pz_key *k)
     hipFree(k->auto_k);
     hipFree(k->auto_s);
     hipFree(k->ksk);
+    hipFree(k->ksk_planes);
     delete k;
     return PZ_OK;
 }
@@ -806,10 +813,11 @@ extern "C" int pz_lwe_key_switch(pz_ctx *c, pz_key *k, const uint64_t *in, uint6
     hipStream_t st = S(c, s);
     const pz_params &p = c->p;
     u32 n1 = p.lwe_n + 1;
+    const bool rm = ks_rowmajor(k->dk);  // the MFMA key switch's digit layout
     launch_prep(PREP_KSONLY, in, nullptr, nullptr, nullptr, nullptr, p.rlwe_n + 1, B,
-                c->d_digits, c->d_ksb, c->d_ks, c->dc, st);
+                c->d_digits, c->d_ksb, c->d_ks, c->dc, st, rm);
     launch_ks(c->d_digits, c->d_ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
-              k->dk.ldk, 0, B, c->d_ks, st);
+              k->dk.ldk, 0, B, c->d_ks, st, rm ? k->dk.ksk_planes : nullptr);
     launch_widen(c->d_ks, out, B * n1, (u32)(p.lwe_q - 1), st);
     HIPCHK(hipGetLastError());
     return PZ_OK;
@@ -884,12 +892,13 @@ static void pbs_sequence(pz_ctx *c, pz_key *k, int mode, const uint64_t *c0, con
     const PbsScratch own{c->d_digits, c->d_ksb, c->d_ks, c->d_stats};
     if (!sc) sc = &own;  // the context's grow-on-demand scratch
     events = events && c->timing;
+    const bool rm = ks_rowmajor(k->dk);  // the MFMA key switch's digit layout
     if (events) hipEventRecord(c->ev[0], st);
     launch_prep(mode, c0, c1, nullptr, nullptr, ops, p.rlwe_n + 1, B, sc->digits, sc->ksb,
-                sc->ks, c->dc, st);
+                sc->ks, c->dc, st, rm);
     if (events) hipEventRecord(c->ev[1], st);
     launch_ks(sc->digits, sc->ksb, k->dk.ksk, k->dk.ksk_bytes, p.rlwe_n * p.lwe_d, n1,
-              k->dk.ldk, 0, B, sc->ks, st);
+              k->dk.ldk, 0, B, sc->ks, st, rm ? k->dk.ksk_planes : nullptr);
     if (events) hipEventRecord(c->ev[2], st);
     int var = BR_ONE, grid = 1;
     if (force_var >= 0) {

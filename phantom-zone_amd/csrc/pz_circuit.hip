@@ -214,15 +214,18 @@ int exec_batch(pz_ctx *c, const std::vector<const pz_key *> &keys, std::vector<G
                        hipMemcpyHostToDevice, st) != hipSuccess)
         return PZ_FAIL(PZ_ERR_HIP, "hipMemcpyAsync failed");
     if (B) {
-        launch_prep(PREP_GATE, pool, pool, d + B, d + 2 * B, d, N1, B, v.digits, v.ksb, v.ks, dc,
-                    st);
         const DevKey *k0 = pz_internal_devkey(keys[pbs[0].key]);
+        // one key: the MFMA key switch when the key has its planes; mixed keys:
+        // the per-tile VALU kernel (tile-major digits)
+        const bool rm = keys.size() == 1 && ks_rowmajor(*k0);
+        launch_prep(PREP_GATE, pool, pool, d + B, d + 2 * B, d, N1, B, v.digits, v.ksb, v.ks, dc,
+                    st, rm);
         if (keys.size() > 1)
             launch_ks_multi(v.digits, v.ksb, g_stage.dk, d + 5 * B + 2 * nn, nw / 4,
                             k0->ksk_bytes, v.p.rlwe_n * v.p.lwe_d, n1, k0->ldk, v.ks, st);
         else
             launch_ks(v.digits, v.ksb, k0->ksk, k0->ksk_bytes, v.p.rlwe_n * v.p.lwe_d, n1,
-                      k0->ldk, 0, B, v.ks, st);
+                      k0->ldk, 0, B, v.ks, st, rm ? k0->ksk_planes : nullptr);
         int var = BR_ONE, grid = 1;
         pz_internal_br_plan(c, B, &var, &grid);
         launch_blind_rotate(v.logn, var, grid, v.ks, d, B, pool, v.stats, dc, *k0,
@@ -892,10 +895,11 @@ extern "C" int pz_circuit_run(pz_circuit *c, pz_key *key, const uint64_t *d_inpu
         const u32 *d = c->d_plan + lp.off;
         const size_t B = lp.B;
         if (B) {
+            const bool rm = ks_rowmajor(*k0);  // the MFMA key switch's digit layout
             launch_prep(PREP_GATE, c->pool, c->pool, d + B, d + 2 * B, d, N1, B, v.digits, v.ksb,
-                        v.ks, dc, st);
+                        v.ks, dc, st, rm);
             launch_ks(v.digits, v.ksb, k0->ksk, k0->ksk_bytes, v.p.rlwe_n * v.p.lwe_d, n1, k0->ldk,
-                      0, B, v.ks, st);
+                      0, B, v.ks, st, rm ? k0->ksk_planes : nullptr);
             int var = BR_ONE, grid = 1;
             pz_internal_br_plan(c->ctx, B, &var, &grid);
             launch_blind_rotate(v.logn, var, grid, v.ks, d, B, c->pool, v.stats, dc, *k0, nullptr,

@@ -95,8 +95,13 @@ struct DevKey {
     const u64 *rgsw, *rgsw_s;  // [n][rgsw_rows][N]
     const u64 *auto_k, *auto_s;  // [w+1][auto_rows][N]
     const void *ksk;           // [N*d][ldk], u16 or u32, columns n+1..ldk-1 zero
+    // the KSK as KS_PLANES 7-bit planes for the MFMA key switch,
+    // [plane][ldk][N*d] int8 (column-major: one column's rows contiguous), or null
+    const int8_t *ksk_planes;
     u32 rgsw_rows, auto_rows, ksk_bytes, ldk;
 };
+// 7-bit planes of a KSK word (q_ks <= 2^21)
+constexpr int KS_PLANES = 3;
 
 // A DevKey read from memory is the same for the whole work group: keep its
 // fields in scalar registers.
@@ -115,6 +120,7 @@ PZ_DEV DevKey uniform_key(const DevKey &k)
     u.auto_k = ufl_ptr(k.auto_k);
     u.auto_s = ufl_ptr(k.auto_s);
     u.ksk = ufl_ptr(k.ksk);
+    u.ksk_planes = ufl_ptr(k.ksk_planes);
     u.rgsw_rows = ufl(k.rgsw_rows);
     u.auto_rows = ufl(k.auto_rows);
     u.ksk_bytes = ufl(k.ksk_bytes);

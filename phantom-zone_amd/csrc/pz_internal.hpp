@@ -37,11 +37,21 @@ int step_split_per_cu(int logn);
 // order[] = the batch's positions sorted by key index (idx >= nkeys last),
 // one work group; false (nothing launched) for more than 8191 keys
 bool launch_key_order(const u32 *idx, size_t B, u32 nkeys, u32 *order, hipStream_t s);
+// rowmajor: digits as [ciphertext][row] (the MFMA key switch's layout)
+// instead of tile-major [B / KS_CT][row][KS_CT]
 void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const u32 *idx1,
                  const u32 *ops, u32 N1, size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out,
-                 const DevCtx &c, hipStream_t s);
+                 const DevCtx &c, hipStream_t s, bool rowmajor = false);
+// With planes (the key's ksk_planes) the MFMA key switch, which takes
+// row-major digits (ks_rowmajor); otherwise the VALU kernel on tile-major ones.
 void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_bytes, u32 rows,
-               u32 n1, u32 ldk, size_t seg_begin, size_t seg_end, u32 *out, hipStream_t s);
+               u32 n1, u32 ldk, size_t seg_begin, size_t seg_end, u32 *out, hipStream_t s,
+               const int8_t *planes = nullptr);
+#ifndef PZ_KS_MFMA
+#define PZ_KS_MFMA 1
+#endif
+// the digit layout a single-key batch with this key uses
+inline bool ks_rowmajor(const DevKey &k) { return PZ_KS_MFMA && k.ksk_planes != nullptr; }
 size_t br_lds_bytes(int logn, bool split, u32 br_q, u32 n_lwe);
 // blind-rotation kernel variants: one work group per bootstrap, or two half
 // groups (d = 1) with the register budget of 2 (latency) or 4 waves per SIMD
@@ -75,6 +85,9 @@ void launch_not(u64 *c0, size_t n, u64 q, hipStream_t s);
 void launch_widen(const u32 *in, u64 *out, size_t n, u32 mask, hipStream_t s);
 void launch_pack_ksk(const u64 *in, void *out, size_t rows, u32 n1, u32 ldk, u32 bytes,
                      hipStream_t s);
+// KSK rows [rows][n1] u64 -> KS_PLANES 7-bit planes [plane][ldk][rows] int8
+void launch_pack_ksk_planes(const u64 *in, int8_t *out, size_t rows, u32 n1, u32 ldk,
+                            hipStream_t s);
 
 }  // namespace pz
 

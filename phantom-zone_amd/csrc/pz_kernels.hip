@@ -108,7 +108,7 @@ PZ_DEV u64 prep_value(int mode, const u64 *c0, const u64 *c1, size_t r0, size_t 
 __global__ __launch_bounds__(256) void k_prep(int mode, const u64 *c0, const u64 *c1,
                                               const u32 *idx0, const u32 *idx1, const u32 *ops,
                                               u32 N1, size_t B, int8_t *digits, u32 *ks_b,
-                                              u32 *ks_out, DevCtx c)
+                                              u32 *ks_out, DevCtx c, bool rowmajor)
 {
     constexpr int CW = KS_CT / 4;  // u32 words of one digit row of a tile
     __shared__ u32 sd[PREP_COLS * PREP_MAXD * CW];
@@ -161,8 +161,32 @@ __global__ __launch_bounds__(256) void k_prep(int mode, const u64 *c0, const u64
         }
     }
     __syncthreads();
-    // the group's columns are one contiguous run of the tile
     const size_t rows = (size_t)(N1 - 1) * d;
+    if (rowmajor) {
+        // [ciphertext][row]: a ciphertext's digits of this group's columns are one
+        // run of PREP_COLS * d bytes from row cb * d; each item gathers 16
+        // consecutive rows of one ciphertext from the LDS words (byte ct % 4 of
+        // word ct / 4 of each row).  The buffer holds whole tiles of KS_CT, and
+        // ciphertexts past B carry zero digits.
+        const u32 chunks = PREP_COLS * d / 16;
+        for (u32 e = tid; e < KS_CT * chunks; e += blockDim.x) {
+            const u32 ct = e / chunks, pos0 = (e % chunks) * 16;
+            const u32 sh = 8 * (ct % 4);
+            u32 wv[4];
+#pragma unroll
+            for (int q4 = 0; q4 < 4; q4++) {
+                u32 word = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    word |= ((sd[(pos0 + 4 * q4 + i) * CW + ct / 4] >> sh) & 0xffu) << (8 * i);
+                wv[q4] = word;
+            }
+            *reinterpret_cast<uint4 *>(digits + (b0 + ct) * rows + (size_t)cb * d + pos0) =
+                make_uint4(wv[0], wv[1], wv[2], wv[3]);
+        }
+        return;
+    }
+    // the group's columns are one contiguous run of the tile
     uint4 *dst = reinterpret_cast<uint4 *>(digits + (b0 / KS_CT) * rows * KS_CT +
                                            (size_t)cb * d * KS_CT);
     const uint4 *src = reinterpret_cast<const uint4 *>(sd);
@@ -300,6 +324,146 @@ __global__ __launch_bounds__(1024) void k_ks(const int8_t *digits, const u32 *ks
             if (unit == 0 && blockIdx.y == 0) v += ks_b[bb];
             atomicAdd(o + unit, v);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_ks_mfma: the LWE key switch (src/lwe.rs:14-44) as an int8 matrix product on
+// the matrix cores.  out[b][col] += sum_row digit[b][row] * ksk[row][col] is an
+// exact integer contraction mod q_ks: the digits are signed and small
+// (|digit| <= 2^(logb-1) <= 8) and each KSK word is split into KS_PLANES 7-bit
+// planes, so every product of v_mfma_i32_16x16x64_i8 is exact and every int32
+// sum stays below 2^31 (rows * 8 * 127 < 2^31 for rows <= 2^21); the planes
+// recombine as P0 + P1 2^7 + P2 2^14 in u32 arithmetic, whose wrap is the
+// reference's mask arithmetic mod q_ks (q_ks divides 2^32).
+// Operands: digits row-major [ciphertext][row] (k_prep, rowmajor), planes
+// [plane][ldk][row] (launch_pack_ksk_planes); both keep 16 consecutive rows of
+// one ciphertext / column together, the operand a lane of the 16x16x64 form
+// holds (lane l: ciphertext or column l % 16, rows 16 (l / 16) .. + 16; output
+// slot i of lane l: ciphertext 4 (l / 16) + i, column l % 16).
+// Work group: KSM_WM x 2 waves, each 32 ciphertexts x 48 columns; a work group
+// covers KSM_WM * 32 ciphertexts x 96 columns over one row range (blockIdx.z;
+// about PZ_KSM_GROUPS work groups per launch).  Measured (one box): 4096 NI_2P
+// ciphertexts 0.32 ms (8 waves x 2: 0.33; 256 or 1024 groups: 0.38 / 0.34), 512
+// NI_8P ciphertexts 0.08 ms, against 1.63 / 0.66 ms for k_ks;
+// each 64-row block of both operands is staged through LDS (rows padded to 80
+// bytes: the 16 lanes of a b128 read start in distinct bank groups) and the
+// next block's loads are issued before the current block's products.  Row
+// ranges meet in u32 atomics on the zeroed output, like k_ks.
+// ---------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+#ifndef PZ_KSM_GROUPS
+#define PZ_KSM_GROUPS 512
+#endif
+#ifndef PZ_KSM_WM
+#define PZ_KSM_WM 4
+#endif
+constexpr int KSM_WM = PZ_KSM_WM;         // waves along the ciphertexts
+constexpr int KSM_TM = 32 * KSM_WM;       // ciphertexts per work group
+constexpr int KSM_TN = 96;                // columns per work group
+constexpr int KSM_KB = 64;                // rows per block
+constexpr int KSM_LD = 80;                // LDS pitch of one ciphertext / column
+constexpr int KSM_THREADS = 128 * KSM_WM;
+constexpr int KSM_BCH = KS_PLANES * KSM_TN * (KSM_KB / 16);  // 16-byte chunks of B per block
+constexpr int KSM_BPT = (KSM_BCH + KSM_THREADS - 1) / KSM_THREADS;
+
+__global__ __launch_bounds__(KSM_THREADS) void k_ks_mfma(const int8_t *digits, const u32 *ks_b,
+                                                       const int8_t *planes, u32 rows, u32 n1,
+                                                       u32 ldk, size_t seg_begin, size_t B,
+                                                       size_t Bpad, u32 kchunk, u32 *out)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sa[KSM_TM * KSM_LD];
+    __shared__ __attribute__((aligned(16))) int8_t sb[KS_PLANES * KSM_TN * KSM_LD];
+    const u32 tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const u32 wm = w % KSM_WM, wn = w / KSM_WM;
+    const size_t ct0 = (size_t)blockIdx.x * KSM_TM;
+    const u32 col0 = blockIdx.y * KSM_TN;
+    const u32 kbeg = blockIdx.z * kchunk;
+    const u32 kend = kbeg + kchunk < rows ? kbeg + kchunk : rows;
+    // staging: A chunk (ciphertext tid / 4, 16 rows at 16 (tid % 4)); B chunks e
+    const size_t a_ct = ct0 + tid / 4;
+    const bool a_ok = a_ct < Bpad;
+    const int8_t *a_src = digits + (a_ct < Bpad ? a_ct : 0) * rows + 16 * (tid % 4);
+    v4i ra, rb[KSM_BPT];
+    const auto load = [&](u32 k0) {
+        ra = a_ok ? *reinterpret_cast<const v4i *>(a_src + k0) : v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < KSM_BPT; i++) {
+            const u32 e = tid + i * KSM_THREADS;
+            const u32 pl = e / (KSM_TN * 4), rem = e % (KSM_TN * 4), col = col0 + rem / 4;
+            rb[i] = (e < (u32)KSM_BCH && col < ldk)
+                        ? *reinterpret_cast<const v4i *>(planes + ((size_t)pl * ldk + col) * rows +
+                                                         k0 + 16 * (rem % 4))
+                        : v4i{0, 0, 0, 0};
+        }
+    };
+    v4i acc[KS_PLANES][2][3];
+#pragma unroll
+    for (int p = 0; p < KS_PLANES; p++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc[p][i][j] = v4i{0, 0, 0, 0};
+    if (kbeg < kend) load(kbeg);
+    for (u32 k0 = kbeg; k0 < kend; k0 += KSM_KB) {
+        __syncthreads();  // the previous block's fragment reads are done
+        *reinterpret_cast<v4i *>(sa + (tid / 4) * KSM_LD + 16 * (tid % 4)) = ra;
+#pragma unroll
+        for (int i = 0; i < KSM_BPT; i++) {
+            const u32 e = tid + i * KSM_THREADS;
+            if (e < (u32)KSM_BCH) {
+                const u32 pl = e / (KSM_TN * 4), rem = e % (KSM_TN * 4);
+                *reinterpret_cast<v4i *>(sb + (pl * KSM_TN + rem / 4) * KSM_LD + 16 * (rem % 4)) =
+                    rb[i];
+            }
+        }
+        __syncthreads();
+        if (k0 + KSM_KB < kend) load(k0 + KSM_KB);  // in flight during the products
+        v4i af[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            af[i] = *reinterpret_cast<const v4i *>(sa + (32 * wm + 16 * i + lane % 16) * KSM_LD +
+                                                   16 * (lane / 16));
+#pragma unroll
+        for (int p = 0; p < KS_PLANES; p++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const v4i bf = *reinterpret_cast<const v4i *>(
+                    sb + (p * KSM_TN + 48 * wn + 16 * j + lane % 16) * KSM_LD + 16 * (lane / 16));
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    acc[p][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf, acc[p][i][j],
+                                                                         0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++) {
+                const size_t ct = ct0 + 32 * wm + 16 * i + 4 * (lane / 16) + sl;
+                const u32 col = col0 + 48 * wn + 16 * j + lane % 16;
+                if (ct < seg_begin || ct >= B || col >= n1) continue;
+                u32 v = 0;
+#pragma unroll
+                for (int p = 0; p < KS_PLANES; p++) v += (u32)acc[p][i][j][sl] << (7 * p);
+                if (col == 0 && blockIdx.z == 0) v += ks_b[ct];
+                atomicAdd(out + ct * n1 + col, v);
+            }
+}
+
+// KSK rows [rows][n1] u64 (below q_ks <= 2^21) -> 7-bit planes [plane][ldk][rows]
+__global__ void k_pack_ksk_planes(const u64 *in, int8_t *out, size_t rows, u32 n1, u32 ldk)
+{
+    const size_t n = (size_t)ldk * rows;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const u32 col = (u32)(i / rows);
+        const size_t r = i % rows;
+        const u64 v = col < n1 ? in[r * n1 + col] : 0;
+#pragma unroll
+        for (int p = 0; p < KS_PLANES; p++) out[(size_t)p * n + i] = (int8_t)((v >> (7 * p)) & 127u);
     }
 }
 
@@ -470,19 +634,36 @@ void launch_decompose(const u64 *in, u64 *digits, size_t count, const Decomp &dc
 
 void launch_prep(int mode, const u64 *c0, const u64 *c1, const u32 *idx0, const u32 *idx1,
                  const u32 *ops, u32 N1, size_t B, int8_t *digits, u32 *ks_b, u32 *ks_out,
-                 const DevCtx &c, hipStream_t s)
+                 const DevCtx &c, hipStream_t s, bool rowmajor)
 {
     if (B == 0) return;
     // (N1 - 1) is a multiple of PREP_COLS for every supported ring size
     dim3 grid((unsigned)((B + KS_CT - 1) / KS_CT), (N1 - 1) / PREP_COLS);
     hipLaunchKernelGGL(k_prep, grid, 256, 0, s, mode, c0, c1, idx0, idx1, ops, N1, B, digits,
-                       ks_b, ks_out, c);
+                       ks_b, ks_out, c, rowmajor);
 }
 
 void launch_ks(const int8_t *digits, const u32 *ks_b, const void *ksk, u32 ksk_bytes, u32 rows,
-               u32 n1, u32 ldk, size_t seg_begin, size_t seg_end, u32 *out, hipStream_t s)
+               u32 n1, u32 ldk, size_t seg_begin, size_t seg_end, u32 *out, hipStream_t s,
+               const int8_t *planes)
 {
     if (seg_end <= seg_begin) return;
+    if (PZ_KS_MFMA && planes) {
+        // the grid covers ciphertexts [0, seg_end); outputs below seg_begin are
+        // left alone.  rows is a multiple of KSM_KB (N * d with N >= 512).
+        const size_t B = seg_end, Bpad = (B + KS_CT - 1) / KS_CT * KS_CT;
+        const u32 gx = (u32)((Bpad + KSM_TM - 1) / KSM_TM), gy = (ldk + KSM_TN - 1) / KSM_TN;
+        // row ranges so that the launch has about PZ_KSM_GROUPS work groups
+        u32 split = (PZ_KSM_GROUPS + gx * gy - 1) / (gx * gy);
+        const u32 blocks = rows / KSM_KB;
+        if (split > blocks) split = blocks;
+        if (split < 1) split = 1;
+        const u32 kchunk = (blocks + split - 1) / split * KSM_KB;
+        split = (rows + kchunk - 1) / kchunk;
+        hipLaunchKernelGGL(k_ks_mfma, dim3(gx, gy, split), KSM_THREADS, 0, s, digits, ks_b, planes,
+                           rows, n1, ldk, seg_begin, B, Bpad, kchunk, out);
+        return;
+    }
     const bool pack = ksk_bytes == 2;
     u32 units = pack ? (n1 + 1) / 2 : n1;
     u32 tpb = (units + 63) / 64 * 64;
@@ -547,6 +728,13 @@ void launch_not(u64 *c0, size_t n, u64 q, hipStream_t s)
 void launch_widen(const u32 *in, u64 *out, size_t n, u32 mask, hipStream_t s)
 {
     hipLaunchKernelGGL(k_widen, grid_for(n, 256), 256, 0, s, in, out, n, mask);
+}
+
+void launch_pack_ksk_planes(const u64 *in, int8_t *out, size_t rows, u32 n1, u32 ldk,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_ksk_planes, grid_for((size_t)ldk * rows, 256), 256, 0, s, in, out,
+                       rows, n1, ldk);
 }
 
 void launch_pack_ksk(const u64 *in, void *out, size_t rows, u32 n1, u32 ldk, u32 bytes,
