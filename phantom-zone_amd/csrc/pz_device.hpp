@@ -67,6 +67,10 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 // of the reference's representative in [0, q) (decomp_digit0, 15): the same
 // residue, for the unreduced transform.
 // The blind rotation's RLWE stays in [0, 2q) between steps.
+// Wave priority of an automorphism's forward transform (0: none)
+#ifndef PZ_AUTO_PRIO
+#define PZ_AUTO_PRIO 3
+#endif
 #ifndef PZ_LOOSE_STEPS
 #define PZ_LOOSE_STEPS 1
 #endif
@@ -458,6 +462,12 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     constexpr int PF = R == 8 ? PZ_TW_PREFETCH : PZ_TW_PREFETCH4;
     PhaseTw<R> tw;  // the first phase's twiddles of the forward, then of the inverse
     if (work) {
+        // an automorphism's transform is the step's critical path (half 1 waits
+        // for it): with two work groups per CU (8 coefficients per thread) its
+        // waves issue ahead of the other work group's (117.4 against 118.5 ms
+        // at B = 4096; one bootstrap per CU gains nothing)
+        constexpr bool PRIO = PZ_AUTO_PRIO && R == 8;
+        if (PRIO && is_auto) __builtin_amdgcn_s_setprio(PZ_AUTO_PRIO);
         if constexpr (PF & 1) {
             fwd_tw_first<LOGN, R>(tw, j, c.tb);
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the digits
@@ -479,6 +489,7 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
         // phase's twiddles instead, measured no faster)
         __builtin_amdgcn_sched_barrier(0);
         load_slice(k1, K + r1, Ks + r1, j);
+        if (PRIO && is_auto) __builtin_amdgcn_s_setprio(0);
     } else {
         load_slice(k1, K + r1, Ks + r1, j);
         ntt_fwd_idle<LOGN, R>();
