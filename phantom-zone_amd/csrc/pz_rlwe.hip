@@ -281,6 +281,122 @@ void launch_rgsw_rows(int logn, u64 *rgsw0, u32 r0, u32 da0, u32 db0, size_t B, 
 }
 
 // ---------------------------------------------------------------------------
+// The blind rotation's op list (src/pbs.rs:240-414), built by the whole work
+// group.  The reference walks the dlog buckets k = nb-1 .. 0 (its -(g^k)
+// buckets, then its +(g^k) ones: k >= br_q/4 is the first half) and emits each
+// bucket's LWE indices in ascending order; within a half, a pending
+// automorphism fires after bucket k (as SCHED_AUTO | v, v the buckets walked
+// since the last one) when bucket k-1 is non-empty, when v reaches w, or at
+// the half's bucket 1; the first half ends with its bucket br_q/4 and the -g
+// automorphism (v = 0).  Whether one fires after bucket k is a function of the
+// bucket sizes alone: with d the distance from k to the half's last natural
+// firing point above it (one past the next non-empty bucket at or above k,
+// below the half's top bucket; the half's top + 1 if none), v = (d-1) mod w + 1
+// and a forced firing happens where d mod w = 0.  Each bucket's op count is
+// then known, and one exclusive scan over them places every op.
+// ---------------------------------------------------------------------------
+
+// 0 when no automorphism follows bucket k, else 1 + v
+PZ_DEV u32 br_auto_after(u32 k, u32 qb4, u32 w, const u32 *cnt, const u64 *occ)
+{
+    const u32 base = k >= qb4 ? qb4 : 0, i = k - base;
+    if (i == 0) return base ? 1u : 0u;
+    const bool nat = i == 1 || cnt[k - 1] != 0;
+    const u32 top = base + qb4 - 1;  // non-empty buckets searched in [k, top)
+    u32 d = top + 1 - k;
+    u32 kk = k;
+    u64 m = occ[k / 64] >> (k % 64);
+    for (;;) {
+        if (m) {
+            const u32 nz = kk + (u32)__builtin_ctzll(m);
+            if (nz < top) d = nz + 1 - k;
+            break;
+        }
+        kk = (kk / 64 + 1) * 64;
+        if (kk >= top) break;
+        m = occ[kk / 64];
+    }
+    return nat || d % w == 0 ? 2u + (d - 1) % w : 0u;
+}
+
+// (3) odd moddown of the key-switched LWE to br_q, dlog buckets, and (5) the
+// op list into sched[0 .. *nsched).  Scratch in the polynomial area of LDS:
+// bucket sizes, non-empty bitmap, per-wave sums, bucket and rank of each LWE
+// index.  bpos: [nb] first op of each bucket.
+template <int THREADS>
+PZ_DEV void br_schedule(const u32 *in, const DevCtx &c, u32 tid, u64 *lds, u16 *bpos,
+                        u16 *sched, u32 *nsched)
+{
+    constexpr u32 WAVES = THREADS / 64;
+    const u32 nb = c.br_q >> 1, qb4 = c.br_q >> 2, n = c.n_lwe, w = c.w;
+    u32 *cnt = reinterpret_cast<u32 *>(lds);            // [nb]
+    u64 *occ = lds + nb / 2;                             // [nb / 64]
+    u32 *wsum = reinterpret_cast<u32 *>(occ + nb / 64);  // [WAVES]
+    u16 *kbin = reinterpret_cast<u16 *>(wsum + WAVES);   // [n]
+    u16 *rk = kbin + n;                                  // [n]
+    for (u32 k = tid; k < nb; k += THREADS) cnt[k] = 0;
+    __syncthreads();
+    for (u32 i = tid; i < n; i += THREADS) {
+        const u32 v = in[1 + i] & (u32)c.lwe_mask;
+        const u32 odd = ((u32)(((u64)v * c.br_q) >> c.lwe_q_log)) | 1u;
+        const u32 k = c.dlog[odd];
+        kbin[i] = (u16)k;
+        rk[i] = (u16)atomicAdd(&cnt[k], 1u);  // arrival order, ranked below
+    }
+    __syncthreads();
+    // non-empty buckets, 64 per wave (nb is a multiple of 64)
+    for (u32 k0 = tid & ~63u; k0 < nb; k0 += THREADS) {
+        const u64 m = __ballot(cnt[k0 + (tid & 63)] != 0);
+        if ((tid & 63) == 0) occ[k0 / 64] = m;
+    }
+    __syncthreads();
+    // thread t takes the E buckets nb-1-t*E .. nb-(t+1)*E (op-list order)
+    const u32 E = nb > THREADS ? nb / THREADS : 1u;
+    const u32 p0 = tid * E;
+    u32 s = 0;
+    for (u32 e = 0; e < E; e++) {
+        const u32 p = p0 + e;
+        if (p < nb) s += cnt[nb - 1 - p] + (br_auto_after(nb - 1 - p, qb4, w, cnt, occ) != 0);
+    }
+    const u32 lane = tid & 63, wv = tid >> 6;
+    u32 x = s;  // inclusive scan over the wave
+#pragma unroll
+    for (u32 o = 1; o < 64; o <<= 1) {
+        const u32 y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    u32 pos = x - s;
+    for (u32 v = 0; v < wv; v++) pos += wsum[v];
+    if (tid == THREADS - 1) *nsched = pos + s;
+    for (u32 e = 0; e < E; e++) {
+        const u32 p = p0 + e;
+        if (p < nb) {
+            const u32 k = nb - 1 - p;
+            const u32 a = br_auto_after(k, qb4, w, cnt, occ);
+            bpos[k] = (u16)pos;
+            pos += cnt[k];
+            if (a) sched[pos++] = (u16)(SCHED_AUTO | (a - 1));
+        }
+    }
+    __syncthreads();
+    // each bucket's indices in arrival order, then each index to its rank
+    // among them (ascending, the reference's order)
+    for (u32 i = tid; i < n; i += THREADS) sched[bpos[kbin[i]] + rk[i]] = (u16)i;
+    __syncthreads();
+    for (u32 i = tid; i < n; i += THREADS) {
+        const u32 k = kbin[i], b0 = bpos[k], nk = cnt[k];
+        u32 r = 0;
+        for (u32 t = 0; t < nk; t++) r += sched[b0 + t] < i;
+        rk[i] = (u16)r;
+    }
+    __syncthreads();
+    for (u32 i = tid; i < n; i += THREADS) sched[bpos[kbin[i]] + rk[i]] = (u16)i;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
 // k_blind_rotate: persistent PBS tail (src/pbs.rs:146-232).
 //   ks:   [B][n+1] key-switched LWE (mod q_ks)
 //   gate: [B] test-vector index
@@ -291,7 +407,7 @@ void launch_rgsw_rows(int logn, u64 *rgsw0, u32 r0, u32 da0, u32 db0, size_t B, 
 // VAR 3, 4 (BR_SPLIT4_LAT, BR_SPLIT4): 2 x N/4 threads, 4 coefficients per
 // thread (radix-4 register phases); budgets for 4 and 8 waves per SIMD.
 // LDS: 2 polynomials (exchange / permutation; SPLIT: one per half), SPLIT: a
-// third for the permuted b of an automorphism, bucket starts + index list.
+// third for the permuted b of an automorphism; bucket positions + op list.
 // ---------------------------------------------------------------------------
 template <int VAR>
 constexpr int br_waves()
@@ -322,11 +438,8 @@ k_blind_rotate(
     const u32 n = c.n_lwe;
     u64 *stash = lds + 2 * W;                               // SPLIT: permuted b
     u32 *nsched = reinterpret_cast<u32 *>(lds + (SPLIT ? 3 : 2) * W);  // op count
-    u16 *bstart = reinterpret_cast<u16 *>(nsched + 2);      // [nb + 1]
-    u16 *bidx = bstart + nb + 1;                            // [n]
-    u16 *sched = bidx + n;                                  // [br_sched_len] op list
-    u32 *cnt = reinterpret_cast<u32 *>(lds);               // scratch in poly area
-    u16 *kbin = reinterpret_cast<u16 *>(cnt + nb + 1);     // scratch in poly area
+    u16 *bpos = reinterpret_cast<u16 *>(nsched + 2);        // [nb] first op of each bucket
+    u16 *sched = bpos + nb;                                 // [br_sched_len] op list
     const u32 tid = threadIdx.x;
     const u32 j = SPLIT ? tid % TPB : tid;
     // half index, uniform over each wave (TPB is a multiple of 64)
@@ -338,49 +451,7 @@ k_blind_rotate(
         // the bootstrap's server key: one key for the batch, or one per
         // ciphertext (sessions batched together)
         const DevKey key = keys ? uniform_key(keys[key_idx[bb]]) : key0;
-        // (3) odd moddown to br_q and dlog buckets (stable counting sort)
-        for (u32 i = tid; i <= nb; i += THREADS) cnt[i] = 0;
-        __syncthreads();
-        for (u32 i = tid; i < n; i += THREADS) {
-            u32 v = in[1 + i] & (u32)c.lwe_mask;
-            u32 odd = ((u32)(((u64)v * c.br_q) >> c.lwe_q_log)) | 1u;
-            u32 k = c.dlog[odd];
-            kbin[i] = (u16)k;
-            atomicAdd(&cnt[k + 1], 1u);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            u32 s = 0;
-            for (u32 i = 0; i <= nb; i++) {
-                s += cnt[i];
-                cnt[i] = s;
-                bstart[i] = (u16)s;
-            }
-            for (u32 i = 0; i < n; i++) bidx[cnt[kbin[i]]++] = (u16)i;
-            // (5) the blind rotation's op order, src/pbs.rs:240-414: RLWE x RGSW
-            // of LWE index s (< 2^15), or SCHED_AUTO | automorphism key v.
-            // Buckets -(g^k) first, then -(g^0) and the -g automorphism, then
-            // +(g^k), then +(g^0); a pending automorphism fires when the next
-            // bucket is non-empty, after w steps, or at the last bucket.
-            const u32 qb4 = c.br_q >> 2;
-            u32 m = 0;
-            for (int half = 0; half < 2; half++) {
-                const u32 off = half == 0 ? qb4 : 0;
-                u32 v = 0;
-                for (u32 i = qb4 - 1; i >= 1; i--) {
-                    for (u32 t = bstart[off + i]; t < bstart[off + i + 1]; t++) sched[m++] = bidx[t];
-                    v += 1;
-                    if (bstart[off + i] != bstart[off + i - 1] || v == c.w || i == 1) {
-                        sched[m++] = (u16)(SCHED_AUTO | v);
-                        v = 0;
-                    }
-                }
-                for (u32 t = bstart[off]; t < bstart[off + 1]; t++) sched[m++] = bidx[t];
-                if (half == 0) sched[m++] = (u16)SCHED_AUTO;
-            }
-            *nsched = m;
-        }
-        __syncthreads();
+        br_schedule<THREADS>(in, c, tid, lds, bpos, sched, nsched);
 
         // (4) test vector times X^{g*b} (monomial_mul, src/pbs.rs:163-211)
         const u32 oddb = ((u32)(((u64)(in[0] & (u32)c.lwe_mask) * c.br_q) >> c.lwe_q_log)) | 1u;
@@ -529,10 +600,12 @@ size_t br_lds_bytes(int logn, bool split, u32 br_q, u32 n_lwe)
 {
     size_t words = (size_t)(1 << logn) + (size_t)(1 << logn) / 8;
     size_t poly = (split ? 3 : 2) * 8 * words;
-    size_t scratch_need = 4 * (size_t)(br_q / 2 + 1) + 2 * (size_t)n_lwe;
+    // br_schedule's scratch: bucket sizes, non-empty bitmap, wave sums (at
+    // most 16 waves), bucket and rank of each LWE index
+    size_t scratch_need = 4 * (size_t)(br_q / 2) + br_q / 16 + 64 + 4 * (size_t)n_lwe;
     if (scratch_need > poly) poly = scratch_need;
-    // op count (8 bytes), bucket starts, index list, op list (u16 each)
-    return poly + 8 + 2 * ((size_t)(br_q / 2 + 1) + n_lwe + br_sched_len(br_q, n_lwe)) + 16;
+    // op count (8 bytes), bucket positions, op list (u16 each)
+    return poly + 8 + 2 * ((size_t)(br_q / 2) + br_sched_len(br_q, n_lwe)) + 16;
 }
 
 // The split work group serves d_a = d_b = d_auto = 1 (every NI set).
