@@ -213,6 +213,34 @@ def test_pbs(pname, mode):
 
 
 @pytest.mark.parametrize("pname,mode", MODES)
+def test_pbs_degenerate_buckets(pname, mode):
+    """Key-switch outputs whose dlog buckets are degenerate: a zero and a
+    trivial ciphertext (every LWE index in one bucket, the op list's longest
+    bucket), ciphertexts with one or a few non-zero mask coefficients (long
+    runs of empty buckets, forced automorphisms every w buckets), next to a
+    uniform one."""
+    o, s, z, key = keys_for(pname)
+    c = ctx_for(pname)
+    k = dkey_for(pname)
+    log("pbs degenerate %s mode=%d" % (pname, mode))
+    c.set_blind_rotate_mode(mode)
+    r = rng(53)
+    x = np.zeros((6, o.N + 1), dtype=np.uint64)
+    x[1, 0] = o.Q // 8                       # trivial: b only
+    x[2, 0], x[2, 5] = o.Q // 3, o.Q // 5    # one mask coefficient
+    x[3, 1:9] = o.Q - 1                      # a few, all rounding to q_ks
+    x[4, 1::97] = uniform_mod(r, o.Q, (len(range(1, o.N + 1, 97)),))
+    x[5] = uniform_mod(r, o.Q, (o.N + 1,))
+    B = x.shape[0]
+    gate = (np.arange(B) % 6).astype(np.uint32)
+    want = np.stack([o.pbs(key, int(gate[b]), x[b])[0] for b in range(B)])
+    d = c.to_device(x)
+    c.pbs(k, c.to_device(gate), d, B)
+    c.set_blind_rotate_mode(0)
+    assert np.array_equal(d.download(), want)
+
+
+@pytest.mark.parametrize("pname,mode", MODES)
 def test_gates_truth_and_parity(pname, mode):
     o, s, z, key = keys_for(pname)
     c = ctx_for(pname)
