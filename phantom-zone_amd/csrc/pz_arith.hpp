@@ -223,7 +223,19 @@ PZ_DEV u64 barrett_mul(const Barrett &br, u64 a, u64 b)
 struct Decomp {
     u64 q;
     u32 logb, d, ignore_bits;
+    // the d = 1 digit's addends (decomp_digit_d1): rc, rc - q, rc - 2q (mod
+    // 2^64), rc = 2^(ignore_bits-1) (0 without ignored bits); set by
+    // decomp_centring on the host
+    u64 cadd0, cadd1, cadd2;
 };
+
+inline void decomp_centring(Decomp &dc)
+{
+    const u64 rc = dc.ignore_bits ? 1ull << (dc.ignore_bits - 1) : 0;
+    dc.cadd0 = rc;
+    dc.cadd1 = rc - dc.q;
+    dc.cadd2 = rc - 2 * dc.q;
+}
 
 // Digit `r` (0-based) of the signed decomposition of v. Release-build
 // semantics: for v == q the !(q - v) + 1 wraps to 0.
@@ -277,19 +289,25 @@ PZ_DEV u64 decomp_digit0(const Decomp &dc, u64 v)
 // the carry rule never changes it: the digit is the value itself
 // (negative values as q + value; the value B/2 with an even next digit stays
 // B/2, the value -B/2 carries into q - B/2).  Needs logb + ignore_bits <= 63
-// and ignore_bits > 0 (round_c = 2^(ignore_bits - 1)).
+// and ignore_bits > 0 (dc.cadd: 2^(ignore_bits - 1) minus 0, q and 2q).
 // LOOSE takes v in [0, 2q] (a representative the reference would hold as
 // v mod q): v_s = v - 2q from q + floor(q/2) on, the reference's rule on
 // v - q there.
+// ca0, ca1, ca2: the decomposer's cadd0, cadd1, cadd2, passed as values (a
+// select between fields of a struct is turned into a load from a stack copy
+// at a selected offset)
 template <bool LOOSE = false>
-PZ_DEV u64 decomp_digit_d1(const Decomp &dc, u64 v, u64 round_c)
+PZ_DEV u64 decomp_digit_d1(u64 q, u32 ignore_bits, u64 ca0, u64 ca1, u64 ca2, u64 v)
 {
-    // v_s + 2^(ib-1) in two's complement, v_s = v - q when v >= floor(q/2)
-    const u64 h = dc.q >> 1;
-    u64 c = v >= h ? round_c - dc.q : round_c;
-    if constexpr (LOOSE) c = v >= dc.q + h ? round_c - 2 * dc.q : c;
+    // v_s + 2^(ib-1) in two's complement, v_s = v - q when v >= floor(q/2):
+    // the addend is selected among the three precomputed ones (computed here
+    // from q, the compiler selects the multiple of q, subtracts it from v and
+    // adds 2^(ib-1) after it: a 64-bit subtraction more per digit)
+    const u64 h = q >> 1;
+    u64 c = v >= h ? ca1 : ca0;
+    if constexpr (LOOSE) c = v >= q + h ? ca2 : c;
     const u64 t = v + c;
-    return dc.q + (u64)((int64_t)t >> dc.ignore_bits);  // floor, then + q
+    return q + (u64)((int64_t)t >> ignore_bits);  // floor, then + q
 }
 
 PZ_DEV u64 decomp_digit(const Decomp &dc, u64 v, u32 r)

@@ -238,6 +238,7 @@ static Decomp make_decomp(uint64_t q, uint32_t logb, uint32_t d)
     dc.logb = logb;
     dc.d = d;
     dc.ignore_bits = host::ref_log2(q) - logb * d;
+    decomp_centring(dc);
     return dc;
 }
 

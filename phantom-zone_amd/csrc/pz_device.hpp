@@ -442,7 +442,12 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     }
     // half 1 of an automorphism has no transform (its digits are not used)
     const bool work = !is_auto || h == 0;
-    const Decomp dc = is_auto ? c.dec_auto : (h ? c.dec_b : c.dec_a);
+    // the step's decomposer, field by field (a select of the whole struct is
+    // built as a stack copy)
+#define PZ_DEC_FIELD(f) (is_auto ? c.dec_auto.f : (h ? c.dec_b.f : c.dec_a.f))
+    const u64 ca0 = PZ_DEC_FIELD(cadd0), ca1 = PZ_DEC_FIELD(cadd1), ca2 = PZ_DEC_FIELD(cadd2);
+    const u32 dib = PZ_DEC_FIELD(ignore_bits);
+#undef PZ_DEC_FIELD
     u64 D[1][R];
     // product rows: RLWE x RGSW -- own digits with the own-part row (half 0:
     // A(-sm), row 0; half 1: B(m), row 3), then the other half's digits with
@@ -473,9 +478,8 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the digits
         }
         // q + digit (below 2q): the unreduced forward transform takes it as is
-        const u64 rc = dc.ignore_bits ? 1ull << (dc.ignore_bits - 1) : 0;
 #pragma unroll
-        for (int r = 0; r < R; r++) D[0][r] = decomp_digit_d1<LOOSE>(dc, p[r], rc);
+        for (int r = 0; r < R; r++) D[0][r] = decomp_digit_d1<LOOSE>(q, dib, ca0, ca1, ca2, p[r]);
         if constexpr (PF & 1)
             ntt_fwd_pf<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(
                 D, j, mine, c.tb, tw);

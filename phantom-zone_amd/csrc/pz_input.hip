@@ -228,6 +228,8 @@ DevCtx pz::rgrg_devctx(const pz_params &p, const DevCtx &dc)
     cc.dec_b.d = p.rgrg_db;
     cc.dec_a.ignore_bits = lq - p.rgrg_logb * p.rgrg_da;
     cc.dec_b.ignore_bits = lq - p.rgrg_logb * p.rgrg_db;
+    decomp_centring(cc.dec_a);
+    decomp_centring(cc.dec_b);
     cc.da = p.rgrg_da;
     cc.db = p.rgrg_db;
     return cc;
