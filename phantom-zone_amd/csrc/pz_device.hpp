@@ -63,7 +63,7 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #define PZ_TW_PREFETCH4 5
 #endif
 
-// Split-step digits are q + digit (decomp_digit_d1, 8 instructions) instead
+// Split-step digits are q + digit (decomp_digit_d1, 9 VALU instructions) instead
 // of the reference's representative in [0, q) (decomp_digit0, 15): the same
 // residue, for the unreduced transform.
 // The blind rotation's RLWE stays in [0, 2q) between steps.
