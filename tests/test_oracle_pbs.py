@@ -179,3 +179,82 @@ def test_schedule_matches_python_restatement(name):
         ops = o.br_schedule(ks)
         got = [(int(x) & 0xFF, (int(x) >> 8) & 0x7FFFFF, bool(int(x) >> 31)) for x in ops]
         assert got == py_br_schedule(ks, o.p)
+
+
+def closed_form_schedule(kbin, nb, w):
+    """The op list as the blind-rotation kernel builds it in parallel
+    (br_schedule in pz_rlwe.hip): bucket sizes; for bucket k with i = k mod
+    br_q/4 >= 1 a pending automorphism fires after it when bucket k-1 is
+    non-empty, at i == 1, or where d mod w == 0, with d = one past the next
+    non-empty bucket at or above k below the half's top bucket, minus k (the
+    half's top + 1 - k if none), and carries v = (d-1) mod w + 1; the first
+    half's bucket br_q/4 ends with the v = 0 automorphism.  Each bucket's op
+    count then places every op by one exclusive scan (k descending), and the
+    indices of a bucket keep ascending order.  Returns (kind, arg) pairs."""
+    qb4 = nb // 2
+    cnt = [0] * nb
+    for k in kbin:
+        cnt[k] += 1
+
+    def auto_after(k):
+        base = qb4 if k >= qb4 else 0
+        i = k - base
+        if i == 0:
+            return 1 if base else 0
+        nat = i == 1 or cnt[k - 1] != 0
+        top = base + qb4 - 1
+        d = top + 1 - k
+        for kk in range(k, top):
+            if cnt[kk]:
+                d = kk + 1 - k
+                break
+        return 2 + (d - 1) % w if nat or d % w == 0 else 0
+
+    pos, bpos, slots = 0, [0] * nb, {}
+    for k in range(nb - 1, -1, -1):
+        a = auto_after(k)
+        bpos[k] = pos
+        pos += cnt[k]
+        if a:
+            slots[pos] = (1, a - 1)
+            pos += 1
+    for i, k in enumerate(kbin):
+        rank = sum(1 for j in range(i) if kbin[j] == k)
+        slots[bpos[k] + rank] = (0, i)
+    assert sorted(slots) == list(range(pos))
+    return [slots[x] for x in range(pos)]
+
+
+@pytest.mark.parametrize("name", ["NI_2P", "NI_4P_HB_FR", "NI_8P", "SP_TEST"])
+def test_parallel_schedule_closed_form(name):
+    """The closed form the GPU kernel uses to build the op list with the whole
+    work group equals the oracle's sequential op order, for uniform key
+    switch outputs, every index in one bucket, and sparse fillings with long
+    runs of empty buckets (forced automorphisms)."""
+    import math
+    from conftest import oracle_for
+    o = oracle_for(name)
+    p = o.p
+    br_q, lq = int(p.br_q), int(p.lwe_q)
+    qb4, g = br_q // 4, int(p.g)
+    dlog = [0] * br_q
+    for i in range(qb4):
+        v = pow(g, i, br_q)
+        dlog[v] = i
+        dlog[br_q - v] = i + qb4
+    r = rng(73)
+    cases = [r.integers(0, lq, size=o.n + 1, dtype=np.uint64) for _ in range(4)]
+    cases.append(np.zeros(o.n + 1, dtype=np.uint64))
+    cases.append(np.full(o.n + 1, lq // 3, dtype=np.uint64))
+    sparse = np.zeros(o.n + 1, dtype=np.uint64)
+    sparse[1::37] = r.integers(0, lq, size=len(sparse[1::37]), dtype=np.uint64)
+    cases.append(sparse)
+    cases.append(r.integers(0, 4, size=o.n + 1, dtype=np.uint64) * (lq // 4) + 1)
+    for ks in cases:
+        kbin = []
+        for v in (int(x) for x in ks[1:]):
+            odd = math.floor((float(v) * float(br_q)) / float(lq))
+            odd += (odd & 1) ^ 1
+            kbin.append(dlog[odd])
+        want = [(int(x) & 0xFF, (int(x) >> 8) & 0x7FFFFF) for x in o.br_schedule(ks)]
+        assert closed_form_schedule(kbin, br_q // 2, int(p.w)) == want
