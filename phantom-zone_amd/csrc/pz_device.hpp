@@ -67,9 +67,9 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 // of the reference's representative in [0, q) (decomp_digit0, 15): the same
 // residue, for the unreduced transform.
 // The blind rotation's RLWE stays in [0, 2q) between steps.
-// Wave priority of an automorphism's forward transform (0: none)
-#ifndef PZ_AUTO_PRIO
-#define PZ_AUTO_PRIO 3
+// Wave priorities of the split step in the throughput variant (0: none)
+#ifndef PZ_STEP_PRIO
+#define PZ_STEP_PRIO 1
 #endif
 #ifndef PZ_LOOSE_STEPS
 #define PZ_LOOSE_STEPS 1
@@ -440,6 +440,16 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
         }
         __syncthreads();  // read before the transform rewrites the region
     }
+    // Wave priorities with two work groups per CU (8 coefficients per thread),
+    // which arbitrate between the two bootstraps on a CU: an automorphism step
+    // (the shorter step, half of whose waves idle through its transform) at 3,
+    // the inverse of an RLWE x RGSW step (its last phase) at 2, the rest at 0.
+    // Measured at B = 4096: 114.0 ms without priorities, 111.3 with the
+    // automorphism step alone, 108.7 with both; the automorphism step must
+    // rank above the inverse (at equal levels 111.8).  One bootstrap per CU
+    // has nothing to arbitrate.
+    constexpr bool PRIO = PZ_STEP_PRIO && R == 8;
+    if (PRIO && is_auto) __builtin_amdgcn_s_setprio(3);
     // half 1 of an automorphism has no transform (its digits are not used)
     const bool work = !is_auto || h == 0;
     // the step's decomposer, field by field (a select of the whole struct is
@@ -467,12 +477,6 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     constexpr int PF = R == 8 ? PZ_TW_PREFETCH : PZ_TW_PREFETCH4;
     PhaseTw<R> tw;  // the first phase's twiddles of the forward, then of the inverse
     if (work) {
-        // an automorphism's transform is the step's critical path (half 1 waits
-        // for it): with two work groups per CU (8 coefficients per thread) its
-        // waves issue ahead of the other work group's (117.4 against 118.5 ms
-        // at B = 4096; one bootstrap per CU gains nothing)
-        constexpr bool PRIO = PZ_AUTO_PRIO && R == 8;
-        if (PRIO && is_auto) __builtin_amdgcn_s_setprio(PZ_AUTO_PRIO);
         if constexpr (PF & 1) {
             fwd_tw_first<LOGN, R>(tw, j, c.tb);
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the digits
@@ -493,7 +497,6 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
         // phase's twiddles instead, measured no faster)
         __builtin_amdgcn_sched_barrier(0);
         load_slice(k1, K + r1, Ks + r1, j);
-        if (PRIO && is_auto) __builtin_amdgcn_s_setprio(0);
     } else {
         load_slice(k1, K + r1, Ks + r1, j);
         ntt_fwd_idle<LOGN, R>();
@@ -518,6 +521,7 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (!is_auto) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k2, D[0], q, q2);
     constexpr int IBND = Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2);  // accumulators below IBND q
+    if (PRIO && !is_auto) __builtin_amdgcn_s_setprio(2);
     if constexpr ((PF & 6) == 4) inv_tw_first<LOGN, R>(tw, j, c.tb);
     if constexpr ((PF & 6) != 0)
         ntt_inv_pf<LOGN, 1, !LOOSE, false, PZ_SPLIT_SM, PZ_INV_FAST, IBND, R, false>(acc, j, mine,
@@ -541,6 +545,7 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
 #pragma unroll
         for (int r = 0; r < R; r++) p[r] = acc[0][r];
     }
+    if (PRIO) __builtin_amdgcn_s_setprio(0);
     // the stash is rewritten only by a following automorphism, after its
     // leading barrier
 }
