@@ -442,12 +442,14 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     }
     // Wave priorities with two work groups per CU (8 coefficients per thread),
     // which arbitrate between the two bootstraps on a CU: an automorphism step
-    // (the shorter step, half of whose waves idle through its transform) at 3,
-    // the inverse of an RLWE x RGSW step (its last phase) at 2, the rest at 0.
-    // Measured at B = 4096: 114.0 ms without priorities, 111.3 with the
-    // automorphism step alone, 108.7 with both; the automorphism step must
-    // rank above the inverse (at equal levels 111.8).  One bootstrap per CU
-    // has nothing to arbitrate.
+    // (the shorter step, half of whose waves idle through its transform) at 3;
+    // an RLWE x RGSW step's forward transform at 1, its products at 0 and its
+    // inverse (the last phase) at 2.  Measured at B = 4096 on one box: 114.0
+    // ms without priorities, 111.3 with the automorphism step alone, 108.7
+    // with the inverse at 2 as well, 107.3 with the forward at 1; the
+    // automorphism step must rank above the inverse (at equal levels 111.8,
+    // and 111.9 with the automorphism's own inverse at 2).  One bootstrap per
+    // CU has nothing to arbitrate.
     constexpr bool PRIO = PZ_STEP_PRIO && R == 8;
     if (PRIO && is_auto) __builtin_amdgcn_s_setprio(3);
     // half 1 of an automorphism has no transform (its digits are not used)
@@ -477,6 +479,7 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     constexpr int PF = R == 8 ? PZ_TW_PREFETCH : PZ_TW_PREFETCH4;
     PhaseTw<R> tw;  // the first phase's twiddles of the forward, then of the inverse
     if (work) {
+        if (PRIO && !is_auto) __builtin_amdgcn_s_setprio(1);
         if constexpr (PF & 1) {
             fwd_tw_first<LOGN, R>(tw, j, c.tb);
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the digits
@@ -497,6 +500,7 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
         // phase's twiddles instead, measured no faster)
         __builtin_amdgcn_sched_barrier(0);
         load_slice(k1, K + r1, Ks + r1, j);
+        if (PRIO && !is_auto) __builtin_amdgcn_s_setprio(0);
     } else {
         load_slice(k1, K + r1, Ks + r1, j);
         ntt_fwd_idle<LOGN, R>();
