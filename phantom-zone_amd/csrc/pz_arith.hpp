@@ -64,6 +64,21 @@ PZ_DEV u64 shoup_quot4(u64 a, u64 ws)
     const u32 x = __umulhi(s0, a1), y = __umulhi(s1, a0);
     return (u64)s1 * a1 + ((u64)x + y);
 }
+// With the twin T = floor(w 2^63 / q) instead (T < 2^63, so its high word is
+// below 2^31) and a < 2^63, the middle column a1*T0 + a0*T1 stays below 2^64
+// and is summed exactly: Q = a1*T1 + floor((a1*T0 + a0*T1) / 2^32) drops only
+// the low product and the column's low word (each below 1 after scaling) and
+// T's truncation (a / 2^64 < 1/2), so floor(a*w / 2q) - 2 <= Q <= floor(a*w /
+// 2q), and c + w*a - 2q*Q lies in c + [0, 6q): three v_mad_u64_u32 (and the
+// move that zero-extends the column's high word) against shoup_quot4's two
+// v_mul_hi_u32, a v_mad_u64_u32 and a 64-bit add.
+PZ_DEV u64 shoup_quot6(u64 a, u64 T)
+{
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), t0 = (u32)T, t1 = (u32)(T >> 32);
+    const u64 col = (u64)a1 * t0 + (u64)a0 * t1;
+    return (u64)a1 * t1 + (col >> 32);
+}
+
 // w*a mod q in [0, 4q) for any a < 2^64 (the two-step form with shoup_quot4)
 PZ_DEV u64 shoup_mul4(u64 a, u64 w, u64 ws, u64 q)
 {

@@ -41,6 +41,12 @@ __host__ __device__ inline u32 br_sched_len(u32 br_q, u32 n) { return n + br_q /
 #ifndef PZ_QUOT4
 #define PZ_QUOT4 1
 #endif
+// The split step's digit transform with the [0,6q) products of FWD_FAST6 (one
+// VALU instruction fewer per butterfly than FWD_FAST4; its outputs, below 68q,
+// feed only the key products, which take any 64-bit input)
+#ifndef PZ_FWD6
+#define PZ_FWD6 1
+#endif
 
 #ifndef PZ_MAC_FENCE
 #define PZ_MAC_FENCE 0
@@ -478,21 +484,20 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
     KeySlice<R> k1, k2;
     constexpr int PF = R == 8 ? PZ_TW_PREFETCH : PZ_TW_PREFETCH4;
     PhaseTw<R> tw;  // the first phase's twiddles of the forward, then of the inverse
+    constexpr int FMODE = PZ_QUOT4 ? (PZ_FWD6 ? FWD_FAST6 : FWD_FAST4) : PZ_FWD_MODE;
     if (work) {
         if (PRIO && !is_auto) __builtin_amdgcn_s_setprio(1);
         if constexpr (PF & 1) {
-            fwd_tw_first<LOGN, R>(tw, j, c.tb);
+            fwd_tw_first<LOGN, R, FMODE>(tw, j, c.tb);
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the digits
         }
         // q + digit (below 2q): the unreduced forward transform takes it as is
 #pragma unroll
         for (int r = 0; r < R; r++) D[0][r] = decomp_digit_d1<LOOSE>(q, dib, ca0, ca1, ca2, p[r]);
         if constexpr (PF & 1)
-            ntt_fwd_pf<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(
-                D, j, mine, c.tb, tw);
+            ntt_fwd_pf<LOGN, 1, FMODE, PZ_SPLIT_SM, R, false>(D, j, mine, c.tb, tw);
         else
-            ntt_fwd<LOGN, 1, PZ_QUOT4 ? FWD_FAST4 : PZ_FWD_MODE, PZ_SPLIT_SM, R, false>(D, j,
-                                                                                      mine, c.tb);
+            ntt_fwd<LOGN, 1, FMODE, PZ_SPLIT_SM, R, false>(D, j, mine, c.tb);
 #pragma unroll
         for (int r = 0; r < R; r++) mine[lds_ix(eval_x<LOGN, R>(j, r))] = D[0][r];
         // not hoisted into the transform: its twiddle loads would wait behind

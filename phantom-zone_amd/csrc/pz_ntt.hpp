@@ -60,6 +60,7 @@ PZ_DEV u64x2 gld2(const u64x2 *p)
 // load fetches both, powers in the reference's bit-reversed order.
 struct Tables {
     const u64x2 *__restrict__ fw;  // forward  (psi^bo, shoup)
+    const u64x2 *__restrict__ fw63;  // forward  (psi^bo, floor(psi^bo 2^63 / q)): FWD_FAST6
     const u64x2 *__restrict__ iw;  // inverse  (psi^-bo, shoup)
     u64 n_inv, n_inv_s;
     u64 q, q2;
@@ -74,7 +75,11 @@ struct Tables {
 // FWD_FAST4 is FWD_FAST with the [0,4q) product (shoup_mul4) and y' = x + 4q - t:
 // from inputs below 2q (the split step's digits, q + digit) stage k leaves
 // values below (4k+2)q (< 46q < 2^60 for N = 2048).
-enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3 };
+// FWD_FAST6 is FWD_FAST4 with the [0,6q) product of shoup_quot6 (twins from
+// tb.fw63, modulus 2q) and y' = x + 6q - t: from inputs below 2q stage k
+// leaves values below (6k+2)q (< 68q < 2^61 for N = 2048), inside the
+// quotient's a < 2^63.
+enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3, FWD_FAST6 = 4 };
 
 // Unreduced tail of the IB = 8 fast inverse (see inv_stage).
 #ifndef PZ_INV_TAIL4
@@ -83,7 +88,8 @@ enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3 };
 
 // One forward stage S inside a phase of stride ST. KIND: 0 = 0_to_4q
 // butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q),
-// 3 = unreduced (FWD_FAST), 4 = unreduced with the [0,4q) product (FWD_FAST4).
+// 3 = unreduced (FWD_FAST), 4 = unreduced with the [0,4q) product (FWD_FAST4),
+// 5 = unreduced with the [0,6q) product (FWD_FAST6).
 // SM selects the two-step Shoup form
 // (shoup_mul<true>, fewer instructions); the one-step form has fewer live
 // registers, which keeps the one-work-group blind rotation within 256 VGPRs.
@@ -99,7 +105,7 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *
         if (r & LD) continue;
         const u32 x = base + ST * r;
         const u32 i = x / (2 * T);
-        const u64x2 tw = PRE ? pre[r] : gld2(tb.fw + M + i);
+        const u64x2 tw = PRE ? pre[r] : gld2((KIND == 5 ? tb.fw63 : tb.fw) + M + i);
         const u64 w = tw.x, ws = tw.y;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -117,6 +123,13 @@ PZ_DEV void fwd_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *
                 const u64 xt = shoup_rem(x, a, w, shoup_quot4(a, ws), tb.q);
                 v[p][r] = xt;
                 v[p][r + LD] = twice_plus(x, tb.q2 << 1) - xt;
+            } else if (KIND == 5) {
+                // the same with t in [0, 6q): the remainder modulo 2q, and
+                // x + 6q - t as 2x + 6q - (x + t)
+                const u64 x = v[p][r], a = v[p][r + LD];
+                const u64 xt = shoup_rem(x, a, w, shoup_quot6(a, ws), tb.q2);
+                v[p][r] = xt;
+                v[p][r + LD] = twice_plus(x, tb.q2 * 3) - xt;
             } else {
                 fb_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
                 if (KIND == 2) {
@@ -236,12 +249,14 @@ template <int R>
 struct PhaseTw {
     u64x2 t[3][R];
 };
-template <int LOGN, int S0, int R, bool INV>
+// TAB: 0 = forward (tb.fw), 1 = inverse (tb.iw), 2 = forward with 2^63
+// twins (tb.fw63, FWD_FAST6)
+template <int LOGN, int S0, int R, int TAB>
 PZ_DEV void phase_tw(PhaseTw<R> &pt, u32 j, const Tables &tb)
 {
     using P = Phase<LOGN, S0, R>;
     const u32 base = P::base(j);
-    const u64x2 *tab = INV ? tb.iw : tb.fw;
+    const u64x2 *tab = TAB == 1 ? tb.iw : (TAB == 2 ? tb.fw63 : tb.fw);
 #pragma unroll
     for (int e = 0; e < P::NS; e++) {
         const int S = S0 + e;
@@ -263,9 +278,12 @@ PZ_DEV void fwd_phase(u64 (&v)[NP][R], u32 j, const Tables &tb, const PhaseTw<R>
     using P = Phase<LOGN, S0, R>;
     const u32 base = P::base(j);
     constexpr int LAST = LOGN - 1;
-    constexpr int K_LAST =
-        MODE == FWD_STRICT ? 2 : (MODE == FWD_LAZY ? 1 : (MODE == FWD_FAST ? 3 : 4));
-    constexpr int K_MID = MODE == FWD_FAST ? 3 : (MODE == FWD_FAST4 ? 4 : 0);
+    constexpr int K_LAST = MODE == FWD_STRICT ? 2
+                         : MODE == FWD_LAZY   ? 1
+                         : MODE == FWD_FAST   ? 3
+                         : MODE == FWD_FAST4  ? 4
+                                              : 5;
+    constexpr int K_MID = MODE == FWD_FAST ? 3 : MODE == FWD_FAST4 ? 4 : MODE == FWD_FAST6 ? 5 : 0;
     fwd_stage<LOGN, S0, P::ST, NP, (S0 == LAST) ? K_LAST : K_MID, SM, R, PRE>(
         v, base, tb, PRE ? pt->t[0] : nullptr);
     if constexpr (P::NS > 1)
@@ -359,7 +377,8 @@ PZ_DEV Tables fresh(const Tables &tb)
 #if PZ_TWIDDLE_RELOAD
     t.fw = uniform_ptr(t.fw);
     t.iw = uniform_ptr(t.iw);
-    asm volatile("" : "+s"(t.fw), "+s"(t.iw));
+    t.fw63 = uniform_ptr(t.fw63);
+    asm volatile("" : "+s"(t.fw), "+s"(t.iw), "+s"(t.fw63));
 #endif
     return t;
 }
@@ -381,7 +400,7 @@ PZ_DEV void fwd_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb,
     if constexpr (S0 + LR < LOGN) {
         if constexpr (PF) {
             PhaseTw<R> nt;
-            phase_tw<LOGN, S0 + LR, R, false>(nt, j, fresh(tb));
+            phase_tw<LOGN, S0 + LR, R, MODE == FWD_FAST6 ? 2 : 0>(nt, j, fresh(tb));
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the exchange
             exchange<LOGN, S0, S0 + LR, NP, R, TAIL>(v, j, lds);
             fwd_from<LOGN, S0 + LR, NP, MODE, SM, R, TAIL, PF>(v, j, lds, tb, &nt);
@@ -408,25 +427,25 @@ template <int LOGN, int NP, int MODE, bool SM = true, int R = 8, bool TAIL = tru
 PZ_DEV void ntt_fwd(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 {
     static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
-    static_assert((MODE != FWD_FAST && MODE != FWD_FAST4) || LOGN <= 11,
-                  "unreduced values must stay below 2^64");
+    static_assert((MODE != FWD_FAST && MODE != FWD_FAST4 && MODE != FWD_FAST6) || LOGN <= 11,
+                  "unreduced values must stay below 2^64 (FWD_FAST6: below 2^63)");
     static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
     fwd_from<LOGN, 0, NP, MODE, SM, R, TAIL>(v, j, lds, tb);
 }
 
 // The same with every phase's twiddles loaded ahead of its exchange; pt holds
 // the first phase's (fwd_tw_first), which the caller issues early.
-template <int LOGN, int R>
+template <int LOGN, int R, int MODE = FWD_FAST4>
 PZ_DEV void fwd_tw_first(PhaseTw<R> &pt, u32 j, const Tables &tb)
 {
-    phase_tw<LOGN, 0, R, false>(pt, j, fresh(tb));
+    phase_tw<LOGN, 0, R, MODE == FWD_FAST6 ? 2 : 0>(pt, j, fresh(tb));
 }
 template <int LOGN, int NP, int MODE, bool SM = true, int R = 8, bool TAIL = true>
 PZ_DEV void ntt_fwd_pf(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb, const PhaseTw<R> &pt)
 {
     static_assert(LOGN >= 9 && LOGN <= 12, "ring size");
-    static_assert((MODE != FWD_FAST && MODE != FWD_FAST4) || LOGN <= 11,
-                  "unreduced values must stay below 2^64");
+    static_assert((MODE != FWD_FAST && MODE != FWD_FAST4 && MODE != FWD_FAST6) || LOGN <= 11,
+                  "unreduced values must stay below 2^64 (FWD_FAST6: below 2^63)");
     static_assert(R == 8 || R == 4, "8 or 4 coefficients per thread");
     fwd_from<LOGN, 0, NP, MODE, SM, R, TAIL, true>(v, j, lds, tb, &pt);
 }
@@ -446,7 +465,7 @@ PZ_DEV void inv_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb,
     if constexpr (S0 > 0) {
         if constexpr (PF) {
             PhaseTw<R> nt;
-            phase_tw<LOGN, S0 - LR, R, true>(nt, j, fresh(tb));
+            phase_tw<LOGN, S0 - LR, R, 1>(nt, j, fresh(tb));
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the exchange
             exchange<LOGN, S0, S0 - LR, NP, R, TAIL>(v, j, lds);
             inv_from<LOGN, S0 - LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL, PF>(v, j, lds, tb,
@@ -476,7 +495,7 @@ template <int LOGN, int R>
 PZ_DEV void inv_tw_first(PhaseTw<R> &pt, u32 j, const Tables &tb)
 {
     constexpr int LR = RadixLog<R>::value;
-    phase_tw<LOGN, ((LOGN - 1) / LR) * LR, R, true>(pt, j, fresh(tb));
+    phase_tw<LOGN, ((LOGN - 1) / LR) * LR, R, 1>(pt, j, fresh(tb));
 }
 template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool FAST = false,
           int IB = 2, int R = 8, bool TAIL = true>
