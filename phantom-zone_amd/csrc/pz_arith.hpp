@@ -304,13 +304,13 @@ PZ_DEV u64 decomp_digit0(const Decomp &dc, u64 v)
 // the carry rule never changes it: the digit is the value itself
 // (negative values as q + value; the value B/2 with an even next digit stays
 // B/2, the value -B/2 carries into q - B/2).  Needs logb + ignore_bits <= 63
-// and ignore_bits > 0 (dc.cadd: 2^(ignore_bits - 1) minus 0, q and 2q).
+// and ignore_bits > 0.  ca0, ca1, ca2: the decomposer's cadd0, cadd1, cadd2
+// (2^(ignore_bits - 1) minus 0, q and 2q), passed as values: a select between
+// fields of a struct is turned into a load from a stack copy at a selected
+// offset.
 // LOOSE takes v in [0, 2q] (a representative the reference would hold as
 // v mod q): v_s = v - 2q from q + floor(q/2) on, the reference's rule on
 // v - q there.
-// ca0, ca1, ca2: the decomposer's cadd0, cadd1, cadd2, passed as values (a
-// select between fields of a struct is turned into a load from a stack copy
-// at a selected offset)
 template <bool LOOSE = false>
 PZ_DEV u64 decomp_digit_d1(u64 q, u32 ignore_bits, u64 ca0, u64 ca1, u64 ca2, u64 v)
 {
