@@ -237,6 +237,7 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *
     }
 }
 
+template <bool F63 = false>
 PZ_DEV struct Tables fresh(const Tables &tb);
 
 // The twiddles of one phase, [stage in the phase][r] (only entries with
@@ -371,14 +372,20 @@ PZ_DEV const u64x2 *uniform_ptr(const u64x2 *p)
     const u32 hi = (u32)__builtin_amdgcn_readfirstlane((int)(u32)(v >> 32));
     return (const u64x2 *)(((u64)hi << 32) | lo);
 }
+// F63: the FWD_FAST6 table as well (passed through the asm only where it is
+// read: one more pointer there cost the transform kernels 40 VGPRs)
+template <bool F63>
 PZ_DEV Tables fresh(const Tables &tb)
 {
     Tables t = tb;
 #if PZ_TWIDDLE_RELOAD
     t.fw = uniform_ptr(t.fw);
     t.iw = uniform_ptr(t.iw);
-    t.fw63 = uniform_ptr(t.fw63);
-    asm volatile("" : "+s"(t.fw), "+s"(t.iw), "+s"(t.fw63));
+    asm volatile("" : "+s"(t.fw), "+s"(t.iw));
+    if constexpr (F63) {
+        t.fw63 = uniform_ptr(t.fw63);
+        asm volatile("" : "+s"(t.fw63));
+    }
 #endif
     return t;
 }
@@ -396,11 +403,12 @@ PZ_DEV void fwd_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb,
 {
     constexpr int LR = RadixLog<R>::value;
     if constexpr (PF) fwd_phase<LOGN, S0, NP, MODE, SM, R, true>(v, j, tb, pt);
-    else fwd_phase<LOGN, S0, NP, MODE, SM, R>(v, j, fresh(tb));
+    else fwd_phase<LOGN, S0, NP, MODE, SM, R>(v, j, fresh<MODE == FWD_FAST6>(tb));
     if constexpr (S0 + LR < LOGN) {
         if constexpr (PF) {
             PhaseTw<R> nt;
-            phase_tw<LOGN, S0 + LR, R, MODE == FWD_FAST6 ? 2 : 0>(nt, j, fresh(tb));
+            phase_tw<LOGN, S0 + LR, R, MODE == FWD_FAST6 ? 2 : 0>(nt, j,
+                                                                  fresh<MODE == FWD_FAST6>(tb));
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the exchange
             exchange<LOGN, S0, S0 + LR, NP, R, TAIL>(v, j, lds);
             fwd_from<LOGN, S0 + LR, NP, MODE, SM, R, TAIL, PF>(v, j, lds, tb, &nt);
@@ -438,7 +446,7 @@ PZ_DEV void ntt_fwd(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 template <int LOGN, int R, int MODE = FWD_FAST4>
 PZ_DEV void fwd_tw_first(PhaseTw<R> &pt, u32 j, const Tables &tb)
 {
-    phase_tw<LOGN, 0, R, MODE == FWD_FAST6 ? 2 : 0>(pt, j, fresh(tb));
+    phase_tw<LOGN, 0, R, MODE == FWD_FAST6 ? 2 : 0>(pt, j, fresh<MODE == FWD_FAST6>(tb));
 }
 template <int LOGN, int NP, int MODE, bool SM = true, int R = 8, bool TAIL = true>
 PZ_DEV void ntt_fwd_pf(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb, const PhaseTw<R> &pt)
