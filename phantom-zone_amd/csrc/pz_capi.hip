@@ -69,7 +69,7 @@ struct pz_ctx {
     DevCtx dc;
     host::HostTables ht;
     // device allocations of the tables
-    u64 *d_tables = nullptr;   // 6N
+    u64 *d_tables = nullptr;   // 8N
     u32 *d_dlog = nullptr;
     u64 *d_tv = nullptr;
     // scratch of the PBS pipeline
@@ -268,9 +268,10 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
         delete c;
         return fail(PZ_ERR_HIP, "stream create failed");
     }
-    // [fw pairs (2N) | iw pairs (2N) | fw63 pairs (2N): the forward powers with
-    // floor(w 2^63 / q), the twins of FWD_FAST6]
-    std::vector<u64> tabs(6 * (size_t)N);
+    // [fw pairs (2N) | iw pairs (2N) | fw63, iw63 pairs (2N each): the powers
+    // with floor(w 2^63 / q), the twins of FWD_FAST6 and of the split step's
+    // fast inverse]
+    std::vector<u64> tabs(8 * (size_t)N);
     for (uint32_t i = 0; i < N; i++) {
         tabs[2 * i] = c->ht.psi_bo[i];
         tabs[2 * i + 1] = c->ht.psi_bo_s[i];
@@ -279,6 +280,9 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
         tabs[4 * N + 2 * i] = c->ht.psi_bo[i];
         tabs[4 * N + 2 * i + 1] =
             (u64)(((unsigned __int128)c->ht.psi_bo[i] << 63) / p->rlwe_q);
+        tabs[6 * N + 2 * i] = c->ht.ipsi_bo[i];
+        tabs[6 * N + 2 * i + 1] =
+            (u64)(((unsigned __int128)c->ht.ipsi_bo[i] << 63) / p->rlwe_q);
     }
     std::vector<uint32_t> dlog = host::dlog_map(*p);
     std::vector<uint64_t> tv = host::test_vectors(*p);
@@ -297,6 +301,7 @@ extern "C" int pz_ctx_create(const pz_params *p, int device, pz_ctx **out)
     d.tb.fw = reinterpret_cast<const u64x2 *>(c->d_tables);
     d.tb.iw = reinterpret_cast<const u64x2 *>(c->d_tables + 2 * N);
     d.tb.fw63 = reinterpret_cast<const u64x2 *>(c->d_tables + 4 * N);
+    d.tb.iw63 = reinterpret_cast<const u64x2 *>(c->d_tables + 6 * N);
     d.tb.n_inv = c->ht.n_inv;
     d.tb.n_inv_s = c->ht.n_inv_s;
     d.tb.q = p->rlwe_q;

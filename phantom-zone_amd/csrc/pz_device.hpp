@@ -526,12 +526,12 @@ PZ_DEV void step_split(u64 (&p)[R], const u64 *__restrict__ K, const u64 *__rest
         for (int r = 0; r < R; r++) D[0][r] = theirs[lds_ix(eval_x<LOGN, R>(j, r))];
     }
     // after the key row, which is needed first
-    if constexpr ((PF & 6) == 2) inv_tw_first<LOGN, R>(tw, j, c.tb);
+    if constexpr ((PF & 6) == 2) inv_tw_first<LOGN, R, Q4>(tw, j, c.tb);
     __syncthreads();  // both halves have read: the regions may be rewritten
     if (!is_auto) fma_slice<PZ_SPLIT_SM, !PZ_LAZY_MAC, Q4>(acc[0], k2, D[0], q, q2);
     constexpr int IBND = Q4 ? 8 : (PZ_LAZY_MAC ? 4 : 2);  // accumulators below IBND q
     if (PRIO && !is_auto) __builtin_amdgcn_s_setprio(2);
-    if constexpr ((PF & 6) == 4) inv_tw_first<LOGN, R>(tw, j, c.tb);
+    if constexpr ((PF & 6) == 4) inv_tw_first<LOGN, R, Q4>(tw, j, c.tb);
     if constexpr ((PF & 6) != 0)
         ntt_inv_pf<LOGN, 1, !LOOSE, false, PZ_SPLIT_SM, PZ_INV_FAST, IBND, R, false>(acc, j, mine,
                                                                                    c.tb, tw);

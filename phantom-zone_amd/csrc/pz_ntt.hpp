@@ -61,6 +61,7 @@ PZ_DEV u64x2 gld2(const u64x2 *p)
 struct Tables {
     const u64x2 *__restrict__ fw;  // forward  (psi^bo, shoup)
     const u64x2 *__restrict__ fw63;  // forward  (psi^bo, floor(psi^bo 2^63 / q)): FWD_FAST6
+    const u64x2 *__restrict__ iw63;  // inverse  (psi^-bo, floor(psi^-bo 2^63 / q)): INV6 stages
     const u64x2 *__restrict__ iw;  // inverse  (psi^-bo, shoup)
     u64 n_inv, n_inv_s;
     u64 q, q2;
@@ -85,6 +86,14 @@ enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3, FWD_FAST6 = 4 
 #ifndef PZ_INV_TAIL4
 #define PZ_INV_TAIL4 1
 #endif
+// The IB = 8 fast inverse's products before stage KR in the [0,6q) form
+// (shoup_quot6, twins from tb.iw63, modulus 2q; see inv_stage)
+#ifndef PZ_INV6
+#define PZ_INV6 1
+#endif
+// executed stage k (1-based) of an IB = 8 fast inverse takes the 2^63 twin
+template <int LOGN, int S>
+constexpr bool inv6_stage() { return PZ_INV6 && LOGN - S < 7; }
 
 // One forward stage S inside a phase of stride ST. KIND: 0 = 0_to_4q
 // butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q),
@@ -184,7 +193,8 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *
         if (r & LD) continue;
         const u32 x = base + ST * r;
         const u32 i = x / (2 * T);
-        const u64x2 tw = PRE ? pre[r] : gld2(tb.iw + M + i);
+        constexpr bool I6 = FAST && IB == 8 && inv6_stage<LOGN, S>();
+        const u64x2 tw = PRE ? pre[r] : gld2((I6 ? tb.iw63 : tb.iw) + M + i);
         const u64 w = tw.x, ws = tw.y;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -200,8 +210,11 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *
                     v[p][r] = K != KR ? sum
                                       : (ONE32 ? shoup_one(sum, (u32)tb.one_s, tb.q)
                                                : shoup_mul<SM>(sum, 1, tb.one_s, tb.q));
-                    v[p][r + LD] = (IB == 8 && K < KR) ? shoup_mul4(dif, w, ws, tb.q)
-                                                       : shoup_mul<SM>(dif, w, ws, tb.q);
+                    if constexpr (I6)  // dif < 2^(K+3) q <= 2^9 q < 2^63; [0,6q) <= 2^K IB q
+                        v[p][r + LD] = shoup_rem(0, dif, w, shoup_quot6(dif, ws), tb.q2);
+                    else
+                        v[p][r + LD] = (IB == 8 && K < KR) ? shoup_mul4(dif, w, ws, tb.q)
+                                                           : shoup_mul<SM>(dif, w, ws, tb.q);
                 }
             } else if constexpr (T4) {
                 const u64 xv = v[p][r], yv = v[p][r + LD];
@@ -251,16 +264,20 @@ struct PhaseTw {
     u64x2 t[3][R];
 };
 // TAB: 0 = forward (tb.fw), 1 = inverse (tb.iw), 2 = forward with 2^63
-// twins (tb.fw63, FWD_FAST6)
+// twins (tb.fw63, FWD_FAST6), 3 = inverse of an IB = 8 fast transform (tb.iw63
+// for its stages before KR, tb.iw after)
 template <int LOGN, int S0, int R, int TAB>
 PZ_DEV void phase_tw(PhaseTw<R> &pt, u32 j, const Tables &tb)
 {
     using P = Phase<LOGN, S0, R>;
     const u32 base = P::base(j);
-    const u64x2 *tab = TAB == 1 ? tb.iw : (TAB == 2 ? tb.fw63 : tb.fw);
 #pragma unroll
     for (int e = 0; e < P::NS; e++) {
         const int S = S0 + e;
+        const u64x2 *tab = TAB == 2                                  ? tb.fw63
+                         : TAB == 3 && PZ_INV6 && LOGN - S < 7       ? tb.iw63
+                         : TAB == 1 || TAB == 3                      ? tb.iw
+                                                                     : tb.fw;
         const int T = (1 << LOGN) >> (S + 1);
         const int LD = T / P::ST;
 #pragma unroll
@@ -384,7 +401,8 @@ PZ_DEV Tables fresh(const Tables &tb)
     asm volatile("" : "+s"(t.fw), "+s"(t.iw));
     if constexpr (F63) {
         t.fw63 = uniform_ptr(t.fw63);
-        asm volatile("" : "+s"(t.fw63));
+        t.iw63 = uniform_ptr(t.iw63);
+        asm volatile("" : "+s"(t.fw63), "+s"(t.iw63));
     }
 #endif
     return t;
@@ -469,11 +487,14 @@ PZ_DEV void inv_from(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb,
 {
     constexpr int LR = RadixLog<R>::value;
     if constexpr (PF) inv_phase<LOGN, S0, NP, STRICT, SCALE, SM, FAST, IB, R, true>(v, j, tb, pt);
-    else inv_phase<LOGN, S0, NP, STRICT, SCALE, SM, FAST, IB, R>(v, j, fresh(tb));
+    else
+        inv_phase<LOGN, S0, NP, STRICT, SCALE, SM, FAST, IB, R>(
+            v, j, fresh<FAST && IB == 8 && PZ_INV6>(tb));
     if constexpr (S0 > 0) {
         if constexpr (PF) {
             PhaseTw<R> nt;
-            phase_tw<LOGN, S0 - LR, R, 1>(nt, j, fresh(tb));
+            phase_tw<LOGN, S0 - LR, R, FAST && IB == 8 ? 3 : 1>(
+                nt, j, fresh<FAST && IB == 8 && PZ_INV6>(tb));
             __builtin_amdgcn_sched_barrier(0);  // issued ahead of the exchange
             exchange<LOGN, S0, S0 - LR, NP, R, TAIL>(v, j, lds);
             inv_from<LOGN, S0 - LR, NP, STRICT, SCALE, SM, FAST, IB, R, TAIL, PF>(v, j, lds, tb,
@@ -499,11 +520,12 @@ PZ_DEV void ntt_inv(u64 (&v)[NP][R], u32 j, u64 *lds, const Tables &tb)
 
 // The inverse with every phase's twiddles loaded ahead of its exchange; pt
 // holds the first phase's (inv_tw_first).
-template <int LOGN, int R>
+// I6: for an IB = 8 fast inverse (the 2^63 twins of its early stages)
+template <int LOGN, int R, bool I6 = false>
 PZ_DEV void inv_tw_first(PhaseTw<R> &pt, u32 j, const Tables &tb)
 {
     constexpr int LR = RadixLog<R>::value;
-    phase_tw<LOGN, ((LOGN - 1) / LR) * LR, R, 1>(pt, j, fresh(tb));
+    phase_tw<LOGN, ((LOGN - 1) / LR) * LR, R, I6 ? 3 : 1>(pt, j, fresh<I6 && PZ_INV6>(tb));
 }
 template <int LOGN, int NP, bool STRICT, bool SCALE = true, bool SM = true, bool FAST = false,
           int IB = 2, int R = 8, bool TAIL = true>
