@@ -69,3 +69,24 @@ def test_fast6_forward_bound():
         assert bound < 2 ** 63
         bound += 6 * Q
     assert bound == 68 * Q and bound < 2 ** 63
+
+
+def test_inv6_stage_bounds():
+    """The IB = 8 fast inverse of the split step: executed stage k receives
+    values below 2^(k-1) 8q and forms dif = x + 2^(k+2) q - y.  Stages 1-6 take
+    the 2^63-twin product, so dif must stay below 2^63 there and the [0, 6q)
+    product within the next stage's input bound 2^k 8q; stage 7 (KR) may see
+    dif up to 2^10 q, beyond the 2^63 operand range, and keeps the 2^64 twin."""
+    for k in range(1, 7):
+        inb = 2 ** (k - 1) * 8 * Q
+        dif_max = inb + 2 ** (k + 2) * Q
+        assert dif_max <= 2 ** (k + 3) * Q < 2 ** 63
+        assert 6 * Q <= 2 ** k * 8 * Q
+    assert 2 ** 6 * 8 * Q + 2 ** 9 * Q > 2 ** 63  # stage 7 stays on the 2^64 twin
+    # the product itself at the largest stage-6 difference
+    r = rng(93)
+    for w in [int(v) for v in r.integers(0, Q, size=300, dtype=np.uint64)] + [1, Q - 1]:
+        t = (w << 63) // Q
+        for a in (2 ** 9 * Q - 1, 2 ** 9 * Q - 2 ** 40, 2 ** 62):
+            res = rem(0, a, w, quot6(a, t), 2 * Q)
+            assert res < 6 * Q and res % Q == (a * w) % Q
