@@ -86,14 +86,20 @@ enum { FWD_LAZY = 0, FWD_STRICT = 1, FWD_FAST = 2, FWD_FAST4 = 3, FWD_FAST6 = 4 
 #ifndef PZ_INV_TAIL4
 #define PZ_INV_TAIL4 1
 #endif
-// The IB = 8 fast inverse's products before stage KR in the [0,6q) form
-// (shoup_quot6, twins from tb.iw63, modulus 2q; see inv_stage)
+// The IB = 8 fast inverse's products in the [0,6q) form (shoup_quot6, twins
+// from tb.iw63, modulus 2q; see inv_stage) in every stage but KR = 7 and the
+// last
 #ifndef PZ_INV6
 #define PZ_INV6 1
 #endif
-// executed stage k (1-based) of an IB = 8 fast inverse takes the 2^63 twin
+// executed stage k = LOGN - S (1-based) of an IB = 8 fast inverse takes the
+// 2^63 twin: before KR its difference is below 2^(k+3) q <= 2^9 q < 2^63 and
+// [0,6q) is within the next stage's bound 2^k 8q; in the unreduced tail (KR <
+// k < LOGN) inputs are below 2^(k-6) q, the difference below 2^(k-5) q, and the
+// next stage's bound 2^(k-5) q >= 8q holds [0,6q); stage KR's difference can
+// reach 2^10 q, beyond 2^63
 template <int LOGN, int S>
-constexpr bool inv6_stage() { return PZ_INV6 && LOGN - S < 7; }
+constexpr bool inv6_stage() { return PZ_INV6 && (LOGN - S < 7 || (LOGN - S > 7 && S > 0)); }
 
 // One forward stage S inside a phase of stride ST. KIND: 0 = 0_to_4q
 // butterfly, 1 = last-stage 0_to_2q (lazy), 2 = last stage + reduce to [0,q),
@@ -228,7 +234,10 @@ PZ_DEV void inv_stage(u64 (&v)[NP][R], u32 base, const Tables &tb, const u64x2 *
                     v[p][r + LD] = STRICT ? umin(yo, yo - tb.q) : yo;
                 } else {
                     v[p][r] = sum;
-                    v[p][r + LD] = shoup_mul4(dif, w, ws, tb.q);
+                    if constexpr (I6)  // [0,6q) within the next stage's 2^(K-KR+2) q >= 8q
+                        v[p][r + LD] = shoup_rem(0, dif, w, shoup_quot6(dif, ws), tb.q2);
+                    else
+                        v[p][r + LD] = shoup_mul4(dif, w, ws, tb.q);
                 }
             } else {
                 ib_2q<SM>(v[p][r], v[p][r + LD], w, ws, tb.q, tb.q2);
@@ -275,7 +284,7 @@ PZ_DEV void phase_tw(PhaseTw<R> &pt, u32 j, const Tables &tb)
     for (int e = 0; e < P::NS; e++) {
         const int S = S0 + e;
         const u64x2 *tab = TAB == 2                                  ? tb.fw63
-                         : TAB == 3 && PZ_INV6 && LOGN - S < 7       ? tb.iw63
+                         : TAB == 3 && PZ_INV6 && (LOGN - S < 7 || (LOGN - S > 7 && S > 0)) ? tb.iw63
                          : TAB == 1 || TAB == 3                      ? tb.iw
                                                                      : tb.fw;
         const int T = (1 << LOGN) >> (S + 1);

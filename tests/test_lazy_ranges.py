@@ -76,13 +76,21 @@ def test_inv6_stage_bounds():
     values below 2^(k-1) 8q and forms dif = x + 2^(k+2) q - y.  Stages 1-6 take
     the 2^63-twin product, so dif must stay below 2^63 there and the [0, 6q)
     product within the next stage's input bound 2^k 8q; stage 7 (KR) may see
-    dif up to 2^10 q, beyond the 2^63 operand range, and keeps the 2^64 twin."""
+    dif up to 2^10 q, beyond the 2^63 operand range, and keeps the 2^64 twin;
+    the tail stages 8-10 take the 2^63 twin again."""
     for k in range(1, 7):
         inb = 2 ** (k - 1) * 8 * Q
         dif_max = inb + 2 ** (k + 2) * Q
         assert dif_max <= 2 ** (k + 3) * Q < 2 ** 63
         assert 6 * Q <= 2 ** k * 8 * Q
     assert 2 ** 6 * 8 * Q + 2 ** 9 * Q > 2 ** 63  # stage 7 stays on the 2^64 twin
+    # the unreduced tail (stages 8-10 of 11): inputs below 2^(k-6) q, the
+    # difference x + 2^(k-6) q - y below 2^(k-5) q; the next stage's bound
+    # 2^(k-5) q takes both the sum and the [0, 6q) product
+    for k in range(8, 11):
+        inb = 2 ** (k - 6) * Q
+        assert inb + 2 ** (k - 6) * Q <= 2 ** (k - 5) * Q < 2 ** 63
+        assert 2 * inb <= 2 ** (k - 5) * Q and 6 * Q <= 2 ** (k - 5) * Q
     # the product itself at the largest stage-6 difference
     r = rng(93)
     for w in [int(v) for v in r.integers(0, Q, size=300, dtype=np.uint64)] + [1, Q - 1]:
